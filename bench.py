@@ -1,0 +1,199 @@
+"""Frames/s of the deferred strand-rasterization path on one MI355X.
+
+A step is one full hair_pass of one groom over resident inputs: clear, camera
+LOD/scan/raster, DOM clear/LOD/scan/raster x2, shade, filter. The default
+workload is the 1080p, 100k-strand x 64-segment wavy groom with a 2048^2 x 16
+deep opacity map and 1 directional + 32 point lights.
+
+Prints one JSON line (rank 0). Per-kernel HIP events are recorded on the
+library's own stream across the timed frames; the dominant kernel's mean
+duration gives the roofline entry. The CPU baseline (the build's C++
+restatement, multithreaded) is timed on a bounded sample of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "1080p frames/s (100k×64 groom): plain vs MLP shading at ≤2% err; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(kernel: str, w, mesh, counts: dict, hit_pixels: int) -> float:
+    """Algorithmic HBM bytes of one launch (per-unit figures x units)."""
+    P = w.width * w.height
+    S = w.dom_size
+    B, N = mesh.bundle_count, mesh.strand_count
+    # Each strand reads its ref (16 B), its bundle (16 B), 64 cage positions and
+    # 64 style texels (16 B each); bundles' data is compulsory once per launch.
+    groom = B * (16 + 64 * 32 + 1024 * 16) + 16 * N
+    if kernel == "clear":
+        return 20.0 * P
+    if kernel == "clear_dom":
+        return 68.0 * S * S
+    if kernel == "cam_raster":
+        c = counts["cam"]
+        return 8.0 * c[0] + 12.0 * c[1] + groom
+    if kernel == "dom_depth":
+        return 4.0 * counts["dom1"][2] + groom
+    if kernel == "dom_opacity":
+        return 8.0 * counts["dom2"][2] + groom
+    if kernel == "shade":
+        return 8.0 * P + 20.0 * P + hit_pixels * (4.0 + 272.0)
+    if kernel == "filter":
+        return 56.0 * P
+    return 0.0
+
+
+def cpu_baseline(sc, w, threads: int, fraction: int) -> dict:
+    """The oracle on 1/fraction of the frame's work, scaled back to a full frame."""
+    import oracle
+
+    ref = oracle.Frame(sc.mesh, w.width, w.height, w.dom_size)
+    cap = max(1, sc.mesh.strand_count)
+    cam = sc.params[0]
+    pa, n_p, da, n_d = sc.light_arrays()
+    t = {}
+    t0 = time.perf_counter()
+    ref.clear()
+    lod = ref.lod(cam, w.control_points, w.strand_fraction)
+    refs, _ = ref.scan(cam, lod, cap)
+    t["setup"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ref.raster(cam, refs[::fraction].copy(), threads=threads)
+    t["cam_raster"] = (time.perf_counter() - t0) * fraction
+    ref.clear_dom()
+    dlod = ref.lod(sc.params[1])
+    drefs, _ = ref.scan(sc.params[1], dlod, cap)
+    sub = drefs[::fraction].copy()
+    t0 = time.perf_counter()
+    ref.raster(sc.params[1], sub, threads=threads)
+    ref.raster(sc.params[2], sub, threads=threads)
+    t["dom"] = (time.perf_counter() - t0) * fraction
+    # shade and filter a band of rows through the middle of the image
+    r0 = w.height // 2 - w.height // (2 * fraction)
+    rows = (r0, r0 + w.height // fraction)
+    t0 = time.perf_counter()
+    ref.shade(cam, pa, n_p, da, n_d, threads=threads, rows=rows)
+    t["shade"] = (time.perf_counter() - t0) * fraction
+    t0 = time.perf_counter()
+    ref.filter(cam, threads=threads, rows=rows)
+    t["filter"] = (time.perf_counter() - t0) * fraction
+    frame_s = sum(t.values())
+    return {"value": round(1.0 / frame_s, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ restatement, not lavapipe) on every {fraction}th strand of the camera and "
+                      f"DOM passes and {rows[1] - rows[0]} of {w.height} rows of shade+filter, scaled x{fraction}",
+            "seconds_per_frame": {k: round(v, 3) for k, v in t.items()}}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-fraction", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this path runs on a single MI355X (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        return 2
+
+    from bevy_hair_amd import _native as N
+    from bevy_hair_amd import plugin
+    from bevy_hair_amd.workload import make_scene
+
+    t_setup = time.perf_counter()
+    sc = make_scene(args.workload)
+    w = sc.w
+    print(f"[bench] workload {w.name}: {sc.mesh.strand_count} strands in {sc.mesh.bundle_count} bundles, "
+          f"setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
+    r = plugin.Renderer(0)
+    r.targets(w.width, w.height, w.dom_size)
+    r.set_scene_depth(None, 0.0)
+    r.set_lights(sc.points, sc.dirs)
+    g = r.upload(sc.mesh)
+    opts = r.frame_opts(control_points=w.control_points, strand_fraction=w.strand_fraction)
+    for _ in range(args.warmup):
+        r.hair_pass(g, sc.params, opts)
+    r.sync()
+    # Work counts of each raster pass (untimed counting launches).
+    r.clear()
+    r.lod(g, sc.params[0], N.PASS_CAMERA, w.control_points, w.strand_fraction)
+    r.scan(g, sc.params[0], N.PASS_CAMERA)
+    counts = {"cam": r.raster_count(g, sc.params[0], N.PASS_CAMERA)}
+    r.hair_pass(g, sc.params, opts)
+    r.lod(g, sc.params[1], N.PASS_DOM)
+    r.scan(g, sc.params[1], N.PASS_DOM)
+    counts["dom1"] = r.raster_count(g, sc.params[1], N.PASS_DOM)
+    counts["dom2"] = r.raster_count(g, sc.params[2], N.PASS_DOM)
+    center, _, _ = r.gbuffer()
+    hit = int((center != np.uint64(0xFFFFFFFFFFFFFFFF)).sum())
+    r.sync()
+
+    # Timed region: K frames between two events, per-kernel events inside.
+    r.timing(args.steps)
+    r.sync()
+    ms_total = r.time_frames(g, sc.params, opts, args.steps)
+    kt = r.kernel_times()
+    r.timing(0)
+    ms = ms_total / args.steps
+    fps = 1000.0 / ms
+
+    dom_k = max(kt, key=kt.get)
+    alg = algorithmic_bytes(dom_k, w, sc.mesh, counts, hit)
+    achieved = alg / (kt[dom_k] * 1e-3) / 1e9
+    atomics_cam = float(counts["cam"][0] + 2 * counts["cam"][1])
+    result = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (u64/u32 integer keys and counts)",
+        "data": "synthetic",
+        "config": {"workload": w.name, "strands": sc.mesh.strand_count, "bundles": sc.mesh.bundle_count,
+                   "segments_per_strand": w.control_points - 1, "width": w.width, "height": w.height,
+                   "dom_size": w.dom_size, "dom_layers": 16, "point_lights": len(sc.points),
+                   "directional_lights": len(sc.dirs), "shading": "plain", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "algorithmic_bytes": alg, "mean_ms": round(kt[dom_k], 5), "traffic": None},
+        "kernels_ms": {k: round(v, 5) for k, v in kt.items()},
+        "work": {"camera_writes": int(counts["cam"][0]), "camera_center_hits": int(counts["cam"][1]),
+                 "camera_atomics": int(atomics_cam), "dom_depth_writes": int(counts["dom1"][2]),
+                 "dom_opacity_increments": int(counts["dom2"][2]), "segments": int(counts["cam"][3]),
+                 "hit_pixels": hit},
+        "atomic_rates": {
+            "cam_raster_atomics_per_s": round(atomics_cam / (kt["cam_raster"] * 1e-3), 1) if kt["cam_raster"] else None,
+            "dom_opacity_atomics_per_s": round(float(counts["dom2"][2]) / (kt["dom_opacity"] * 1e-3), 1)
+            if kt["dom_opacity"] else None},
+    }
+    g.free()
+    r.close()
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        print(f"[bench] cpu baseline on {threads} threads ...", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(sc, w, threads, args.cpu_fraction)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

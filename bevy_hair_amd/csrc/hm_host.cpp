@@ -1,0 +1,935 @@
+// Host side of the C ABI (include/hair_mi.h): context and buffer ownership,
+// HairParams construction and the per-groom dispatch sequence of hair_pass.
+//
+// Reference (paths relative to the reference repository):
+//   GpuGroom::upload   src/render/pass.rs:98-167
+//   ensure_targets     src/render/pass.rs:789-815
+//   hair_pass          src/render/pass.rs:375-643 (per-groom sequence)
+//   make_params        src/render/pass.rs:676-709
+//   key_direction      src/render/pass.rs:711-719
+//   world_extent       src/render/pass.rs:721-745
+//   light_fit          src/render/pass.rs:747-776
+//   depth_is_reverse   src/render/pass.rs:778-787
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hm_kernels.h"
+
+struct hm_groom {
+    uint32_t nb = 0, nl = 0, ns = 0, strands = 0, capacity = 0;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    hm_bundle* bundles = nullptr;
+    hm_layer* layers = nullptr;
+    float* style = nullptr;
+    hm_lod_record* lod[2] = {nullptr, nullptr};
+    hm_strand_ref* refs[2] = {nullptr, nullptr};
+    uint32_t* prefix[2] = {nullptr, nullptr};
+    uint32_t* indirect[2] = {nullptr, nullptr};
+};
+
+struct hm_ctx {
+    int device = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    uint32_t width = 0, height = 0, dom_size = 0;
+    void* gb_slab = nullptr;  // center | conservative | beta, cleared by one fill
+    size_t gb_words = 0;
+    uint64_t* center = nullptr;
+    uint64_t* cons = nullptr;
+    uint32_t* beta = nullptr;
+    float4* shaded = nullptr;
+    float* shaded_depth = nullptr;
+    float4* filtered = nullptr;
+    float* filtered_depth = nullptr;
+    uint32_t* dom = nullptr;
+    float* scene_depth = nullptr;
+    float scene_clear = 0.0f;
+    hm_point_light* points = nullptr;
+    uint32_t point_count = 0, point_cap = 0;
+    hm_directional_light* dirs = nullptr;
+    uint32_t dir_count = 0, dir_cap = 0;
+    uint64_t* counters = nullptr;
+    // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
+    uint32_t slots = 0, frame = 0;
+    std::vector<hipEvent_t> ev;
+    std::vector<uint8_t> ran;
+    hipEvent_t frame_ev[2] = {nullptr, nullptr};
+    std::string err;
+};
+
+namespace {
+
+hm_status fail(hm_ctx* ctx, hm_status st, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return st;
+}
+
+hm_status hip_fail(hm_ctx* ctx, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(ctx, e == hipErrorOutOfMemory ? HM_ERR_OOM : HM_ERR_HIP, m);
+}
+
+#define HM_HIP(ctx, call)                                              \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #call);       \
+    } while (0)
+
+template <class T> hipError_t dev_alloc(T** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    return hipMalloc(reinterpret_cast<void**>(p), bytes);
+}
+
+template <class T> void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// ---- column-major float mat4 helpers (glam conventions) -------------------------
+
+struct M4 { float m[16]; };
+struct V3 { float x, y, z; };
+
+V3 v3(float x, float y, float z) { return {x, y, z}; }
+V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3 scale(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+float length(V3 a) { return std::sqrt(dot(a, a)); }
+V3 cross(V3 a, V3 b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+V3 normalize_or(V3 v, V3 fallback) {  // glam Vec3::normalize_or
+    float rcp = 1.0f / length(v);
+    if (std::isfinite(rcp) && rcp > 0.0f) return scale(v, rcp);
+    return fallback;
+}
+
+M4 mul(const M4& a, const M4& b) {
+    M4 r;
+    for (int c = 0; c < 4; ++c)
+        for (int row = 0; row < 4; ++row)
+            r.m[c * 4 + row] = a.m[0 * 4 + row] * b.m[c * 4 + 0] + a.m[1 * 4 + row] * b.m[c * 4 + 1] +
+                               a.m[2 * 4 + row] * b.m[c * 4 + 2] + a.m[3 * 4 + row] * b.m[c * 4 + 3];
+    return r;
+}
+
+void mul4(const M4& a, const float v[4], float out[4]) {
+    for (int row = 0; row < 4; ++row)
+        out[row] = a.m[0 * 4 + row] * v[0] + a.m[1 * 4 + row] * v[1] + a.m[2 * 4 + row] * v[2] + a.m[3 * 4 + row] * v[3];
+}
+
+V3 transform_point3(const M4& a, V3 p) {
+    float v[4] = {p.x, p.y, p.z, 1.0f}, o[4];
+    mul4(a, v, o);
+    return {o[0], o[1], o[2]};
+}
+
+V3 transform_vector3(const M4& a, V3 p) {
+    float v[4] = {p.x, p.y, p.z, 0.0f}, o[4];
+    mul4(a, v, o);
+    return {o[0], o[1], o[2]};
+}
+
+// General 4x4 inverse by cofactor expansion.
+M4 inverse(const M4& a) {
+    const float* m = a.m;
+    float inv[16];
+    inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+             m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+             m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+             m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+              m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+             m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+             m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+             m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+              m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+             m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+             m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+              m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+              m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+             m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+             m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+              m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+              m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    float det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+    M4 r;
+    float rdet = 1.0f / det;
+    for (int i = 0; i < 16; ++i) r.m[i] = inv[i] * rdet;
+    return r;
+}
+
+M4 load_m4(const float* f) {
+    M4 r;
+    std::memcpy(r.m, f, sizeof(r.m));
+    return r;
+}
+
+void world_corners(const float bmin[3], const float bmax[3], const M4& wfm, V3 out[8]) {  // pass.rs:721-735
+    int index = 0;
+    for (int ix = 0; ix < 2; ++ix)
+        for (int iy = 0; iy < 2; ++iy)
+            for (int iz = 0; iz < 2; ++iz)
+                out[index++] = transform_point3(wfm, v3(ix ? bmax[0] : bmin[0], iy ? bmax[1] : bmin[1],
+                                                        iz ? bmax[2] : bmin[2]));
+}
+
+void world_extent(const float bmin[3], const float bmax[3], const M4& wfm, V3& center, float& radius) {
+    // pass.rs:737-745: mean of the 8 corners; radius folds from 1.0.
+    V3 c[8];
+    world_corners(bmin, bmax, wfm, c);
+    V3 sum = v3(0, 0, 0);
+    for (int i = 0; i < 8; ++i) sum = add(sum, c[i]);
+    center = scale(sum, 1.0f / 8.0f);
+    radius = 1.0f;
+    for (int i = 0; i < 8; ++i) radius = std::fmax(radius, length(sub(c[i], center)));
+}
+
+// Transform::looking_to(forward, up) then to_matrix().inverse(): the view
+// matrix whose -Z axis is `forward` (bevy_transform; glam look_to_rh).
+M4 look_to_view(V3 eye, V3 forward, V3 up) {
+    V3 back = scale(normalize_or(forward, v3(0, 0, -1)), -1.0f);
+    V3 right = normalize_or(cross(up, back), v3(1, 0, 0));
+    V3 upv = cross(back, right);
+    M4 v;
+    // rows of the rotation are right, up, back; translation -R * eye
+    v.m[0] = right.x; v.m[4] = right.y; v.m[8] = right.z; v.m[12] = -dot(right, eye);
+    v.m[1] = upv.x;   v.m[5] = upv.y;   v.m[9] = upv.z;   v.m[13] = -dot(upv, eye);
+    v.m[2] = back.x;  v.m[6] = back.y;  v.m[10] = back.z; v.m[14] = -dot(back, eye);
+    v.m[3] = 0;       v.m[7] = 0;       v.m[11] = 0;      v.m[15] = 1;
+    return v;
+}
+
+// Right-handed orthographic projection with depth in [0, 1] (glam
+// Mat4::orthographic_rh). Only its x, y and w rows reach the hair path.
+M4 orthographic(float l, float r, float b, float t, float n, float f) {
+    float rw = 1.0f / (r - l), rh = 1.0f / (t - b), rd = 1.0f / (n - f);
+    M4 m;
+    std::memset(m.m, 0, sizeof(m.m));
+    m.m[0] = rw + rw;
+    m.m[5] = rh + rh;
+    m.m[10] = rd;
+    m.m[12] = -(l + r) * rw;
+    m.m[13] = -(t + b) * rh;
+    m.m[14] = rd * n;
+    m.m[15] = 1.0f;
+    return m;
+}
+
+void light_fit(const float bmin[3], const float bmax[3], const M4& wfm, V3 direction_to_light, V3& eye, V3& fwd,
+               M4& clip, float& near_out, float& far_out) {  // pass.rs:747-776
+    V3 center;
+    float radius;
+    world_extent(bmin, bmax, wfm, center, radius);
+    V3 to_light = normalize_or(direction_to_light, v3(0, 1, 0));
+    fwd = scale(to_light, -1.0f);
+    eye = add(center, scale(to_light, radius));
+    float near_ = 3.402823466e38f, far_ = -3.402823466e38f;
+    V3 c[8];
+    world_corners(bmin, bmax, wfm, c);
+    for (int i = 0; i < 8; ++i) {
+        float depth = dot(sub(c[i], eye), fwd);
+        near_ = std::fmin(near_, depth);
+        far_ = std::fmax(far_, depth);
+    }
+    near_ = std::fmax(near_ - radius * 0.05f, 0.05f);
+    far_ = std::fmax(far_ + radius * 0.05f, near_ + 1.0f);
+    V3 up = std::fabs(dot(fwd, v3(0, 1, 0))) > 0.95f ? v3(0, 0, 1) : v3(0, 1, 0);
+    M4 view = look_to_view(eye, fwd, up);
+    M4 proj = orthographic(-radius, radius, -radius, radius, near_, far_);
+    clip = mul(proj, view);
+    near_out = near_;
+    far_out = far_;
+}
+
+float clip_z(const M4& clip, V3 p) {  // pass.rs:784-787
+    float v[4] = {p.x, p.y, p.z, 1.0f}, c[4];
+    mul4(clip, v, c);
+    return c[2] / std::fmax(c[3], 1e-5f);
+}
+
+struct ViewArgs {
+    M4 clip_from_world, world_from_clip, world_from_model, light_clip;
+    V3 camera_pos, camera_forward;
+    float screen[4];
+    V3 light_forward;
+    float dom_near, dom_far, dom_scale;
+    uint32_t pass_mode[4];
+    V3 light_eye;
+    uint32_t dom_size;
+};
+
+void make_params(const ViewArgs& a, const hm_groom_config& g, hm_params& out) {  // pass.rs:676-709
+    std::memset(&out, 0, sizeof(out));
+    std::memcpy(out.clip_from_world, a.clip_from_world.m, 64);
+    std::memcpy(out.world_from_clip, a.world_from_clip.m, 64);
+    std::memcpy(out.world_from_model, a.world_from_model.m, 64);
+    std::memcpy(out.light_clip_from_world, a.light_clip.m, 64);
+    float cp[4] = {a.camera_pos.x, a.camera_pos.y, a.camera_pos.z, 1.0f};
+    float cf[4] = {a.camera_forward.x, a.camera_forward.y, a.camera_forward.z, 0.0f};
+    std::memcpy(out.camera_pos, cp, 16);
+    std::memcpy(out.camera_forward, cf, 16);
+    std::memcpy(out.screen, a.screen, 16);
+    float app[4] = {g.roughness, g.tilt, g.lambda, g.center_diameter};
+    std::memcpy(out.appearance, app, 16);
+    float fp[4] = {1.0f, 0.2f, std::fmax(a.dom_scale, 1e-3f), HM_DOM_SIGMA};
+    std::memcpy(out.filter_params, fp, 16);
+    float alb[4] = {g.albedo_linear[0], g.albedo_linear[1], g.albedo_linear[2], 1.0f};
+    std::memcpy(out.albedo, alb, 16);
+    uint32_t flags[4] = {g.lod ? 1u : 0u, g.filter ? 1u : 0u, g.ambient_occlusion ? 1u : 0u, g.deep_opacity ? 1u : 0u};
+    std::memcpy(out.flags, flags, 16);
+    std::memcpy(out.pass_mode, a.pass_mode, 16);
+    float le[4] = {a.light_eye.x, a.light_eye.y, a.light_eye.z, 1.0f};
+    float lf[4] = {a.light_forward.x, a.light_forward.y, a.light_forward.z, 0.0f};
+    std::memcpy(out.light_eye, le, 16);
+    std::memcpy(out.light_forward, lf, 16);
+    float di[4] = {(float)a.dom_size, (float)a.dom_size, a.dom_near, a.dom_far};
+    std::memcpy(out.dom_info, di, 16);
+}
+
+void free_targets(hm_ctx* c) {
+    dev_free(c->gb_slab);
+    c->center = nullptr;
+    c->cons = nullptr;
+    c->beta = nullptr;
+    dev_free(c->shaded);
+    dev_free(c->shaded_depth);
+    dev_free(c->filtered);
+    dev_free(c->filtered_depth);
+    dev_free(c->dom);
+    dev_free(c->scene_depth);
+    c->width = c->height = c->dom_size = 0;
+}
+
+int raster_blocks(const hm_ctx* c, uint32_t capacity) {
+    uint32_t want = (capacity + 3u) / 4u;
+    uint32_t cap = (uint32_t)c->cu_count * 16u;
+    return (int)(want < cap ? (want ? want : 1u) : cap);
+}
+
+// Records an event pair around one kernel into the current frame slot.
+struct Timer {
+    hm_ctx* c;
+    size_t base = 0;
+    bool on = false;
+    Timer(hm_ctx* ctx, int kernel) : c(ctx) {
+        uint32_t slot = c->frame;
+        on = c->slots && slot < c->slots;
+        if (!on) return;
+        base = ((size_t)slot * HM_K_COUNT + (size_t)kernel) * 2;
+        (void)hipEventRecord(c->ev[base], c->stream);
+    }
+    ~Timer() {
+        if (!on) return;
+        (void)hipEventRecord(c->ev[base + 1], c->stream);
+        c->ran[base / 2] = 1;
+    }
+};
+
+hm_status check_targets(hm_ctx* ctx) {
+    if (!ctx) return HM_ERR_INVALID;
+    if (!ctx->gb_slab) return fail(ctx, HM_ERR_NOT_READY, "targets not allocated (hm_targets)");
+    return HM_OK;
+}
+
+hm_status check_pass_params(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass) {
+    if (!ctx || !g || !p || (pass != HM_PASS_CAMERA && pass != HM_PASS_DOM))
+        return fail(ctx, HM_ERR_INVALID, "null handle or bad pass");
+    if (p->pass_mode[1] > g->nb) return fail(ctx, HM_ERR_INVALID, "pass_mode.y exceeds the groom's bundle count");
+    return HM_OK;
+}
+
+}  // namespace
+
+// ---- ABI ------------------------------------------------------------------------
+
+extern "C" {
+
+const char* hm_status_string(hm_status s) {
+    switch (s) {
+        case HM_OK: return "ok";
+        case HM_ERR_INVALID: return "invalid argument";
+        case HM_ERR_OOM: return "out of device memory";
+        case HM_ERR_HIP: return "HIP runtime error";
+        case HM_ERR_NO_DEVICE: return "no usable gfx950 device";
+        case HM_ERR_NOT_READY: return "not ready";
+        case HM_ERR_CAPACITY: return "capacity exceeded";
+    }
+    return "unknown status";
+}
+
+const char* hm_last_error(const hm_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+hm_status hm_create(int device, hm_ctx** out) {
+    if (!out) return HM_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return HM_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return HM_ERR_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return HM_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return HM_ERR_NO_DEVICE;
+    hm_ctx* c = new hm_ctx();
+    c->device = device;
+    c->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return HM_ERR_HIP;
+    }
+    (void)hipEventCreate(&c->frame_ev[0]);
+    (void)hipEventCreate(&c->frame_ev[1]);
+    if (dev_alloc(&c->counters, 64) != hipSuccess) {
+        hm_destroy(c);
+        return HM_ERR_OOM;
+    }
+    *out = c;
+    return HM_OK;
+}
+
+void hm_destroy(hm_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_targets(c);
+    dev_free(c->points);
+    dev_free(c->dirs);
+    dev_free(c->counters);
+    for (hipEvent_t e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (int j = 0; j < 2; ++j)
+        if (c->frame_ev[j]) (void)hipEventDestroy(c->frame_ev[j]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+hm_status hm_groom_upload(hm_ctx* ctx, const hm_groom_desc* d, hm_groom** out) {
+    if (!ctx || !d || !out) return fail(ctx, HM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if ((d->bundle_count && !d->bundles) || (d->layer_count && !d->layers) || (d->style_count && !d->style))
+        return fail(ctx, HM_ERR_INVALID, "null array with nonzero count");
+    // Every bundle must address 16x4 corners and 8x8x16 texels inside the arrays.
+    uint64_t strands = 0;
+    for (uint32_t b = 0; b < d->bundle_count; ++b) {
+        const hm_bundle& B = d->bundles[b];
+        if ((uint64_t)B.layer_offset + HM_LAYER_COUNT * 4u > d->layer_count ||
+            (uint64_t)B.style_offset + HM_STYLE_TEXELS > d->style_count)
+            return fail(ctx, HM_ERR_INVALID, "bundle " + std::to_string(b) + " addresses outside the cage/style arrays");
+        strands += B.strand_count;
+    }
+    if (strands > 0x7fffffffull) return fail(ctx, HM_ERR_CAPACITY, "more than 2^31 strands");
+    (void)hipSetDevice(ctx->device);
+    hm_groom* g = new hm_groom();
+    // An empty mesh keeps one zero bundle, like the 16-byte placeholder
+    // buffers of GpuGroom::upload (pass.rs:118-119, pass.rs:170-175).
+    g->nb = d->bundle_count ? d->bundle_count : 1u;
+    g->nl = d->bundle_count ? d->layer_count : HM_LAYER_COUNT * 4u;
+    g->ns = d->bundle_count ? d->style_count : HM_STYLE_TEXELS;
+    g->strands = (uint32_t)strands;
+    g->capacity = g->strands ? g->strands : 1u;
+    std::memcpy(g->bmin, d->bounds_min, 12);
+    std::memcpy(g->bmax, d->bounds_max, 12);
+    hipError_t e = hipSuccess;
+    auto A = [&](auto** p, size_t bytes) {
+        if (e == hipSuccess) e = dev_alloc(p, bytes);
+    };
+    A(&g->bundles, (size_t)g->nb * sizeof(hm_bundle));
+    A(&g->layers, (size_t)g->nl * sizeof(hm_layer));
+    A(&g->style, (size_t)g->ns * 16);
+    for (int k = 0; k < 2; ++k) {
+        A(&g->lod[k], (size_t)g->nb * sizeof(hm_lod_record));
+        A(&g->refs[k], (size_t)g->capacity * sizeof(hm_strand_ref));
+        A(&g->prefix[k], ((size_t)g->nb + 1) * 4);
+        A(&g->indirect[k], 16);
+    }
+    if (e == hipSuccess) {
+        if (d->bundle_count) {
+            e = hipMemcpy(g->bundles, d->bundles, (size_t)g->nb * sizeof(hm_bundle), hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(g->layers, d->layers, (size_t)g->nl * sizeof(hm_layer), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(g->style, d->style, (size_t)g->ns * 16, hipMemcpyHostToDevice);
+        } else {
+            e = hipMemset(g->bundles, 0, sizeof(hm_bundle));
+            if (e == hipSuccess) e = hipMemset(g->layers, 0, (size_t)g->nl * sizeof(hm_layer));
+            if (e == hipSuccess) e = hipMemset(g->style, 0, (size_t)g->ns * 16);
+        }
+    }
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMemset(g->indirect[k], 0, 16);
+    if (e != hipSuccess) {
+        hm_groom_free(ctx, g);
+        return hip_fail(ctx, e, "hm_groom_upload");
+    }
+    *out = g;
+    return HM_OK;
+}
+
+void hm_groom_free(hm_ctx* ctx, hm_groom* g) {
+    if (!g) return;
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    dev_free(g->bundles);
+    dev_free(g->layers);
+    dev_free(g->style);
+    for (int k = 0; k < 2; ++k) {
+        dev_free(g->lod[k]);
+        dev_free(g->refs[k]);
+        dev_free(g->prefix[k]);
+        dev_free(g->indirect[k]);
+    }
+    delete g;
+}
+
+uint32_t hm_groom_bundle_count(const hm_groom* g) { return g ? g->nb : 0u; }
+uint32_t hm_groom_strand_count(const hm_groom* g) { return g ? g->strands : 0u; }
+
+hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_size) {
+    if (!ctx) return HM_ERR_INVALID;
+    if (width == 0 || height == 0) return fail(ctx, HM_ERR_NOT_READY, "zero-size target");  // pass.rs:320-322
+    if (dom_size == 0 || (dom_size & 1u) || dom_size > 16384u)
+        return fail(ctx, HM_ERR_INVALID, "dom_size must be even and in [2, 16384]");
+    if ((uint64_t)width * height > (1ull << 31)) return fail(ctx, HM_ERR_CAPACITY, "target larger than 2^31 pixels");
+    if (ctx->gb_slab && ctx->width == width && ctx->height == height && ctx->dom_size == dom_size) return HM_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    float keep_clear = ctx->scene_clear;
+    free_targets(ctx);
+    size_t px = (size_t)width * height;
+    size_t dom_words = (size_t)dom_size * dom_size * (1 + HM_DOM_LAYERS);
+    // center (8 B) | conservative (8 B) | beta (4 B, padded to 16 B)
+    size_t beta_words = (px + 3) & ~(size_t)3;
+    size_t gb_words = px * 2 + px * 2 + beta_words;
+    hipError_t e = dev_alloc(&ctx->gb_slab, gb_words * 4);
+    if (e == hipSuccess) e = dev_alloc(&ctx->shaded, px * 16);
+    if (e == hipSuccess) e = dev_alloc(&ctx->shaded_depth, px * 4);
+    if (e == hipSuccess) e = dev_alloc(&ctx->filtered, px * 16);
+    if (e == hipSuccess) e = dev_alloc(&ctx->filtered_depth, px * 4);
+    if (e == hipSuccess) e = dev_alloc(&ctx->dom, dom_words * 4);
+    if (e == hipSuccess) e = hipMemset(ctx->dom, 0, dom_words * 4);
+    if (e != hipSuccess) {
+        free_targets(ctx);
+        return hip_fail(ctx, e, "hm_targets");
+    }
+    uint32_t* slab = static_cast<uint32_t*>(ctx->gb_slab);
+    ctx->center = reinterpret_cast<uint64_t*>(slab);
+    ctx->cons = reinterpret_cast<uint64_t*>(slab + px * 2);
+    ctx->beta = slab + px * 4;
+    ctx->gb_words = gb_words;
+    ctx->width = width;
+    ctx->height = height;
+    ctx->dom_size = dom_size;
+    ctx->scene_clear = keep_clear;
+    return HM_OK;
+}
+
+hm_status hm_set_scene_depth(hm_ctx* ctx, const float* host_depth, float clear_value) {
+    if (!ctx) return HM_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    ctx->scene_clear = clear_value;
+    if (!host_depth) {
+        (void)hipStreamSynchronize(ctx->stream);
+        dev_free(ctx->scene_depth);
+        return HM_OK;
+    }
+    hm_status st = check_targets(ctx);
+    if (st != HM_OK) return st;
+    size_t bytes = (size_t)ctx->width * ctx->height * 4;
+    if (!ctx->scene_depth) HM_HIP(ctx, dev_alloc(&ctx->scene_depth, bytes));
+    HM_HIP(ctx, hipMemcpyAsync(ctx->scene_depth, host_depth, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HM_OK;
+}
+
+hm_status hm_set_lights(hm_ctx* ctx, const hm_point_light* points, uint32_t np, const hm_directional_light* dirs,
+                        uint32_t nd) {
+    if (!ctx || (np && !points) || (nd && !dirs)) return fail(ctx, HM_ERR_INVALID, "null light array");
+    (void)hipSetDevice(ctx->device);
+    HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (np > ctx->point_cap) {
+        dev_free(ctx->points);
+        ctx->point_cap = 0;
+        HM_HIP(ctx, dev_alloc(&ctx->points, (size_t)np * sizeof(hm_point_light)));
+        ctx->point_cap = np;
+    }
+    if (nd > ctx->dir_cap) {
+        dev_free(ctx->dirs);
+        ctx->dir_cap = 0;
+        HM_HIP(ctx, dev_alloc(&ctx->dirs, (size_t)nd * sizeof(hm_directional_light)));
+        ctx->dir_cap = nd;
+    }
+    if (np) HM_HIP(ctx, hipMemcpy(ctx->points, points, (size_t)np * sizeof(hm_point_light), hipMemcpyHostToDevice));
+    if (nd) HM_HIP(ctx, hipMemcpy(ctx->dirs, dirs, (size_t)nd * sizeof(hm_directional_light), hipMemcpyHostToDevice));
+    ctx->point_count = np;
+    ctx->dir_count = nd;
+    return HM_OK;
+}
+
+void hm_key_direction(const hm_directional_light* lights, uint32_t count, float out[3]) {  // pass.rs:711-719
+    V3 d = v3(0.35f, 0.82f, 0.45f);
+    const hm_directional_light* pick = nullptr;
+    for (uint32_t i = 0; i < count && !pick; ++i)
+        if (lights[i].flags & HM_LIGHT_FLAG_SHADOWS) pick = &lights[i];
+    if (!pick && count) pick = &lights[0];
+    if (pick) d = v3(pick->direction_to_light[0], pick->direction_to_light[1], pick->direction_to_light[2]);
+    d = normalize_or(d, v3(0, 1, 0));
+    out[0] = d.x;
+    out[1] = d.y;
+    out[2] = d.z;
+}
+
+hm_status hm_make_params(const hm_view* view, const hm_groom_config* groom, const float bounds_min[3],
+                         const float bounds_max[3], uint32_t bundle_count, const float direction_to_light[3],
+                         uint32_t width, uint32_t height, uint32_t dom_size, hm_params out[3]) {
+    if (!view || !groom || !bounds_min || !bounds_max || !direction_to_light || !out) return HM_ERR_INVALID;
+    M4 wfv = load_m4(view->world_from_view);
+    V3 camera_pos = transform_point3(wfv, v3(0, 0, 0));
+    V3 forward = normalize_or(scale(transform_vector3(wfv, v3(0, 0, 1)), -1.0f), v3(0, 0, -1));
+    M4 clip = load_m4(view->clip_from_world);
+    M4 wfc = inverse(clip);
+    bool reverse = clip_z(clip, add(camera_pos, scale(forward, 0.5f))) > clip_z(clip, add(camera_pos, scale(forward, 30.0f)));
+    M4 wfm = load_m4(groom->world_from_model);
+    V3 light_eye, light_dir;
+    M4 light_clip;
+    float light_near, light_far;
+    light_fit(bounds_min, bounds_max, wfm, v3(direction_to_light[0], direction_to_light[1], direction_to_light[2]),
+              light_eye, light_dir, light_clip, light_near, light_far);
+    V3 center;
+    float radius;
+    world_extent(bounds_min, bounds_max, wfm, center, radius);
+    float dist = std::fmax(length(sub(camera_pos, center)), 0.5f);
+    const float cam_near = 0.05f;
+    float cam_far = std::fmax(dist + radius * 4.0f, cam_near + 1.0f);
+    float dom_scale = (light_far - light_near) / (float)HM_LAYER_COUNT;
+
+    ViewArgs cam;
+    cam.clip_from_world = clip;
+    cam.world_from_clip = wfc;
+    cam.world_from_model = wfm;
+    cam.light_clip = light_clip;
+    cam.camera_pos = camera_pos;
+    cam.camera_forward = forward;
+    float scr[4] = {(float)width, (float)height, cam_near, cam_far};
+    std::memcpy(cam.screen, scr, 16);
+    cam.light_forward = light_dir;
+    cam.dom_near = light_near;
+    cam.dom_far = light_far;
+    cam.dom_scale = dom_scale;
+    uint32_t pm0[4] = {0u, bundle_count, reverse ? 1u : 0u, 0u};
+    std::memcpy(cam.pass_mode, pm0, 16);
+    cam.light_eye = light_eye;
+    cam.dom_size = dom_size;
+    make_params(cam, *groom, out[0]);
+
+    ViewArgs dom = cam;
+    dom.clip_from_world = light_clip;
+    dom.world_from_clip = inverse(light_clip);
+    dom.camera_pos = light_eye;
+    dom.camera_forward = light_dir;
+    float dscr[4] = {(float)dom_size, (float)dom_size, light_near, light_far};
+    std::memcpy(dom.screen, dscr, 16);
+    uint32_t pm1[4] = {1u, bundle_count, reverse ? 1u : 0u, 1u};
+    std::memcpy(dom.pass_mode, pm1, 16);
+    make_params(dom, *groom, out[1]);
+    out[2] = out[1];
+    out[2].pass_mode[0] = 2u;
+    return HM_OK;
+}
+
+void hm_shadow_light_from_params(const hm_params* dom_params, hm_shadow_light* out) {  // pass.rs:574-579
+    if (!dom_params || !out) return;
+    std::memcpy(out->light_clip_from_world, dom_params->clip_from_world, 64);
+    float eye[4] = {dom_params->light_eye[0], dom_params->light_eye[1], dom_params->light_eye[2], 1.0f};
+    float fwd[4] = {dom_params->light_forward[0], dom_params->light_forward[1], dom_params->light_forward[2], 0.0f};
+    float depth[4] = {dom_params->dom_info[2], dom_params->dom_info[3], HM_DOM_SIGMA, 1.0f};
+    std::memcpy(out->light_eye, eye, 16);
+    std::memcpy(out->light_forward, fwd, 16);
+    std::memcpy(out->depth, depth, 16);
+}
+
+hm_status hm_clear(hm_ctx* ctx) {
+    hm_status st = check_targets(ctx);
+    if (st != HM_OK) return st;
+    Timer t(ctx, HM_K_CLEAR);
+    HM_HIP(ctx, hm::launch_fill32(ctx->stream, static_cast<uint32_t*>(ctx->gb_slab), ctx->gb_words, 0xffffffffu,
+                                  ctx->cu_count * 8));
+    return HM_OK;
+}
+
+hm_status hm_clear_dom(hm_ctx* ctx) {
+    hm_status st = check_targets(ctx);
+    if (st != HM_OK) return st;
+    Timer t(ctx, HM_K_CLEAR_DOM);
+    size_t texels = (size_t)ctx->dom_size * ctx->dom_size;
+    HM_HIP(ctx, hm::launch_fill32(ctx->stream, ctx->dom, texels, 0xffffffffu, ctx->cu_count * 8));
+    HM_HIP(ctx, hm::launch_fill32(ctx->stream, ctx->dom + texels, texels * HM_DOM_LAYERS, 0u, ctx->cu_count * 8));
+    return HM_OK;
+}
+
+hm_status hm_lod(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass, const hm_lod_override* ovr) {
+    hm_status st = check_pass_params(ctx, g, p, pass);
+    if (st != HM_OK) return st;
+    Timer t(ctx, pass == HM_PASS_CAMERA ? HM_K_CAM_LOD : HM_K_DOM_LOD);
+    uint32_t oc = ovr ? ovr->control_points : 0u;
+    float of = ovr ? ovr->strand_fraction : 0.0f;
+    HM_HIP(ctx, hm::launch_lod(ctx->stream, *p, g->bundles, g->layers, g->lod[pass], p->pass_mode[1], oc, of));
+    return HM_OK;
+}
+
+hm_status hm_scan(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass) {
+    hm_status st = check_pass_params(ctx, g, p, pass);
+    if (st != HM_OK) return st;
+    Timer t(ctx, pass == HM_PASS_CAMERA ? HM_K_CAM_SCAN : HM_K_DOM_SCAN);
+    HM_HIP(ctx, hm::launch_scan(ctx->stream, g->lod[pass], p->pass_mode[1], g->capacity, g->prefix[pass],
+                                g->indirect[pass], g->refs[pass]));
+    return HM_OK;
+}
+
+static hm_status raster_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass, hm::RasterArgs& a) {
+    hm_status st = check_pass_params(ctx, g, p, pass);
+    if (st != HM_OK) return st;
+    st = check_targets(ctx);
+    if (st != HM_OK) return st;
+    uint32_t mode = p->pass_mode[0];
+    if (mode > 2u || (mode == 0u) != (pass == HM_PASS_CAMERA))
+        return fail(ctx, HM_ERR_INVALID, "pass_mode.x does not match the pass");
+    if (mode == 0u) {
+        if ((uint32_t)p->screen[0] != ctx->width || (uint32_t)p->screen[1] != ctx->height)
+            return fail(ctx, HM_ERR_INVALID, "camera params screen size differs from the targets");
+    } else if ((uint32_t)p->dom_info[0] != ctx->dom_size || (uint32_t)p->dom_info[1] != ctx->dom_size ||
+               (uint32_t)p->screen[0] != ctx->dom_size || (uint32_t)p->screen[1] != ctx->dom_size) {
+        return fail(ctx, HM_ERR_INVALID, "DOM params size differs from the targets' DOM size");
+    }
+    std::memcpy(&a.view, p, sizeof(hm_params));
+    a.bundles = g->bundles;
+    a.layers = g->layers;
+    a.style = g->style;
+    a.refs = g->refs[pass];
+    a.indirect = g->indirect[pass];
+    a.capacity = g->capacity;
+    a.center = ctx->center;
+    a.cons = ctx->cons;
+    a.beta = ctx->beta;
+    a.dom = ctx->dom;
+    a.scene_depth = ctx->scene_depth;
+    a.scene_clear = ctx->scene_clear;
+    a.counters = ctx->counters;
+    return HM_OK;
+}
+
+hm_status hm_raster(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass) {
+    hm::RasterArgs a;
+    hm_status st = raster_args(ctx, g, p, pass, a);
+    if (st != HM_OK) return st;
+    int mode = (int)p->pass_mode[0];
+    Timer t(ctx, mode == 0 ? HM_K_CAM_RASTER : (mode == 1 ? HM_K_DOM_DEPTH : HM_K_DOM_OPACITY));
+    HM_HIP(ctx, hm::launch_raster(ctx->stream, a, mode, false, raster_blocks(ctx, g->capacity)));
+    return HM_OK;
+}
+
+hm_status hm_raster_count(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass, uint64_t counts[4]) {
+    if (!counts) return fail(ctx, HM_ERR_INVALID, "null counts");
+    hm::RasterArgs a;
+    hm_status st = raster_args(ctx, g, p, pass, a);
+    if (st != HM_OK) return st;
+    HM_HIP(ctx, hm::launch_fill32(ctx->stream, reinterpret_cast<uint32_t*>(ctx->counters), 8, 0u, 1));
+    HM_HIP(ctx, hm::launch_raster(ctx->stream, a, (int)p->pass_mode[0], true, raster_blocks(ctx, g->capacity)));
+    HM_HIP(ctx, hipMemcpyAsync(counts, ctx->counters, 32, hipMemcpyDeviceToHost, ctx->stream));
+    HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HM_OK;
+}
+
+hm_status hm_shade(hm_ctx* ctx, const hm_params* p, float exposure) {
+    hm_status st = check_targets(ctx);
+    if (st != HM_OK) return st;
+    if (!p || (uint32_t)p->screen[0] != ctx->width || (uint32_t)p->screen[1] != ctx->height)
+        return fail(ctx, HM_ERR_INVALID, "shade params screen size differs from the targets");
+    if ((uint32_t)p->dom_info[0] != ctx->dom_size || (uint32_t)p->dom_info[1] != ctx->dom_size)
+        return fail(ctx, HM_ERR_INVALID, "shade params DOM size differs from the targets");
+    hm::ShadeArgs a;
+    std::memcpy(&a.view, p, sizeof(hm_params));
+    a.center = ctx->center;
+    a.beta = ctx->beta;
+    a.dom = ctx->dom;
+    a.points = ctx->points;
+    a.point_count = ctx->point_count;
+    a.directionals = ctx->dirs;
+    a.directional_count = ctx->dir_count;
+    a.exposure = exposure;
+    a.shaded = ctx->shaded;
+    a.shaded_depth = ctx->shaded_depth;
+    Timer t(ctx, HM_K_SHADE);
+    HM_HIP(ctx, hm::launch_shade(ctx->stream, a, ctx->width, ctx->height));
+    return HM_OK;
+}
+
+hm_status hm_filter(hm_ctx* ctx, const hm_params* p) {
+    hm_status st = check_targets(ctx);
+    if (st != HM_OK) return st;
+    if (!p || (uint32_t)p->screen[0] != ctx->width || (uint32_t)p->screen[1] != ctx->height)
+        return fail(ctx, HM_ERR_INVALID, "filter params screen size differs from the targets");
+    hm::FilterArgs a;
+    std::memcpy(&a.view, p, sizeof(hm_params));
+    a.center = ctx->center;
+    a.cons = ctx->cons;
+    a.shaded = ctx->shaded;
+    a.shaded_depth = ctx->shaded_depth;
+    a.filtered = ctx->filtered;
+    a.filtered_depth = ctx->filtered_depth;
+    Timer t(ctx, HM_K_FILTER);
+    HM_HIP(ctx, hm::launch_filter(ctx->stream, a, ctx->width, ctx->height));
+    return HM_OK;
+}
+
+hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm_frame_opts* opts) {
+    if (!ctx || !g || !params) return fail(ctx, HM_ERR_INVALID, "null argument");
+    hm_status st;
+    float exposure = opts ? opts->exposure : 1.0f;
+    const hm_lod_override* ovr = opts ? &opts->camera_lod : nullptr;
+    struct FrameSlot {
+        hm_ctx* c;
+        ~FrameSlot() {
+            if (c->slots && c->frame < c->slots) c->frame++;
+        }
+    } slot{ctx};
+    const hm_params& cam = params[0];
+    if ((st = hm_clear(ctx)) != HM_OK) return st;
+    if ((st = hm_lod(ctx, g, &cam, HM_PASS_CAMERA, ovr)) != HM_OK) return st;
+    if ((st = hm_scan(ctx, g, &cam, HM_PASS_CAMERA)) != HM_OK) return st;
+    if ((st = hm_raster(ctx, g, &cam, HM_PASS_CAMERA)) != HM_OK) return st;
+    if (cam.flags[3]) {
+        if ((st = hm_clear_dom(ctx)) != HM_OK) return st;
+        if ((st = hm_lod(ctx, g, &params[1], HM_PASS_DOM, nullptr)) != HM_OK) return st;
+        if ((st = hm_scan(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
+        if ((st = hm_raster(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
+        if ((st = hm_raster(ctx, g, &params[2], HM_PASS_DOM)) != HM_OK) return st;
+    }
+    if ((st = hm_shade(ctx, &cam, exposure)) != HM_OK) return st;
+    if (cam.flags[1]) {
+        if ((st = hm_filter(ctx, &cam)) != HM_OK) return st;
+    }
+    return HM_OK;
+}
+
+static bool target_buffer(hm_ctx* ctx, hm_groom* g, hm_buffer which, void** ptr, size_t* bytes) {
+    size_t px = (size_t)ctx->width * ctx->height;
+    size_t dom = (size_t)ctx->dom_size * ctx->dom_size * (1 + HM_DOM_LAYERS) * 4;
+    switch (which) {
+        case HM_BUF_CENTER: *ptr = ctx->center; *bytes = px * 8; return ctx->center != nullptr;
+        case HM_BUF_CONSERVATIVE: *ptr = ctx->cons; *bytes = px * 8; return ctx->cons != nullptr;
+        case HM_BUF_BETA: *ptr = ctx->beta; *bytes = px * 4; return ctx->beta != nullptr;
+        case HM_BUF_SHADED: *ptr = ctx->shaded; *bytes = px * 16; return ctx->shaded != nullptr;
+        case HM_BUF_SHADED_DEPTH: *ptr = ctx->shaded_depth; *bytes = px * 4; return ctx->shaded_depth != nullptr;
+        case HM_BUF_FILTERED: *ptr = ctx->filtered; *bytes = px * 16; return ctx->filtered != nullptr;
+        case HM_BUF_FILTERED_DEPTH: *ptr = ctx->filtered_depth; *bytes = px * 4; return ctx->filtered_depth != nullptr;
+        case HM_BUF_DOM: *ptr = ctx->dom; *bytes = dom; return ctx->dom != nullptr;
+        case HM_BUF_SCENE_DEPTH: *ptr = ctx->scene_depth; *bytes = px * 4; return ctx->scene_depth != nullptr;
+        default: break;
+    }
+    if (!g) return false;
+    int pass = (which >= HM_BUF_DOM_LOD) ? 1 : 0;
+    switch (which) {
+        case HM_BUF_CAMERA_LOD:
+        case HM_BUF_DOM_LOD: *ptr = g->lod[pass]; *bytes = (size_t)g->nb * sizeof(hm_lod_record); return true;
+        case HM_BUF_CAMERA_REFS:
+        case HM_BUF_DOM_REFS: *ptr = g->refs[pass]; *bytes = (size_t)g->capacity * sizeof(hm_strand_ref); return true;
+        case HM_BUF_CAMERA_INDIRECT:
+        case HM_BUF_DOM_INDIRECT: *ptr = g->indirect[pass]; *bytes = 16; return true;
+        default: return false;
+    }
+}
+
+hm_status hm_read(hm_ctx* ctx, hm_groom* g, hm_buffer which, void* dst, size_t bytes) {
+    if (!ctx || !dst) return fail(ctx, HM_ERR_INVALID, "null argument");
+    void* src = nullptr;
+    size_t size = 0;
+    if (!target_buffer(ctx, g, which, &src, &size)) return fail(ctx, HM_ERR_NOT_READY, "buffer not allocated");
+    if (bytes > size) return fail(ctx, HM_ERR_INVALID, "read larger than the buffer");
+    (void)hipSetDevice(ctx->device);
+    HM_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HM_OK;
+}
+
+hm_status hm_device_ptr(hm_ctx* ctx, hm_groom* g, hm_buffer which, void** ptr, size_t* bytes) {
+    if (!ctx || !ptr || !bytes) return fail(ctx, HM_ERR_INVALID, "null argument");
+    if (!target_buffer(ctx, g, which, ptr, bytes)) return fail(ctx, HM_ERR_NOT_READY, "buffer not allocated");
+    return HM_OK;
+}
+
+hm_status hm_sync(hm_ctx* ctx) {
+    if (!ctx) return HM_ERR_INVALID;
+    HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HM_OK;
+}
+
+hm_status hm_timing_enable(hm_ctx* ctx, int frames) {
+    if (!ctx || frames < 0) return HM_ERR_INVALID;
+    HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t want = (uint32_t)frames;
+    size_t need = (size_t)want * HM_K_COUNT * 2;
+    while (ctx->ev.size() < need) {
+        hipEvent_t e = nullptr;
+        HM_HIP(ctx, hipEventCreate(&e));
+        ctx->ev.push_back(e);
+    }
+    ctx->ran.assign((size_t)want * HM_K_COUNT, 0);
+    ctx->slots = want;
+    ctx->frame = 0;
+    return HM_OK;
+}
+
+hm_status hm_timing_read(hm_ctx* ctx, float* ms, uint32_t count) {
+    if (!ctx || !ms) return HM_ERR_INVALID;
+    HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t frames = ctx->frame < ctx->slots ? ctx->frame : ctx->slots;
+    if (frames == 0 && ctx->slots) frames = 1;  // kernels launched outside hm_frame land in slot 0
+    for (uint32_t k = 0; k < count; ++k) {
+        ms[k] = 0.0f;
+        if (k >= HM_K_COUNT) continue;
+        double sum = 0.0;
+        uint32_t n = 0;
+        for (uint32_t f = 0; f < frames; ++f) {
+            size_t i = (size_t)f * HM_K_COUNT + k;
+            if (!ctx->ran[i]) continue;
+            float v = 0.0f;
+            HM_HIP(ctx, hipEventElapsedTime(&v, ctx->ev[i * 2], ctx->ev[i * 2 + 1]));
+            sum += v;
+            ++n;
+        }
+        ms[k] = n ? (float)(sum / n) : 0.0f;
+    }
+    return HM_OK;
+}
+
+hm_status hm_time_frames(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm_frame_opts* opts,
+                         uint32_t frames, float* total_ms) {
+    if (!ctx || !total_ms) return HM_ERR_INVALID;
+    HM_HIP(ctx, hipEventRecord(ctx->frame_ev[0], ctx->stream));
+    for (uint32_t f = 0; f < frames; ++f) {
+        hm_status st = hm_frame(ctx, g, params, opts);
+        if (st != HM_OK) return st;
+    }
+    HM_HIP(ctx, hipEventRecord(ctx->frame_ev[1], ctx->stream));
+    HM_HIP(ctx, hipEventSynchronize(ctx->frame_ev[1]));
+    HM_HIP(ctx, hipEventElapsedTime(total_ms, ctx->frame_ev[0], ctx->frame_ev[1]));
+    return HM_OK;
+}
+
+}  // extern "C"

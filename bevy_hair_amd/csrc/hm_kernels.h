@@ -1,0 +1,63 @@
+// Launch interface between the host driver (hm_host.cpp) and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hair_mi.h"
+
+namespace hm {
+
+// Everything one raster pass reads or writes (the raster.wgsl bind group,
+// pipeline.rs:110-170, plus the work list bound and a counter block).
+struct RasterArgs {
+    hm_params view;
+    const hm_bundle* bundles;
+    const hm_layer* layers;
+    const float* style;             // float4 per texel
+    const hm_strand_ref* refs;
+    const uint32_t* indirect;       // [0] = strands to draw
+    uint32_t capacity;              // refs length
+    uint64_t* center;
+    uint64_t* cons;
+    uint32_t* beta;
+    uint32_t* dom;
+    const float* scene_depth;       // W*H or null
+    float scene_clear;
+    uint64_t* counters;             // counting variant only
+};
+
+struct ShadeArgs {
+    hm_params view;
+    const uint64_t* center;
+    const uint32_t* beta;
+    const uint32_t* dom;
+    const hm_point_light* points;
+    uint32_t point_count;
+    const hm_directional_light* directionals;
+    uint32_t directional_count;
+    float exposure;
+    float4* shaded;
+    float* shaded_depth;
+};
+
+struct FilterArgs {
+    hm_params view;
+    const uint64_t* center;
+    const uint64_t* cons;
+    const float4* shaded;
+    const float* shaded_depth;
+    float4* filtered;
+    float* filtered_depth;
+};
+
+hipError_t launch_fill32(hipStream_t s, uint32_t* dst, size_t words, uint32_t value, int max_blocks);
+hipError_t launch_lod(hipStream_t s, const hm_params& prm, const hm_bundle* bundles, const hm_layer* layers,
+                      hm_lod_record* out, uint32_t nb, uint32_t ovr_cps, float ovr_frac);
+hipError_t launch_scan(hipStream_t s, const hm_lod_record* lod, uint32_t nb, uint32_t capacity, uint32_t* prefix,
+                       uint32_t* indirect, hm_strand_ref* refs);
+hipError_t launch_raster(hipStream_t s, const RasterArgs& a, int mode, bool count, int blocks);
+hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, uint32_t width, uint32_t height);
+hipError_t launch_filter(hipStream_t s, const FilterArgs& a, uint32_t width, uint32_t height);
+
+}  // namespace hm

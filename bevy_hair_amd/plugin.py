@@ -1,0 +1,261 @@
+"""Host-side mirror of bevy_hair's plugin surface over the C ABI.
+
+HairGroom           src/lib.rs:32-64 (same fields and defaults)
+Renderer.upload     GpuGroom::upload, src/render/pass.rs:98-167
+Renderer.hair_pass  hair_pass's per-groom sequence, src/render/pass.rs:375-643
+Renderer.shadow     HairShadow publishing of the DOM + ShadowLight, pass.rs:571-580
+
+Everything runs on the GPU through libhairmi.so; there is no CPU path here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import _native as N
+from .groom import HairMesh
+from .scene import View, cols
+
+
+def srgb_to_linear(c: float) -> float:
+    """Bevy Color::srgb(..).to_linear() per channel."""
+    return c / 12.92 if c <= 0.04045 else ((c + 0.055) / 1.055) ** 2.4
+
+
+@dataclasses.dataclass
+class HairGroom:
+    """A groom instance (lib.rs:45-64): shading parameters and feature toggles."""
+
+    lambda_: float = 3.0
+    albedo_srgb: tuple = (0.16, 0.07, 0.035)
+    albedo_linear: tuple | None = None  # overrides albedo_srgb when set
+    roughness: float = 0.2
+    tilt: float = 0.08
+    lod: bool = True
+    filter: bool = True
+    ambient_occlusion: bool = True
+    deep_opacity: bool = True
+    center_diameter: float = 1.0
+    world_from_model: np.ndarray = dataclasses.field(default_factory=lambda: np.eye(4))
+
+    def config(self) -> N.HmGroomConfig:
+        g = N.HmGroomConfig()
+        g.world_from_model[:] = cols(self.world_from_model)
+        g.lambda_ = self.lambda_
+        lin = self.albedo_linear or tuple(srgb_to_linear(c) for c in self.albedo_srgb)
+        g.albedo_linear[:] = [float(x) for x in lin]
+        g.roughness, g.tilt, g.center_diameter = self.roughness, self.tilt, self.center_diameter
+        g.lod, g.filter = int(self.lod), int(self.filter)
+        g.ambient_occlusion, g.deep_opacity = int(self.ambient_occlusion), int(self.deep_opacity)
+        return g
+
+
+def point_light(d: dict) -> N.HmPointLight:
+    L = N.HmPointLight()
+    L.position_radius[:] = d["position_radius"]
+    L.color_inverse_square_range[:] = d["color_inverse_square_range"]
+    L.light_custom_data[:] = d.get("light_custom_data", [0, 0, 0, 0])
+    L.flags, L.kind = d.get("flags", 0), d.get("kind", 0)
+    return L
+
+
+def directional_light(d: dict) -> N.HmDirectionalLight:
+    L = N.HmDirectionalLight()
+    L.direction_to_light[:] = d["direction_to_light"]
+    L.color[:] = d["color"]
+    L.flags = d.get("flags", 0)
+    return L
+
+
+def light_arrays(points: list[dict], dirs: list[dict]):
+    pa = (N.HmPointLight * max(1, len(points)))(*[point_light(p) for p in points])
+    da = (N.HmDirectionalLight * max(1, len(dirs)))(*[directional_light(d) for d in dirs])
+    return pa, len(points), da, len(dirs)
+
+
+class GpuGroom:
+    """Device copy of one HairMesh (owned by a Renderer)."""
+
+    def __init__(self, renderer: "Renderer", mesh: HairMesh):
+        mesh.validate()
+        self.renderer, self.mesh = renderer, mesh
+        self._arrays = (np.ascontiguousarray(mesh.bundles), np.ascontiguousarray(mesh.layers),
+                        np.ascontiguousarray(mesh.style))
+        d = N.HmGroomDesc()
+        d.bundles, d.bundle_count = N.ptr(self._arrays[0]), mesh.bundle_count
+        d.layers, d.layer_count = N.ptr(self._arrays[1]), len(mesh.layers)
+        d.style, d.style_count = N.ptr(self._arrays[2]), len(mesh.style)
+        d.bounds_min[:] = mesh.bounds_min.tolist()
+        d.bounds_max[:] = mesh.bounds_max.tolist()
+        h = C.c_void_p()
+        N.check(N.lib().hm_groom_upload(renderer.ctx, C.byref(d), C.byref(h)), renderer.ctx, "hm_groom_upload")
+        self.handle = h
+
+    @property
+    def bundle_count(self) -> int:
+        return int(N.lib().hm_groom_bundle_count(self.handle))
+
+    def free(self):
+        if self.handle:
+            N.lib().hm_groom_free(self.renderer.ctx, self.handle)
+            self.handle = None
+
+
+class Renderer:
+    """One device context (hm_ctx): targets, lights, grooms, frames."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        st = N.lib().hm_create(device, C.byref(h))
+        N.check(st, None, f"hm_create(device={device})")
+        self.ctx = h
+        self.width = self.height = self.dom_size = 0
+
+    def close(self):
+        if self.ctx:
+            N.lib().hm_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st, what):
+        N.check(st, self.ctx, what)
+
+    def upload(self, mesh: HairMesh) -> GpuGroom:
+        return GpuGroom(self, mesh)
+
+    def targets(self, width: int, height: int, dom_size: int):
+        self._check(N.lib().hm_targets(self.ctx, width, height, dom_size), "hm_targets")
+        self.width, self.height, self.dom_size = width, height, dom_size
+
+    def set_scene_depth(self, depth: np.ndarray | None, clear_value: float = 0.0):
+        if depth is None:
+            self._check(N.lib().hm_set_scene_depth(self.ctx, None, clear_value), "hm_set_scene_depth")
+        else:
+            d = np.ascontiguousarray(depth, dtype=np.float32)
+            assert d.size == self.width * self.height
+            self._check(N.lib().hm_set_scene_depth(self.ctx, N.ptr(d), clear_value), "hm_set_scene_depth")
+
+    def set_lights(self, points: list[dict], directionals: list[dict]):
+        pa, np_, da, nd = light_arrays(points, directionals)
+        self._check(N.lib().hm_set_lights(self.ctx, pa, np_, da, nd), "hm_set_lights")
+        self.directionals = directionals
+
+    @staticmethod
+    def make_params(view: View, groom: HairGroom, mesh: HairMesh, dom_size: int, directionals: list[dict]):
+        """The camera / DOM-depth / DOM-opacity HairParams (pass.rs:399-453)."""
+        v = N.HmView()
+        v.clip_from_world[:] = cols(view.clip_from_world)
+        v.world_from_view[:] = cols(view.world_from_view)
+        cfg = groom.config()
+        da = (N.HmDirectionalLight * max(1, len(directionals)))(*[directional_light(d) for d in directionals])
+        key = (C.c_float * 3)()
+        N.lib().hm_key_direction(da, len(directionals), key)
+        out = (N.HmParams * 3)()
+        bmin = (C.c_float * 3)(*mesh.bounds_min.tolist())
+        bmax = (C.c_float * 3)(*mesh.bounds_max.tolist())
+        st = N.lib().hm_make_params(C.byref(v), C.byref(cfg), bmin, bmax, max(1, mesh.bundle_count), key,
+                                    view.width, view.height, dom_size, out)
+        N.check(st, None, "hm_make_params")
+        return out
+
+    # -- kernel-level passes (one per WGSL dispatch) --
+    def clear(self):
+        self._check(N.lib().hm_clear(self.ctx), "hm_clear")
+
+    def clear_dom(self):
+        self._check(N.lib().hm_clear_dom(self.ctx), "hm_clear_dom")
+
+    def lod(self, g: GpuGroom, params, pass_: int, control_points: int = 0, strand_fraction: float = 0.0):
+        o = N.HmLodOverride(control_points, strand_fraction)
+        self._check(N.lib().hm_lod(self.ctx, g.handle, C.byref(params), pass_, C.byref(o)), "hm_lod")
+
+    def scan(self, g: GpuGroom, params, pass_: int):
+        self._check(N.lib().hm_scan(self.ctx, g.handle, C.byref(params), pass_), "hm_scan")
+
+    def raster(self, g: GpuGroom, params, pass_: int):
+        self._check(N.lib().hm_raster(self.ctx, g.handle, C.byref(params), pass_), "hm_raster")
+
+    def raster_count(self, g: GpuGroom, params, pass_: int) -> np.ndarray:
+        c = (C.c_uint64 * 4)()
+        self._check(N.lib().hm_raster_count(self.ctx, g.handle, C.byref(params), pass_, c), "hm_raster_count")
+        return np.array(list(c), dtype=np.uint64)
+
+    def shade(self, params, exposure: float = 1.0):
+        self._check(N.lib().hm_shade(self.ctx, C.byref(params), exposure), "hm_shade")
+
+    def filter(self, params):
+        self._check(N.lib().hm_filter(self.ctx, C.byref(params)), "hm_filter")
+
+    # -- frame driver --
+    @staticmethod
+    def frame_opts(exposure=1.0, control_points=0, strand_fraction=0.0) -> N.HmFrameOpts:
+        return N.HmFrameOpts(exposure, N.HmLodOverride(control_points, strand_fraction))
+
+    def hair_pass(self, g: GpuGroom, params, opts: N.HmFrameOpts | None = None):
+        opts = opts or self.frame_opts()
+        self._check(N.lib().hm_frame(self.ctx, g.handle, params, C.byref(opts)), "hm_frame")
+
+    def time_frames(self, g: GpuGroom, params, opts, frames: int) -> float:
+        ms = C.c_float()
+        self._check(N.lib().hm_time_frames(self.ctx, g.handle, params, C.byref(opts), frames, C.byref(ms)),
+                    "hm_time_frames")
+        return float(ms.value)
+
+    def timing(self, on: bool):
+        self._check(N.lib().hm_timing_enable(self.ctx, int(on)), "hm_timing_enable")
+
+    def kernel_times(self) -> dict:
+        ms = (C.c_float * len(N.KERNELS))()
+        self._check(N.lib().hm_timing_read(self.ctx, ms, len(N.KERNELS)), "hm_timing_read")
+        return dict(zip(N.KERNELS, list(ms)))
+
+    def sync(self):
+        self._check(N.lib().hm_sync(self.ctx), "hm_sync")
+
+    # -- readback --
+    def read(self, which: int, dtype, count: int, groom: GpuGroom | None = None) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        self._check(N.lib().hm_read(self.ctx, groom.handle if groom else None, which, N.ptr(out), out.nbytes),
+                    "hm_read")
+        return out
+
+    def gbuffer(self):
+        n = self.width * self.height
+        return (self.read(N.BUF_CENTER, np.uint64, n), self.read(N.BUF_CONSERVATIVE, np.uint64, n),
+                self.read(N.BUF_BETA, np.uint32, n))
+
+    def dom(self) -> np.ndarray:
+        return self.read(N.BUF_DOM, np.uint32, self.dom_size * self.dom_size * 17)
+
+    def shaded(self):
+        n = self.width * self.height
+        return (self.read(N.BUF_SHADED, np.float32, n * 4).reshape(n, 4),
+                self.read(N.BUF_SHADED_DEPTH, np.float32, n))
+
+    def filtered(self):
+        n = self.width * self.height
+        return (self.read(N.BUF_FILTERED, np.float32, n * 4).reshape(n, 4),
+                self.read(N.BUF_FILTERED_DEPTH, np.float32, n))
+
+    def lod_records(self, g: GpuGroom, pass_: int) -> np.ndarray:
+        which = N.BUF_CAMERA_LOD if pass_ == 0 else N.BUF_DOM_LOD
+        return self.read(which, np.uint32, g.bundle_count * 4, g).reshape(-1, 4)
+
+    def refs(self, g: GpuGroom, pass_: int) -> tuple[np.ndarray, int]:
+        ind = self.read(N.BUF_CAMERA_INDIRECT if pass_ == 0 else N.BUF_DOM_INDIRECT, np.uint32, 4, g)
+        total = int(ind[0])
+        cap = max(1, g.mesh.strand_count)
+        refs = self.read(N.BUF_CAMERA_REFS if pass_ == 0 else N.BUF_DOM_REFS, np.uint32, cap * 4, g).reshape(-1, 4)
+        return refs[:total], total
+
+    def shadow_light(self, dom_params) -> N.HmShadowLight:
+        out = N.HmShadowLight()
+        N.lib().hm_shadow_light_from_params(C.byref(dom_params), C.byref(out))
+        return out

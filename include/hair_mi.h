@@ -1,0 +1,320 @@
+/*
+ * hair_mi.h — C ABI of the MI355X (gfx950) deferred strand rasterizer.
+ *
+ * The library replaces the GPU side of bevy_hair's `hair_pass`
+ * (src/render/pass.rs:259-657): the WGSL compute dispatches clear, clear_dom,
+ * lod, scan, raster (x3), the fullscreen shade pass and the reconnection
+ * filter. A Rust host links this header through `extern "C"` and keeps its own
+ * Bevy plugin surface (HairPlugin, HairGroom, HairMesh, HairShadowMaterial).
+ *
+ * Conventions
+ *  - All structs are plain C, little-endian, with the byte layout of the
+ *    reference's `#[repr(C)]` Pod types; sizes are checked with static asserts.
+ *  - Matrices are column-major float[16]: m[col*4 + row] (glam `to_cols_array`).
+ *  - Every entry point returns hm_status; none aborts. The reference has no
+ *    error codes and silently skips (pass.rs:290-322); a Rust caller maps any
+ *    non-OK status onto that same skip.
+ *  - One hm_ctx owns one HIP stream and all device allocations made through
+ *    it. A context is single-threaded; use one context per host thread.
+ */
+#ifndef HAIR_MI_H
+#define HAIR_MI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HM_LAYER_COUNT 16u  /* mesh.rs:10, common.wgsl:1 */
+#define HM_STYLE_U 8u       /* mesh.rs:11 */
+#define HM_STYLE_V 8u       /* mesh.rs:12 */
+#define HM_STYLE_W 16u      /* mesh.rs:13 */
+#define HM_STYLE_TEXELS (HM_STYLE_U * HM_STYLE_V * HM_STYLE_W)
+#define HM_DOM_LAYERS 16u   /* pass.rs:24, common.wgsl:7 */
+#define HM_DOM_SIGMA 0.05f  /* pass.rs:26 */
+#define HM_DOM_EMPTY 0xffffffffu
+#define HM_GB_EMPTY 0xffffffffffffffffull
+
+typedef enum hm_status {
+    HM_OK = 0,
+    HM_ERR_INVALID = 1,     /* bad argument, size mismatch, null handle */
+    HM_ERR_OOM = 2,         /* device allocation failed */
+    HM_ERR_HIP = 3,         /* a HIP runtime call failed */
+    HM_ERR_NO_DEVICE = 4,   /* no gfx950 device at that index */
+    HM_ERR_NOT_READY = 5,   /* targets or groom missing (the reference's silent skip) */
+    HM_ERR_CAPACITY = 6     /* a fixed-size limit was exceeded */
+} hm_status;
+
+/* HairParams, 432 bytes (pass.rs:42-62, common.wgsl:10-28). */
+typedef struct hm_params {
+    float clip_from_world[16];
+    float world_from_clip[16];
+    float world_from_model[16];
+    float light_clip_from_world[16];
+    float camera_pos[4];
+    float camera_forward[4];
+    float screen[4];        /* width, height, near, far */
+    float appearance[4];    /* roughness, tilt, lambda, center_diameter (px) */
+    float filter_params[4]; /* s_par, s_perp, dom_scale, dom_sigma */
+    float albedo[4];        /* linear rgb, 1 */
+    uint32_t flags[4];      /* lod, filter, ambient_occlusion, deep_opacity */
+    uint32_t pass_mode[4];  /* mode (0 camera, 1 DOM depth, 2 DOM opacity), bundle count, reverse-Z, is_dom */
+    float light_eye[4];
+    float light_forward[4];
+    float dom_info[4];      /* dom width, dom height, light near, light far */
+} hm_params;
+
+/* GpuBundle, 16 bytes (pass.rs:71-78). */
+typedef struct hm_bundle {
+    uint32_t layer_offset;  /* index of the bundle's first hm_layer */
+    uint32_t style_offset;  /* index of the bundle's first style texel */
+    uint32_t strand_count;
+    uint32_t _pad;
+} hm_bundle;
+
+/* GpuLayer, 32 bytes (pass.rs:64-69): one cage corner. 16 layers x 4 corners
+ * per bundle, layer-major, corner order c00, c10, c01, c11 (mesh.rs:34-36). */
+typedef struct hm_layer {
+    float pos_ao[4];
+    float tangent[4];
+} hm_layer;
+
+/* LodRecord, 16 bytes (common.wgsl:42-47). */
+typedef struct hm_lod_record {
+    uint32_t n_lod;
+    uint32_t control_points;
+    uint32_t strand_count;
+    uint32_t _pad;
+} hm_lod_record;
+
+/* StrandRef, 16 bytes (common.wgsl:49-54). */
+typedef struct hm_strand_ref {
+    uint32_t bundle;
+    uint32_t strand;
+    uint32_t n_lod;
+    uint32_t control_points;
+} hm_strand_ref;
+
+/* ShadowLight, 112 bytes (pass.rs:31-40): the DOM's light view for receivers. */
+typedef struct hm_shadow_light {
+    float light_clip_from_world[16];
+    float light_eye[4];
+    float light_forward[4];
+    float depth[4];         /* near, far, extinction per strand fragment, 1 once a map exists */
+} hm_shadow_light;
+
+/* A clusterable light as shade.wesl:452-480 reads it from Bevy's
+ * `clustered_lights` (64 bytes). kind 0 = point, 1 = spot. */
+#define HM_LIGHT_POINT 0u
+#define HM_LIGHT_SPOT 1u
+#define HM_LIGHT_FLAG_SHADOWS 1u            /* POINT/DIRECTIONAL_LIGHT_FLAGS_SHADOWS_ENABLED_BIT */
+#define HM_LIGHT_FLAG_SPOT_Y_NEGATIVE 2u    /* POINT_LIGHT_FLAGS_SPOT_LIGHT_Y_NEGATIVE */
+typedef struct hm_point_light {
+    float position_radius[4];
+    float color_inverse_square_range[4];   /* rgb radiance scale, w = 1 / range^2 */
+    float light_custom_data[4];            /* spot: dir.x, dir.z, cone scale, cone offset */
+    uint32_t flags;
+    uint32_t kind;
+    uint32_t _pad[2];
+} hm_point_light;
+
+/* A directional light as shade.wesl:484-501 reads it (48 bytes). */
+typedef struct hm_directional_light {
+    float direction_to_light[4];
+    float color[4];
+    uint32_t flags;         /* HM_LIGHT_FLAG_SHADOWS marks a shadow caster */
+    uint32_t _pad[3];
+} hm_directional_light;
+
+/* HairGroom (lib.rs:32-64) after extract_hair (mod.rs:126-139). */
+typedef struct hm_groom_config {
+    float world_from_model[16];
+    float lambda;
+    float albedo_linear[3];
+    float roughness;
+    float tilt;
+    float center_diameter;
+    uint32_t lod;
+    uint32_t filter;
+    uint32_t ambient_occlusion;
+    uint32_t deep_opacity;
+} hm_groom_config;
+
+/* The camera view (ExtractedView) hair_pass reads (pass.rs:342-349). */
+typedef struct hm_view {
+    float clip_from_world[16];
+    float world_from_view[16];
+} hm_view;
+
+/* Cage + style of one HairMesh asset (mesh.rs:31-41). */
+typedef struct hm_groom_desc {
+    const hm_bundle* bundles;
+    uint32_t bundle_count;
+    const hm_layer* layers;
+    uint32_t layer_count;       /* 64 per bundle */
+    const float* style;         /* 4 floats per texel */
+    uint32_t style_count;       /* 1024 per bundle */
+    float bounds_min[3];
+    float bounds_max[3];
+} hm_groom_desc;
+
+/* Harness-level overrides of the LOD record (camera pass only). They exist
+ * because the measured workloads ask for fixed strand and control-point
+ * counts that the reference's screen-space LOD would not pick; a zero value
+ * keeps the reference behaviour of lod.wgsl:41-58. Culled bundles stay culled. */
+typedef struct hm_lod_override {
+    uint32_t control_points;   /* 0 = Eqs. 4-5; else forced count (2..127) */
+    float strand_fraction;     /* 0 = Eq. 1; else n_lod = clamp(ceil(f * N), 1, N) */
+} hm_lod_override;
+
+typedef struct hm_ctx hm_ctx;
+typedef struct hm_groom hm_groom;
+
+/* Which per-pass buffers of a groom an entry point uses. */
+#define HM_PASS_CAMERA 0
+#define HM_PASS_DOM 1
+
+/* Buffers that hm_read / hm_device_ptr expose. */
+typedef enum hm_buffer {
+    HM_BUF_CENTER = 0,          /* u64[W*H] */
+    HM_BUF_CONSERVATIVE = 1,    /* u64[W*H] */
+    HM_BUF_BETA = 2,            /* u32[W*H] */
+    HM_BUF_SHADED = 3,          /* float4[W*H] */
+    HM_BUF_SHADED_DEPTH = 4,    /* float[W*H] */
+    HM_BUF_FILTERED = 5,        /* float4[W*H] */
+    HM_BUF_FILTERED_DEPTH = 6,  /* float[W*H] */
+    HM_BUF_DOM = 7,             /* u32[S*S*17]: S*S fronts, then 16 counts per texel */
+    HM_BUF_SCENE_DEPTH = 8,     /* float[W*H] */
+    HM_BUF_CAMERA_LOD = 16,     /* hm_lod_record[B] of the groom */
+    HM_BUF_CAMERA_REFS = 17,    /* hm_strand_ref[N] */
+    HM_BUF_CAMERA_INDIRECT = 18,/* u32[4]: x, 1, 1, 0 */
+    HM_BUF_DOM_LOD = 19,
+    HM_BUF_DOM_REFS = 20,
+    HM_BUF_DOM_INDIRECT = 21
+} hm_buffer;
+
+/* Kernel ids for hm_timing_read. */
+typedef enum hm_kernel {
+    HM_K_CLEAR = 0,
+    HM_K_CAM_LOD = 1,
+    HM_K_CAM_SCAN = 2,
+    HM_K_CAM_RASTER = 3,
+    HM_K_CLEAR_DOM = 4,
+    HM_K_DOM_LOD = 5,
+    HM_K_DOM_SCAN = 6,
+    HM_K_DOM_DEPTH = 7,
+    HM_K_DOM_OPACITY = 8,
+    HM_K_SHADE = 9,
+    HM_K_FILTER = 10,
+    HM_K_COUNT = 11
+} hm_kernel;
+
+/* Per-frame options of hm_frame (pass.rs:375-643). */
+typedef struct hm_frame_opts {
+    float exposure;                 /* view.exposure (shade.wesl:588) */
+    hm_lod_override camera_lod;     /* see hm_lod_override */
+} hm_frame_opts;
+
+/* ---- lifetime ----------------------------------------------------------- */
+
+/* Opens `device` (a gfx950 GPU) and creates the context's stream. */
+hm_status hm_create(int device, hm_ctx** out);
+void hm_destroy(hm_ctx* ctx);
+const char* hm_status_string(hm_status status);
+/* Detail of the last failure on this context (static storage, never NULL). */
+const char* hm_last_error(const hm_ctx* ctx);
+
+/* Replaces GpuGroom::upload (pass.rs:98-167): copies the cage, the style
+ * volume and allocates the per-pass LOD/ref/indirect buffers. */
+hm_status hm_groom_upload(hm_ctx* ctx, const hm_groom_desc* desc, hm_groom** out);
+void hm_groom_free(hm_ctx* ctx, hm_groom* groom);
+uint32_t hm_groom_bundle_count(const hm_groom* groom);
+uint32_t hm_groom_strand_count(const hm_groom* groom);
+
+/* Replaces ensure_targets (pass.rs:789-815) with a runtime DOM size (the
+ * reference hard-codes 512, pass.rs:22). Reallocates only when sizes change. */
+hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_size);
+
+/* The scene depth texture raster tests against (raster.wgsl:93). NULL means
+ * no scene geometry: every texel reads `clear_value` (0 for reverse-Z). */
+hm_status hm_set_scene_depth(hm_ctx* ctx, const float* host_depth, float clear_value);
+
+/* The light list shade reads in place of Bevy's view bindings. */
+hm_status hm_set_lights(hm_ctx* ctx, const hm_point_light* points, uint32_t point_count,
+                        const hm_directional_light* directionals, uint32_t directional_count);
+
+/* ---- host-side parameter construction ----------------------------------- */
+
+/* The three HairParams of one groom: camera, DOM depth, DOM opacity
+ * (pass.rs:399-453 with make_params pass.rs:676-709, light_fit pass.rs:747-776,
+ * world_extent pass.rs:737-745, depth_is_reverse pass.rs:778-787).
+ * `direction_to_light` is key_direction()'s result (pass.rs:711-719). */
+hm_status hm_make_params(const hm_view* view, const hm_groom_config* groom,
+                         const float bounds_min[3], const float bounds_max[3],
+                         uint32_t bundle_count, const float direction_to_light[3],
+                         uint32_t width, uint32_t height, uint32_t dom_size,
+                         hm_params out[3]);
+
+/* key_direction (pass.rs:711-719): first shadow caster, else first light,
+ * else (0.35, 0.82, 0.45); normalized. */
+void hm_key_direction(const hm_directional_light* lights, uint32_t count, float out[3]);
+
+/* The ShadowLight published with the DOM (pass.rs:574-579). */
+void hm_shadow_light_from_params(const hm_params* dom_params, hm_shadow_light* out);
+
+/* ---- kernel-level entry points (one per WGSL dispatch) ------------------ */
+
+hm_status hm_clear(hm_ctx* ctx);                                        /* clear.wgsl */
+hm_status hm_clear_dom(hm_ctx* ctx);                                    /* clear_dom.wgsl */
+hm_status hm_lod(hm_ctx* ctx, hm_groom* groom, const hm_params* params,
+                 int pass, const hm_lod_override* override_or_null);    /* lod.wgsl */
+hm_status hm_scan(hm_ctx* ctx, hm_groom* groom, const hm_params* params, int pass); /* scan.wgsl */
+hm_status hm_raster(hm_ctx* ctx, hm_groom* groom, const hm_params* params, int pass); /* raster.wgsl */
+hm_status hm_shade(hm_ctx* ctx, const hm_params* params, float exposure); /* shade.wesl:fs */
+hm_status hm_filter(hm_ctx* ctx, const hm_params* params);              /* filter.wgsl */
+
+/* Counts the raster's algorithmic work for one pass without writing any
+ * target: counts[0] = camera write_pixel calls that reach the atomics,
+ * counts[1] = of those, centre hits, counts[2] = DOM writes that pass the
+ * depth range, counts[3] = segments drawn. Synchronous. */
+hm_status hm_raster_count(hm_ctx* ctx, hm_groom* groom, const hm_params* params,
+                          int pass, uint64_t counts[4]);
+
+/* ---- frame driver ------------------------------------------------------- */
+
+/* One groom of hair_pass (pass.rs:379-619): clear, camera lod/scan/raster,
+ * [clear_dom, DOM lod/scan/raster x2 if flags.w], shade, [filter if flags.y].
+ * `params` are hm_make_params's three records. Asynchronous on the stream. */
+hm_status hm_frame(hm_ctx* ctx, hm_groom* groom, const hm_params params[3],
+                   const hm_frame_opts* opts);
+
+/* ---- data access and timing --------------------------------------------- */
+
+/* Copies a buffer to host memory (synchronizes the stream). Groom buffers
+ * (HM_BUF_*_LOD/REFS/INDIRECT) need `groom`; target buffers ignore it. */
+hm_status hm_read(hm_ctx* ctx, hm_groom* groom, hm_buffer which, void* dst, size_t bytes);
+/* Device address and size of a buffer, for zero-copy sharing (the DOM for
+ * HairShadow receivers, pass.rs:571-573). */
+hm_status hm_device_ptr(hm_ctx* ctx, hm_groom* groom, hm_buffer which, void** ptr, size_t* bytes);
+hm_status hm_sync(hm_ctx* ctx);
+
+/* With frames > 0, the next `frames` hm_frame calls record a HIP event pair
+ * around each kernel on the context's stream (0 turns recording off).
+ * hm_timing_read returns each kernel's mean milliseconds over the recorded
+ * frames (HM_K_COUNT floats, 0 for kernels that did not run). */
+hm_status hm_timing_enable(hm_ctx* ctx, int frames);
+hm_status hm_timing_read(hm_ctx* ctx, float* ms, uint32_t count);
+
+/* Times `frames` back-to-back hm_frame calls with one event pair around all
+ * of them; returns total milliseconds. Per-kernel recording, if enabled,
+ * continues inside. */
+hm_status hm_time_frames(hm_ctx* ctx, hm_groom* groom, const hm_params params[3],
+                         const hm_frame_opts* opts, uint32_t frames, float* total_ms);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* HAIR_MI_H */

@@ -1,0 +1,158 @@
+"""HIP path vs the oracle on identical seeded inputs (bit-exact keys, 1e-3 RGB).
+
+Each stage is checked in isolation: the oracle stage consumes the GPU's own
+outputs of the previous stage, so a failure names the kernel that diverged.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from bevy_hair_amd import _native as N
+from tests.scenes import EMPTY64, make_scene, rel_err
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["tiny", "c1", "small_lod"]
+RGB_TOL = 1e-3
+
+
+@pytest.fixture(scope="module", params=CASES)
+def run(request, renderer):
+    sc = make_scene(request.param)
+    w = sc.w
+    print(f"\n[gpu parity] case={request.param} strands={w.strands} {w.width}x{w.height} dom={w.dom_size}", flush=True)
+    r = renderer
+    r.targets(w.width, w.height, w.dom_size)
+    r.set_scene_depth(None, 0.0)
+    r.set_lights(sc.points, sc.dirs)
+    g = r.upload(sc.mesh)
+    cam, dom1, dom2 = sc.params[0], sc.params[1], sc.params[2]
+    out = {"scene": sc, "g": g}
+    r.clear()
+    r.lod(g, cam, N.PASS_CAMERA, w.control_points, w.strand_fraction)
+    r.scan(g, cam, N.PASS_CAMERA)
+    out["cam_lod"] = r.lod_records(g, N.PASS_CAMERA)
+    out["cam_refs"] = r.refs(g, N.PASS_CAMERA)
+    r.raster(g, cam, N.PASS_CAMERA)
+    out["gb"] = r.gbuffer()
+    out["cam_counts"] = r.raster_count(g, cam, N.PASS_CAMERA)
+    r.clear_dom()
+    r.lod(g, dom1, N.PASS_DOM)
+    r.scan(g, dom1, N.PASS_DOM)
+    out["dom_lod"] = r.lod_records(g, N.PASS_DOM)
+    out["dom_refs"] = r.refs(g, N.PASS_DOM)
+    out["dom_counts1"] = r.raster_count(g, dom1, N.PASS_DOM)
+    r.raster(g, dom1, N.PASS_DOM)
+    out["dom_front"] = r.dom()
+    out["dom_counts2"] = r.raster_count(g, dom2, N.PASS_DOM)
+    r.raster(g, dom2, N.PASS_DOM)
+    out["dom"] = r.dom()
+    r.shade(cam, 1.0)
+    out["shaded"] = r.shaded()
+    r.filter(cam)
+    out["filtered"] = r.filtered()
+    yield out
+    g.free()
+
+
+def ref_frame(sc):
+    return oracle.Frame(sc.mesh, sc.w.width, sc.w.height, sc.w.dom_size)
+
+
+def test_lod_and_scan(run):
+    sc = run["scene"]
+    ref = ref_frame(sc)
+    cap = max(1, sc.mesh.strand_count)
+    lod = ref.lod(sc.params[0], sc.w.control_points, sc.w.strand_fraction)
+    assert np.array_equal(run["cam_lod"], lod)
+    refs, total = ref.scan(sc.params[0], lod, cap)
+    got, gtotal = run["cam_refs"]
+    assert gtotal == total and np.array_equal(got, refs)
+    dlod = ref.lod(sc.params[1])
+    assert np.array_equal(run["dom_lod"], dlod)
+    drefs, dtotal = ref.scan(sc.params[1], dlod, cap)
+    got, gtotal = run["dom_refs"]
+    assert gtotal == dtotal and np.array_equal(got, drefs)
+
+
+def test_camera_gbuffer_bit_exact(run):
+    sc = run["scene"]
+    ref = ref_frame(sc)
+    ref.clear()
+    counts = ref.raster(sc.params[0], run["cam_refs"][0])
+    center, cons, beta = run["gb"]
+    assert (center != EMPTY64).sum() > 0, "nothing rasterized"
+    assert np.array_equal(center, ref.center), f"{(center != ref.center).sum()} centre words differ"
+    assert np.array_equal(cons, ref.cons), f"{(cons != ref.cons).sum()} conservative words differ"
+    assert np.array_equal(beta, ref.beta), f"{(beta != ref.beta).sum()} beta words differ"
+    assert np.array_equal(run["cam_counts"][:2], counts[:2])
+    assert run["cam_counts"][3] == counts[3]
+
+
+def test_dom_bit_exact(run):
+    sc = run["scene"]
+    ref = ref_frame(sc)
+    S = sc.w.dom_size
+    ref.clear_dom()
+    c1 = ref.raster(sc.params[1], run["dom_refs"][0])
+    assert np.array_equal(run["dom_front"][: S * S], ref.dom[: S * S])
+    c2 = ref.raster(sc.params[2], run["dom_refs"][0])
+    assert np.array_equal(run["dom"], ref.dom), f"{(run['dom'] != ref.dom).sum()} DOM words differ"
+    assert run["dom_counts1"][2] == c1[2] and run["dom_counts2"][2] == c2[2]
+    # every depth-pass write is one opacity increment
+    assert int(ref.dom[S * S:].sum()) == int(c2[2]) == int(c1[2])
+
+
+def test_shade_within_tolerance(run):
+    sc = run["scene"]
+    ref = ref_frame(sc)
+    ref.center[:], ref.cons[:], ref.beta[:] = run["gb"]
+    ref.dom[:] = run["dom"]
+    pa, n_p, da, n_d = sc.light_arrays()
+    ref.shade(sc.params[0], pa, n_p, da, n_d)
+    shaded, depth = run["shaded"]
+    assert np.array_equal(depth, ref.shaded_depth)
+    e = rel_err(shaded, ref.shaded)
+    print(f"shade max rel err {e.max():.3e}")
+    assert e.max() <= RGB_TOL
+
+
+def test_filter_within_tolerance(run):
+    sc = run["scene"]
+    ref = ref_frame(sc)
+    ref.center[:], ref.cons[:], ref.beta[:] = run["gb"]
+    ref.shaded[:], ref.shaded_depth[:] = run["shaded"]
+    ref.filter(sc.params[0])
+    filt, depth = run["filtered"]
+    assert np.array_equal(depth, ref.filtered_depth)
+    assert np.array_equal(filt[:, 3], ref.filtered[:, 3]), "coverage alpha differs"
+    e = rel_err(filt, ref.filtered)
+    print(f"filter max rel err {e.max():.3e}")
+    assert e.max() <= RGB_TOL
+
+
+def test_frame_matches_oracle_and_is_deterministic(renderer):
+    sc = make_scene("c1")
+    w, r = sc.w, renderer
+    r.targets(w.width, w.height, w.dom_size)
+    r.set_scene_depth(None, 0.0)
+    r.set_lights(sc.points, sc.dirs)
+    g = r.upload(sc.mesh)
+    opts = r.frame_opts(control_points=w.control_points)
+    r.hair_pass(g, sc.params, opts)
+    first = (*r.gbuffer(), r.dom(), r.filtered()[0])
+    r.hair_pass(g, sc.params, opts)
+    second = (*r.gbuffer(), r.dom(), r.filtered()[0])
+    g.free()
+    for a, b in zip(first, second):
+        assert np.array_equal(a, b)
+    ref = ref_frame(sc)
+    pa, n_p, da, n_d = sc.light_arrays()
+    ref.hair_pass(sc.params, pa, n_p, da, n_d, control_points=w.control_points)
+    assert np.array_equal(first[0], ref.center)
+    assert np.array_equal(first[1], ref.cons)
+    assert np.array_equal(first[2], ref.beta)
+    assert np.array_equal(first[3], ref.dom)
+    assert rel_err(first[4], ref.filtered).max() <= RGB_TOL
