@@ -96,7 +96,11 @@ class HmLodOverride(C.Structure):
 
 
 class HmFrameOpts(C.Structure):
-    _fields_ = [("exposure", f32), ("camera_lod", HmLodOverride)]
+    _fields_ = [("exposure", f32), ("camera_lod", HmLodOverride), ("shading", u32), ("mlp_tolerance", f32)]
+
+
+SHADING_PLAIN, SHADING_MLP = 0, 1
+MLP_NW, MLP_NBIAS = 11264, 208
 
 
 class HmShadowLight(C.Structure):
@@ -155,6 +159,18 @@ ABI = {
     "hm_timing_enable": (C.c_int, [P, C.c_int]),
     "hm_timing_read": (C.c_int, [P, C.POINTER(f32), u32]),
     "hm_time_frames": (C.c_int, [P, P, C.POINTER(HmParams), C.POINTER(HmFrameOpts), u32, C.POINTER(f32)]),
+    "hm_mlp_init": (C.c_int, [P, u32]),
+    "hm_mlp_set_weights": (C.c_int, [P, P, P]),
+    "hm_mlp_get_weights": (C.c_int, [P, P, P]),
+    "hm_mlp_forward": (C.c_int, [P, P, u32, P]),
+    "hm_mlp_gradients": (C.c_int, [P, P, P, u32, P, C.POINTER(f32), P, P]),
+    "hm_mlp_reserve_samples": (C.c_int, [P, u32]),
+    "hm_mlp_collect": (C.c_int, [P, C.POINTER(HmParams), f32, u32, C.POINTER(u32)]),
+    "hm_mlp_read_samples": (C.c_int, [P, P, P, u32]),
+    "hm_mlp_set_samples": (C.c_int, [P, P, P, u32]),
+    "hm_mlp_set_range": (C.c_int, [P, P, P]),
+    "hm_mlp_train": (C.c_int, [P, u32, u32, f32, P, u32, C.POINTER(f32)]),
+    "hm_shade_mlp": (C.c_int, [P, C.POINTER(HmParams), f32, f32, C.POINTER(u32)]),
 }
 
 _lib = None

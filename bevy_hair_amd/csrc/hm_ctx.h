@@ -1,0 +1,58 @@
+// Private state behind the opaque hm_ctx / hm_groom handles.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "hm_kernels.h"
+
+struct hm_mlp_state;  // learned shading (hm_mlp_host.cpp)
+
+struct hm_groom {
+    uint32_t nb = 0, nl = 0, ns = 0, strands = 0, capacity = 0;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    hm_bundle* bundles = nullptr;
+    hm_layer* layers = nullptr;
+    float* style = nullptr;
+    hm_lod_record* lod[2] = {nullptr, nullptr};
+    hm_strand_ref* refs[2] = {nullptr, nullptr};
+    uint32_t* prefix[2] = {nullptr, nullptr};
+    uint32_t* indirect[2] = {nullptr, nullptr};
+};
+
+struct hm_ctx {
+    int device = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    uint32_t width = 0, height = 0, dom_size = 0;
+    void* gb_slab = nullptr;  // center | conservative | beta, cleared by one fill
+    size_t gb_words = 0;
+    uint64_t* center = nullptr;
+    uint64_t* cons = nullptr;
+    uint32_t* beta = nullptr;
+    float4* shaded = nullptr;
+    float* shaded_depth = nullptr;
+    float4* filtered = nullptr;
+    float* filtered_depth = nullptr;
+    uint32_t* dom = nullptr;
+    float* scene_depth = nullptr;
+    float scene_clear = 0.0f;
+    hm_point_light* points = nullptr;
+    uint32_t point_count = 0, point_cap = 0;
+    hm_directional_light* dirs = nullptr;
+    uint32_t dir_count = 0, dir_cap = 0;
+    uint64_t* counters = nullptr;
+    // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
+    uint32_t slots = 0, frame = 0;
+    std::vector<hipEvent_t> ev;
+    std::vector<uint8_t> ran;
+    hipEvent_t frame_ev[2] = {nullptr, nullptr};
+    hm_mlp_state* mlp = nullptr;
+    std::string err;
+};
+
+
+// Shared helpers of the host translation units.
+hm_status hm_fail(hm_ctx* ctx, hm_status st, const std::string& msg);
+hm_status hm_hip_fail(hm_ctx* ctx, hipError_t e, const char* what);
+void hm_mlp_release(hm_ctx* ctx);

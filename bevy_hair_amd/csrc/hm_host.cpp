@@ -18,47 +18,7 @@
 
 #include "hm_kernels.h"
 
-struct hm_groom {
-    uint32_t nb = 0, nl = 0, ns = 0, strands = 0, capacity = 0;
-    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
-    hm_bundle* bundles = nullptr;
-    hm_layer* layers = nullptr;
-    float* style = nullptr;
-    hm_lod_record* lod[2] = {nullptr, nullptr};
-    hm_strand_ref* refs[2] = {nullptr, nullptr};
-    uint32_t* prefix[2] = {nullptr, nullptr};
-    uint32_t* indirect[2] = {nullptr, nullptr};
-};
-
-struct hm_ctx {
-    int device = 0;
-    int cu_count = 256;
-    hipStream_t stream = nullptr;
-    uint32_t width = 0, height = 0, dom_size = 0;
-    void* gb_slab = nullptr;  // center | conservative | beta, cleared by one fill
-    size_t gb_words = 0;
-    uint64_t* center = nullptr;
-    uint64_t* cons = nullptr;
-    uint32_t* beta = nullptr;
-    float4* shaded = nullptr;
-    float* shaded_depth = nullptr;
-    float4* filtered = nullptr;
-    float* filtered_depth = nullptr;
-    uint32_t* dom = nullptr;
-    float* scene_depth = nullptr;
-    float scene_clear = 0.0f;
-    hm_point_light* points = nullptr;
-    uint32_t point_count = 0, point_cap = 0;
-    hm_directional_light* dirs = nullptr;
-    uint32_t dir_count = 0, dir_cap = 0;
-    uint64_t* counters = nullptr;
-    // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
-    uint32_t slots = 0, frame = 0;
-    std::vector<hipEvent_t> ev;
-    std::vector<uint8_t> ran;
-    hipEvent_t frame_ev[2] = {nullptr, nullptr};
-    std::string err;
-};
+#include "hm_ctx.h"
 
 namespace {
 
@@ -358,6 +318,9 @@ hm_status check_pass_params(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pa
 
 }  // namespace
 
+hm_status hm_fail(hm_ctx* ctx, hm_status st, const std::string& msg) { return fail(ctx, st, msg); }
+hm_status hm_hip_fail(hm_ctx* ctx, hipError_t e, const char* what) { return hip_fail(ctx, e, what); }
+
 // ---- ABI ------------------------------------------------------------------------
 
 extern "C" {
@@ -408,6 +371,7 @@ void hm_destroy(hm_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_targets(c);
+    hm_mlp_release(c);
     dev_free(c->points);
     dev_free(c->dirs);
     dev_free(c->counters);
@@ -820,7 +784,11 @@ hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm
         if ((st = hm_raster(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
         if ((st = hm_raster(ctx, g, &params[2], HM_PASS_DOM)) != HM_OK) return st;
     }
-    if ((st = hm_shade(ctx, &cam, exposure)) != HM_OK) return st;
+    if (opts && opts->shading == HM_SHADING_MLP) {
+        if ((st = hm_shade_mlp(ctx, &cam, exposure, opts->mlp_tolerance, nullptr)) != HM_OK) return st;
+    } else if ((st = hm_shade(ctx, &cam, exposure)) != HM_OK) {
+        return st;
+    }
     if (cam.flags[1]) {
         if ((st = hm_filter(ctx, &cam)) != HM_OK) return st;
     }

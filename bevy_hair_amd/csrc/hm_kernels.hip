@@ -12,6 +12,7 @@
 #include <cstring>
 
 #include "hm_device.h"
+#include "hm_shade.h"
 
 namespace hm {
 
@@ -441,192 +442,6 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs a) {
 
 // ---- shade.wesl -----------------------------------------------------------------
 
-struct Lobes { F3 r, tt, trt; };
-struct Shape { F3 alpha, beta; };
-struct DualScatter { Lobes fwd; F3 a_f, beta_f2, a_back, delta_b, sigma_b2; };
-
-__device__ __forceinline__ F3 vexp(F3 a) { return {expf(a.x), expf(a.y), expf(a.z)}; }
-__device__ __forceinline__ F3 vsqrt(F3 a) { return {sqrtf(a.x), sqrtf(a.y), sqrtf(a.z)}; }
-
-__device__ __forceinline__ F3 gaussian3(F3 b, F3 x) {
-    F3 bb = vmax(b, splat3(1e-3f));
-    const float s2pi = sqrtf(2.0f * kPi);
-    return vexp(-0.5f * x * x / (bb * bb)) / (s2pi * bb);
-}
-
-__device__ __forceinline__ float fresnel(float cos_t) {
-    const float ior = 1.55f;
-    float r0 = (1.0f - ior) / (1.0f + ior);
-    float f0 = r0 * r0;
-    float x = 1.0f - clampf(cos_t, 0.0f, 1.0f);
-    float x2 = x * x;
-    return f0 + (1.0f - f0) * (x2 * x2 * x);
-}
-
-__device__ __forceinline__ F3 absorption_from_color(F3 color, float roughness) {
-    float b = clampf(roughness, 0.0f, 1.0f);
-    float b2 = b * b, b3 = b2 * b;
-    float d = 5.969f - 0.215f * b + 2.532f * b2 - 10.73f * b3 + 5.574f * b2 * b2 + 0.245f * b3 * b2;
-    F3 c = vclamp(color, 1e-4f, 1.0f);
-    F3 l = {logf(c.x) / d, logf(c.y) / d, logf(c.z) / d};
-    return l * l;
-}
-
-__device__ __forceinline__ Lobes azimuthal(float cos_theta_d, float cos_phi, F3 sigma_a) {
-    float cos_half_phi = sqrtf(clampf(0.5f + 0.5f * cos_phi, 0.0f, 1.0f));
-    float n_r = 0.25f * cos_half_phi * fresnel(cos_theta_d * cos_half_phi);
-    float eta = 1.19f / cos_theta_d + 0.36f * cos_theta_d;
-    float a = 1.0f / eta;
-    float h = cos_half_phi * (1.0f + a * (0.6f - 0.8f * cos_phi));
-    float f_tt = fresnel(cos_theta_d * sqrtf(clampf(1.0f - h * h, 0.0f, 1.0f)));
-    float s_tt = sqrtf(clampf(1.0f - h * h * a * a, 0.0f, 1.0f));
-    F3 t_tt = vexp(sigma_a * -1.0f * 2.0f * s_tt / cos_theta_d);
-    F3 n_tt = t_tt * (expf(-3.65f * cos_phi - 3.98f) * (1.0f - f_tt) * (1.0f - f_tt));
-    float f_trt = fresnel(cos_theta_d * 0.5f);
-    F3 t_trt = vexp(sigma_a * -1.0f * 3.2f / cos_theta_d);
-    F3 n_trt = t_trt * (expf(17.0f * cos_phi - 16.78f) * (1.0f - f_trt) * (1.0f - f_trt) * f_trt);
-    return {splat3(n_r), n_tt, n_trt};
-}
-
-__device__ __forceinline__ F3 lobe_mean(const Lobes& l, F3 total, F3 q) {
-    return (l.r * q.x + l.tt * q.y + l.trt * q.z) / vmax(total, splat3(1e-5f));
-}
-
-__device__ DualScatter dual_scatter(float cos_theta_d, F3 sigma_a, const Shape& shape) {
-    Lobes fwd = {splat3(0), splat3(0), splat3(0)}, back = fwd;
-    for (uint32_t i = 0; i < 8u; ++i) {
-        float u = (u2f(i) + 0.5f) / 8.0f;
-        Lobes b = azimuthal(cos_theta_d, cosf(u * 0.5f * kPi), sigma_a);
-        Lobes f = azimuthal(cos_theta_d, cosf((1.0f + u) * 0.5f * kPi), sigma_a);
-        back = {back.r + b.r, back.tt + b.tt, back.trt + b.trt};
-        fwd = {fwd.r + f.r, fwd.tt + f.tt, fwd.trt + f.trt};
-    }
-    float w = kPi / 8.0f;
-    fwd = {fwd.r * w, fwd.tt * w, fwd.trt * w};
-    back = {back.r * w, back.tt * w, back.trt * w};
-    F3 a_f = vmin(fwd.r + fwd.tt + fwd.trt, splat3(0.95f));
-    F3 a_b = back.r + back.tt + back.trt;
-    F3 beta2 = shape.beta * shape.beta;
-    F3 beta_f2 = lobe_mean(fwd, a_f, beta2);
-    F3 alpha_b = lobe_mean(back, a_b, shape.alpha);
-    F3 beta_b2 = lobe_mean(back, a_b, beta2);
-    F3 af2 = a_f * a_f;
-    F3 rest = splat3(1.0f) - af2;
-    F3 ab2 = a_b * a_b, ab3 = ab2 * a_b;
-    F3 rest3 = rest * rest * rest;
-    F3 a_back = a_b * af2 / rest + ab3 * af2 / rest3;
-    F3 delta_b = alpha_b * (splat3(1.0f) - 2.0f * ab2 / (rest * rest)) +
-                 alpha_b * (2.0f * rest * rest + 4.0f * af2 * ab2) / rest3;
-    F3 beta_f = vsqrt(beta_f2), beta_b = vsqrt(beta_b2);
-    F3 sigma_b = (splat3(1.0f) + 0.7f * af2) *
-                 (a_b * vsqrt(2.0f * beta_f2 + beta_b2) + ab3 * vsqrt(2.0f * beta_f2 + 3.0f * beta_b2)) /
-                 vmax(a_b + ab3 * (2.0f * beta_f + 3.0f * beta_b), splat3(1e-5f));
-    return {fwd, a_f, beta_f2, a_back, delta_b, sigma_b * sigma_b};
-}
-
-__device__ F3 hair_bsdf(F3 t, F3 v, F3 light, F3 sigma_a, const Shape& shape, const DualScatter& ds, float depth) {
-    // t and v arrive normalized (safe_normalize is idempotent on them up to rounding).
-    t = safe_normalize(t);
-    v = safe_normalize(v);
-    F3 l = safe_normalize(light);
-    float sin_i = clampf(dot3(t, l), -1.0f, 1.0f);
-    float sin_o = clampf(dot3(t, v), -1.0f, 1.0f);
-    float theta_i = asinf(sin_i), theta_o = asinf(sin_o);
-    float theta_h = theta_i + theta_o;
-    float cos_theta_d = mx(cosf(0.5f * fabsf(theta_i - theta_o)), 1e-3f);
-    F3 t_i = l - t * sin_i;
-    F3 t_o = v - t * sin_o;
-    float proj = sqrtf(mx(dot3(t_i, t_i) * dot3(t_o, t_o), 1e-8f));
-    float cos_phi = clampf(dot3(t_i, t_o) / proj, -1.0f, 1.0f);
-    Lobes n = azimuthal(cos_theta_d, cos_phi, sigma_a);
-    F3 m = gaussian3(shape.beta, splat3(theta_h) - shape.alpha);
-    F3 single = m.x * n.r + m.y * n.tt + m.z * n.trt;
-    F3 back = 2.0f * ds.a_back * gaussian3(vsqrt(ds.sigma_b2), splat3(theta_h) - ds.delta_b) / kPi;
-    float direct = expf(-depth);
-    F3 color = direct * (single + 0.7f * back);
-    if (depth > 1e-3f) {
-        F3 sigma_f2 = depth * ds.beta_f2;
-        F3 transmit = {powf(ds.a_f.x, depth), powf(ds.a_f.y, depth), powf(ds.a_f.z, depth)};
-        F3 beta2 = shape.beta * shape.beta;
-        F3 x = splat3(theta_h);
-        F3 spread = gaussian3(vsqrt(splat3(beta2.x) + sigma_f2), x - splat3(shape.alpha.x)) * ds.fwd.r +
-                    gaussian3(vsqrt(splat3(beta2.y) + sigma_f2), x - splat3(shape.alpha.y)) * ds.fwd.tt +
-                    gaussian3(vsqrt(splat3(beta2.z) + sigma_f2), x - splat3(shape.alpha.z)) * ds.fwd.trt;
-        F3 back_spread = 2.0f * ds.a_back * gaussian3(vsqrt(ds.sigma_b2 + sigma_f2), x - ds.delta_b) / kPi;
-        color = color + (1.0f - direct) * transmit * 0.7f * (spread / kPi + kPi * 0.7f * back_spread);
-    }
-    return color;
-}
-
-__device__ __forceinline__ float dom_texel_visibility(const View& p, const uint32_t* __restrict__ dom, int32_t x,
-                                                      int32_t y, float dist, float near_, float far_, uint32_t dom_w,
-                                                      uint32_t dom_h) {
-    if (x < 0 || y < 0 || x >= (int32_t)dom_w || y >= (int32_t)dom_h) return 1.0f;
-    uint32_t pix = (uint32_t)y * dom_w + (uint32_t)x;
-    uint32_t front_q = dom[pix];
-    if (front_q == HM_DOM_EMPTY) return 1.0f;
-    float z_front = dequantize_depth(front_q, near_, far_);
-    float thickness = rdiv(mx(far_ - near_, 1e-3f), 16.0f);
-    float rel = mx(dist - z_front - thickness * 0.05f, 0.0f);
-    float t = rdiv(rel, thickness);
-    const uint4* c4 = reinterpret_cast<const uint4*>(dom + (size_t)dom_w * dom_h + (size_t)pix * HM_DOM_LAYERS);
-    float occ = 0.0f;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        uint4 c = c4[q];
-        uint32_t cs[4] = {c.x, c.y, c.z, c.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float i = (float)(q * 4 + k);
-            float count = u2f(cs[k]);
-            if (i + 1.0f <= t) occ = occ + count;
-            else if (i < t) occ = occ + count * (t - i);
-        }
-    }
-    return expf(-occ * p.filter_params.w);
-}
-
-__device__ float dom_visibility(const View& p, const uint32_t* __restrict__ dom, F3 world, float beta) {
-    if (p.flags[3] == 0u) return 1.0f;
-    float cx = mat_row(p.lclip, 0, world.x, world.y, world.z, 1.0f);
-    float cy = mat_row(p.lclip, 1, world.x, world.y, world.z, 1.0f);
-    float cw = mat_row(p.lclip, 3, world.x, world.y, world.z, 1.0f);
-    if (cw <= 0.0f) return 1.0f;
-    float ndc_x = rdiv(cx, cw), ndc_y = rdiv(cy, cw);
-    float ux = ndc_x * 0.5f + 0.5f;
-    float uy = 1.0f - (ndc_y * 0.5f + 0.5f);
-    if (ux < 0.0f || uy < 0.0f || ux > 1.0f || uy > 1.0f) return 1.0f;
-    uint32_t dom_w = to_u32(p.dom_info.x), dom_h = to_u32(p.dom_info.y);
-    float near_ = p.dom_info.z, far_ = p.dom_info.w;
-    float dist = dot3(world - xyz(p.light_eye), xyz(p.light_forward)) - (-logf(mx(beta, 1e-4f)) * p.filter_params.z);
-    float fx = ux * u2f(dom_w) - 0.5f;
-    float fy = uy * u2f(dom_h) - 0.5f;
-    float flx = floorf(fx), fly = floorf(fy);
-    int32_t x0 = to_i32(flx), y0 = to_i32(fly);
-    float tx = fx - flx, ty = fy - fly;
-    float v00 = dom_texel_visibility(p, dom, x0, y0, dist, near_, far_, dom_w, dom_h);
-    float v10 = dom_texel_visibility(p, dom, x0 + 1, y0, dist, near_, far_, dom_w, dom_h);
-    float v01 = dom_texel_visibility(p, dom, x0, y0 + 1, dist, near_, far_, dom_w, dom_h);
-    float v11 = dom_texel_visibility(p, dom, x0 + 1, y0 + 1, dist, near_, far_, dom_w, dom_h);
-    return mixs(mixs(v00, v10, tx), mixs(v01, v11, tx), ty);
-}
-
-__device__ __forceinline__ float distance_attenuation(float dist2, float inv_range2) {
-    float factor = dist2 * inv_range2;
-    float smooth = clampf(1.0f - factor * factor, 0.0f, 1.0f);
-    return smooth * smooth * 1.0f / mx(dist2, 1e-4f);
-}
-
-__device__ __forceinline__ float spot_mask(const hm_point_light& L, F3 world) {
-    F3 sd = {L.light_custom_data[0], 0.0f, L.light_custom_data[1]};
-    sd.y = sqrtf(mx(0.0f, 1.0f - sd.x * sd.x - sd.z * sd.z));
-    if (L.flags & HM_LIGHT_FLAG_SPOT_Y_NEGATIVE) sd.y = -sd.y;
-    F3 ltf = F3{L.position_radius[0], L.position_radius[1], L.position_radius[2]} - world;
-    float cd = dot3(sd * -1.0f, safe_normalize(ltf));
-    float att = clampf(cd * L.light_custom_data[2] + L.light_custom_data[3], 0.0f, 1.0f);
-    return att * att;
-}
-
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     const View& p = *reinterpret_cast<const View*>(&a.view);
     uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
@@ -640,57 +455,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         a.shaded_depth[pix] = 0.0f;
         return;
     }
-    uint32_t low = (uint32_t)word;
-    float view_dist = dequantize_depth(gb_depth(word), p.screen.z, p.screen.w);
-    F3 world = reconstruct_world(p, (int32_t)x, (int32_t)y, view_dist);
-    F3 tangent = oct_decode(gb_tx(word), gb_ty(word));
-    float uvw_z = u2f((low >> 6u) & 0x3fu) / 63.0f;
-    float beta = u2f(a.beta[pix] & 0xffu) / 255.0f;
-    float roughness = mx(p.appearance.x * (0.65f + 0.7f * uvw_z), 0.04f);
-    float tilt = p.appearance.y;
-    F3 albedo = xyz(p.albedo) * (0.65f + 0.35f * uvw_z);
-    bool ortho = p.clip[3] == 0.0f && p.clip[7] == 0.0f && p.clip[11] == 0.0f && p.clip[15] == 1.0f;
-    F3 view = ortho ? safe_normalize(F3{p.clip[2], p.clip[6], p.clip[10]}) : safe_normalize(xyz(p.camera_pos) - world);
-
-    float bb = clampf(roughness * roughness, 0.02f, 1.0f);
-    Shape shape = {{tilt, -0.5f * tilt, -1.5f * tilt}, {bb, 0.5f * bb, 2.0f * bb}};
-    float tv = dot3(tangent, view);
-    float cos_theta_o = sqrtf(mx(1.0f - tv * tv, 1e-6f));
-    F3 sigma_a = absorption_from_color(albedo, roughness);
-    DualScatter ds = dual_scatter(cos_theta_o, sigma_a, shape);
-    F3 color = splat3(0.0f);
-    for (int pass = 0; pass < 2; ++pass) {
-        for (uint32_t i = 0; i < a.point_count; ++i) {
-            const hm_point_light& L = a.points[i];
-            if ((pass == 0) != (L.kind == HM_LIGHT_POINT)) continue;
-            F3 ltf = F3{L.position_radius[0], L.position_radius[1], L.position_radius[2]} - world;
-            float dist2 = dot3(ltf, ltf);
-            float att = distance_attenuation(dist2, L.color_inverse_square_range[3]);
-            if (att == 0.0f) continue;  // outside the light's range: the cluster would not list it
-            F3 incident = safe_normalize(ltf);
-            F3 lobe = hair_bsdf(tangent, view, incident, sigma_a, shape, ds, 0.0f);
-            F3 c = lobe * F3{L.color_inverse_square_range[0], L.color_inverse_square_range[1],
-                             L.color_inverse_square_range[2]} * att;
-            if (pass == 1) c = c * spot_mask(L, world);
-            color = color + c;
-        }
-    }
-    bool dom_left = true;
-    for (uint32_t i = 0; i < a.directional_count; ++i) {
-        const hm_directional_light& L = a.directionals[i];
-        bool casts = (L.flags & HM_LIGHT_FLAG_SHADOWS) != 0u;
-        float depth = 0.0f;
-        if (dom_left && casts) {
-            depth = -logf(mx(dom_visibility(p, a.dom, world, beta), 1e-6f));
-            dom_left = false;
-        }
-        F3 lobe = hair_bsdf(tangent, view, F3{L.direction_to_light[0], L.direction_to_light[1], L.direction_to_light[2]},
-                            sigma_a, shape, ds, depth);
-        color = color + lobe * F3{L.color[0], L.color[1], L.color[2]};
-    }
-    color = color * a.exposure;
+    PixelShade px = pixel_setup(p, word, a.beta[pix], x, y);
+    F3 color = plain_pixel_color(a, p, px);
     a.shaded[pix] = make_float4(color.x, color.y, color.z, 1.0f);
-    a.shaded_depth[pix] = view_dist;
+    a.shaded_depth[pix] = px.view_dist;
 }
 
 // ---- filter.wgsl ----------------------------------------------------------------

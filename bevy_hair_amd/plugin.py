@@ -195,8 +195,10 @@ class Renderer:
 
     # -- frame driver --
     @staticmethod
-    def frame_opts(exposure=1.0, control_points=0, strand_fraction=0.0) -> N.HmFrameOpts:
-        return N.HmFrameOpts(exposure, N.HmLodOverride(control_points, strand_fraction))
+    def frame_opts(exposure=1.0, control_points=0, strand_fraction=0.0, mlp: bool = False,
+                   mlp_tolerance: float = 0.05) -> N.HmFrameOpts:
+        return N.HmFrameOpts(exposure, N.HmLodOverride(control_points, strand_fraction),
+                             N.SHADING_MLP if mlp else N.SHADING_PLAIN, mlp_tolerance)
 
     def hair_pass(self, g: GpuGroom, params, opts: N.HmFrameOpts | None = None):
         opts = opts or self.frame_opts()
@@ -254,6 +256,80 @@ class Renderer:
         cap = max(1, g.mesh.strand_count)
         refs = self.read(N.BUF_CAMERA_REFS if pass_ == 0 else N.BUF_DOM_REFS, np.uint32, cap * 4, g).reshape(-1, 4)
         return refs[:total], total
+
+    # -- learned shading --
+    def mlp_init(self, seed: int = 0):
+        self._check(N.lib().hm_mlp_init(self.ctx, seed), "hm_mlp_init")
+
+    def mlp_set_weights(self, w: np.ndarray, b: np.ndarray):
+        w = np.ascontiguousarray(w, np.float32).reshape(-1)
+        b = np.ascontiguousarray(b, np.float32).reshape(-1)
+        assert w.size == N.MLP_NW and b.size == N.MLP_NBIAS
+        self._check(N.lib().hm_mlp_set_weights(self.ctx, N.ptr(w), N.ptr(b)), "hm_mlp_set_weights")
+
+    def mlp_get_weights(self):
+        w = np.empty(N.MLP_NW, np.float32)
+        b = np.empty(N.MLP_NBIAS, np.float32)
+        self._check(N.lib().hm_mlp_get_weights(self.ctx, N.ptr(w), N.ptr(b)), "hm_mlp_get_weights")
+        return w, b
+
+    def mlp_forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty((x.shape[0], 16), np.float32)
+        self._check(N.lib().hm_mlp_forward(self.ctx, N.ptr(x), x.shape[0], N.ptr(y)), "hm_mlp_forward")
+        return y
+
+    def mlp_gradients(self, x: np.ndarray, t: np.ndarray, scale: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        scale = np.ascontiguousarray(scale, np.float32)
+        dw = np.empty(N.MLP_NW, np.float32)
+        db = np.empty(N.MLP_NBIAS, np.float32)
+        loss = C.c_float()
+        self._check(N.lib().hm_mlp_gradients(self.ctx, N.ptr(x), N.ptr(t), x.shape[0], N.ptr(scale), C.byref(loss),
+                                              N.ptr(dw), N.ptr(db)), "hm_mlp_gradients")
+        return float(loss.value), dw, db
+
+    def mlp_reserve_samples(self, capacity: int):
+        self._check(N.lib().hm_mlp_reserve_samples(self.ctx, capacity), "hm_mlp_reserve_samples")
+
+    def mlp_collect(self, params, keep: float, seed: int) -> int:
+        total = C.c_uint32()
+        self._check(N.lib().hm_mlp_collect(self.ctx, C.byref(params), keep, seed, C.byref(total)), "hm_mlp_collect")
+        return int(total.value)
+
+    def mlp_read_samples(self, n: int):
+        x = np.empty((n, 32), np.float32)
+        t = np.empty((n, 12), np.float32)
+        self._check(N.lib().hm_mlp_read_samples(self.ctx, N.ptr(x), N.ptr(t), n), "hm_mlp_read_samples")
+        return x, t
+
+    def mlp_set_samples(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        assert x.shape[1] == 32 and t.shape[1] == 12 and x.shape[0] == t.shape[0]
+        self._check(N.lib().hm_mlp_set_samples(self.ctx, N.ptr(x), N.ptr(t), x.shape[0]), "hm_mlp_set_samples")
+
+    def mlp_set_range(self, lo, hi):
+        if lo is None:
+            self._check(N.lib().hm_mlp_set_range(self.ctx, None, None), "hm_mlp_set_range")
+            return
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        self._check(N.lib().hm_mlp_set_range(self.ctx, N.ptr(lo), N.ptr(hi)), "hm_mlp_set_range")
+
+    def mlp_train(self, epochs: int, batch: int, lr: float, scale, seed: int = 0) -> float:
+        scale = np.ascontiguousarray(scale, np.float32)
+        loss = C.c_float()
+        self._check(N.lib().hm_mlp_train(self.ctx, epochs, batch, lr, N.ptr(scale), seed, C.byref(loss)),
+                    "hm_mlp_train")
+        return float(loss.value)
+
+    def shade_mlp(self, params, exposure: float = 1.0, tolerance: float = 0.05, count: bool = True):
+        n = C.c_uint32()
+        self._check(N.lib().hm_shade_mlp(self.ctx, C.byref(params), exposure, tolerance,
+                                         C.byref(n) if count else None), "hm_shade_mlp")
+        return int(n.value) if count else None
 
     def shadow_light(self, dom_params) -> N.HmShadowLight:
         out = N.HmShadowLight()

@@ -211,10 +211,15 @@ typedef enum hm_kernel {
     HM_K_COUNT = 11
 } hm_kernel;
 
+#define HM_SHADING_PLAIN 0u
+#define HM_SHADING_MLP 1u
+
 /* Per-frame options of hm_frame (pass.rs:375-643). */
 typedef struct hm_frame_opts {
     float exposure;                 /* view.exposure (shade.wesl:588) */
     hm_lod_override camera_lod;     /* see hm_lod_override */
+    uint32_t shading;               /* HM_SHADING_PLAIN or HM_SHADING_MLP */
+    float mlp_tolerance;            /* per-tile probe bound for HM_SHADING_MLP */
 } hm_frame_opts;
 
 /* ---- lifetime ----------------------------------------------------------- */
@@ -312,6 +317,51 @@ hm_status hm_timing_read(hm_ctx* ctx, float* ms, uint32_t count);
  * continues inside. */
 hm_status hm_time_frames(hm_ctx* ctx, hm_groom* groom, const hm_params params[3],
                          const hm_frame_opts* opts, uint32_t frames, float* total_ms);
+
+/* ---- learned shading (no counterpart in the reference) -------------------
+ * A 29 -> 64 -> 64 -> 64 -> 12 MLP (bf16 weights on the matrix cores, f32
+ * accumulation) stands in for dual_scatter (shade.wesl:153-190) and the key
+ * light's multiple-scattering part of hair_bsdf (shade.wesl:220-242); single
+ * scattering stays analytic. Features per pixel (float[32]): tangent[3],
+ * view[3], key light direction[3], 16 DOM visibility samples at the pixel's
+ * light depth shifted by (k - 8) / 4 opacity layers, sigma_a * 0.25 [3],
+ * roughness, 3 zeros. Outputs (float[16], 12 used): key-light multiple
+ * scattering per unit light colour [3], a_back[3], delta_b[3], sigma_b^2[3].
+ * Weights: float[11264] (W0 64x32, W1 64x64, W2 64x64, W3 16x64, out x in
+ * row-major), biases float[208]. */
+
+/* He-uniform initialisation from `seed`; zero biases. */
+hm_status hm_mlp_init(hm_ctx* ctx, uint32_t seed);
+hm_status hm_mlp_set_weights(hm_ctx* ctx, const float* weights, const float* biases);
+hm_status hm_mlp_get_weights(hm_ctx* ctx, float* weights, float* biases);
+/* y[n][16] = network(x[n][32]); host arrays, synchronous. */
+hm_status hm_mlp_forward(hm_ctx* ctx, const float* x, uint32_t n, float* y);
+/* Loss sum_o ((y_o - t_o) * scale_o)^2 / (12 n) over the 12 outputs and its
+ * gradients (float[11264], float[208]) for one batch; synchronous. */
+hm_status hm_mlp_gradients(hm_ctx* ctx, const float* x, const float* t, uint32_t n, const float scale[12],
+                           float* loss, float* grad_weights, float* grad_biases);
+/* Sample buffer for training (features + plain-method targets); resets the count. */
+hm_status hm_mlp_reserve_samples(hm_ctx* ctx, uint32_t capacity);
+/* Appends samples from the current G-buffer/DOM (after the raster passes of
+ * a frame): each hit pixel is kept with probability `keep`. */
+hm_status hm_mlp_collect(hm_ctx* ctx, const hm_params* camera_params, float keep, uint32_t seed, uint32_t* total);
+hm_status hm_mlp_read_samples(hm_ctx* ctx, float* x, float* t, uint32_t n);
+/* Replaces the held samples with host ones (x[n][32], t[n][12]); n <= capacity. */
+hm_status hm_mlp_set_samples(hm_ctx* ctx, const float* x, const float* t, uint32_t n);
+/* Feature range the network was trained on (NULL clears it); tiles with a
+ * pixel outside it are shaded by the plain method. */
+hm_status hm_mlp_set_range(hm_ctx* ctx, const float lo[32], const float hi[32]);
+/* Adam over the collected samples for `epochs` passes; returns the last
+ * epoch's mean loss. */
+hm_status hm_mlp_train(hm_ctx* ctx, uint32_t epochs, uint32_t batch, float learning_rate, const float scale[12],
+                       uint32_t seed, float* final_loss);
+/* Network shading of the frame's G-buffer into HM_BUF_SHADED. Per 16x4 tile,
+ * one probe pixel is shaded by the plain method; tiles whose probe misses by
+ * more than `tolerance` (relative, L1 over RGB) or that leave the trained
+ * range are re-shaded by the plain method. `fallback_tiles` (may be NULL,
+ * then the call is asynchronous) receives their count. */
+hm_status hm_shade_mlp(hm_ctx* ctx, const hm_params* params, float exposure, float tolerance,
+                       uint32_t* fallback_tiles);
 
 #ifdef __cplusplus
 }  /* extern "C" */
