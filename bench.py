@@ -101,7 +101,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-fraction", type=int, default=8)
+    ap.add_argument("--cpu-fraction", type=int, default=2)
+    ap.add_argument("--no-probes", action="store_true", help="skip the copy/atomic microbenchmarks")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this path runs on a single MI355X (asked for --gpus {args.gpus})",
@@ -183,6 +184,20 @@ def main() -> int:
             "dom_opacity_atomics_per_s": round(float(counts["dom2"][2]) / (kt["dom_opacity"] * 1e-3), 1)
             if kt["dom_opacity"] else None},
     }
+    if not args.no_probes:
+        # Measured denominators: STREAM copy and random-address atomics over the
+        # footprints the raster passes hit (G-buffer words, DOM fronts, DOM counts).
+        P, S = w.width * w.height, w.dom_size
+        peaks = {"copy_GBs": round(r.bench_copy(1 << 30, 5), 1),
+                 "u64_min_per_s": r.bench_atomics(8 * P, 0, 1 << 26),
+                 "u32_min_per_s": r.bench_atomics(4 * S * S, 1, 1 << 26),
+                 "u32_add_per_s": r.bench_atomics(64 * S * S, 2, 1 << 26)}
+        result["measured_peaks"] = peaks
+        rates = result["atomic_rates"]
+        if rates["cam_raster_atomics_per_s"]:
+            rates["cam_raster_frac_of_u64_min_peak"] = round(rates["cam_raster_atomics_per_s"] / peaks["u64_min_per_s"], 4)
+        if rates["dom_opacity_atomics_per_s"]:
+            rates["dom_opacity_frac_of_u32_add_peak"] = round(rates["dom_opacity_atomics_per_s"] / peaks["u32_add_per_s"], 4)
     g.free()
     r.close()
     if not args.no_cpu:

@@ -103,6 +103,13 @@ SHADING_PLAIN, SHADING_MLP = 0, 1
 MLP_NW, MLP_NBIAS = 11264, 208
 
 
+class HmTuning(C.Structure):
+    _fields_ = [("raster_flags", u32 * 3), ("raster_blocks_per_cu", u32)]
+
+
+RASTER_PRECHECK, RASTER_XCD = 1, 2
+
+
 class HmShadowLight(C.Structure):
     _fields_ = [
         ("light_clip_from_world", f32 * 16),
@@ -159,6 +166,11 @@ ABI = {
     "hm_timing_enable": (C.c_int, [P, C.c_int]),
     "hm_timing_read": (C.c_int, [P, C.POINTER(f32), u32]),
     "hm_time_frames": (C.c_int, [P, P, C.POINTER(HmParams), C.POINTER(HmFrameOpts), u32, C.POINTER(f32)]),
+    "hm_get_tuning": (C.c_int, [P, C.POINTER(HmTuning)]),
+    "hm_set_tuning": (C.c_int, [P, C.POINTER(HmTuning)]),
+    "hm_autotune": (C.c_int, [P, P, C.POINTER(HmParams), C.POINTER(HmFrameOpts), u32, C.POINTER(HmTuning)]),
+    "hm_bench_copy": (C.c_int, [P, C.c_size_t, u32, C.POINTER(f32)]),
+    "hm_bench_atomics": (C.c_int, [P, C.c_size_t, u32, C.c_uint64, C.POINTER(f32)]),
     "hm_mlp_init": (C.c_int, [P, u32]),
     "hm_mlp_set_weights": (C.c_int, [P, P, P]),
     "hm_mlp_get_weights": (C.c_int, [P, P, P]),

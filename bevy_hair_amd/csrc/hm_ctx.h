@@ -35,6 +35,8 @@ struct hm_ctx {
     float4* filtered = nullptr;
     float* filtered_depth = nullptr;
     uint32_t* dom = nullptr;
+    uint32_t* pix_list = nullptr;   // covered-pixel list (shade, then filter)
+    uint32_t* pix_count = nullptr;
     float* scene_depth = nullptr;
     float scene_clear = 0.0f;
     hm_point_light* points = nullptr;
@@ -42,6 +44,7 @@ struct hm_ctx {
     hm_directional_light* dirs = nullptr;
     uint32_t dir_count = 0, dir_cap = 0;
     uint64_t* counters = nullptr;
+    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, 16u};
     // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
     uint32_t slots = 0, frame = 0;
     std::vector<hipEvent_t> ev;

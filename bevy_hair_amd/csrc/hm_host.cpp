@@ -274,13 +274,15 @@ void free_targets(hm_ctx* c) {
     dev_free(c->filtered);
     dev_free(c->filtered_depth);
     dev_free(c->dom);
+    dev_free(c->pix_list);
+    dev_free(c->pix_count);
     dev_free(c->scene_depth);
     c->width = c->height = c->dom_size = 0;
 }
 
 int raster_blocks(const hm_ctx* c, uint32_t capacity) {
     uint32_t want = (capacity + 3u) / 4u;
-    uint32_t cap = (uint32_t)c->cu_count * 16u;
+    uint32_t cap = (uint32_t)c->cu_count * (c->tuning.raster_blocks_per_cu ? c->tuning.raster_blocks_per_cu : 16u);
     return (int)(want < cap ? (want ? want : 1u) : cap);
 }
 
@@ -486,6 +488,8 @@ hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_
     if (e == hipSuccess) e = dev_alloc(&ctx->filtered, px * 16);
     if (e == hipSuccess) e = dev_alloc(&ctx->filtered_depth, px * 4);
     if (e == hipSuccess) e = dev_alloc(&ctx->dom, dom_words * 4);
+    if (e == hipSuccess) e = dev_alloc(&ctx->pix_list, px * 4);
+    if (e == hipSuccess) e = dev_alloc(&ctx->pix_count, 64);
     if (e == hipSuccess) e = hipMemset(ctx->dom, 0, dom_words * 4);
     if (e != hipSuccess) {
         free_targets(ctx);
@@ -694,6 +698,7 @@ static hm_status raster_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int p
     a.scene_depth = ctx->scene_depth;
     a.scene_clear = ctx->scene_clear;
     a.counters = ctx->counters;
+    a.opt = ctx->tuning.raster_flags[p->pass_mode[0] < 3u ? p->pass_mode[0] : 0u];
     return HM_OK;
 }
 
@@ -739,7 +744,10 @@ hm_status hm_shade(hm_ctx* ctx, const hm_params* p, float exposure) {
     a.shaded = ctx->shaded;
     a.shaded_depth = ctx->shaded_depth;
     Timer t(ctx, HM_K_SHADE);
-    HM_HIP(ctx, hm::launch_shade(ctx->stream, a, ctx->width, ctx->height));
+    uint32_t px = ctx->width * ctx->height;
+    HM_HIP(ctx, hm::launch_compact(ctx->stream, 0, ctx->center, ctx->cons, px, ctx->pix_list, ctx->pix_count,
+                                   ctx->shaded, ctx->shaded_depth));
+    HM_HIP(ctx, hm::launch_shade(ctx->stream, a, ctx->pix_list, ctx->pix_count, px, ctx->cu_count * 8));
     return HM_OK;
 }
 
@@ -757,7 +765,10 @@ hm_status hm_filter(hm_ctx* ctx, const hm_params* p) {
     a.filtered = ctx->filtered;
     a.filtered_depth = ctx->filtered_depth;
     Timer t(ctx, HM_K_FILTER);
-    HM_HIP(ctx, hm::launch_filter(ctx->stream, a, ctx->width, ctx->height));
+    uint32_t px = ctx->width * ctx->height;
+    HM_HIP(ctx, hm::launch_compact(ctx->stream, 1, ctx->center, ctx->cons, px, ctx->pix_list, ctx->pix_count,
+                                   ctx->filtered, ctx->filtered_depth));
+    HM_HIP(ctx, hm::launch_filter(ctx->stream, a, ctx->pix_list, ctx->pix_count, px, ctx->cu_count * 8));
     return HM_OK;
 }
 
@@ -883,6 +894,119 @@ hm_status hm_timing_read(hm_ctx* ctx, float* ms, uint32_t count) {
         }
         ms[k] = n ? (float)(sum / n) : 0.0f;
     }
+    return HM_OK;
+}
+
+hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out) {
+    if (!ctx || !out) return HM_ERR_INVALID;
+    *out = ctx->tuning;
+    return HM_OK;
+}
+
+hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
+    if (!ctx || !t || t->raster_blocks_per_cu == 0 || t->raster_blocks_per_cu > 64)
+        return fail(ctx, HM_ERR_INVALID, "raster_blocks_per_cu must be in [1, 64]");
+    for (int m = 0; m < 3; ++m)
+        if (t->raster_flags[m] & ~(HM_RASTER_PRECHECK | HM_RASTER_XCD)) return fail(ctx, HM_ERR_INVALID, "unknown flag");
+    ctx->tuning = *t;
+    return HM_OK;
+}
+
+hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm_frame_opts* opts, uint32_t reps,
+                      hm_tuning* best) {
+    if (!ctx || !g || !params || reps == 0) return fail(ctx, HM_ERR_INVALID, "null argument");
+    hm_status st;
+    const uint32_t grids[4] = {4u, 8u, 16u, 32u};
+    hm_tuning keep = ctx->tuning, out = ctx->tuning;
+    uint32_t saved_slots = ctx->slots;
+    ctx->slots = 0;  // no per-kernel events while tuning
+    const hm_lod_override* ovr = opts ? &opts->camera_lod : nullptr;
+    // Work lists of both passes, and the DOM front the opacity pass reads.
+    if ((st = hm_clear(ctx)) != HM_OK) return st;
+    if ((st = hm_lod(ctx, g, &params[0], HM_PASS_CAMERA, ovr)) != HM_OK) return st;
+    if ((st = hm_scan(ctx, g, &params[0], HM_PASS_CAMERA)) != HM_OK) return st;
+    if ((st = hm_clear_dom(ctx)) != HM_OK) return st;
+    if ((st = hm_lod(ctx, g, &params[1], HM_PASS_DOM, nullptr)) != HM_OK) return st;
+    if ((st = hm_scan(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
+    if ((st = hm_raster(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
+    float best_ms[3] = {1e30f, 1e30f, 1e30f};
+    uint32_t best_grid[3] = {16u, 16u, 16u};
+    for (int mode = 0; mode < 3; ++mode) {
+        const hm_params* p = &params[mode];
+        int pass = mode == 0 ? HM_PASS_CAMERA : HM_PASS_DOM;
+        for (uint32_t flags = 0; flags < 4u; ++flags) {
+            for (uint32_t gi = 0; gi < 4; ++gi) {
+                ctx->tuning.raster_flags[mode] = flags;
+                ctx->tuning.raster_blocks_per_cu = grids[gi];
+                // the depth pass re-runs on its own output; mins are idempotent
+                if ((st = hm_raster(ctx, g, p, pass)) != HM_OK) return st;  // warm
+                HM_HIP(ctx, hipEventRecord(ctx->frame_ev[0], ctx->stream));
+                for (uint32_t r = 0; r < reps; ++r)
+                    if ((st = hm_raster(ctx, g, p, pass)) != HM_OK) return st;
+                HM_HIP(ctx, hipEventRecord(ctx->frame_ev[1], ctx->stream));
+                HM_HIP(ctx, hipEventSynchronize(ctx->frame_ev[1]));
+                float ms = 0.0f;
+                HM_HIP(ctx, hipEventElapsedTime(&ms, ctx->frame_ev[0], ctx->frame_ev[1]));
+                if (ms < best_ms[mode]) {
+                    best_ms[mode] = ms;
+                    out.raster_flags[mode] = flags;
+                    best_grid[mode] = grids[gi];
+                }
+            }
+        }
+    }
+    // One grid size for all passes: the one the camera pass (the largest) prefers.
+    out.raster_blocks_per_cu = best_grid[0];
+    ctx->tuning = out;
+    ctx->slots = saved_slots;
+    (void)keep;
+    if (best) *best = out;
+    return HM_OK;
+}
+
+hm_status hm_bench_copy(hm_ctx* ctx, size_t bytes, uint32_t reps, float* gbps) {
+    if (!ctx || !gbps || bytes < 16 || reps == 0) return HM_ERR_INVALID;
+    bytes &= ~(size_t)15;
+    (void)hipSetDevice(ctx->device);
+    void *a = nullptr, *b = nullptr;
+    HM_HIP(ctx, hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hm::launch_fill32(ctx->stream, (uint32_t*)a, bytes / 4, 1u, ctx->cu_count * 8);
+    if (e == hipSuccess) e = hm::launch_copy(ctx->stream, a, b, bytes, ctx->cu_count * 8);  // warm
+    if (e == hipSuccess) e = hipEventRecord(ctx->frame_ev[0], ctx->stream);
+    for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) e = hm::launch_copy(ctx->stream, a, b, bytes, ctx->cu_count * 8);
+    if (e == hipSuccess) e = hipEventRecord(ctx->frame_ev[1], ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->frame_ev[1]);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->frame_ev[0], ctx->frame_ev[1]);
+    (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hm_bench_copy");
+    *gbps = (float)(2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9);  // read + write
+    return HM_OK;
+}
+
+hm_status hm_bench_atomics(hm_ctx* ctx, size_t target_bytes, uint32_t kind, uint64_t count, float* per_second) {
+    if (!ctx || !per_second || kind > 2 || count == 0) return HM_ERR_INVALID;
+    size_t word = kind == 0 ? 8 : 4;
+    size_t words = target_bytes / word;
+    if (words == 0) return HM_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    void* buf = nullptr;
+    HM_HIP(ctx, hipMalloc(&buf, words * word));
+    uint32_t threads = (uint32_t)ctx->cu_count * 2048u;
+    uint32_t per = (uint32_t)((count + threads - 1) / threads);
+    hipError_t e = hm::launch_fill32(ctx->stream, (uint32_t*)buf, words * word / 4, 0xffffffffu, ctx->cu_count * 8);
+    if (e == hipSuccess) e = hm::launch_atomics(ctx->stream, buf, words, threads, per, kind);  // warm
+    if (e == hipSuccess) e = hipEventRecord(ctx->frame_ev[0], ctx->stream);
+    if (e == hipSuccess) e = hm::launch_atomics(ctx->stream, buf, words, threads, per, kind);
+    if (e == hipSuccess) e = hipEventRecord(ctx->frame_ev[1], ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->frame_ev[1]);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->frame_ev[0], ctx->frame_ev[1]);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hm_bench_atomics");
+    *per_second = (float)((double)threads * per / (ms * 1e-3));
     return HM_OK;
 }
 

@@ -8,6 +8,9 @@
 
 namespace hm {
 
+constexpr uint32_t RASTER_PRECHECK = 1u;  // screen each atomic min with a plain load
+constexpr uint32_t RASTER_XCD = 2u;       // contiguous strand ranges per XCD
+
 // Everything one raster pass reads or writes (the raster.wgsl bind group,
 // pipeline.rs:110-170, plus the work list bound and a counter block).
 struct RasterArgs {
@@ -25,6 +28,7 @@ struct RasterArgs {
     const float* scene_depth;       // W*H or null
     float scene_clear;
     uint64_t* counters;             // counting variant only
+    uint32_t opt;                   // RASTER_* flags
 };
 
 struct ShadeArgs {
@@ -51,13 +55,19 @@ struct FilterArgs {
     float* filtered_depth;
 };
 
+hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes, int blocks);
+hipError_t launch_atomics(hipStream_t s, void* buf, size_t words, uint32_t threads, uint32_t per_thread, uint32_t kind);
 hipError_t launch_fill32(hipStream_t s, uint32_t* dst, size_t words, uint32_t value, int max_blocks);
 hipError_t launch_lod(hipStream_t s, const hm_params& prm, const hm_bundle* bundles, const hm_layer* layers,
                       hm_lod_record* out, uint32_t nb, uint32_t ovr_cps, float ovr_frac);
 hipError_t launch_scan(hipStream_t s, const hm_lod_record* lod, uint32_t nb, uint32_t capacity, uint32_t* prefix,
                        uint32_t* indirect, hm_strand_ref* refs);
 hipError_t launch_raster(hipStream_t s, const RasterArgs& a, int mode, bool count, int blocks);
-hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, uint32_t width, uint32_t height);
-hipError_t launch_filter(hipStream_t s, const FilterArgs& a, uint32_t width, uint32_t height);
+hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const uint64_t* cons, uint32_t pixels,
+                          uint32_t* list, uint32_t* count, float4* color_out, float* depth_out);
+hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, const uint32_t* list, const uint32_t* count,
+                        uint32_t pixels, int blocks);
+hipError_t launch_filter(hipStream_t s, const FilterArgs& a, const uint32_t* list, const uint32_t* count,
+                         uint32_t pixels, int blocks);
 
 }  // namespace hm

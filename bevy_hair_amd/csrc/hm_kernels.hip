@@ -30,6 +30,27 @@ __global__ __launch_bounds__(256) void k_fill32(uint32_t* __restrict__ dst, size
     if (tail < words) dst[tail] = value;
 }
 
+// ---- microbenchmarks (roofline denominators) ----------------------------------
+
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// Random-address atomics over `words` targets: kind 0 u64 min, 1 u32 min, 2 u32 add.
+__global__ __launch_bounds__(256) void k_atomics(void* __restrict__ buf, size_t words, uint32_t per_thread,
+                                                 uint32_t kind, uint32_t seed) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t h = pcg(id ^ seed);
+    for (uint32_t k = 0; k < per_thread; ++k) {
+        h = pcg(h);
+        size_t i = (size_t)(((uint64_t)h * words) >> 32);
+        if (kind == 0) atomicMin((unsigned long long*)buf + i, (unsigned long long)h);
+        else if (kind == 1) atomicMin((uint32_t*)buf + i, h);
+        else atomicAdd((uint32_t*)buf + i, 1u);
+    }
+}
+
 // ---- lod.wgsl -----------------------------------------------------------------
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -150,13 +171,14 @@ __global__ __launch_bounds__(256) void k_expand(const hm_lod_record* __restrict_
 
 // ---- raster.wgsl ----------------------------------------------------------------
 
-__device__ __forceinline__ void gmin64(uint64_t* a, uint64_t v) {
-    // Targets only decrease during a pass, so a value read here is an upper
-    // bound of the current one and a payload not below it cannot win.
-    if (v < *a) atomicMin((unsigned long long*)a, (unsigned long long)v);
+// With `check`, a plain load screens the atomic: targets only decrease during
+// a pass, so a value read here bounds the current one from above and a payload
+// that is not below it cannot change the word.
+__device__ __forceinline__ void gmin64(uint64_t* a, uint64_t v, bool check) {
+    if (!check || v < *a) atomicMin((unsigned long long*)a, (unsigned long long)v);
 }
-__device__ __forceinline__ void gmin32(uint32_t* a, uint32_t v) {
-    if (v < *a) atomicMin(a, v);
+__device__ __forceinline__ void gmin32(uint32_t* a, uint32_t v, bool check) {
+    if (!check || v < *a) atomicMin(a, v);
 }
 
 __device__ __forceinline__ F3 shfl3(F3 v, int src) { return {__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src)}; }
@@ -195,10 +217,11 @@ struct Drawer {
             if (center_hit) c.cam_centers++;
             return;
         }
-        gmin64(a.cons + pix, payload);
+        const bool chk = (a.opt & RASTER_PRECHECK) != 0u;
+        gmin64(a.cons + pix, payload, chk);
         if (center_hit) {
-            gmin64(a.center + pix, payload);
-            gmin32(a.beta + pix, beta_word);
+            gmin64(a.center + pix, payload, chk);
+            gmin32(a.beta + pix, beta_word, chk);
         }
     }
 
@@ -269,7 +292,7 @@ struct Drawer {
         uint32_t q = quantize_depth(dist, near_, far_);
         if (MODE == 1) {
             if (COUNT) c.dom_writes++;
-            else gmin32(a.dom + pix, q);
+            else gmin32(a.dom + pix, q, (a.opt & RASTER_PRECHECK) != 0u);
             return;
         }
         uint32_t front_q = a.dom[pix];  // final: written by the previous (depth) pass
@@ -324,13 +347,27 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs a) {
     const View& p = *reinterpret_cast<const View*>(&a.view);
     const int lane = (int)(threadIdx.x & 63u);
     const uint32_t waves_per_block = blockDim.x >> 6;
-    const uint32_t wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * waves_per_block + (threadIdx.x >> 6));
-    const uint32_t nwaves = gridDim.x * waves_per_block;
     uint32_t total = a.indirect[0];
     total = total < a.capacity ? total : a.capacity;
+    // Strand schedule. Default: wave w takes strands w, w + W, ... With
+    // RASTER_XCD, workgroup b runs on XCD b % 8 and each XCD takes one
+    // contiguous eighth of the bundle-major strand list, so strands that hit
+    // the same screen region share that XCD's L2.
+    uint32_t first = 0, end = total;
+    uint32_t wave0 = blockIdx.x * waves_per_block + (threadIdx.x >> 6);
+    uint32_t nwaves = gridDim.x * waves_per_block;
+    if ((a.opt & RASTER_XCD) && gridDim.x >= 8u) {
+        uint32_t xcd = blockIdx.x & 7u;
+        uint32_t blocks_here = (gridDim.x - xcd + 7u) / 8u;
+        first = (uint32_t)((uint64_t)total * xcd / 8u);
+        end = (uint32_t)((uint64_t)total * (xcd + 1u) / 8u);
+        wave0 = first + (blockIdx.x >> 3) * waves_per_block + (threadIdx.x >> 6);
+        nwaves = blocks_here * waves_per_block;
+    }
+    wave0 = __builtin_amdgcn_readfirstlane(wave0);
     Drawer<MODE, COUNT> dr{p, a, {0, 0, 0, 0}};
 
-    for (uint32_t s = wave0; s < total; s += nwaves) {
+    for (uint32_t s = wave0; s < end; s += nwaves) {
         const hm_strand_ref ref = a.refs[s];
         const hm_bundle bundle = a.bundles[ref.bundle];
         const F2 uv = pcg_uv(ref.strand ^ (ref.bundle * 0x85ebca6bu));
@@ -442,42 +479,61 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs a) {
 
 // ---- shade.wesl -----------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
-    const View& p = *reinterpret_cast<const View*>(&a.view);
-    uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
-    uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
-    uint32_t width = to_u32(p.screen.x), height = to_u32(p.screen.y);
-    if (x >= width || y >= height) return;
-    size_t pix = (size_t)y * width + x;
-    uint64_t word = a.center[pix];
-    if (gb_empty(word)) {
-        a.shaded[pix] = make_float4(0, 0, 0, 0);
-        a.shaded_depth[pix] = 0.0f;
-        return;
+// Covered-pixel lists: one pass over the G-buffer writes the empty result of
+// every uncovered pixel and appends covered pixels to a list (one atomic per
+// wave), so the shading and filter kernels run on full waves.
+// mode 0: shade list (centre word present); mode 1: filter list (centre or
+// conservative word present).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ center, const uint64_t* __restrict__ cons,
+                                                 uint32_t pixels, uint32_t* __restrict__ list,
+                                                 uint32_t* __restrict__ count, float4* __restrict__ color_out,
+                                                 float* __restrict__ depth_out) {
+    uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    bool inb = pix < pixels;
+    bool hit = false;
+    if (inb) {
+        hit = !gb_empty(center[pix]);
+        if (MODE == 1 && !hit) hit = !gb_empty(cons[pix]);
+        if (!hit) {
+            color_out[pix] = make_float4(0, 0, 0, 0);
+            depth_out[pix] = 0.0f;
+        }
     }
-    PixelShade px = pixel_setup(p, word, a.beta[pix], x, y);
-    F3 color = plain_pixel_color(a, p, px);
-    a.shaded[pix] = make_float4(color.x, color.y, color.z, 1.0f);
-    a.shaded_depth[pix] = px.view_dist;
+    uint64_t mask = __ballot(hit);
+    uint32_t lane = threadIdx.x & 63u;
+    uint32_t total = (uint32_t)__popcll(mask);
+    uint32_t base = 0;
+    if (total) {
+        if (lane == 0) base = atomicAdd(count, total);
+        base = __shfl(base, 0);
+    }
+    if (hit) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = pix;
+}
+
+__global__ __launch_bounds__(256) void k_shade(ShadeArgs a, const uint32_t* __restrict__ list,
+                                               const uint32_t* __restrict__ count) {
+    const View& p = *reinterpret_cast<const View*>(&a.view);
+    uint32_t n = *count;
+    uint32_t width = to_u32(p.screen.x);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t pix = list[i];
+        uint32_t x = pix % width, y = pix / width;
+        PixelShade px = pixel_setup(p, a.center[pix], a.beta[pix], x, y);
+        F3 color = plain_pixel_color(a, p, px);
+        a.shaded[pix] = make_float4(color.x, color.y, color.z, 1.0f);
+        a.shaded_depth[pix] = px.view_dist;
+    }
 }
 
 // ---- filter.wgsl ----------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_filter(FilterArgs a) {
-    const View& p = *reinterpret_cast<const View*>(&a.view);
-    int32_t x = (int32_t)(blockIdx.x * 16u + (threadIdx.x & 15u));
-    int32_t y = (int32_t)(blockIdx.y * 16u + (threadIdx.x >> 4));
+__device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p, int32_t x, int32_t y) {
     int32_t width = to_i32(p.screen.x), height = to_i32(p.screen.y);
-    if (x >= width || y >= height) return;
     size_t pix = (size_t)y * (uint32_t)width + (uint32_t)x;
     uint64_t center_word = a.center[pix];
     uint64_t cons_word = a.cons[pix];
     bool center_hit = !gb_empty(center_word), cons_hit = !gb_empty(cons_word);
-    if (!center_hit && !cons_hit) {
-        a.filtered[pix] = make_float4(0, 0, 0, 0);
-        a.filtered_depth[pix] = 0.0f;
-        return;
-    }
     uint64_t w0 = center_hit ? center_word : cons_word;
     F3 tangent = oct_decode(gb_tx(w0), gb_ty(w0));
     uint32_t depth_q = gb_depth(w0);
@@ -550,6 +606,17 @@ __global__ __launch_bounds__(256) void k_filter(FilterArgs a) {
     a.filtered_depth[pix] = view_dist;
 }
 
+__global__ __launch_bounds__(256) void k_filter(FilterArgs a, const uint32_t* __restrict__ list,
+                                                const uint32_t* __restrict__ count) {
+    const View& p = *reinterpret_cast<const View*>(&a.view);
+    uint32_t n = *count;
+    uint32_t width = to_u32(p.screen.x);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t pix = list[i];
+        filter_pixel(a, p, (int32_t)(pix % width), (int32_t)(pix / width));
+    }
+}
+
 }  // namespace hm
 
 // ---- launchers ----------------------------------------------------------------
@@ -563,6 +630,16 @@ hipError_t launch_fill32(hipStream_t s, uint32_t* dst, size_t words, uint32_t va
     if (blocks > (size_t)max_blocks) blocks = (size_t)max_blocks;
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_fill32, dim3((unsigned)blocks), dim3(256), 0, s, dst, words, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes, int blocks) {
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, bytes / 16);
+    return hipGetLastError();
+}
+
+hipError_t launch_atomics(hipStream_t s, void* buf, size_t words, uint32_t threads, uint32_t per_thread, uint32_t kind) {
+    hipLaunchKernelGGL(k_atomics, dim3((threads + 255) / 256), dim3(256), 0, s, buf, words, per_thread, kind, 12345u);
     return hipGetLastError();
 }
 
@@ -599,15 +676,30 @@ hipError_t launch_raster(hipStream_t s, const RasterArgs& a, int mode, bool coun
     return hipGetLastError();
 }
 
-hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, uint32_t width, uint32_t height) {
-    if (width == 0 || height == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_shade, dim3((width + 15) / 16, (height + 15) / 16), dim3(256), 0, s, a);
+hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const uint64_t* cons, uint32_t pixels,
+                          uint32_t* list, uint32_t* count, float4* color_out, float* depth_out) {
+    if (pixels == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u);
+    dim3 g((pixels + 255) / 256), b(256);
+    if (mode == 0) hipLaunchKernelGGL(k_compact<0>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out);
+    else hipLaunchKernelGGL(k_compact<1>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out);
     return hipGetLastError();
 }
 
-hipError_t launch_filter(hipStream_t s, const FilterArgs& a, uint32_t width, uint32_t height) {
-    if (width == 0 || height == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_filter, dim3((width + 15) / 16, (height + 15) / 16), dim3(256), 0, s, a);
+hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, const uint32_t* list, const uint32_t* count,
+                        uint32_t pixels, int blocks) {
+    if (pixels == 0) return hipSuccess;
+    uint32_t want = (pixels + 255) / 256;
+    hipLaunchKernelGGL(k_shade, dim3(want < (uint32_t)blocks ? want : (uint32_t)blocks), dim3(256), 0, s, a, list, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter(hipStream_t s, const FilterArgs& a, const uint32_t* list, const uint32_t* count,
+                         uint32_t pixels, int blocks) {
+    if (pixels == 0) return hipSuccess;
+    uint32_t want = (pixels + 255) / 256;
+    hipLaunchKernelGGL(k_filter, dim3(want < (uint32_t)blocks ? want : (uint32_t)blocks), dim3(256), 0, s, a, list,
+                       count);
     return hipGetLastError();
 }
 

@@ -257,6 +257,32 @@ class Renderer:
         refs = self.read(N.BUF_CAMERA_REFS if pass_ == 0 else N.BUF_DOM_REFS, np.uint32, cap * 4, g).reshape(-1, 4)
         return refs[:total], total
 
+    # -- tuning --
+    def get_tuning(self) -> dict:
+        t = N.HmTuning()
+        self._check(N.lib().hm_get_tuning(self.ctx, C.byref(t)), "hm_get_tuning")
+        return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": int(t.raster_blocks_per_cu)}
+
+    def set_tuning(self, raster_flags, raster_blocks_per_cu: int):
+        t = N.HmTuning((C.c_uint32 * 3)(*raster_flags), raster_blocks_per_cu)
+        self._check(N.lib().hm_set_tuning(self.ctx, C.byref(t)), "hm_set_tuning")
+
+    def autotune(self, g: GpuGroom, params, opts, reps: int = 3) -> dict:
+        t = N.HmTuning()
+        self._check(N.lib().hm_autotune(self.ctx, g.handle, params, C.byref(opts), reps, C.byref(t)), "hm_autotune")
+        return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": int(t.raster_blocks_per_cu)}
+
+    # -- roofline microbenchmarks --
+    def bench_copy(self, nbytes: int, reps: int = 5) -> float:
+        v = C.c_float()
+        self._check(N.lib().hm_bench_copy(self.ctx, nbytes, reps, C.byref(v)), "hm_bench_copy")
+        return float(v.value)
+
+    def bench_atomics(self, target_bytes: int, kind: int, count: int) -> float:
+        v = C.c_float()
+        self._check(N.lib().hm_bench_atomics(self.ctx, target_bytes, kind, count, C.byref(v)), "hm_bench_atomics")
+        return float(v.value)
+
     # -- learned shading --
     def mlp_init(self, seed: int = 0):
         self._check(N.lib().hm_mlp_init(self.ctx, seed), "hm_mlp_init")

@@ -318,6 +318,34 @@ hm_status hm_timing_read(hm_ctx* ctx, float* ms, uint32_t count);
 hm_status hm_time_frames(hm_ctx* ctx, hm_groom* groom, const hm_params params[3],
                          const hm_frame_opts* opts, uint32_t frames, float* total_ms);
 
+/* ---- tuning ------------------------------------------------------------- */
+
+#define HM_RASTER_PRECHECK 1u   /* screen each atomic min with a plain load */
+#define HM_RASTER_XCD 2u        /* one contiguous strand range per XCD (shared L2) */
+
+/* Launch choices of the raster passes; results do not depend on them. */
+typedef struct hm_tuning {
+    uint32_t raster_flags[3];      /* per pass mode (camera, DOM depth, DOM opacity) */
+    uint32_t raster_blocks_per_cu; /* persistent raster grid, 256-thread workgroups per CU */
+} hm_tuning;
+
+hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);
+hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* tuning);
+/* Times every raster variant (flags x grid size) on this groom and view for
+ * `reps` launches each, keeps the fastest per pass and returns it. Leaves the
+ * targets dirty: run a frame afterwards. */
+hm_status hm_autotune(hm_ctx* ctx, hm_groom* groom, const hm_params params[3], const hm_frame_opts* opts,
+                      uint32_t reps, hm_tuning* best);
+
+/* ---- roofline microbenchmarks -------------------------------------------- */
+
+/* STREAM-style copy of `bytes` between two fresh buffers, `reps` times;
+ * returns read + write GB/s. */
+hm_status hm_bench_copy(hm_ctx* ctx, size_t bytes, uint32_t reps, float* gbps);
+/* `count` random-address atomics over a `target_bytes` buffer: kind 0 u64
+ * atomicMin, 1 u32 atomicMin, 2 u32 atomicAdd; returns atomics per second. */
+hm_status hm_bench_atomics(hm_ctx* ctx, size_t target_bytes, uint32_t kind, uint64_t count, float* per_second);
+
 /* ---- learned shading (no counterpart in the reference) -------------------
  * A 29 -> 64 -> 64 -> 64 -> 12 MLP (bf16 weights on the matrix cores, f32
  * accumulation) stands in for dual_scatter (shade.wesl:153-190) and the key
