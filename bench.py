@@ -103,6 +103,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-fraction", type=int, default=2)
     ap.add_argument("--no-probes", action="store_true", help="skip the copy/atomic microbenchmarks")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default raster launch choices")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this path runs on a single MI355X (asked for --gpus {args.gpus})",
@@ -126,6 +127,10 @@ def main() -> int:
     opts = r.frame_opts(control_points=w.control_points, strand_fraction=w.strand_fraction)
     for _ in range(args.warmup):
         r.hair_pass(g, sc.params, opts)
+    r.sync()
+    # Launch choices of the raster passes picked on this groom and view (untimed).
+    tuning = r.autotune(g, sc.params, opts, reps=3) if not args.no_tune else r.get_tuning()
+    r.hair_pass(g, sc.params, opts)
     r.sync()
     # Work counts of each raster pass (untimed counting launches).
     r.clear()
@@ -175,6 +180,7 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "algorithmic_bytes": alg, "mean_ms": round(kt[dom_k], 5), "traffic": None},
         "kernels_ms": {k: round(v, 5) for k, v in kt.items()},
+        "tuning": tuning,
         "work": {"camera_writes": int(counts["cam"][0]), "camera_center_hits": int(counts["cam"][1]),
                  "camera_atomics": int(atomics_cam), "dom_depth_writes": int(counts["dom1"][2]),
                  "dom_opacity_increments": int(counts["dom2"][2]), "segments": int(counts["cam"][3]),

@@ -104,7 +104,7 @@ MLP_NW, MLP_NBIAS = 11264, 208
 
 
 class HmTuning(C.Structure):
-    _fields_ = [("raster_flags", u32 * 3), ("raster_blocks_per_cu", u32)]
+    _fields_ = [("raster_flags", u32 * 3), ("raster_blocks_per_cu", u32 * 3)]
 
 
 RASTER_PRECHECK, RASTER_XCD = 1, 2

@@ -1,0 +1,359 @@
+// Binned deep-opacity-map raster. The two DOM passes of raster.wgsl
+// (draw_dom_segment / write_dom_pixel, raster.wgsl:118-190, run with
+// pass_mode.x = 1 and then 2 by pass.rs:560-583) are evaluated per light-space
+// tile in LDS, and each tile's finished texels are written to HBM once.
+//
+//   k_fill32         zero the status words and per-tile counts
+//   k_bin_gen        one wave per strand: control points (gen_strand) to HBM,
+//                    the tile box of every segment, wave-aggregated counts
+//   k_bin_scan       exclusive scan of the counts; gate words
+//   k_bin_fill       segment ids into per-tile record lists
+//   k_fill32         DOM fronts to empty              (only with long segments)
+//   k_dom_long<1>    long segments, global atomic min  (same)
+//   k_dom_tiles<T>   per tile: depth pass (LDS min), front to HBM, opacity
+//                    pass (LDS add), counts to HBM: the whole DOM is written,
+//                    so no separate clear is needed
+//   k_dom_long<2>    long segments, global atomic add  (same)
+// When the records outgrow their buffer the gates switch every binned kernel
+// off and the direct kernels (clear + k_raster<1> + k_raster<2>) run instead.
+// DOM words are built from min and add only, so the result does not depend on
+// the order in which segments reach a texel and equals the direct path's bit
+// for bit.
+#include "hm_kernels.h"
+
+#include "hm_device.h"
+#include "hm_raster.h"
+
+namespace hm {
+
+namespace {
+
+__device__ __forceinline__ const View& bin_view(const BinArgs& a) { return *reinterpret_cast<const View*>(&a.view); }
+
+__device__ __forceinline__ F3 ld3(const float* q) { return {q[0], q[1], q[2]}; }
+__device__ __forceinline__ int64_t mx64(int64_t a, int64_t b) { return a > b ? a : b; }
+__device__ __forceinline__ int64_t mn64(int64_t a, int64_t b) { return a < b ? a : b; }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Texels the Bresenham walk of Drawer::dom_segment can write, clipped to the
+// map, as a tile box. The walk moves each coordinate by at most one per step
+// toward its end point and draws at most 1024 pixels; when both spans are
+// below 2^20 the error term cannot overflow and the walk stays inside the end
+// points' bounding box.
+__device__ __forceinline__ uint32_t segment_box(const View& p, F3 p0, F3 p1, uint32_t shift, uint32_t box_limit,
+                                                bool& is_long) {
+    is_long = false;
+    Projected A = project_world(p, p0);
+    Projected B = project_world(p, p1);
+    if (!A.in_front && !B.in_front) return BIN_NONE;
+    int64_t x0 = to_i32(floorf(A.screen.x)), y0 = to_i32(floorf(A.screen.y));
+    int64_t x1 = to_i32(floorf(B.screen.x)), y1 = to_i32(floorf(B.screen.y));
+    int64_t lx = x0 - 1023, hx = x0 + 1023, ly = y0 - 1023, hy = y0 + 1023;
+    int64_t spanx = x1 > x0 ? x1 - x0 : x0 - x1, spany = y1 > y0 ? y1 - y0 : y0 - y1;
+    if (spanx < (1 << 20) && spany < (1 << 20)) {
+        lx = mx64(lx, x0 < x1 ? x0 : x1);
+        hx = mn64(hx, x0 < x1 ? x1 : x0);
+        ly = mx64(ly, y0 < y1 ? y0 : y1);
+        hy = mn64(hy, y0 < y1 ? y1 : y0);
+    }
+    const int64_t w = to_i32(p.screen.x), h = to_i32(p.screen.y);
+    lx = mx64(lx, 0);
+    ly = mx64(ly, 0);
+    hx = mn64(hx, w - 1);
+    hy = mn64(hy, h - 1);
+    if (lx > hx || ly > hy) return BIN_NONE;
+    uint32_t tx0 = (uint32_t)lx >> shift, tx1 = (uint32_t)hx >> shift;
+    uint32_t ty0 = (uint32_t)ly >> shift, ty1 = (uint32_t)hy >> shift;
+    uint32_t bw = tx1 - tx0 + 1u, bh = ty1 - ty0 + 1u;
+    if (bw * bh > box_limit) {
+        is_long = true;
+        return BIN_NONE;
+    }
+    return tx0 | (ty0 << 10) | ((bw - 1u) << 20) | ((bh - 1u) << 26);
+}
+
+__device__ __forceinline__ uint32_t box_tiles(uint32_t box) { return (((box >> 20) & 63u) + 1u) * (((box >> 26) & 63u) + 1u); }
+
+__device__ __forceinline__ uint32_t box_tile(uint32_t box, uint32_t i, uint32_t tiles_x) {
+    uint32_t bw = ((box >> 20) & 63u) + 1u;
+    uint32_t tx = (box & 1023u) + i % bw, ty = ((box >> 10) & 1023u) + i / bw;
+    return ty * tiles_x + tx;
+}
+
+// Walks each lane's (up to two) boxes tile by tile. All lanes that stand on
+// the same tile are served by one atomic of the lowest such lane; `emit`
+// receives each lane's slot (base + rank) for its current tile.
+template <class Emit>
+__device__ __forceinline__ void aggregate_tiles(uint32_t box0, uint32_t box1, uint32_t tiles_x, uint32_t* ctr,
+                                                Emit emit) {
+    const int lane = (int)(threadIdx.x & 63u);
+    uint32_t k = box0 != BIN_NONE ? 0u : (box1 != BIN_NONE ? 1u : 2u);
+    uint32_t i = 0u;
+    while (true) {
+        uint32_t box = k == 0u ? box0 : box1;
+        uint32_t mine = k < 2u ? box_tile(box, i, tiles_x) : 0xffffffffu;
+        uint64_t active = __ballot(mine != 0xffffffffu);
+        if (active == 0u) break;
+        int leader = __ffsll((unsigned long long)active) - 1;
+        uint32_t tile = __shfl(mine, leader);
+        uint64_t same = __ballot(mine == tile);
+        uint32_t base = 0u;
+        if (lane == leader) base = atomicAdd(ctr + tile, (uint32_t)__popcll(same));
+        base = __shfl(base, leader);
+        if (mine == tile) {
+            emit(k, base + lanes_below(same));
+            if (++i == box_tiles(box)) {
+                i = 0u;
+                k = (k == 0u && box1 != BIN_NONE) ? 1u : 2u;
+            }
+        }
+    }
+}
+
+// Appends flagged lanes' segment ids to the long list with one atomic per wave.
+__device__ __forceinline__ void append_long(bool flag, uint32_t id, uint32_t* count, uint32_t* list, uint32_t cap) {
+    uint64_t m = __ballot(flag);
+    if (m == 0u) return;
+    const int lane = (int)(threadIdx.x & 63u);
+    int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    uint32_t pos = base + lanes_below(m);
+    if (flag && pos < cap) list[pos] = id;
+}
+
+struct StrandSchedule {
+    uint32_t first, end, stride;
+};
+
+__device__ __forceinline__ StrandSchedule strand_schedule(const BinArgs& a) {
+    uint32_t total = a.indirect[0];
+    total = total < a.capacity ? total : a.capacity;
+    uint32_t wpb = blockDim.x >> 6;
+    uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (threadIdx.x >> 6));
+    return {w, total, gridDim.x * wpb};
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
+    const View& p = bin_view(a);
+    const int lane = (int)(threadIdx.x & 63u);
+    uint32_t* tile_count = a.meta + BIN_META_WORDS;
+    StrandSchedule sch = strand_schedule(a);
+    for (uint32_t s = sch.first; s < sch.end; s += sch.stride) {
+        const hm_strand_ref ref = a.refs[s];
+        StrandCps sc = gen_strand(p, ref, a.bundles, a.layers, a.style, lane);
+        const uint32_t ncp = sc.ncp;
+        float* base = a.cps + (size_t)s * (BIN_STRIDE * 3u);
+        if ((uint32_t)lane < ncp) {
+            float* q = base + lane * 3;
+            q[0] = sc.cw[0].x;
+            q[1] = sc.cw[0].y;
+            q[2] = sc.cw[0].z;
+        }
+        if ((uint32_t)lane + 64u < ncp) {
+            float* q = base + (lane + 64) * 3;
+            q[0] = sc.cw[1].x;
+            q[1] = sc.cw[1].y;
+            q[2] = sc.cw[1].z;
+        }
+        F3 n0 = shfl3(sc.cw[0], (lane + 1) & 63), n1 = shfl3(sc.cw[1], (lane + 1) & 63);
+        F3 e0 = lane < 63 ? n0 : n1;
+        const uint32_t id0 = s * BIN_STRIDE + (uint32_t)lane, id1 = id0 + 64u;
+        bool long0 = false, long1 = false;
+        uint32_t box0 = BIN_NONE, box1 = BIN_NONE;
+        if ((uint32_t)lane + 1u < ncp) {
+            box0 = segment_box(p, sc.cw[0], e0, a.tile_shift, a.box_limit, long0);
+            a.seg_box[id0] = box0;
+        }
+        if ((uint32_t)lane + 65u < ncp) {
+            box1 = segment_box(p, sc.cw[1], n1, a.tile_shift, a.box_limit, long1);
+            a.seg_box[id1] = box1;
+        }
+        append_long(long0, id0, a.meta + BIN_LONG_COUNT, a.long_list, a.long_capacity);
+        append_long(long1, id1, a.meta + BIN_LONG_COUNT, a.long_list, a.long_capacity);
+        aggregate_tiles(box0, box1, a.tiles_x, tile_count, [](uint32_t, uint32_t) {});
+    }
+}
+
+// Exclusive scan of the per-tile counts (one block), then the gate words.
+__global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tiles, uint32_t record_capacity,
+                                                   uint32_t long_capacity) {
+    __shared__ uint32_t part[1024];
+    const uint32_t tid = threadIdx.x;
+    uint32_t* cnt = meta + BIN_META_WORDS;
+    uint32_t* off = cnt + tiles;
+    uint32_t* cur = off + tiles + 1u;
+    const uint32_t per = (tiles + 1023u) / 1024u;
+    const uint32_t b = tid * per < tiles ? tid * per : tiles;
+    const uint32_t e = b + per < tiles ? b + per : tiles;
+    uint32_t sum = 0u;
+    for (uint32_t i = b; i < e; ++i) sum += cnt[i];
+    part[tid] = sum;
+    __syncthreads();
+    for (uint32_t d = 1u; d < 1024u; d <<= 1) {  // inclusive Hillis-Steele scan
+        uint32_t v = tid >= d ? part[tid - d] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[tid] - sum;
+    for (uint32_t i = b; i < e; ++i) {
+        off[i] = run;
+        cur[i] = run;
+        run += cnt[i];
+    }
+    if (tid == 1023u) {
+        uint32_t total = part[1023];
+        off[tiles] = total;
+        uint32_t longs = meta[BIN_LONG_COUNT];
+        uint32_t ok = (total <= record_capacity && longs <= long_capacity) ? 1u : 0u;
+        meta[BIN_RECORDS] = total;
+        meta[BIN_OK] = ok;
+        meta[BIN_FALLBACK] = 1u - ok;
+        meta[BIN_LONG] = (ok && longs) ? 1u : 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_fill(BinArgs a) {
+    if (a.meta[BIN_OK] == 0u) return;
+    const int lane = (int)(threadIdx.x & 63u);
+    uint32_t* cursor = a.meta + BIN_META_WORDS + 2u * a.tiles + 1u;
+    StrandSchedule sch = strand_schedule(a);
+    for (uint32_t s = sch.first; s < sch.end; s += sch.stride) {
+        uint32_t ncp = a.refs[s].control_points;
+        ncp = ncp < 2u ? 2u : (ncp > 127u ? 127u : ncp);
+        const uint32_t id0 = s * BIN_STRIDE + (uint32_t)lane, id1 = id0 + 64u;
+        uint32_t box0 = (uint32_t)lane + 1u < ncp ? a.seg_box[id0] : BIN_NONE;
+        uint32_t box1 = (uint32_t)lane + 65u < ncp ? a.seg_box[id1] : BIN_NONE;
+        uint32_t* records = a.records;
+        aggregate_tiles(box0, box1, a.tiles_x, cursor,
+                        [&](uint32_t k, uint32_t slot) { records[slot] = k == 0u ? id0 : id1; });
+    }
+}
+
+// LDS tile of the map: fronts and 16 counts per texel.
+template <int T>
+struct DomTileSink {
+    uint32_t* front;
+    uint32_t* counts;
+    int32_t ox, oy;
+    __device__ __forceinline__ bool accepts(int32_t x, int32_t y) const {
+        return (uint32_t)(x - ox) < (uint32_t)T && (uint32_t)(y - oy) < (uint32_t)T;
+    }
+    // The walk is monotone in x and y: once past the tile in its direction of
+    // travel it cannot come back.
+    __device__ __forceinline__ bool walked_out(int32_t x, int32_t y, int32_t sx, int32_t sy) const {
+        return (sx > 0 ? x >= ox + T : x < ox) || (sy > 0 ? y >= oy + T : y < oy);
+    }
+    __device__ __forceinline__ uint32_t at(int32_t x, int32_t y) const { return (uint32_t)((y - oy) * T + (x - ox)); }
+    __device__ __forceinline__ void dom_min(size_t, int32_t x, int32_t y, uint32_t q, Counters&) {
+        atomicMin(front + at(x, y), q);
+    }
+    __device__ __forceinline__ uint32_t dom_front(size_t, int32_t x, int32_t y) const { return front[at(x, y)]; }
+    __device__ __forceinline__ void dom_add(size_t, int32_t x, int32_t y, uint32_t slice, Counters&) {
+        atomicAdd(counts + at(x, y) * HM_DOM_LAYERS + slice, 1u);
+    }
+};
+
+template <int T>
+__global__ __launch_bounds__(256) void k_dom_tiles(BinArgs a) {
+    if (a.meta[BIN_OK] == 0u) return;
+    __shared__ uint32_t front[T * T];
+    __shared__ __align__(16) uint32_t counts[T * T * HM_DOM_LAYERS];
+    const View& p = bin_view(a);
+    const uint32_t tile = blockIdx.x;
+    const int32_t ox = (int32_t)(tile % a.tiles_x) * T, oy = (int32_t)(tile / a.tiles_x) * T;
+    const int32_t S = (int32_t)a.dom_size;
+    const bool has_long = a.meta[BIN_LONG] != 0u;
+    uint32_t* dom_front = a.dom;
+    uint4* dom_counts = reinterpret_cast<uint4*>(a.dom + (size_t)S * (size_t)S);
+    for (int i = (int)threadIdx.x; i < T * T; i += 256) {
+        int32_t x = ox + i % T, y = oy + i / T;
+        front[i] = (has_long && x < S && y < S) ? dom_front[(size_t)y * S + x] : HM_DOM_EMPTY;
+    }
+    for (int i = (int)threadIdx.x; i < T * T * 4; i += 256) reinterpret_cast<uint4*>(counts)[i] = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+
+    const uint32_t* off = a.meta + BIN_META_WORDS + a.tiles;
+    const uint32_t begin = off[tile], end = off[tile + 1u];
+    DomTileSink<T> sink{front, counts, ox, oy};
+    {
+        Drawer<1, DomTileSink<T>> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
+        for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) {
+            uint32_t id = a.records[r];
+            const float* q = a.cps + (size_t)id * 3u;
+            d.dom_segment(ld3(q), ld3(q + 3), (id & (BIN_STRIDE - 1u)) == 0u);
+        }
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < T * T; i += 256) {
+        int32_t x = ox + i % T, y = oy + i / T;
+        if (x < S && y < S) dom_front[(size_t)y * S + x] = front[i];
+    }
+    {
+        Drawer<2, DomTileSink<T>> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
+        for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) {
+            uint32_t id = a.records[r];
+            const float* q = a.cps + (size_t)id * 3u;
+            d.dom_segment(ld3(q), ld3(q + 3), (id & (BIN_STRIDE - 1u)) == 0u);
+        }
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < T * T * 4; i += 256) {
+        int texel = i >> 2;
+        int32_t x = ox + texel % T, y = oy + texel / T;
+        if (x < S && y < S) dom_counts[((size_t)y * S + x) * 4u + (uint32_t)(i & 3)] = reinterpret_cast<uint4*>(counts)[i];
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_dom_long(BinArgs a) {
+    if (a.meta[BIN_LONG] == 0u) return;
+    const View& p = bin_view(a);
+    const uint32_t n = a.meta[BIN_LONG_COUNT];
+    const size_t S = a.dom_size;
+    GlobalSink sink{nullptr, nullptr, nullptr, a.dom, S * S, true};
+    Drawer<MODE, GlobalSink> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t id = a.long_list[i];
+        const float* q = a.cps + (size_t)id * 3u;
+        d.dom_segment(ld3(q), ld3(q + 3), (id & (BIN_STRIDE - 1u)) == 0u);
+    }
+}
+
+hipError_t launch_bin(hipStream_t s, const BinArgs& a, int blocks) {
+    hipError_t e = launch_fill32(s, a.meta, BIN_META_WORDS + a.tiles, 0u, 64);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bin_gen, dim3(blocks), dim3(256), 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, a.meta, a.tiles, a.record_capacity, a.long_capacity);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bin_fill, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks) {
+    const size_t texels = (size_t)a.dom_size * a.dom_size;
+    hipError_t e = launch_fill32(s, a.dom, texels, HM_DOM_EMPTY, long_blocks, a.meta + BIN_LONG);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dom_long<1>, dim3(long_blocks), dim3(256), 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (a.tile_shift == 4u) {
+        hipLaunchKernelGGL(k_dom_tiles<16>, dim3(a.tiles), dim3(256), 0, s, a);
+    } else if (a.tile_shift == 5u) {
+        hipLaunchKernelGGL(k_dom_tiles<32>, dim3(a.tiles), dim3(256), 0, s, a);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dom_long<2>, dim3(long_blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace hm

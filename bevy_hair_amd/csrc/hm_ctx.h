@@ -44,7 +44,7 @@ struct hm_ctx {
     hm_directional_light* dirs = nullptr;
     uint32_t dir_count = 0, dir_cap = 0;
     uint64_t* counters = nullptr;
-    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, 16u};
+    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}};
     // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
     uint32_t slots = 0, frame = 0;
     std::vector<hipEvent_t> ev;

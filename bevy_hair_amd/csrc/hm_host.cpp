@@ -280,9 +280,10 @@ void free_targets(hm_ctx* c) {
     c->width = c->height = c->dom_size = 0;
 }
 
-int raster_blocks(const hm_ctx* c, uint32_t capacity) {
+int raster_blocks(const hm_ctx* c, uint32_t capacity, uint32_t mode) {
     uint32_t want = (capacity + 3u) / 4u;
-    uint32_t cap = (uint32_t)c->cu_count * (c->tuning.raster_blocks_per_cu ? c->tuning.raster_blocks_per_cu : 16u);
+    uint32_t bpc = c->tuning.raster_blocks_per_cu[mode < 3u ? mode : 0u];
+    uint32_t cap = (uint32_t)c->cu_count * (bpc ? bpc : 16u);
     return (int)(want < cap ? (want ? want : 1u) : cap);
 }
 
@@ -699,6 +700,7 @@ static hm_status raster_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int p
     a.scene_clear = ctx->scene_clear;
     a.counters = ctx->counters;
     a.opt = ctx->tuning.raster_flags[p->pass_mode[0] < 3u ? p->pass_mode[0] : 0u];
+    a.gate = nullptr;
     return HM_OK;
 }
 
@@ -708,7 +710,7 @@ hm_status hm_raster(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass) {
     if (st != HM_OK) return st;
     int mode = (int)p->pass_mode[0];
     Timer t(ctx, mode == 0 ? HM_K_CAM_RASTER : (mode == 1 ? HM_K_DOM_DEPTH : HM_K_DOM_OPACITY));
-    HM_HIP(ctx, hm::launch_raster(ctx->stream, a, mode, false, raster_blocks(ctx, g->capacity)));
+    HM_HIP(ctx, hm::launch_raster(ctx->stream, a, mode, false, raster_blocks(ctx, g->capacity, p->pass_mode[0])));
     return HM_OK;
 }
 
@@ -718,7 +720,7 @@ hm_status hm_raster_count(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass
     hm_status st = raster_args(ctx, g, p, pass, a);
     if (st != HM_OK) return st;
     HM_HIP(ctx, hm::launch_fill32(ctx->stream, reinterpret_cast<uint32_t*>(ctx->counters), 8, 0u, 1));
-    HM_HIP(ctx, hm::launch_raster(ctx->stream, a, (int)p->pass_mode[0], true, raster_blocks(ctx, g->capacity)));
+    HM_HIP(ctx, hm::launch_raster(ctx->stream, a, (int)p->pass_mode[0], true, raster_blocks(ctx, g->capacity, p->pass_mode[0])));
     HM_HIP(ctx, hipMemcpyAsync(counts, ctx->counters, 32, hipMemcpyDeviceToHost, ctx->stream));
     HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return HM_OK;
@@ -904,10 +906,12 @@ hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out) {
 }
 
 hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
-    if (!ctx || !t || t->raster_blocks_per_cu == 0 || t->raster_blocks_per_cu > 64)
-        return fail(ctx, HM_ERR_INVALID, "raster_blocks_per_cu must be in [1, 64]");
-    for (int m = 0; m < 3; ++m)
+    if (!ctx || !t) return HM_ERR_INVALID;
+    for (int m = 0; m < 3; ++m) {
+        if (t->raster_blocks_per_cu[m] == 0 || t->raster_blocks_per_cu[m] > 64)
+            return fail(ctx, HM_ERR_INVALID, "raster_blocks_per_cu must be in [1, 64]");
         if (t->raster_flags[m] & ~(HM_RASTER_PRECHECK | HM_RASTER_XCD)) return fail(ctx, HM_ERR_INVALID, "unknown flag");
+    }
     ctx->tuning = *t;
     return HM_OK;
 }
@@ -921,42 +925,48 @@ hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const
     uint32_t saved_slots = ctx->slots;
     ctx->slots = 0;  // no per-kernel events while tuning
     const hm_lod_override* ovr = opts ? &opts->camera_lod : nullptr;
-    // Work lists of both passes, and the DOM front the opacity pass reads.
-    if ((st = hm_clear(ctx)) != HM_OK) return st;
+    // Work lists of both passes.
     if ((st = hm_lod(ctx, g, &params[0], HM_PASS_CAMERA, ovr)) != HM_OK) return st;
     if ((st = hm_scan(ctx, g, &params[0], HM_PASS_CAMERA)) != HM_OK) return st;
-    if ((st = hm_clear_dom(ctx)) != HM_OK) return st;
     if ((st = hm_lod(ctx, g, &params[1], HM_PASS_DOM, nullptr)) != HM_OK) return st;
     if ((st = hm_scan(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
-    if ((st = hm_raster(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
     float best_ms[3] = {1e30f, 1e30f, 1e30f};
-    uint32_t best_grid[3] = {16u, 16u, 16u};
+    hipEvent_t e0 = ctx->frame_ev[0], e1 = ctx->frame_ev[1];
     for (int mode = 0; mode < 3; ++mode) {
         const hm_params* p = &params[mode];
         int pass = mode == 0 ? HM_PASS_CAMERA : HM_PASS_DOM;
         for (uint32_t flags = 0; flags < 4u; ++flags) {
             for (uint32_t gi = 0; gi < 4; ++gi) {
+                ctx->tuning = out;
                 ctx->tuning.raster_flags[mode] = flags;
-                ctx->tuning.raster_blocks_per_cu = grids[gi];
-                // the depth pass re-runs on its own output; mins are idempotent
-                if ((st = hm_raster(ctx, g, p, pass)) != HM_OK) return st;  // warm
-                HM_HIP(ctx, hipEventRecord(ctx->frame_ev[0], ctx->stream));
-                for (uint32_t r = 0; r < reps; ++r)
+                ctx->tuning.raster_blocks_per_cu[mode] = grids[gi];
+                // Each repetition starts from the state a frame sees: cleared
+                // targets (and, for the opacity pass, a finished depth pass).
+                float total = 0.0f;
+                for (uint32_t r = 0; r <= reps; ++r) {
+                    if (mode == 0) {
+                        if ((st = hm_clear(ctx)) != HM_OK) return st;
+                    } else {
+                        if ((st = hm_clear_dom(ctx)) != HM_OK) return st;
+                        if (mode == 2 && (st = hm_raster(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
+                    }
+                    HM_HIP(ctx, hipEventRecord(e0, ctx->stream));
                     if ((st = hm_raster(ctx, g, p, pass)) != HM_OK) return st;
-                HM_HIP(ctx, hipEventRecord(ctx->frame_ev[1], ctx->stream));
-                HM_HIP(ctx, hipEventSynchronize(ctx->frame_ev[1]));
-                float ms = 0.0f;
-                HM_HIP(ctx, hipEventElapsedTime(&ms, ctx->frame_ev[0], ctx->frame_ev[1]));
-                if (ms < best_ms[mode]) {
-                    best_ms[mode] = ms;
+                    HM_HIP(ctx, hipEventRecord(e1, ctx->stream));
+                    HM_HIP(ctx, hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    HM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+                    if (r > 0) total += ms;  // repetition 0 warms up
+                }
+                if (total < best_ms[mode]) {
+                    best_ms[mode] = total;
                     out.raster_flags[mode] = flags;
-                    best_grid[mode] = grids[gi];
+                    out.raster_blocks_per_cu[mode] = grids[gi];
                 }
             }
         }
     }
-    // One grid size for all passes: the one the camera pass (the largest) prefers.
-    out.raster_blocks_per_cu = best_grid[0];
+
     ctx->tuning = out;
     ctx->slots = saved_slots;
     (void)keep;

@@ -29,6 +29,7 @@ struct RasterArgs {
     float scene_clear;
     uint64_t* counters;             // counting variant only
     uint32_t opt;                   // RASTER_* flags
+    const uint32_t* gate;           // when set and zero, the launch does nothing
 };
 
 struct ShadeArgs {
@@ -55,9 +56,45 @@ struct FilterArgs {
     float* filtered_depth;
 };
 
+// Binned DOM raster (hm_binned.hip). Segment ids are strand slot * BIN_STRIDE
+// + index of the segment's first control point.
+constexpr uint32_t BIN_STRIDE = 128u;
+constexpr uint32_t BIN_NONE = 0xffffffffu;  // no box (behind the view, off the map, or long)
+// Status words at the head of BinArgs::meta.
+constexpr uint32_t BIN_RECORDS = 0u;     // records the tiles asked for
+constexpr uint32_t BIN_OK = 1u;          // binned kernels run
+constexpr uint32_t BIN_FALLBACK = 2u;    // direct kernels run
+constexpr uint32_t BIN_LONG = 3u;        // binned and long segments exist
+constexpr uint32_t BIN_LONG_COUNT = 4u;
+constexpr uint32_t BIN_META_WORDS = 8u;  // then count[tiles], offset[tiles + 1], cursor[tiles]
+
+struct BinArgs {
+    hm_params view;                 // DOM view (the two DOM passes share it)
+    const hm_bundle* bundles;
+    const hm_layer* layers;
+    const float* style;
+    const hm_strand_ref* refs;
+    const uint32_t* indirect;
+    uint32_t capacity;              // refs length
+    float* cps;                     // world control points, 3 floats, [slot * BIN_STRIDE + idx]
+    uint32_t* seg_box;              // per segment id: tile box or BIN_NONE
+    uint32_t* meta;
+    uint32_t* records;              // segment ids grouped by tile
+    uint32_t record_capacity;
+    uint32_t* long_list;            // segments whose box spans more than box_limit tiles
+    uint32_t long_capacity;
+    uint32_t tiles_x, tiles, tile_shift, box_limit;
+    uint32_t dom_size;
+    uint32_t* dom;                  // fronts S*S, then counts S*S*16
+};
+
+hipError_t launch_bin(hipStream_t s, const BinArgs& a, int blocks);
+hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks);
+
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes, int blocks);
 hipError_t launch_atomics(hipStream_t s, void* buf, size_t words, uint32_t threads, uint32_t per_thread, uint32_t kind);
-hipError_t launch_fill32(hipStream_t s, uint32_t* dst, size_t words, uint32_t value, int max_blocks);
+hipError_t launch_fill32(hipStream_t s, uint32_t* dst, size_t words, uint32_t value, int max_blocks,
+                         const uint32_t* gate = nullptr);
 hipError_t launch_lod(hipStream_t s, const hm_params& prm, const hm_bundle* bundles, const hm_layer* layers,
                       hm_lod_record* out, uint32_t nb, uint32_t ovr_cps, float ovr_frac);
 hipError_t launch_scan(hipStream_t s, const hm_lod_record* lod, uint32_t nb, uint32_t capacity, uint32_t* prefix,

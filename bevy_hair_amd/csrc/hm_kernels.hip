@@ -12,7 +12,10 @@
 #include <cstring>
 
 #include "hm_device.h"
+#include "hm_raster.h"
 #include "hm_shade.h"
+
+#include <type_traits>
 
 namespace hm {
 
@@ -20,7 +23,9 @@ static_assert(sizeof(View) == sizeof(hm_params), "View must mirror hm_params byt
 
 // ---- fills ------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_fill32(uint32_t* __restrict__ dst, size_t words, uint32_t value) {
+__global__ __launch_bounds__(256) void k_fill32(uint32_t* __restrict__ dst, size_t words, uint32_t value,
+                                                const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
     size_t quads = words / 4;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     uint4 v = make_uint4(value, value, value, value);
@@ -171,303 +176,69 @@ __global__ __launch_bounds__(256) void k_expand(const hm_lod_record* __restrict_
 
 // ---- raster.wgsl ----------------------------------------------------------------
 
-// With `check`, a plain load screens the atomic: targets only decrease during
-// a pass, so a value read here bounds the current one from above and a payload
-// that is not below it cannot change the word.
-__device__ __forceinline__ void gmin64(uint64_t* a, uint64_t v, bool check) {
-    if (!check || v < *a) atomicMin((unsigned long long*)a, (unsigned long long)v);
-}
-__device__ __forceinline__ void gmin32(uint32_t* a, uint32_t v, bool check) {
-    if (!check || v < *a) atomicMin(a, v);
-}
-
-__device__ __forceinline__ F3 shfl3(F3 v, int src) { return {__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src)}; }
-__device__ __forceinline__ F4 shfl4(F4 v, int src) {
-    return {__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src), __shfl(v.w, src)};
-}
-
-struct Counters {
-    uint64_t cam_writes, cam_centers, dom_writes, segments;
-};
-
-template <int MODE, bool COUNT>
-struct Drawer {
-    const View& p;
-    const RasterArgs& a;
-    Counters c;
-
-    __device__ __forceinline__ float scene_at(int32_t x, int32_t y) const {
-        if (!a.scene_depth) return a.scene_clear;
-        return a.scene_depth[(size_t)(uint32_t)y * (uint32_t)to_i32(p.screen.x) + (uint32_t)x];
-    }
-
-    // One camera write of a given payload (raster.wgsl:93-115 after the
-    // payload is built); the payload depends only on t, not on the pixel.
-    __device__ __forceinline__ void put(int32_t x, int32_t y, int32_t width, float clip_depth, uint64_t payload,
-                                        bool center_hit, uint32_t beta_word) {
-        float scene = scene_at(x, y);
-        if (p.pass_mode[2] == 1u) {
-            if (clip_depth < scene) return;
-        } else if (clip_depth > scene) {
-            return;
-        }
-        size_t pix = (size_t)(uint32_t)y * (uint32_t)width + (uint32_t)x;
-        if (COUNT) {
-            c.cam_writes++;
-            if (center_hit) c.cam_centers++;
-            return;
-        }
-        const bool chk = (a.opt & RASTER_PRECHECK) != 0u;
-        gmin64(a.cons + pix, payload, chk);
-        if (center_hit) {
-            gmin64(a.center + pix, payload, chk);
-            gmin32(a.beta + pix, beta_word, chk);
-        }
-    }
-
-    __device__ void segment(F3 p0, F3 p1, F3 t0, F3 t1, F3 uvw0, F3 uvw1, float ao0, float ao1, float beta) {
-        // raster.wgsl:192-223
-        Projected A = project_world(p, p0);
-        Projected B = project_world(p, p1);
-        if (!A.in_front && !B.in_front) return;
-        if (COUNT) c.segments++;
-        F2 delta = B.screen - A.screen;
-        int32_t steps = to_i32(ceilf(mx(fabsf(delta.x), fabsf(delta.y))));
-        int32_t n = steps < 1 ? 1 : (steps > 96 ? 96 : steps);
-        float diameter = mx(p.appearance.w, 0.05f);
-        const int32_t width = to_i32(p.screen.x), height = to_i32(p.screen.y);
-        F2 seg = B.screen - A.screen;
-        float denom = mx(dot2(seg, seg), 1e-6f);
-        uint32_t beta_q = to_u32(clampf(beta, 0.0f, 1.0f) * 255.0f + 0.5f);
-        int32_t ppx = 0, ppy = 0;
-        for (int32_t i = 0; i <= n; ++i) {
-            float t = rdiv(i2f(i), i2f(n));
-            F2 screen = mix2(A.screen, B.screen, t);
-            int32_t px = to_i32(floorf(screen.x)), py = to_i32(floorf(screen.y));
-            bool diag = i > 0 && ppx != px && ppy != py;
-            bool in0 = px >= 0 && py >= 0 && px < width && py < height;
-            bool in1 = diag && ppx >= 0 && py >= 0 && ppx < width && py < height;
-            bool in2 = diag && px >= 0 && ppy >= 0 && px < width && ppy < height;
-            if (in0 || in1 || in2) {
-                F3 world = mix3(p0, p1, t);
-                float vd = dot3(world - xyz(p.camera_pos), xyz(p.camera_forward));
-                if (vd > p.screen.z) {
-                    float cz = mat_row(p.clip, 2, world.x, world.y, world.z, 1.0f);
-                    float cw = mat_row(p.clip, 3, world.x, world.y, world.z, 1.0f);
-                    float clip_depth = rdiv(cz, mx(cw, 1e-5f));
-                    F3 tangent = safe_normalize(mix3(t0, t1, t));
-                    F3 uvw = mix3(uvw0, uvw1, t);
-                    float ao = mixs(ao0, ao1, t);
-                    uint32_t depth = quantize_depth(vd, p.screen.z, p.screen.w);
-                    uint32_t ox, oy;
-                    oct_encode(tangent, ox, oy);
-                    uint32_t ao_q = to_u32(clampf(ao, 0.0f, 1.0f) * 63.0f + 0.5f);
-                    if (p.flags[2] == 0u) ao_q = 63u;
-                    uint64_t payload = pack_gb(depth, ox, oy, quant63(uvw.x), quant63(uvw.y), quant63(uvw.z), ao_q);
-                    uint32_t beta_word = (depth << 8u) | beta_q;
-                    if (in0) {
-                        F2 center = {i2f(px) + 0.5f, i2f(py) + 0.5f};
-                        float closest = clampf(rdiv(dot2(center - A.screen, seg), denom), 0.0f, 1.0f);
-                        float dist = len2(center - (A.screen + seg * closest));
-                        bool hit = dist <= diameter * 0.5f;
-                        put(px, py, width, clip_depth, payload, hit, beta_word);
-                    }
-                    if (in1) put(ppx, py, width, clip_depth, payload, false, beta_word);
-                    if (in2) put(px, ppy, width, clip_depth, payload, false, beta_word);
-                }
-            }
-            ppx = px;
-            ppy = py;
-        }
-    }
-
-    __device__ __forceinline__ void dom_pixel(int32_t x, int32_t y, F3 world) {  // raster.wgsl:118-148
-        int32_t width = to_i32(p.screen.x), height = to_i32(p.screen.y);
-        if (x < 0 || y < 0 || x >= width || y >= height) return;
-        float near_ = p.screen.z, far_ = p.screen.w;
-        float dist = dot3(world - xyz(p.camera_pos), xyz(p.camera_forward));
-        if (dist <= near_ || dist >= far_) return;
-        uint32_t dom_w = to_u32(p.dom_info.x);
-        size_t pix = (size_t)(uint32_t)y * dom_w + (uint32_t)x;
-        uint32_t q = quantize_depth(dist, near_, far_);
-        if (MODE == 1) {
-            if (COUNT) c.dom_writes++;
-            else gmin32(a.dom + pix, q, (a.opt & RASTER_PRECHECK) != 0u);
-            return;
-        }
-        uint32_t front_q = a.dom[pix];  // final: written by the previous (depth) pass
-        if (front_q == HM_DOM_EMPTY) return;
-        float z_front = dequantize_depth(front_q, near_, far_);
-        float thickness = rdiv(mx(far_ - near_, 1e-3f), 16.0f);
-        float t = rdiv(mx(dist - z_front, 0.0f), thickness);
-        uint32_t slice = to_u32(t);
-        slice = slice < 15u ? slice : 15u;
-        size_t pixels = (size_t)dom_w * to_u32(p.dom_info.y);
-        if (COUNT) c.dom_writes++;
-        else atomicAdd(a.dom + pixels + pix * HM_DOM_LAYERS + slice, 1u);
-    }
-
-    __device__ void dom_segment(F3 p0, F3 p1, bool include_start) {  // raster.wgsl:150-190
-        Projected A = project_world(p, p0);
-        Projected B = project_world(p, p1);
-        if (!A.in_front && !B.in_front) return;
-        if (COUNT) c.segments++;
-        int32_t x0 = to_i32(floorf(A.screen.x)), y0 = to_i32(floorf(A.screen.y));
-        int32_t x1 = to_i32(floorf(B.screen.x)), y1 = to_i32(floorf(B.screen.y));
-        int32_t dx = wabs(wsub(x1, x0)), dy = wabs(wsub(y1, y0));
-        int32_t sx = x0 < x1 ? 1 : -1, sy = y0 < y1 ? 1 : -1;
-        int32_t err = wsub(dx, dy);
-        F2 seg = B.screen - A.screen;
-        float seg_len2 = mx(dot2(seg, seg), 1e-6f);
-        bool skip = !include_start && (x0 != x1 || y0 != y1);
-        int32_t ndy = wsub(0, dy);
-        for (int step = 0; step < 1024; ++step) {
-            if (!skip) {
-                F2 center = {i2f(x0) + 0.5f, i2f(y0) + 0.5f};
-                float t = clampf(rdiv(dot2(center - A.screen, seg), seg_len2), 0.0f, 1.0f);
-                dom_pixel(x0, y0, mix3(p0, p1, t));
-            }
-            skip = false;
-            if (x0 == x1 && y0 == y1) break;
-            int32_t e2 = wadd(err, err);
-            if (e2 > ndy) {
-                err = wsub(err, dy);
-                x0 = wadd(x0, sx);
-            }
-            if (e2 < dx) {
-                err = wadd(err, dx);
-                y0 = wadd(y0, sy);
-            }
-        }
-    }
-};
-
 template <int MODE, bool COUNT>
 __global__ __launch_bounds__(256) void k_raster(RasterArgs a) {
     const View& p = *reinterpret_cast<const View*>(&a.view);
     const int lane = (int)(threadIdx.x & 63u);
     const uint32_t waves_per_block = blockDim.x >> 6;
+    if (a.gate && *a.gate == 0u) return;
     uint32_t total = a.indirect[0];
     total = total < a.capacity ? total : a.capacity;
     // Strand schedule. Default: wave w takes strands w, w + W, ... With
     // RASTER_XCD, workgroup b runs on XCD b % 8 and each XCD takes one
     // contiguous eighth of the bundle-major strand list, so strands that hit
     // the same screen region share that XCD's L2.
-    uint32_t first = 0, end = total;
+    uint32_t end = total;
     uint32_t wave0 = blockIdx.x * waves_per_block + (threadIdx.x >> 6);
     uint32_t nwaves = gridDim.x * waves_per_block;
     if ((a.opt & RASTER_XCD) && gridDim.x >= 8u) {
         uint32_t xcd = blockIdx.x & 7u;
         uint32_t blocks_here = (gridDim.x - xcd + 7u) / 8u;
-        first = (uint32_t)((uint64_t)total * xcd / 8u);
+        uint32_t first = (uint32_t)((uint64_t)total * xcd / 8u);
         end = (uint32_t)((uint64_t)total * (xcd + 1u) / 8u);
         wave0 = first + (blockIdx.x >> 3) * waves_per_block + (threadIdx.x >> 6);
         nwaves = blocks_here * waves_per_block;
     }
     wave0 = __builtin_amdgcn_readfirstlane(wave0);
-    Drawer<MODE, COUNT> dr{p, a, {0, 0, 0, 0}};
+    typedef typename std::conditional<COUNT, CountSink, GlobalSink>::type Sink;
+    Sink sink;
+    if constexpr (COUNT) {
+        sink.dom = a.dom;
+    } else {
+        sink.center = a.center;
+        sink.cons = a.cons;
+        sink.beta = a.beta;
+        sink.dom = a.dom;
+        sink.dom_pixels = (size_t)to_u32(p.dom_info.x) * to_u32(p.dom_info.y);
+        sink.check = (a.opt & RASTER_PRECHECK) != 0u;
+    }
+    Drawer<MODE, Sink> dr{p, a.scene_depth, a.scene_clear, sink, {0, 0, 0, 0}};
 
     for (uint32_t s = wave0; s < end; s += nwaves) {
         const hm_strand_ref ref = a.refs[s];
-        const hm_bundle bundle = a.bundles[ref.bundle];
-        const F2 uv = pcg_uv(ref.strand ^ (ref.bundle * 0x85ebca6bu));
-
-        // Lanes 0..15 sample layer `lane` of the cage (raster.wgsl:22-32;
-        // only pos_ao is kept: the layer tangent is recomputed below).
-        // Lanes 16..31 sample level `lane-16` of the style volume at this uv
-        // (the u/v part of raster.wgsl:34-60; the w blend happens per point).
-        F4 lay = {0, 0, 0, 0};
-        F3 sty = {0, 0, 0};
-        if (lane < 16) {
-            const hm_layer* L = a.layers + bundle.layer_offset + (uint32_t)lane * 4u;
-            lay = bilinear4(ld4(L[0].pos_ao), ld4(L[1].pos_ao), ld4(L[2].pos_ao), ld4(L[3].pos_ao), uv);
-        } else if (lane < 32) {
-            uint32_t k = (uint32_t)lane - 16u;
-            float u = clampf(uv.x, 0.0f, 1.0f) * 7.0f;
-            float v = clampf(uv.y, 0.0f, 1.0f) * 7.0f;
-            uint32_t u0 = to_u32(floorf(u)), v0 = to_u32(floorf(v));
-            uint32_t u1 = u0 + 1u < 7u ? u0 + 1u : 7u;
-            uint32_t v1 = v0 + 1u < 7u ? v0 + 1u : 7u;
-            float tu = u - u2f(u0), tv = v - u2f(v0);
-            const float* S = a.style;
-            uint32_t off = bundle.style_offset + k * 64u;
-            F3 c000 = xyz(ld4(S + 4u * (off + v0 * 8u + u0)));
-            F3 c100 = xyz(ld4(S + 4u * (off + v0 * 8u + u1)));
-            F3 c010 = xyz(ld4(S + 4u * (off + v1 * 8u + u0)));
-            F3 c110 = xyz(ld4(S + 4u * (off + v1 * 8u + u1)));
-            sty = mix3(mix3(c000, c100, tu), mix3(c010, c110, tu), tv);
-        }
-        sty = shfl3(sty, (lane + 16) & 63);  // style level k now sits in lane k
-
-        // Layer tangents (raster.wgsl:244-267).
-        F3 lp = xyz(lay);
-        F3 prev_l = shfl3(lp, lane > 0 ? lane - 1 : 0);
-        F3 next_l = shfl3(lp, lane < 15 ? lane + 1 : 15);
-        F3 ltan = calc_tangent((uint32_t)lane, HM_LAYER_COUNT, prev_l, lp, next_l);
-
-        // Control points idx = lane and lane + 64 (raster.wgsl:270-284).
-        const uint32_t ncp = ref.control_points < 2u ? 2u : (ref.control_points > 127u ? 127u : ref.control_points);
-        const float inv_den = u2f(ncp - 1u > 1u ? ncp - 1u : 1u);
-        F3 cp[2], cw[2];
-        float cp_w[2], cp_ao[2];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            uint32_t idx = (uint32_t)lane + 64u * (uint32_t)j;
-            float w = rdiv(u2f(idx), inv_den);
-            float layers_f = w * 15.0f;
-            uint32_t i0 = to_u32(floorf(layers_f));
-            i0 = i0 < 15u ? i0 : 15u;
-            uint32_t i1 = i0 + 1u < 15u ? i0 + 1u : 15u;
-            float t = layers_f - u2f(i0);
-            F4 a0 = shfl4(lay, (int)i0), a1 = shfl4(lay, (int)i1);
-            F3 m0 = shfl3(ltan, (int)i0), m1 = shfl3(ltan, (int)i1);
-            F3 base = hermite(xyz(a0), xyz(a1), m0, m1, t);
-            float ao = mixs(a0.w, a1.w, t);
-            // Style: w in [0, 1], so its level index and blend equal i0/i1/t.
-            F3 s0 = shfl3(sty, (int)i0), s1 = shfl3(sty, (int)i1);
-            F3 residual = mix3(s0, s1, t);
-            cp[j] = base + residual;
-            cp_w[j] = w;
-            cp_ao[j] = ao;
-        }
-
-        // Control-point tangents (raster.wgsl:286-317).
-        F3 up0 = shfl3(cp[0], (lane + 63) & 63), up1 = shfl3(cp[1], (lane + 63) & 63);
-        F3 dn0 = shfl3(cp[0], (lane + 1) & 63), dn1 = shfl3(cp[1], (lane + 1) & 63);
-        F3 ctan[2];
-        {
-            uint32_t idx = (uint32_t)lane;
-            F3 prev = lane > 0 ? up0 : cp[0];
-            F3 next = idx + 1u < ncp ? (lane < 63 ? dn0 : dn1) : cp[0];
-            ctan[0] = safe_normalize(calc_tangent(idx, ncp, prev, cp[0], next));
-            idx += 64u;
-            prev = lane > 0 ? up1 : up0;
-            next = idx + 1u < ncp ? dn1 : cp[1];
-            ctan[1] = safe_normalize(calc_tangent(idx, ncp, prev, cp[1], next));
-        }
-
-        const float beta = rdiv(u2f(ref.n_lod), u2f(bundle.strand_count > 1u ? bundle.strand_count : 1u));
-#pragma unroll
-        for (int j = 0; j < 2; ++j) cw[j] = to_world(p, cp[j]);
-        // Segment idx -> idx+1: the end point lives in the next lane (slot 0 of
-        // lane 63 continues in slot 1 of lane 0).
-        F3 n0 = shfl3(cw[0], (lane + 1) & 63), n1 = shfl3(cw[1], (lane + 1) & 63);
+        StrandCps sc = gen_strand(p, ref, a.bundles, a.layers, a.style, lane);
+        const uint32_t ncp = sc.ncp;
+        // Segment idx -> idx + 1: the end point lives in the next lane (slot 0
+        // of lane 63 continues in slot 1 of lane 0).
+        F3 n0 = shfl3(sc.cw[0], (lane + 1) & 63), n1 = shfl3(sc.cw[1], (lane + 1) & 63);
         F3 e0 = lane < 63 ? n0 : n1;
         if (MODE != 0) {
-            if ((uint32_t)lane + 1u < ncp) dr.dom_segment(cw[0], e0, lane == 0);
-            if ((uint32_t)lane + 65u < ncp) dr.dom_segment(cw[1], n1, false);
+            if ((uint32_t)lane + 1u < ncp) dr.dom_segment(sc.cw[0], e0, lane == 0);
+            if ((uint32_t)lane + 65u < ncp) dr.dom_segment(sc.cw[1], n1, false);
         } else {
-            F3 wt0 = dir_world(p, ctan[0]), wt1 = dir_world(p, ctan[1]);
+            F3 wt0 = dir_world(p, sc.ct[0]), wt1 = dir_world(p, sc.ct[1]);
             F3 nt0 = shfl3(wt0, (lane + 1) & 63), nt1 = shfl3(wt1, (lane + 1) & 63);
-            float nw0 = __shfl(cp_w[0], (lane + 1) & 63), nw1 = __shfl(cp_w[1], (lane + 1) & 63);
-            float na0 = __shfl(cp_ao[0], (lane + 1) & 63), na1 = __shfl(cp_ao[1], (lane + 1) & 63);
+            float nw0 = __shfl(sc.w[0], (lane + 1) & 63), nw1 = __shfl(sc.w[1], (lane + 1) & 63);
+            float na0 = __shfl(sc.ao[0], (lane + 1) & 63), na1 = __shfl(sc.ao[1], (lane + 1) & 63);
+            const F2 uv = sc.uv;
             if ((uint32_t)lane + 1u < ncp) {
                 bool wrap = lane == 63;
-                dr.segment(cw[0], e0, wt0, wrap ? nt1 : nt0, F3{uv.x, uv.y, cp_w[0]},
-                           F3{uv.x, uv.y, wrap ? nw1 : nw0}, cp_ao[0], wrap ? na1 : na0, beta);
+                dr.segment(sc.cw[0], e0, wt0, wrap ? nt1 : nt0, F3{uv.x, uv.y, sc.w[0]},
+                           F3{uv.x, uv.y, wrap ? nw1 : nw0}, sc.ao[0], wrap ? na1 : na0, sc.beta);
             }
             if ((uint32_t)lane + 65u < ncp)
-                dr.segment(cw[1], n1, wt1, nt1, F3{uv.x, uv.y, cp_w[1]}, F3{uv.x, uv.y, nw1}, cp_ao[1], na1, beta);
+                dr.segment(sc.cw[1], n1, wt1, nt1, F3{uv.x, uv.y, sc.w[1]}, F3{uv.x, uv.y, nw1}, sc.ao[1], na1,
+                           sc.beta);
         }
     }
     if (COUNT) {
@@ -623,13 +394,14 @@ __global__ __launch_bounds__(256) void k_filter(FilterArgs a, const uint32_t* __
 
 namespace hm {
 
-hipError_t launch_fill32(hipStream_t s, uint32_t* dst, size_t words, uint32_t value, int max_blocks) {
+hipError_t launch_fill32(hipStream_t s, uint32_t* dst, size_t words, uint32_t value, int max_blocks,
+                         const uint32_t* gate) {
     if (words == 0) return hipSuccess;
     size_t quads = (words + 3) / 4;
     size_t blocks = (quads + 255) / 256;
     if (blocks > (size_t)max_blocks) blocks = (size_t)max_blocks;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_fill32, dim3((unsigned)blocks), dim3(256), 0, s, dst, words, value);
+    hipLaunchKernelGGL(k_fill32, dim3((unsigned)blocks), dim3(256), 0, s, dst, words, value, gate);
     return hipGetLastError();
 }
 
@@ -679,7 +451,7 @@ hipError_t launch_raster(hipStream_t s, const RasterArgs& a, int mode, bool coun
 hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const uint64_t* cons, uint32_t pixels,
                           uint32_t* list, uint32_t* count, float4* color_out, float* depth_out) {
     if (pixels == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u);
+    hipLaunchKernelGGL(k_fill32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u, (const uint32_t*)nullptr);
     dim3 g((pixels + 255) / 256), b(256);
     if (mode == 0) hipLaunchKernelGGL(k_compact<0>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out);
     else hipLaunchKernelGGL(k_compact<1>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out);

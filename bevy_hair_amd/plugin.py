@@ -261,16 +261,18 @@ class Renderer:
     def get_tuning(self) -> dict:
         t = N.HmTuning()
         self._check(N.lib().hm_get_tuning(self.ctx, C.byref(t)), "hm_get_tuning")
-        return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": int(t.raster_blocks_per_cu)}
+        return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": list(t.raster_blocks_per_cu)}
 
-    def set_tuning(self, raster_flags, raster_blocks_per_cu: int):
-        t = N.HmTuning((C.c_uint32 * 3)(*raster_flags), raster_blocks_per_cu)
+    def set_tuning(self, raster_flags, raster_blocks_per_cu):
+        if isinstance(raster_blocks_per_cu, int):
+            raster_blocks_per_cu = [raster_blocks_per_cu] * 3
+        t = N.HmTuning((C.c_uint32 * 3)(*raster_flags), (C.c_uint32 * 3)(*raster_blocks_per_cu))
         self._check(N.lib().hm_set_tuning(self.ctx, C.byref(t)), "hm_set_tuning")
 
     def autotune(self, g: GpuGroom, params, opts, reps: int = 3) -> dict:
         t = N.HmTuning()
         self._check(N.lib().hm_autotune(self.ctx, g.handle, params, C.byref(opts), reps, C.byref(t)), "hm_autotune")
-        return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": int(t.raster_blocks_per_cu)}
+        return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": list(t.raster_blocks_per_cu)}
 
     # -- roofline microbenchmarks --
     def bench_copy(self, nbytes: int, reps: int = 5) -> float:

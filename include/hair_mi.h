@@ -326,14 +326,14 @@ hm_status hm_time_frames(hm_ctx* ctx, hm_groom* groom, const hm_params params[3]
 /* Launch choices of the raster passes; results do not depend on them. */
 typedef struct hm_tuning {
     uint32_t raster_flags[3];      /* per pass mode (camera, DOM depth, DOM opacity) */
-    uint32_t raster_blocks_per_cu; /* persistent raster grid, 256-thread workgroups per CU */
+    uint32_t raster_blocks_per_cu[3]; /* persistent raster grid per pass, 256-thread workgroups per CU */
 } hm_tuning;
 
 hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);
 hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* tuning);
-/* Times every raster variant (flags x grid size) on this groom and view for
- * `reps` launches each, keeps the fastest per pass and returns it. Leaves the
- * targets dirty: run a frame afterwards. */
+/* Times every raster variant (flags x grid size) on this groom and view,
+ * `reps` launches each from cleared targets, keeps the fastest per pass and
+ * returns it. Leaves the targets dirty: run a frame afterwards. */
 hm_status hm_autotune(hm_ctx* ctx, hm_groom* groom, const hm_params params[3], const hm_frame_opts* opts,
                       uint32_t reps, hm_tuning* best);
 

@@ -156,3 +156,29 @@ def test_frame_matches_oracle_and_is_deterministic(renderer):
     assert np.array_equal(first[2], ref.beta)
     assert np.array_equal(first[3], ref.dom)
     assert rel_err(first[4], ref.filtered).max() <= RGB_TOL
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_raster_variants_give_identical_bytes(renderer, flags):
+    """Every launch choice the autotuner may pick writes the same G-buffer and DOM."""
+    sc = make_scene("small_lod")
+    w, r = sc.w, renderer
+    print(f"\n[gpu parity] raster flags={flags}", flush=True)
+    r.targets(w.width, w.height, w.dom_size)
+    r.set_scene_depth(None, 0.0)
+    r.set_lights(sc.points, sc.dirs)
+    g = r.upload(sc.mesh)
+    keep = r.get_tuning()
+    try:
+        for bpc in (4, 32):
+            r.set_tuning([flags] * 3, bpc)
+            r.hair_pass(g, sc.params, r.frame_opts(control_points=w.control_points))
+            got = (*r.gbuffer(), r.dom())
+            ref = ref_frame(sc)
+            pa, n_p, da, n_d = sc.light_arrays()
+            ref.hair_pass(sc.params, pa, n_p, da, n_d, control_points=w.control_points)
+            for a, b in zip(got, (ref.center, ref.cons, ref.beta, ref.dom)):
+                assert np.array_equal(a, b)
+    finally:
+        r.set_tuning(keep["raster_flags"], keep["raster_blocks_per_cu"])
+        g.free()
