@@ -46,6 +46,14 @@ def algorithmic_bytes(kernel: str, w, mesh, counts: dict, hit_pixels: int) -> fl
         return 4.0 * counts["dom1"][2] + groom
     if kernel == "dom_opacity":
         return 8.0 * counts["dom2"][2] + groom
+    if kernel == "dom_bin":
+        # groom in, control points out (12 B), tile box out and back (4 + 4 B
+        # per segment), one 4 B record per (segment, tile)
+        return groom + 12.0 * counts["dom_cps"] + 8.0 * counts["dom_segments"] + 4.0 * counts["dom_records"]
+    if kernel == "dom_tiles":
+        # the map written once (front + 16 counts per texel); per record its
+        # id and two control points
+        return 68.0 * S * S + 28.0 * counts["dom_records"]
     if kernel == "shade":
         return 8.0 * P + 20.0 * P + hit_pixels * (4.0 + 272.0)
     if kernel == "filter":
@@ -142,6 +150,13 @@ def main() -> int:
     r.scan(g, sc.params[1], N.PASS_DOM)
     counts["dom1"] = r.raster_count(g, sc.params[1], N.PASS_DOM)
     counts["dom2"] = r.raster_count(g, sc.params[2], N.PASS_DOM)
+    dom_refs, _ = r.refs(g, N.PASS_DOM)
+    ncp = np.clip(dom_refs[:, 3].astype(np.int64), 2, 127)
+    counts["dom_cps"] = int(ncp.sum())
+    counts["dom_segments"] = int((ncp - 1).sum())
+    r.dom(g, sc.params[1], sc.params[2])
+    bin_status = r.dom_bin_status() if tuning.get("dom_tile") else None
+    counts["dom_records"] = bin_status["records"] if bin_status else 0
     center, _, _ = r.gbuffer()
     hit = int((center != np.uint64(0xFFFFFFFFFFFFFFFF)).sum())
     r.sync()
@@ -184,11 +199,11 @@ def main() -> int:
         "work": {"camera_writes": int(counts["cam"][0]), "camera_center_hits": int(counts["cam"][1]),
                  "camera_atomics": int(atomics_cam), "dom_depth_writes": int(counts["dom1"][2]),
                  "dom_opacity_increments": int(counts["dom2"][2]), "segments": int(counts["cam"][3]),
-                 "hit_pixels": hit},
+                 "hit_pixels": hit, "dom_bin": bin_status},
         "atomic_rates": {
             "cam_raster_atomics_per_s": round(atomics_cam / (kt["cam_raster"] * 1e-3), 1) if kt["cam_raster"] else None,
             "dom_opacity_atomics_per_s": round(float(counts["dom2"][2]) / (kt["dom_opacity"] * 1e-3), 1)
-            if kt["dom_opacity"] else None},
+            if kt["dom_opacity"] and not bin_status else None},
     }
     if not args.no_probes:
         # Measured denominators: STREAM copy and random-address atomics over the

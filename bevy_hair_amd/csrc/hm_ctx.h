@@ -20,6 +20,18 @@ struct hm_groom {
     uint32_t* indirect[2] = {nullptr, nullptr};
 };
 
+// Work buffers of the binned DOM (sized for the largest groom and map seen).
+struct hm_bin_state {
+    float* cps = nullptr;           // world control points, [slot * 128 + idx] x 3 floats
+    uint32_t* seg_box = nullptr;    // per segment id
+    uint32_t* long_list = nullptr;
+    size_t slots = 0;               // strand slots cps / seg_box / long_list hold
+    uint32_t* meta = nullptr;       // status words, per-tile count / offset / cursor
+    size_t meta_words = 0;
+    uint32_t* records = nullptr;
+    size_t record_capacity = 0;
+};
+
 struct hm_ctx {
     int device = 0;
     int cu_count = 256;
@@ -44,7 +56,8 @@ struct hm_ctx {
     hm_directional_light* dirs = nullptr;
     uint32_t dir_count = 0, dir_cap = 0;
     uint64_t* counters = nullptr;
-    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}};
+    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 32u, 8u, 0u};
+    hm_bin_state bin;
     // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
     uint32_t slots = 0, frame = 0;
     std::vector<hipEvent_t> ev;

@@ -10,6 +10,7 @@
 //   world_extent       src/render/pass.rs:721-745
 //   light_fit          src/render/pass.rs:747-776
 //   depth_is_reverse   src/render/pass.rs:778-787
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -280,6 +281,49 @@ void free_targets(hm_ctx* c) {
     c->width = c->height = c->dom_size = 0;
 }
 
+void free_bin(hm_ctx* c) {
+    hm_bin_state& b = c->bin;
+    dev_free(b.cps);
+    dev_free(b.seg_box);
+    dev_free(b.long_list);
+    dev_free(b.meta);
+    dev_free(b.records);
+    b = hm_bin_state();
+}
+
+// Grows the binned-DOM buffers to hold `slots` strand slots, `meta_words`
+// status + per-tile words and `records` tile records.
+hipError_t ensure_bin(hm_ctx* c, size_t slots, size_t meta_words, size_t records) {
+    hm_bin_state& b = c->bin;
+    if (slots <= b.slots && meta_words <= b.meta_words && records <= b.record_capacity) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(c->stream);  // earlier frames may still read them
+    if (e != hipSuccess) return e;
+    if (slots > b.slots) {
+        dev_free(b.cps);
+        dev_free(b.seg_box);
+        dev_free(b.long_list);
+        b.slots = 0;
+        size_t segs = slots * hm::BIN_STRIDE;
+        if ((e = dev_alloc(&b.cps, segs * 3 * sizeof(float))) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.seg_box, segs * 4)) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.long_list, segs * 4)) != hipSuccess) return e;
+        b.slots = slots;
+    }
+    if (meta_words > b.meta_words) {
+        dev_free(b.meta);
+        b.meta_words = 0;
+        if ((e = dev_alloc(&b.meta, meta_words * 4)) != hipSuccess) return e;
+        b.meta_words = meta_words;
+    }
+    if (records > b.record_capacity) {
+        dev_free(b.records);
+        b.record_capacity = 0;
+        if ((e = dev_alloc(&b.records, records * 4)) != hipSuccess) return e;
+        b.record_capacity = records;
+    }
+    return hipSuccess;
+}
+
 int raster_blocks(const hm_ctx* c, uint32_t capacity, uint32_t mode) {
     uint32_t want = (capacity + 3u) / 4u;
     uint32_t bpc = c->tuning.raster_blocks_per_cu[mode < 3u ? mode : 0u];
@@ -374,6 +418,7 @@ void hm_destroy(hm_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_targets(c);
+    free_bin(c);
     hm_mlp_release(c);
     dev_free(c->points);
     dev_free(c->dirs);
@@ -726,6 +771,82 @@ hm_status hm_raster_count(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass
     return HM_OK;
 }
 
+hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params* op) {
+    hm::RasterArgs a1, a2;
+    hm_status st = raster_args(ctx, g, dp, HM_PASS_DOM, a1);
+    if (st != HM_OK) return st;
+    if ((st = raster_args(ctx, g, op, HM_PASS_DOM, a2)) != HM_OK) return st;
+    if (dp->pass_mode[0] != 1u || op->pass_mode[0] != 2u)
+        return fail(ctx, HM_ERR_INVALID, "hm_dom takes the depth (pass_mode.x 1) then the opacity (2) params");
+    const uint32_t T = ctx->tuning.dom_tile;
+    const uint32_t S = ctx->dom_size;
+    const uint32_t tiles_x = T ? (S + T - 1u) / T : 0u;
+    // The tile kernel evaluates both passes with one view: the params must
+    // agree except for the mode.
+    hm_params same = *op;
+    same.pass_mode[0] = dp->pass_mode[0];
+    const bool binned = T != 0u && tiles_x <= 1023u && std::memcmp(&same, dp, sizeof(hm_params)) == 0 &&
+                        (uint64_t)g->capacity * hm::BIN_STRIDE <= 0xffffffffull;
+    if (!binned) {
+        if ((st = hm_clear_dom(ctx)) != HM_OK) return st;
+        if ((st = hm_raster(ctx, g, dp, HM_PASS_DOM)) != HM_OK) return st;
+        return hm_raster(ctx, g, op, HM_PASS_DOM);
+    }
+    const uint32_t tiles = tiles_x * tiles_x;
+    const size_t slots = g->capacity ? g->capacity : 1u;
+    const size_t want_records = ctx->tuning.bin_records ? ctx->tuning.bin_records : slots * hm::BIN_STRIDE * 2u;
+    HM_HIP(ctx, ensure_bin(ctx, slots, hm::BIN_META_WORDS + 3u * (size_t)tiles + 1u, want_records));
+    hm::BinArgs b;
+    std::memcpy(&b.view, dp, sizeof(hm_params));
+    b.bundles = g->bundles;
+    b.layers = g->layers;
+    b.style = g->style;
+    b.refs = g->refs[HM_PASS_DOM];
+    b.indirect = g->indirect[HM_PASS_DOM];
+    b.capacity = g->capacity;
+    b.cps = ctx->bin.cps;
+    b.seg_box = ctx->bin.seg_box;
+    b.meta = ctx->bin.meta;
+    b.records = ctx->bin.records;
+    b.record_capacity = (uint32_t)std::min<size_t>(want_records, 0xffffffffu);
+    b.long_list = ctx->bin.long_list;
+    b.long_capacity = (uint32_t)std::min<size_t>(slots * hm::BIN_STRIDE, 0xffffffffu);
+    b.tiles_x = tiles_x;
+    b.tiles = tiles;
+    b.tile_shift = T == 16u ? 4u : 5u;
+    b.box_limit = ctx->tuning.bin_box_limit;
+    b.dom_size = S;
+    b.dom = ctx->dom;
+    {
+        Timer t(ctx, HM_K_DOM_BIN);
+        HM_HIP(ctx, hm::launch_bin(ctx->stream, b, raster_blocks(ctx, g->capacity, 1u)));
+    }
+    {
+        Timer t(ctx, HM_K_DOM_TILES);
+        HM_HIP(ctx, hm::launch_dom_tiles(ctx->stream, b, ctx->cu_count));
+    }
+    // Direct kernels, switched on by the device only when the records overflowed.
+    const uint32_t* fallback = ctx->bin.meta + hm::BIN_FALLBACK;
+    const size_t texels = (size_t)S * S;
+    {
+        Timer t(ctx, HM_K_CLEAR_DOM);
+        HM_HIP(ctx, hm::launch_fill32(ctx->stream, ctx->dom, texels, 0xffffffffu, ctx->cu_count * 8, fallback));
+        HM_HIP(ctx, hm::launch_fill32(ctx->stream, ctx->dom + texels, texels * HM_DOM_LAYERS, 0u, ctx->cu_count * 8,
+                                      fallback));
+    }
+    a1.gate = fallback;
+    a2.gate = fallback;
+    {
+        Timer t(ctx, HM_K_DOM_DEPTH);
+        HM_HIP(ctx, hm::launch_raster(ctx->stream, a1, 1, false, raster_blocks(ctx, g->capacity, 1u)));
+    }
+    {
+        Timer t(ctx, HM_K_DOM_OPACITY);
+        HM_HIP(ctx, hm::launch_raster(ctx->stream, a2, 2, false, raster_blocks(ctx, g->capacity, 2u)));
+    }
+    return HM_OK;
+}
+
 hm_status hm_shade(hm_ctx* ctx, const hm_params* p, float exposure) {
     hm_status st = check_targets(ctx);
     if (st != HM_OK) return st;
@@ -791,11 +912,9 @@ hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm
     if ((st = hm_scan(ctx, g, &cam, HM_PASS_CAMERA)) != HM_OK) return st;
     if ((st = hm_raster(ctx, g, &cam, HM_PASS_CAMERA)) != HM_OK) return st;
     if (cam.flags[3]) {
-        if ((st = hm_clear_dom(ctx)) != HM_OK) return st;
         if ((st = hm_lod(ctx, g, &params[1], HM_PASS_DOM, nullptr)) != HM_OK) return st;
         if ((st = hm_scan(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
-        if ((st = hm_raster(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
-        if ((st = hm_raster(ctx, g, &params[2], HM_PASS_DOM)) != HM_OK) return st;
+        if ((st = hm_dom(ctx, g, &params[1], &params[2])) != HM_OK) return st;
     }
     if (opts && opts->shading == HM_SHADING_MLP) {
         if ((st = hm_shade_mlp(ctx, &cam, exposure, opts->mlp_tolerance, nullptr)) != HM_OK) return st;
@@ -821,6 +940,7 @@ static bool target_buffer(hm_ctx* ctx, hm_groom* g, hm_buffer which, void** ptr,
         case HM_BUF_FILTERED_DEPTH: *ptr = ctx->filtered_depth; *bytes = px * 4; return ctx->filtered_depth != nullptr;
         case HM_BUF_DOM: *ptr = ctx->dom; *bytes = dom; return ctx->dom != nullptr;
         case HM_BUF_SCENE_DEPTH: *ptr = ctx->scene_depth; *bytes = px * 4; return ctx->scene_depth != nullptr;
+        case HM_BUF_DOM_BIN_STATUS: *ptr = ctx->bin.meta; *bytes = hm::BIN_META_WORDS * 4; return ctx->bin.meta != nullptr;
         default: break;
     }
     if (!g) return false;
@@ -912,6 +1032,10 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
             return fail(ctx, HM_ERR_INVALID, "raster_blocks_per_cu must be in [1, 64]");
         if (t->raster_flags[m] & ~(HM_RASTER_PRECHECK | HM_RASTER_XCD)) return fail(ctx, HM_ERR_INVALID, "unknown flag");
     }
+    if (t->dom_tile != 0u && t->dom_tile != 16u && t->dom_tile != 32u)
+        return fail(ctx, HM_ERR_INVALID, "dom_tile must be 0, 16 or 32");
+    if (t->bin_box_limit == 0u || t->bin_box_limit > 64u)
+        return fail(ctx, HM_ERR_INVALID, "bin_box_limit must be in [1, 64]");
     ctx->tuning = *t;
     return HM_OK;
 }
@@ -963,6 +1087,29 @@ hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const
                     out.raster_flags[mode] = flags;
                     out.raster_blocks_per_cu[mode] = grids[gi];
                 }
+            }
+        }
+    }
+    // DOM: direct kernels against binned tiles of either size, whole hm_dom timed.
+    if (params[1].flags[3]) {
+        const uint32_t tile_sizes[3] = {0u, 16u, 32u};
+        float best_dom = 1e30f;
+        for (uint32_t ti = 0; ti < 3; ++ti) {
+            ctx->tuning = out;
+            ctx->tuning.dom_tile = tile_sizes[ti];
+            float total = 0.0f;
+            for (uint32_t r = 0; r <= reps; ++r) {
+                HM_HIP(ctx, hipEventRecord(e0, ctx->stream));
+                if ((st = hm_dom(ctx, g, &params[1], &params[2])) != HM_OK) return st;
+                HM_HIP(ctx, hipEventRecord(e1, ctx->stream));
+                HM_HIP(ctx, hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) total += ms;
+            }
+            if (total < best_dom) {
+                best_dom = total;
+                out.dom_tile = tile_sizes[ti];
             }
         }
     }

@@ -182,6 +182,15 @@ class Renderer:
     def raster(self, g: GpuGroom, params, pass_: int):
         self._check(N.lib().hm_raster(self.ctx, g.handle, C.byref(params), pass_), "hm_raster")
 
+    def dom(self, g: GpuGroom, depth_params, opacity_params):
+        """clear_dom + DOM depth + DOM opacity in one call (binned when tuned so)."""
+        self._check(N.lib().hm_dom(self.ctx, g.handle, C.byref(depth_params), C.byref(opacity_params)), "hm_dom")
+
+    def dom_bin_status(self) -> dict:
+        s = self.read(N.BUF_DOM_BIN_STATUS, np.uint32, 8)
+        return {"records": int(s[0]), "binned": bool(s[1]), "fallback": bool(s[2]), "long": bool(s[3]),
+                "long_count": int(s[4])}
+
     def raster_count(self, g: GpuGroom, params, pass_: int) -> np.ndarray:
         c = (C.c_uint64 * 4)()
         self._check(N.lib().hm_raster_count(self.ctx, g.handle, C.byref(params), pass_, c), "hm_raster_count")
@@ -233,7 +242,7 @@ class Renderer:
         return (self.read(N.BUF_CENTER, np.uint64, n), self.read(N.BUF_CONSERVATIVE, np.uint64, n),
                 self.read(N.BUF_BETA, np.uint32, n))
 
-    def dom(self) -> np.ndarray:
+    def dom_map(self) -> np.ndarray:
         return self.read(N.BUF_DOM, np.uint32, self.dom_size * self.dom_size * 17)
 
     def shaded(self):
@@ -258,21 +267,37 @@ class Renderer:
         return refs[:total], total
 
     # -- tuning --
+    @staticmethod
+    def _tuning_dict(t: N.HmTuning) -> dict:
+        return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": list(t.raster_blocks_per_cu),
+                "dom_tile": int(t.dom_tile), "bin_box_limit": int(t.bin_box_limit),
+                "bin_records": int(t.bin_records)}
+
     def get_tuning(self) -> dict:
         t = N.HmTuning()
         self._check(N.lib().hm_get_tuning(self.ctx, C.byref(t)), "hm_get_tuning")
-        return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": list(t.raster_blocks_per_cu)}
+        return self._tuning_dict(t)
 
-    def set_tuning(self, raster_flags, raster_blocks_per_cu):
-        if isinstance(raster_blocks_per_cu, int):
-            raster_blocks_per_cu = [raster_blocks_per_cu] * 3
-        t = N.HmTuning((C.c_uint32 * 3)(*raster_flags), (C.c_uint32 * 3)(*raster_blocks_per_cu))
+    def set_tuning(self, raster_flags=None, raster_blocks_per_cu=None, dom_tile=None, bin_box_limit=None,
+                   bin_records=None):
+        """Changes the given launch choices; None keeps the current one."""
+        cur = self.get_tuning()
+        if raster_flags is not None:
+            cur["raster_flags"] = [raster_flags] * 3 if isinstance(raster_flags, int) else list(raster_flags)
+        if raster_blocks_per_cu is not None:
+            b = raster_blocks_per_cu
+            cur["raster_blocks_per_cu"] = [b] * 3 if isinstance(b, int) else list(b)
+        for k, v in (("dom_tile", dom_tile), ("bin_box_limit", bin_box_limit), ("bin_records", bin_records)):
+            if v is not None:
+                cur[k] = int(v)
+        t = N.HmTuning((C.c_uint32 * 3)(*cur["raster_flags"]), (C.c_uint32 * 3)(*cur["raster_blocks_per_cu"]),
+                       cur["dom_tile"], cur["bin_box_limit"], cur["bin_records"])
         self._check(N.lib().hm_set_tuning(self.ctx, C.byref(t)), "hm_set_tuning")
 
     def autotune(self, g: GpuGroom, params, opts, reps: int = 3) -> dict:
         t = N.HmTuning()
         self._check(N.lib().hm_autotune(self.ctx, g.handle, params, C.byref(opts), reps, C.byref(t)), "hm_autotune")
-        return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": list(t.raster_blocks_per_cu)}
+        return self._tuning_dict(t)
 
     # -- roofline microbenchmarks --
     def bench_copy(self, nbytes: int, reps: int = 5) -> float:

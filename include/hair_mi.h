@@ -192,7 +192,9 @@ typedef enum hm_buffer {
     HM_BUF_CAMERA_INDIRECT = 18,/* u32[4]: x, 1, 1, 0 */
     HM_BUF_DOM_LOD = 19,
     HM_BUF_DOM_REFS = 20,
-    HM_BUF_DOM_INDIRECT = 21
+    HM_BUF_DOM_INDIRECT = 21,
+    HM_BUF_DOM_BIN_STATUS = 24  /* u32[8] of the last hm_dom: records, binned, fallback,
+                                   long segments present, long segment count */
 } hm_buffer;
 
 /* Kernel ids for hm_timing_read. */
@@ -208,7 +210,9 @@ typedef enum hm_kernel {
     HM_K_DOM_OPACITY = 8,
     HM_K_SHADE = 9,
     HM_K_FILTER = 10,
-    HM_K_COUNT = 11
+    HM_K_DOM_BIN = 11,          /* binned DOM: control points, tile counts, scan, records */
+    HM_K_DOM_TILES = 12,        /* binned DOM: both passes per tile (+ long segments) */
+    HM_K_COUNT = 13
 } hm_kernel;
 
 #define HM_SHADING_PLAIN 0u
@@ -280,6 +284,18 @@ hm_status hm_raster(hm_ctx* ctx, hm_groom* groom, const hm_params* params, int p
 hm_status hm_shade(hm_ctx* ctx, const hm_params* params, float exposure); /* shade.wesl:fs */
 hm_status hm_filter(hm_ctx* ctx, const hm_params* params);              /* filter.wgsl */
 
+/* The deep opacity map of one groom in one call: clear_dom, the depth pass
+ * (`depth_params`, pass_mode.x = 1) and the opacity pass (`opacity_params`,
+ * pass_mode.x = 2) over the DOM work list that hm_lod/hm_scan built
+ * (pass.rs:524-583 after the lod and scan dispatches). The result equals
+ * hm_clear_dom + hm_raster + hm_raster bit for bit. With hm_tuning.dom_tile
+ * set and the two params differing only in pass_mode it runs binned: one
+ * workgroup per light-space tile evaluates both passes in LDS; when the tile
+ * records do not fit their buffer the direct kernels run instead, decided on
+ * the device (HM_BUF_DOM_BIN_STATUS says which ran). Asynchronous. */
+hm_status hm_dom(hm_ctx* ctx, hm_groom* groom, const hm_params* depth_params,
+                 const hm_params* opacity_params);
+
 /* Counts the raster's algorithmic work for one pass without writing any
  * target: counts[0] = camera write_pixel calls that reach the atomics,
  * counts[1] = of those, centre hits, counts[2] = DOM writes that pass the
@@ -290,7 +306,7 @@ hm_status hm_raster_count(hm_ctx* ctx, hm_groom* groom, const hm_params* params,
 /* ---- frame driver ------------------------------------------------------- */
 
 /* One groom of hair_pass (pass.rs:379-619): clear, camera lod/scan/raster,
- * [clear_dom, DOM lod/scan/raster x2 if flags.w], shade, [filter if flags.y].
+ * [DOM lod/scan, hm_dom if flags.w], shade, [filter if flags.y].
  * `params` are hm_make_params's three records. Asynchronous on the stream. */
 hm_status hm_frame(hm_ctx* ctx, hm_groom* groom, const hm_params params[3],
                    const hm_frame_opts* opts);
@@ -327,11 +343,16 @@ hm_status hm_time_frames(hm_ctx* ctx, hm_groom* groom, const hm_params params[3]
 typedef struct hm_tuning {
     uint32_t raster_flags[3];      /* per pass mode (camera, DOM depth, DOM opacity) */
     uint32_t raster_blocks_per_cu[3]; /* persistent raster grid per pass, 256-thread workgroups per CU */
+    uint32_t dom_tile;             /* hm_dom: 0 = direct kernels, 16 or 32 = binned, texels per tile side */
+    uint32_t bin_box_limit;        /* tiles a segment's box may span before it is drawn with
+                                      global atomics instead (1..64) */
+    uint32_t bin_records;          /* tile record buffer length; 0 = 2 per segment slot */
 } hm_tuning;
 
 hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);
 hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* tuning);
-/* Times every raster variant (flags x grid size) on this groom and view,
+/* Times every raster variant (flags x grid size, and the DOM's tile size)
+ * on this groom and view,
  * `reps` launches each from cleared targets, keeps the fastest per pass and
  * returns it. Leaves the targets dirty: run a frame afterwards. */
 hm_status hm_autotune(hm_ctx* ctx, hm_groom* groom, const hm_params params[3], const hm_frame_opts* opts,

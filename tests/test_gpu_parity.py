@@ -38,6 +38,8 @@ def run(request, renderer):
     r.raster(g, cam, N.PASS_CAMERA)
     out["gb"] = r.gbuffer()
     out["cam_counts"] = r.raster_count(g, cam, N.PASS_CAMERA)
+    keep = r.get_tuning()
+    r.set_tuning(dom_tile=0)  # kernel-by-kernel DOM below; hm_dom is tested separately
     r.clear_dom()
     r.lod(g, dom1, N.PASS_DOM)
     r.scan(g, dom1, N.PASS_DOM)
@@ -45,14 +47,15 @@ def run(request, renderer):
     out["dom_refs"] = r.refs(g, N.PASS_DOM)
     out["dom_counts1"] = r.raster_count(g, dom1, N.PASS_DOM)
     r.raster(g, dom1, N.PASS_DOM)
-    out["dom_front"] = r.dom()
+    out["dom_front"] = r.dom_map()
     out["dom_counts2"] = r.raster_count(g, dom2, N.PASS_DOM)
     r.raster(g, dom2, N.PASS_DOM)
-    out["dom"] = r.dom()
+    out["dom"] = r.dom_map()
     r.shade(cam, 1.0)
     out["shaded"] = r.shaded()
     r.filter(cam)
     out["filtered"] = r.filtered()
+    r.set_tuning(**keep)
     yield out
     g.free()
 
@@ -142,9 +145,9 @@ def test_frame_matches_oracle_and_is_deterministic(renderer):
     g = r.upload(sc.mesh)
     opts = r.frame_opts(control_points=w.control_points)
     r.hair_pass(g, sc.params, opts)
-    first = (*r.gbuffer(), r.dom(), r.filtered()[0])
+    first = (*r.gbuffer(), r.dom_map(), r.filtered()[0])
     r.hair_pass(g, sc.params, opts)
-    second = (*r.gbuffer(), r.dom(), r.filtered()[0])
+    second = (*r.gbuffer(), r.dom_map(), r.filtered()[0])
     g.free()
     for a, b in zip(first, second):
         assert np.array_equal(a, b)
@@ -171,14 +174,70 @@ def test_raster_variants_give_identical_bytes(renderer, flags):
     keep = r.get_tuning()
     try:
         for bpc in (4, 32):
-            r.set_tuning([flags] * 3, bpc)
+            r.set_tuning([flags] * 3, bpc, dom_tile=0)
             r.hair_pass(g, sc.params, r.frame_opts(control_points=w.control_points))
-            got = (*r.gbuffer(), r.dom())
+            got = (*r.gbuffer(), r.dom_map())
             ref = ref_frame(sc)
             pa, n_p, da, n_d = sc.light_arrays()
             ref.hair_pass(sc.params, pa, n_p, da, n_d, control_points=w.control_points)
             for a, b in zip(got, (ref.center, ref.cons, ref.beta, ref.dom)):
                 assert np.array_equal(a, b)
     finally:
-        r.set_tuning(keep["raster_flags"], keep["raster_blocks_per_cu"])
+        r.set_tuning(**keep)
+        g.free()
+
+
+_DOM_REF = {}
+
+
+def oracle_dom(name):
+    """The oracle's DOM after both passes over its own DOM work list."""
+    if name not in _DOM_REF:
+        sc = make_scene(name)
+        ref = ref_frame(sc)
+        cap = max(1, sc.mesh.strand_count)
+        lod = ref.lod(sc.params[1])
+        refs, _ = ref.scan(sc.params[1], lod, cap)
+        ref.clear_dom()
+        counts = ref.raster(sc.params[1], refs)
+        ref.raster(sc.params[2], refs)
+        _DOM_REF[name] = (ref.dom.copy(), int(counts[2]))
+    return _DOM_REF[name]
+
+
+# (tile, box limit, record buffer): binned with few / many long segments, and
+# a record buffer too small, which must switch to the direct kernels on the device.
+DOM_VARIANTS = [(16, 8, 0), (32, 8, 0), (16, 1, 0), (32, 64, 0), (32, 8, 1)]
+
+
+@pytest.mark.parametrize("case", ["tiny", "c1", "small_lod"])
+@pytest.mark.parametrize("variant", DOM_VARIANTS, ids=[f"t{t}-box{b}-rec{n}" for t, b, n in DOM_VARIANTS])
+def test_binned_dom_bit_exact(renderer, case, variant):
+    tile, box, records = variant
+    sc = make_scene(case)
+    w, r = sc.w, renderer
+    print(f"\n[gpu parity] hm_dom case={case} tile={tile} box={box} records={records}", flush=True)
+    r.targets(w.width, w.height, w.dom_size)
+    g = r.upload(sc.mesh)
+    keep = r.get_tuning()
+    try:
+        r.set_tuning(dom_tile=tile, bin_box_limit=box, bin_records=records)
+        r.lod(g, sc.params[1], N.PASS_DOM)
+        r.scan(g, sc.params[1], N.PASS_DOM)
+        # twice: the second call must not depend on what the first left behind
+        for _ in range(2):
+            r.dom(g, sc.params[1], sc.params[2])
+            got = r.dom_map()
+            st = r.dom_bin_status()
+            want, writes = oracle_dom(case)
+            assert np.array_equal(got, want), f"{(got != want).sum()} DOM words differ"
+        assert st["fallback"] == (records == 1) and st["binned"] == (records != 1)
+        if records != 1:
+            assert st["records"] > 0
+        if box == 1 and records != 1:
+            assert st["long_count"] > 0 and st["long"]
+        S = w.dom_size
+        assert int(got[S * S:].sum()) == writes
+    finally:
+        r.set_tuning(**keep)
         g.free()
