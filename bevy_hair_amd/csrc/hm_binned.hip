@@ -4,10 +4,13 @@
 // tile in LDS, and each tile's finished texels are written to HBM once.
 //
 //   k_fill32         zero the status words and per-tile counts
-//   k_bin_gen        one wave per strand: control points (gen_strand) to HBM,
-//                    the tile box of every segment, wave-aggregated counts
+//   k_bin_gen        one wave per strand: control points (gen_strand) and
+//                    their light-space projection to HBM, each segment's tile box
+//   k_bin_count      per-tile counts: LDS histogram per workgroup, one global
+//                    atomic per (workgroup, tile)
 //   k_bin_scan       exclusive scan of the counts; gate words
-//   k_bin_fill       segment ids into per-tile record lists
+//   k_bin_fill       segment ids into per-tile record lists (same histogram,
+//                    one global reservation per (workgroup, tile))
 //   k_fill32         DOM fronts to empty              (only with long segments)
 //   k_dom_long<1>    long segments, global atomic min  (same)
 //   k_dom_tiles<T>   per tile: depth pass (LDS min), front to HBM, opacity
@@ -30,7 +33,6 @@ namespace {
 
 __device__ __forceinline__ const View& bin_view(const BinArgs& a) { return *reinterpret_cast<const View*>(&a.view); }
 
-__device__ __forceinline__ F3 ld3(const float* q) { return {q[0], q[1], q[2]}; }
 __device__ __forceinline__ int64_t mx64(int64_t a, int64_t b) { return a > b ? a : b; }
 __device__ __forceinline__ int64_t mn64(int64_t a, int64_t b) { return a < b ? a : b; }
 
@@ -38,19 +40,16 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Texels the Bresenham walk of Drawer::dom_segment can write, clipped to the
-// map, as a tile box. The walk moves each coordinate by at most one per step
-// toward its end point and draws at most 1024 pixels; when both spans are
-// below 2^20 the error term cannot overflow and the walk stays inside the end
-// points' bounding box.
-__device__ __forceinline__ uint32_t segment_box(const View& p, F3 p0, F3 p1, uint32_t shift, uint32_t box_limit,
+// Texels the Bresenham walk of Drawer::dom_walk can write between projected
+// end points a and b, clipped to the map, as a tile box. The walk moves each
+// coordinate by at most one per step toward its end point and draws at most
+// 1024 pixels; when both spans are below 2^20 the error term cannot overflow
+// and the walk stays inside the end points' bounding box.
+__device__ __forceinline__ uint32_t segment_box(const View& p, F2 a, F2 b, uint32_t shift, uint32_t box_limit,
                                                 bool& is_long) {
     is_long = false;
-    Projected A = project_world(p, p0);
-    Projected B = project_world(p, p1);
-    if (!A.in_front && !B.in_front) return BIN_NONE;
-    int64_t x0 = to_i32(floorf(A.screen.x)), y0 = to_i32(floorf(A.screen.y));
-    int64_t x1 = to_i32(floorf(B.screen.x)), y1 = to_i32(floorf(B.screen.y));
+    int64_t x0 = to_i32(floorf(a.x)), y0 = to_i32(floorf(a.y));
+    int64_t x1 = to_i32(floorf(b.x)), y1 = to_i32(floorf(b.y));
     int64_t lx = x0 - 1023, hx = x0 + 1023, ly = y0 - 1023, hy = y0 + 1023;
     int64_t spanx = x1 > x0 ? x1 - x0 : x0 - x1, spany = y1 > y0 ? y1 - y0 : y0 - y1;
     if (spanx < (1 << 20) && spany < (1 << 20)) {
@@ -83,36 +82,6 @@ __device__ __forceinline__ uint32_t box_tile(uint32_t box, uint32_t i, uint32_t 
     return ty * tiles_x + tx;
 }
 
-// Walks each lane's (up to two) boxes tile by tile. All lanes that stand on
-// the same tile are served by one atomic of the lowest such lane; `emit`
-// receives each lane's slot (base + rank) for its current tile.
-template <class Emit>
-__device__ __forceinline__ void aggregate_tiles(uint32_t box0, uint32_t box1, uint32_t tiles_x, uint32_t* ctr,
-                                                Emit emit) {
-    const int lane = (int)(threadIdx.x & 63u);
-    uint32_t k = box0 != BIN_NONE ? 0u : (box1 != BIN_NONE ? 1u : 2u);
-    uint32_t i = 0u;
-    while (true) {
-        uint32_t box = k == 0u ? box0 : box1;
-        uint32_t mine = k < 2u ? box_tile(box, i, tiles_x) : 0xffffffffu;
-        uint64_t active = __ballot(mine != 0xffffffffu);
-        if (active == 0u) break;
-        int leader = __ffsll((unsigned long long)active) - 1;
-        uint32_t tile = __shfl(mine, leader);
-        uint64_t same = __ballot(mine == tile);
-        uint32_t base = 0u;
-        if (lane == leader) base = atomicAdd(ctr + tile, (uint32_t)__popcll(same));
-        base = __shfl(base, leader);
-        if (mine == tile) {
-            emit(k, base + lanes_below(same));
-            if (++i == box_tiles(box)) {
-                i = 0u;
-                k = (k == 0u && box1 != BIN_NONE) ? 1u : 2u;
-            }
-        }
-    }
-}
-
 // Appends flagged lanes' segment ids to the long list with one atomic per wave.
 __device__ __forceinline__ void append_long(bool flag, uint32_t id, uint32_t* count, uint32_t* list, uint32_t cap) {
     uint64_t m = __ballot(flag);
@@ -126,16 +95,18 @@ __device__ __forceinline__ void append_long(bool flag, uint32_t id, uint32_t* co
     if (flag && pos < cap) list[pos] = id;
 }
 
-struct StrandSchedule {
-    uint32_t first, end, stride;
+// Workgroup b takes the contiguous run [first, end) of `n` items; its threads
+// interleave within the run.
+struct Run {
+    uint32_t first, end;
 };
 
-__device__ __forceinline__ StrandSchedule strand_schedule(const BinArgs& a) {
-    uint32_t total = a.indirect[0];
-    total = total < a.capacity ? total : a.capacity;
-    uint32_t wpb = blockDim.x >> 6;
-    uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (threadIdx.x >> 6));
-    return {w, total, gridDim.x * wpb};
+__device__ __forceinline__ Run workgroup_run(uint32_t n) {
+    uint32_t per = (n + gridDim.x - 1u) / gridDim.x;
+    uint32_t first = blockIdx.x * per;
+    first = first < n ? first : n;
+    uint32_t end = first + per < n ? first + per : n;
+    return {first, end};
 }
 
 }  // namespace
@@ -143,42 +114,67 @@ __device__ __forceinline__ StrandSchedule strand_schedule(const BinArgs& a) {
 __global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
     const View& p = bin_view(a);
     const int lane = (int)(threadIdx.x & 63u);
-    uint32_t* tile_count = a.meta + BIN_META_WORDS;
-    StrandSchedule sch = strand_schedule(a);
-    for (uint32_t s = sch.first; s < sch.end; s += sch.stride) {
+    uint32_t total = a.indirect[0];
+    total = total < a.capacity ? total : a.capacity;
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (threadIdx.x >> 6));
+    for (uint32_t s = w0; s < total; s += gridDim.x * wpb) {
         const hm_strand_ref ref = a.refs[s];
         StrandCps sc = gen_strand(p, ref, a.bundles, a.layers, a.style, lane);
         const uint32_t ncp = sc.ncp;
-        float* base = a.cps + (size_t)s * (BIN_STRIDE * 3u);
+        Projected pr0 = project_world(p, sc.cw[0]), pr1 = project_world(p, sc.cw[1]);
+        const size_t base = (size_t)s * BIN_STRIDE;
         if ((uint32_t)lane < ncp) {
-            float* q = base + lane * 3;
-            q[0] = sc.cw[0].x;
-            q[1] = sc.cw[0].y;
-            q[2] = sc.cw[0].z;
+            a.cps[base + lane] = make_float4(sc.cw[0].x, sc.cw[0].y, sc.cw[0].z, pr0.screen.x);
+            a.cpy[base + lane] = pr0.screen.y;
         }
         if ((uint32_t)lane + 64u < ncp) {
-            float* q = base + (lane + 64) * 3;
-            q[0] = sc.cw[1].x;
-            q[1] = sc.cw[1].y;
-            q[2] = sc.cw[1].z;
+            a.cps[base + lane + 64] = make_float4(sc.cw[1].x, sc.cw[1].y, sc.cw[1].z, pr1.screen.x);
+            a.cpy[base + lane + 64] = pr1.screen.y;
         }
-        F3 n0 = shfl3(sc.cw[0], (lane + 1) & 63), n1 = shfl3(sc.cw[1], (lane + 1) & 63);
-        F3 e0 = lane < 63 ? n0 : n1;
+        // Segment idx -> idx + 1: the end point's projection lives in the next lane.
+        const int nl = (lane + 1) & 63;
+        F2 s0n = {__shfl(pr0.screen.x, nl), __shfl(pr0.screen.y, nl)};
+        F2 s1n = {__shfl(pr1.screen.x, nl), __shfl(pr1.screen.y, nl)};
+        bool f0n = __shfl((int)pr0.in_front, nl) != 0, f1n = __shfl((int)pr1.in_front, nl) != 0;
+        F2 e0 = lane < 63 ? s0n : s1n;
+        bool ef0 = lane < 63 ? f0n : f1n;
         const uint32_t id0 = s * BIN_STRIDE + (uint32_t)lane, id1 = id0 + 64u;
         bool long0 = false, long1 = false;
         uint32_t box0 = BIN_NONE, box1 = BIN_NONE;
-        if ((uint32_t)lane + 1u < ncp) {
-            box0 = segment_box(p, sc.cw[0], e0, a.tile_shift, a.box_limit, long0);
-            a.seg_box[id0] = box0;
-        }
-        if ((uint32_t)lane + 65u < ncp) {
-            box1 = segment_box(p, sc.cw[1], n1, a.tile_shift, a.box_limit, long1);
-            a.seg_box[id1] = box1;
-        }
+        if ((uint32_t)lane + 1u < ncp && (pr0.in_front || ef0))
+            box0 = segment_box(p, pr0.screen, e0, a.tile_shift, a.box_limit, long0);
+        if ((uint32_t)lane + 65u < ncp && (pr1.in_front || f1n))
+            box1 = segment_box(p, pr1.screen, s1n, a.tile_shift, a.box_limit, long1);
+        // Every slot of the strand is written, so the binning passes can walk
+        // segment ids without the strand's control-point count.
+        a.seg_box[id0] = box0;
+        a.seg_box[id1] = box1;
         append_long(long0, id0, a.meta + BIN_LONG_COUNT, a.long_list, a.long_capacity);
         append_long(long1, id1, a.meta + BIN_LONG_COUNT, a.long_list, a.long_capacity);
-        aggregate_tiles(box0, box1, a.tiles_x, tile_count, [](uint32_t, uint32_t) {});
     }
+}
+
+// Per-workgroup tile histogram in LDS (dynamic size: one word per tile).
+// Each workgroup adds up the (segment, tile) pairs of one contiguous run of
+// segment ids, then adds each nonzero entry to the global count once.
+__global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
+    extern __shared__ uint32_t hist[];
+    for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x) hist[t] = 0u;
+    __syncthreads();
+    uint32_t total = a.indirect[0];
+    total = total < a.capacity ? total : a.capacity;
+    const Run run = workgroup_run(total * BIN_STRIDE);
+    for (uint32_t id = run.first + threadIdx.x; id < run.end; id += blockDim.x) {
+        const uint32_t box = a.seg_box[id];
+        if (box == BIN_NONE) continue;
+        const uint32_t n = box_tiles(box);
+        for (uint32_t i = 0; i < n; ++i) atomicAdd(hist + box_tile(box, i, a.tiles_x), 1u);
+    }
+    __syncthreads();
+    uint32_t* count = a.meta + BIN_META_WORDS;
+    for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x)
+        if (hist[t]) atomicAdd(count + t, hist[t]);
 }
 
 // Exclusive scan of the per-tile counts (one block), then the gate words.
@@ -220,20 +216,32 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
     }
 }
 
+// Same runs as k_bin_count: count again in LDS, reserve one block of slots
+// per touched tile with one global atomic, then hand the slots out in LDS.
 __global__ __launch_bounds__(256) void k_bin_fill(BinArgs a) {
     if (a.meta[BIN_OK] == 0u) return;
-    const int lane = (int)(threadIdx.x & 63u);
+    extern __shared__ uint32_t hist[];
+    for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x) hist[t] = 0u;
+    __syncthreads();
+    uint32_t total = a.indirect[0];
+    total = total < a.capacity ? total : a.capacity;
+    const Run run = workgroup_run(total * BIN_STRIDE);
+    for (uint32_t id = run.first + threadIdx.x; id < run.end; id += blockDim.x) {
+        const uint32_t box = a.seg_box[id];
+        if (box == BIN_NONE) continue;
+        const uint32_t n = box_tiles(box);
+        for (uint32_t i = 0; i < n; ++i) atomicAdd(hist + box_tile(box, i, a.tiles_x), 1u);
+    }
+    __syncthreads();
     uint32_t* cursor = a.meta + BIN_META_WORDS + 2u * a.tiles + 1u;
-    StrandSchedule sch = strand_schedule(a);
-    for (uint32_t s = sch.first; s < sch.end; s += sch.stride) {
-        uint32_t ncp = a.refs[s].control_points;
-        ncp = ncp < 2u ? 2u : (ncp > 127u ? 127u : ncp);
-        const uint32_t id0 = s * BIN_STRIDE + (uint32_t)lane, id1 = id0 + 64u;
-        uint32_t box0 = (uint32_t)lane + 1u < ncp ? a.seg_box[id0] : BIN_NONE;
-        uint32_t box1 = (uint32_t)lane + 65u < ncp ? a.seg_box[id1] : BIN_NONE;
-        uint32_t* records = a.records;
-        aggregate_tiles(box0, box1, a.tiles_x, cursor,
-                        [&](uint32_t k, uint32_t slot) { records[slot] = k == 0u ? id0 : id1; });
+    for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x)
+        if (hist[t]) hist[t] = atomicAdd(cursor + t, hist[t]);
+    __syncthreads();
+    for (uint32_t id = run.first + threadIdx.x; id < run.end; id += blockDim.x) {
+        const uint32_t box = a.seg_box[id];
+        if (box == BIN_NONE) continue;
+        const uint32_t n = box_tiles(box);
+        for (uint32_t i = 0; i < n; ++i) a.records[atomicAdd(hist + box_tile(box, i, a.tiles_x), 1u)] = id;
     }
 }
 
@@ -261,6 +269,15 @@ struct DomTileSink {
     }
 };
 
+// One segment of the map from the stored control points (world position and
+// light-space projection of both ends, raster.wgsl:150-156).
+template <class D>
+__device__ __forceinline__ void walk_record(D& d, const BinArgs& a, uint32_t id) {
+    const float4 c0 = a.cps[id], c1 = a.cps[id + 1u];
+    const float y0 = a.cpy[id], y1 = a.cpy[id + 1u];
+    d.dom_walk(F3{c0.x, c0.y, c0.z}, F3{c1.x, c1.y, c1.z}, F2{c0.w, y0}, F2{c1.w, y1}, (id & (BIN_STRIDE - 1u)) == 0u);
+}
+
 template <int T>
 __global__ __launch_bounds__(256) void k_dom_tiles(BinArgs a) {
     if (a.meta[BIN_OK] == 0u) return;
@@ -285,11 +302,7 @@ __global__ __launch_bounds__(256) void k_dom_tiles(BinArgs a) {
     DomTileSink<T> sink{front, counts, ox, oy};
     {
         Drawer<1, DomTileSink<T>> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
-        for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) {
-            uint32_t id = a.records[r];
-            const float* q = a.cps + (size_t)id * 3u;
-            d.dom_segment(ld3(q), ld3(q + 3), (id & (BIN_STRIDE - 1u)) == 0u);
-        }
+        for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) walk_record(d, a, a.records[r]);
     }
     __syncthreads();
     for (int i = (int)threadIdx.x; i < T * T; i += 256) {
@@ -298,11 +311,7 @@ __global__ __launch_bounds__(256) void k_dom_tiles(BinArgs a) {
     }
     {
         Drawer<2, DomTileSink<T>> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
-        for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) {
-            uint32_t id = a.records[r];
-            const float* q = a.cps + (size_t)id * 3u;
-            d.dom_segment(ld3(q), ld3(q + 3), (id & (BIN_STRIDE - 1u)) == 0u);
-        }
+        for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) walk_record(d, a, a.records[r]);
     }
     __syncthreads();
     for (int i = (int)threadIdx.x; i < T * T * 4; i += 256) {
@@ -320,21 +329,22 @@ __global__ __launch_bounds__(256) void k_dom_long(BinArgs a) {
     const size_t S = a.dom_size;
     GlobalSink sink{nullptr, nullptr, nullptr, a.dom, S * S, true};
     Drawer<MODE, GlobalSink> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t id = a.long_list[i];
-        const float* q = a.cps + (size_t)id * 3u;
-        d.dom_segment(ld3(q), ld3(q + 3), (id & (BIN_STRIDE - 1u)) == 0u);
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        walk_record(d, a, a.long_list[i]);
 }
 
-hipError_t launch_bin(hipStream_t s, const BinArgs& a, int blocks) {
+hipError_t launch_bin(hipStream_t s, const BinArgs& a, int gen_blocks, int bin_blocks) {
+    if (a.tiles > BIN_MAX_TILES) return hipErrorInvalidValue;
     hipError_t e = launch_fill32(s, a.meta, BIN_META_WORDS + a.tiles, 0u, 64);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_gen, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_bin_gen, dim3(gen_blocks), dim3(256), 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const size_t lds = (size_t)a.tiles * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_bin_count, dim3(bin_blocks), dim3(256), lds, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, a.meta, a.tiles, a.record_capacity, a.long_capacity);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_fill, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_bin_fill, dim3(bin_blocks), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 

@@ -22,7 +22,8 @@ struct hm_groom {
 
 // Work buffers of the binned DOM (sized for the largest groom and map seen).
 struct hm_bin_state {
-    float* cps = nullptr;           // world control points, [slot * 128 + idx] x 3 floats
+    float4* cps = nullptr;          // [slot * 128 + idx]: world xyz, light-space x
+    float* cpy = nullptr;           // light-space y
     uint32_t* seg_box = nullptr;    // per segment id
     uint32_t* long_list = nullptr;
     size_t slots = 0;               // strand slots cps / seg_box / long_list hold
@@ -56,7 +57,7 @@ struct hm_ctx {
     hm_directional_light* dirs = nullptr;
     uint32_t dir_count = 0, dir_cap = 0;
     uint64_t* counters = nullptr;
-    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 32u, 8u, 0u};
+    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 8u, 0u, 2u};
     hm_bin_state bin;
     // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
     uint32_t slots = 0, frame = 0;

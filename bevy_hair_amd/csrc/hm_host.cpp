@@ -284,6 +284,7 @@ void free_targets(hm_ctx* c) {
 void free_bin(hm_ctx* c) {
     hm_bin_state& b = c->bin;
     dev_free(b.cps);
+    dev_free(b.cpy);
     dev_free(b.seg_box);
     dev_free(b.long_list);
     dev_free(b.meta);
@@ -300,11 +301,13 @@ hipError_t ensure_bin(hm_ctx* c, size_t slots, size_t meta_words, size_t records
     if (e != hipSuccess) return e;
     if (slots > b.slots) {
         dev_free(b.cps);
+        dev_free(b.cpy);
         dev_free(b.seg_box);
         dev_free(b.long_list);
         b.slots = 0;
         size_t segs = slots * hm::BIN_STRIDE;
-        if ((e = dev_alloc(&b.cps, segs * 3 * sizeof(float))) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.cps, segs * sizeof(float4))) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.cpy, segs * sizeof(float))) != hipSuccess) return e;
         if ((e = dev_alloc(&b.seg_box, segs * 4)) != hipSuccess) return e;
         if ((e = dev_alloc(&b.long_list, segs * 4)) != hipSuccess) return e;
         b.slots = slots;
@@ -785,7 +788,7 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
     // agree except for the mode.
     hm_params same = *op;
     same.pass_mode[0] = dp->pass_mode[0];
-    const bool binned = T != 0u && tiles_x <= 1023u && std::memcmp(&same, dp, sizeof(hm_params)) == 0 &&
+    const bool binned = T != 0u && tiles_x * tiles_x <= hm::BIN_MAX_TILES && std::memcmp(&same, dp, sizeof(hm_params)) == 0 &&
                         (uint64_t)g->capacity * hm::BIN_STRIDE <= 0xffffffffull;
     if (!binned) {
         if ((st = hm_clear_dom(ctx)) != HM_OK) return st;
@@ -805,6 +808,7 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
     b.indirect = g->indirect[HM_PASS_DOM];
     b.capacity = g->capacity;
     b.cps = ctx->bin.cps;
+    b.cpy = ctx->bin.cpy;
     b.seg_box = ctx->bin.seg_box;
     b.meta = ctx->bin.meta;
     b.records = ctx->bin.records;
@@ -819,7 +823,8 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
     b.dom = ctx->dom;
     {
         Timer t(ctx, HM_K_DOM_BIN);
-        HM_HIP(ctx, hm::launch_bin(ctx->stream, b, raster_blocks(ctx, g->capacity, 1u)));
+        HM_HIP(ctx, hm::launch_bin(ctx->stream, b, raster_blocks(ctx, g->capacity, 1u),
+                                   ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu));
     }
     {
         Timer t(ctx, HM_K_DOM_TILES);
@@ -1036,6 +1041,9 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
         return fail(ctx, HM_ERR_INVALID, "dom_tile must be 0, 16 or 32");
     if (t->bin_box_limit == 0u || t->bin_box_limit > 64u)
         return fail(ctx, HM_ERR_INVALID, "bin_box_limit must be in [1, 64]");
+    if (t->bin_blocks_per_cu == 0u || t->bin_blocks_per_cu > 64u)
+        return fail(ctx, HM_ERR_INVALID, "bin_blocks_per_cu must be in [1, 64]");
+
     ctx->tuning = *t;
     return HM_OK;
 }
@@ -1092,11 +1100,12 @@ hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const
     }
     // DOM: direct kernels against binned tiles of either size, whole hm_dom timed.
     if (params[1].flags[3]) {
-        const uint32_t tile_sizes[3] = {0u, 16u, 32u};
+        const uint32_t tile_sizes[7][2] = {{0u, 0u}, {16u, 1u}, {16u, 2u}, {16u, 4u}, {32u, 1u}, {32u, 2u}, {32u, 4u}};
         float best_dom = 1e30f;
-        for (uint32_t ti = 0; ti < 3; ++ti) {
+        for (uint32_t ti = 0; ti < 7; ++ti) {
             ctx->tuning = out;
-            ctx->tuning.dom_tile = tile_sizes[ti];
+            ctx->tuning.dom_tile = tile_sizes[ti][0];
+            if (tile_sizes[ti][1]) ctx->tuning.bin_blocks_per_cu = tile_sizes[ti][1];
             float total = 0.0f;
             for (uint32_t r = 0; r <= reps; ++r) {
                 HM_HIP(ctx, hipEventRecord(e0, ctx->stream));
@@ -1109,7 +1118,8 @@ hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const
             }
             if (total < best_dom) {
                 best_dom = total;
-                out.dom_tile = tile_sizes[ti];
+                out.dom_tile = ctx->tuning.dom_tile;
+                out.bin_blocks_per_cu = ctx->tuning.bin_blocks_per_cu;
             }
         }
     }

@@ -67,6 +67,7 @@ constexpr uint32_t BIN_FALLBACK = 2u;    // direct kernels run
 constexpr uint32_t BIN_LONG = 3u;        // binned and long segments exist
 constexpr uint32_t BIN_LONG_COUNT = 4u;
 constexpr uint32_t BIN_META_WORDS = 8u;  // then count[tiles], offset[tiles + 1], cursor[tiles]
+constexpr uint32_t BIN_MAX_TILES = 16384u;  // one LDS word per tile in the binning workgroups
 
 struct BinArgs {
     hm_params view;                 // DOM view (the two DOM passes share it)
@@ -76,7 +77,8 @@ struct BinArgs {
     const hm_strand_ref* refs;
     const uint32_t* indirect;
     uint32_t capacity;              // refs length
-    float* cps;                     // world control points, 3 floats, [slot * BIN_STRIDE + idx]
+    float4* cps;                    // per control point [slot * BIN_STRIDE + idx]: world xyz, light-space x
+    float* cpy;                     // light-space y
     uint32_t* seg_box;              // per segment id: tile box or BIN_NONE
     uint32_t* meta;
     uint32_t* records;              // segment ids grouped by tile
@@ -88,7 +90,7 @@ struct BinArgs {
     uint32_t* dom;                  // fronts S*S, then counts S*S*16
 };
 
-hipError_t launch_bin(hipStream_t s, const BinArgs& a, int blocks);
+hipError_t launch_bin(hipStream_t s, const BinArgs& a, int gen_blocks, int bin_blocks);
 hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks);
 
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes, int blocks);

@@ -247,13 +247,12 @@ struct Drawer {
 
     // raster.wgsl:118-148. The sample position (raster.wgsl:176-179) is only
     // formed for pixels that pass the bounds test.
-    __device__ __forceinline__ void dom_pixel(int32_t x, int32_t y, const Projected& A, F2 seg, float seg_len2, F3 p0,
-                                              F3 p1) {
+    __device__ __forceinline__ void dom_pixel(int32_t x, int32_t y, F2 a, F2 seg, float seg_len2, F3 p0, F3 p1) {
         int32_t width = to_i32(p.screen.x), height = to_i32(p.screen.y);
         if (x < 0 || y < 0 || x >= width || y >= height) return;
         if (!sink.accepts(x, y)) return;
         F2 center = {i2f(x) + 0.5f, i2f(y) + 0.5f};
-        float ts = clampf(rdiv(dot2(center - A.screen, seg), seg_len2), 0.0f, 1.0f);
+        float ts = clampf(rdiv(dot2(center - a, seg), seg_len2), 0.0f, 1.0f);
         F3 world = mix3(p0, p1, ts);
         float near_ = p.screen.z, far_ = p.screen.w;
         float dist = dot3(world - xyz(p.camera_pos), xyz(p.camera_forward));
@@ -280,17 +279,23 @@ struct Drawer {
         Projected B = project_world(p, p1);
         if (!A.in_front && !B.in_front) return;
         c.segments++;
-        int32_t x0 = to_i32(floorf(A.screen.x)), y0 = to_i32(floorf(A.screen.y));
-        int32_t x1 = to_i32(floorf(B.screen.x)), y1 = to_i32(floorf(B.screen.y));
+        dom_walk(p0, p1, A.screen, B.screen, include_start);
+    }
+
+    // The Bresenham walk of raster.wgsl:157-189 between projected end points
+    // a = project(p0).screen and b = project(p1).screen.
+    __device__ void dom_walk(F3 p0, F3 p1, F2 a, F2 b, bool include_start) {
+        int32_t x0 = to_i32(floorf(a.x)), y0 = to_i32(floorf(a.y));
+        int32_t x1 = to_i32(floorf(b.x)), y1 = to_i32(floorf(b.y));
         int32_t dx = wabs(wsub(x1, x0)), dy = wabs(wsub(y1, y0));
         int32_t sx = x0 < x1 ? 1 : -1, sy = y0 < y1 ? 1 : -1;
         int32_t err = wsub(dx, dy);
-        F2 seg = B.screen - A.screen;
+        F2 seg = b - a;
         float seg_len2 = mx(dot2(seg, seg), 1e-6f);
         bool skip = !include_start && (x0 != x1 || y0 != y1);
         int32_t ndy = wsub(0, dy);
         for (int step = 0; step < 1024; ++step) {
-            if (!skip) dom_pixel(x0, y0, A, seg, seg_len2, p0, p1);
+            if (!skip) dom_pixel(x0, y0, a, seg, seg_len2, p0, p1);
             skip = false;
             if (x0 == x1 && y0 == y1) break;
             if (sink.walked_out(x0, y0, sx, sy)) break;

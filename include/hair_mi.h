@@ -289,7 +289,8 @@ hm_status hm_filter(hm_ctx* ctx, const hm_params* params);              /* filte
  * pass_mode.x = 2) over the DOM work list that hm_lod/hm_scan built
  * (pass.rs:524-583 after the lod and scan dispatches). The result equals
  * hm_clear_dom + hm_raster + hm_raster bit for bit. With hm_tuning.dom_tile
- * set and the two params differing only in pass_mode it runs binned: one
+ * set, at most 16384 tiles and the two params differing only in pass_mode
+ * it runs binned: one
  * workgroup per light-space tile evaluates both passes in LDS; when the tile
  * records do not fit their buffer the direct kernels run instead, decided on
  * the device (HM_BUF_DOM_BIN_STATUS says which ran). Asynchronous. */
@@ -347,6 +348,8 @@ typedef struct hm_tuning {
     uint32_t bin_box_limit;        /* tiles a segment's box may span before it is drawn with
                                       global atomics instead (1..64) */
     uint32_t bin_records;          /* tile record buffer length; 0 = 2 per segment slot */
+    uint32_t bin_blocks_per_cu;    /* binning grid: 256-thread workgroups per CU, each counting
+                                      one contiguous run of segments in LDS (1..64) */
 } hm_tuning;
 
 hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);

@@ -205,23 +205,24 @@ def oracle_dom(name):
     return _DOM_REF[name]
 
 
-# (tile, box limit, record buffer): binned with few / many long segments, and
-# a record buffer too small, which must switch to the direct kernels on the device.
-DOM_VARIANTS = [(16, 8, 0), (32, 8, 0), (16, 1, 0), (32, 64, 0), (32, 8, 1)]
+# (tile, box limit, record buffer, binning workgroups per CU): binned with few
+# / many long segments, short and long segment runs per workgroup, and a
+# record buffer too small, which must switch to the direct kernels on the device.
+DOM_VARIANTS = [(16, 8, 0, 2), (32, 8, 0, 4), (16, 1, 0, 2), (32, 64, 0, 1), (16, 8, 0, 16), (32, 8, 1, 2)]
 
 
 @pytest.mark.parametrize("case", ["tiny", "c1", "small_lod"])
-@pytest.mark.parametrize("variant", DOM_VARIANTS, ids=[f"t{t}-box{b}-rec{n}" for t, b, n in DOM_VARIANTS])
+@pytest.mark.parametrize("variant", DOM_VARIANTS, ids=[f"t{t}-box{b}-rec{n}-wg{w}" for t, b, n, w in DOM_VARIANTS])
 def test_binned_dom_bit_exact(renderer, case, variant):
-    tile, box, records = variant
+    tile, box, records, wg = variant
     sc = make_scene(case)
     w, r = sc.w, renderer
-    print(f"\n[gpu parity] hm_dom case={case} tile={tile} box={box} records={records}", flush=True)
+    print(f"\n[gpu parity] hm_dom case={case} tile={tile} box={box} records={records} wg={wg}", flush=True)
     r.targets(w.width, w.height, w.dom_size)
     g = r.upload(sc.mesh)
     keep = r.get_tuning()
     try:
-        r.set_tuning(dom_tile=tile, bin_box_limit=box, bin_records=records)
+        r.set_tuning(dom_tile=tile, bin_box_limit=box, bin_records=records, bin_blocks_per_cu=wg)
         r.lod(g, sc.params[1], N.PASS_DOM)
         r.scan(g, sc.params[1], N.PASS_DOM)
         # twice: the second call must not depend on what the first left behind
