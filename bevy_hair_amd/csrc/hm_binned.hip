@@ -74,12 +74,14 @@ __device__ __forceinline__ uint32_t segment_box(const View& p, F2 a, F2 b, uint3
     return tx0 | (ty0 << 10) | ((bw - 1u) << 20) | ((bh - 1u) << 26);
 }
 
-__device__ __forceinline__ uint32_t box_tiles(uint32_t box) { return (((box >> 20) & 63u) + 1u) * (((box >> 26) & 63u) + 1u); }
-
-__device__ __forceinline__ uint32_t box_tile(uint32_t box, uint32_t i, uint32_t tiles_x) {
-    uint32_t bw = ((box >> 20) & 63u) + 1u;
-    uint32_t tx = (box & 1023u) + i % bw, ty = ((box >> 10) & 1023u) + i / bw;
-    return ty * tiles_x + tx;
+// Calls f(tile) for every tile of a box, row by row.
+template <class F>
+__device__ __forceinline__ void for_box_tiles(uint32_t box, uint32_t tiles_x, F f) {
+    if (box == BIN_NONE) return;
+    const uint32_t bw = ((box >> 20) & 63u) + 1u, bh = ((box >> 26) & 63u) + 1u;
+    uint32_t row = ((box >> 10) & 1023u) * tiles_x + (box & 1023u);
+    for (uint32_t y = 0; y < bh; ++y, row += tiles_x)
+        for (uint32_t x = 0; x < bw; ++x) f(row + x);
 }
 
 // Appends flagged lanes' segment ids to the long list with one atomic per wave.
@@ -95,17 +97,19 @@ __device__ __forceinline__ void append_long(bool flag, uint32_t id, uint32_t* co
     if (flag && pos < cap) list[pos] = id;
 }
 
-// Workgroup b takes the contiguous run [first, end) of `n` items; its threads
-// interleave within the run.
+// Workgroup b takes the contiguous run [first, end) of the n segment-box
+// words, in whole uint4 groups (n is a multiple of 4: 128 slots per strand);
+// its threads interleave within the run.
 struct Run {
-    uint32_t first, end;
+    uint32_t first, end;  // in uint4 groups
 };
 
 __device__ __forceinline__ Run workgroup_run(uint32_t n) {
-    uint32_t per = (n + gridDim.x - 1u) / gridDim.x;
+    const uint32_t groups = n / 4u;
+    const uint32_t per = (groups + gridDim.x - 1u) / gridDim.x;
     uint32_t first = blockIdx.x * per;
-    first = first < n ? first : n;
-    uint32_t end = first + per < n ? first + per : n;
+    first = first < groups ? first : groups;
+    const uint32_t end = first + per < groups ? first + per : groups;
     return {first, end};
 }
 
@@ -157,58 +161,90 @@ __global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
 
 // Per-workgroup tile histogram in LDS (dynamic size: one word per tile).
 // Each workgroup adds up the (segment, tile) pairs of one contiguous run of
-// segment ids, then adds each nonzero entry to the global count once.
+// segment ids, adds each nonzero entry to the global count once, and keeps
+// the count's previous value, its rank within the tile, as a (tile, rank)
+// list for k_bin_fill.
 __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
     extern __shared__ uint32_t hist[];
+    __shared__ uint32_t touched, list_base, list_pos;
     for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x) hist[t] = 0u;
+    if (threadIdx.x == 0) {
+        touched = 0u;
+        list_pos = 0u;
+    }
     __syncthreads();
     uint32_t total = a.indirect[0];
     total = total < a.capacity ? total : a.capacity;
     const Run run = workgroup_run(total * BIN_STRIDE);
-    for (uint32_t id = run.first + threadIdx.x; id < run.end; id += blockDim.x) {
-        const uint32_t box = a.seg_box[id];
-        if (box == BIN_NONE) continue;
-        const uint32_t n = box_tiles(box);
-        for (uint32_t i = 0; i < n; ++i) atomicAdd(hist + box_tile(box, i, a.tiles_x), 1u);
+    const uint4* boxes = reinterpret_cast<const uint4*>(a.seg_box);
+    for (uint32_t q = run.first + threadIdx.x; q < run.end; q += blockDim.x) {
+        const uint4 b = boxes[q];
+        const uint32_t bx[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { atomicAdd(hist + t, 1u); });
+    }
+    __syncthreads();
+    uint32_t mine = 0u;
+    for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x) mine += hist[t] ? 1u : 0u;
+    if (mine) atomicAdd(&touched, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        list_base = touched ? atomicAdd(a.meta + BIN_LIST_COUNT, touched) : 0u;
+        a.wg_list[blockIdx.x] = make_uint2(list_base, touched);
     }
     __syncthreads();
     uint32_t* count = a.meta + BIN_META_WORDS;
-    for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x)
-        if (hist[t]) atomicAdd(count + t, hist[t]);
+    for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x) {
+        const uint32_t h = hist[t];
+        if (!h) continue;
+        const uint32_t rank = atomicAdd(count + t, h);
+        const uint32_t pos = list_base + atomicAdd(&list_pos, 1u);
+        if (pos < a.list_capacity) a.lists[pos] = make_uint2(t, rank);
+    }
 }
 
-// Exclusive scan of the per-tile counts (one block), then the gate words.
+// Exclusive scan of the per-tile counts (one block of 1024 threads, 16 tiles
+// per thread), then the gate words.
 __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tiles, uint32_t record_capacity,
-                                                   uint32_t long_capacity) {
-    __shared__ uint32_t part[1024];
-    const uint32_t tid = threadIdx.x;
-    uint32_t* cnt = meta + BIN_META_WORDS;
-    uint32_t* off = cnt + tiles;
+                                                   uint32_t long_capacity, uint32_t list_capacity) {
+    constexpr uint32_t PER = BIN_MAX_TILES / 1024u;
+    __shared__ uint32_t wave_total[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t* cnt = meta + BIN_META_WORDS;
+    uint32_t* off = meta + BIN_META_WORDS + tiles;
     uint32_t* cur = off + tiles + 1u;
-    const uint32_t per = (tiles + 1023u) / 1024u;
-    const uint32_t b = tid * per < tiles ? tid * per : tiles;
-    const uint32_t e = b + per < tiles ? b + per : tiles;
+    const uint32_t b = tid * PER;
+    uint32_t v[PER];
     uint32_t sum = 0u;
-    for (uint32_t i = b; i < e; ++i) sum += cnt[i];
-    part[tid] = sum;
-    __syncthreads();
-    for (uint32_t d = 1u; d < 1024u; d <<= 1) {  // inclusive Hillis-Steele scan
-        uint32_t v = tid >= d ? part[tid - d] : 0u;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        v[k] = b + k < tiles ? cnt[b + k] : 0u;
+        sum += v[k];
     }
-    uint32_t run = part[tid] - sum;
-    for (uint32_t i = b; i < e; ++i) {
-        off[i] = run;
-        cur[i] = run;
-        run += cnt[i];
+    uint32_t inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(inc, d);
+        if ((int)lane >= d) inc += y;
+    }
+    if (lane == 63u) wave_total[wave] = inc;
+    __syncthreads();
+    uint32_t run = inc - sum;
+    for (uint32_t w = 0; w < wave; ++w) run += wave_total[w];
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        if (b + k < tiles) {
+            off[b + k] = run;
+            cur[b + k] = run;
+        }
+        run += v[k];
     }
     if (tid == 1023u) {
-        uint32_t total = part[1023];
+        const uint32_t total = run;
         off[tiles] = total;
-        uint32_t longs = meta[BIN_LONG_COUNT];
-        uint32_t ok = (total <= record_capacity && longs <= long_capacity) ? 1u : 0u;
+        const uint32_t longs = meta[BIN_LONG_COUNT];
+        const uint32_t ok =
+            (total <= record_capacity && longs <= long_capacity && meta[BIN_LIST_COUNT] <= list_capacity) ? 1u : 0u;
         meta[BIN_RECORDS] = total;
         meta[BIN_OK] = ok;
         meta[BIN_FALLBACK] = 1u - ok;
@@ -216,32 +252,31 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
     }
 }
 
-// Same runs as k_bin_count: count again in LDS, reserve one block of slots
-// per touched tile with one global atomic, then hand the slots out in LDS.
+// Same runs as k_bin_count: the workgroup's (tile, rank) list gives the first
+// slot of its block in each touched tile; slots are handed out in LDS.
 __global__ __launch_bounds__(256) void k_bin_fill(BinArgs a) {
     if (a.meta[BIN_OK] == 0u) return;
     extern __shared__ uint32_t hist[];
-    for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x) hist[t] = 0u;
+    const uint32_t* off = a.meta + BIN_META_WORDS + a.tiles;
+    const uint2 wl = a.wg_list[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < wl.y; i += blockDim.x) {
+        const uint2 e = a.lists[wl.x + i];
+        hist[e.x] = off[e.x] + e.y;
+    }
     __syncthreads();
     uint32_t total = a.indirect[0];
     total = total < a.capacity ? total : a.capacity;
     const Run run = workgroup_run(total * BIN_STRIDE);
-    for (uint32_t id = run.first + threadIdx.x; id < run.end; id += blockDim.x) {
-        const uint32_t box = a.seg_box[id];
-        if (box == BIN_NONE) continue;
-        const uint32_t n = box_tiles(box);
-        for (uint32_t i = 0; i < n; ++i) atomicAdd(hist + box_tile(box, i, a.tiles_x), 1u);
-    }
-    __syncthreads();
-    uint32_t* cursor = a.meta + BIN_META_WORDS + 2u * a.tiles + 1u;
-    for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x)
-        if (hist[t]) hist[t] = atomicAdd(cursor + t, hist[t]);
-    __syncthreads();
-    for (uint32_t id = run.first + threadIdx.x; id < run.end; id += blockDim.x) {
-        const uint32_t box = a.seg_box[id];
-        if (box == BIN_NONE) continue;
-        const uint32_t n = box_tiles(box);
-        for (uint32_t i = 0; i < n; ++i) a.records[atomicAdd(hist + box_tile(box, i, a.tiles_x), 1u)] = id;
+    const uint4* boxes = reinterpret_cast<const uint4*>(a.seg_box);
+    uint32_t* records = a.records;
+    for (uint32_t q = run.first + threadIdx.x; q < run.end; q += blockDim.x) {
+        const uint4 b = boxes[q];
+        const uint32_t bx[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t id = q * 4u + (uint32_t)k;
+            for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { records[atomicAdd(hist + t, 1u)] = id; });
+        }
     }
 }
 
@@ -342,7 +377,8 @@ hipError_t launch_bin(hipStream_t s, const BinArgs& a, int gen_blocks, int bin_b
     const size_t lds = (size_t)a.tiles * sizeof(uint32_t);
     hipLaunchKernelGGL(k_bin_count, dim3(bin_blocks), dim3(256), lds, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, a.meta, a.tiles, a.record_capacity, a.long_capacity);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, a.meta, a.tiles, a.record_capacity, a.long_capacity,
+                       a.list_capacity);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin_fill, dim3(bin_blocks), dim3(256), lds, s, a);
     return hipGetLastError();

@@ -30,7 +30,10 @@ struct hm_bin_state {
     uint32_t* meta = nullptr;       // status words, per-tile count / offset / cursor
     size_t meta_words = 0;
     uint32_t* records = nullptr;
+    uint2* lists = nullptr;         // same length as records
     size_t record_capacity = 0;
+    uint2* wg_list = nullptr;       // one per binning workgroup
+    size_t wg_capacity = 0;
 };
 
 struct hm_ctx {

@@ -289,14 +289,17 @@ void free_bin(hm_ctx* c) {
     dev_free(b.long_list);
     dev_free(b.meta);
     dev_free(b.records);
+    dev_free(b.lists);
+    dev_free(b.wg_list);
     b = hm_bin_state();
 }
 
 // Grows the binned-DOM buffers to hold `slots` strand slots, `meta_words`
-// status + per-tile words and `records` tile records.
-hipError_t ensure_bin(hm_ctx* c, size_t slots, size_t meta_words, size_t records) {
+// status + per-tile words, `records` tile records and `wgs` binning workgroups.
+hipError_t ensure_bin(hm_ctx* c, size_t slots, size_t meta_words, size_t records, size_t wgs) {
     hm_bin_state& b = c->bin;
-    if (slots <= b.slots && meta_words <= b.meta_words && records <= b.record_capacity) return hipSuccess;
+    if (slots <= b.slots && meta_words <= b.meta_words && records <= b.record_capacity && wgs <= b.wg_capacity)
+        return hipSuccess;
     hipError_t e = hipStreamSynchronize(c->stream);  // earlier frames may still read them
     if (e != hipSuccess) return e;
     if (slots > b.slots) {
@@ -320,9 +323,17 @@ hipError_t ensure_bin(hm_ctx* c, size_t slots, size_t meta_words, size_t records
     }
     if (records > b.record_capacity) {
         dev_free(b.records);
+        dev_free(b.lists);
         b.record_capacity = 0;
         if ((e = dev_alloc(&b.records, records * 4)) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.lists, records * sizeof(uint2))) != hipSuccess) return e;
         b.record_capacity = records;
+    }
+    if (wgs > b.wg_capacity) {
+        dev_free(b.wg_list);
+        b.wg_capacity = 0;
+        if ((e = dev_alloc(&b.wg_list, wgs * sizeof(uint2))) != hipSuccess) return e;
+        b.wg_capacity = wgs;
     }
     return hipSuccess;
 }
@@ -798,7 +809,8 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
     const uint32_t tiles = tiles_x * tiles_x;
     const size_t slots = g->capacity ? g->capacity : 1u;
     const size_t want_records = ctx->tuning.bin_records ? ctx->tuning.bin_records : slots * hm::BIN_STRIDE * 2u;
-    HM_HIP(ctx, ensure_bin(ctx, slots, hm::BIN_META_WORDS + 3u * (size_t)tiles + 1u, want_records));
+    const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
+    HM_HIP(ctx, ensure_bin(ctx, slots, hm::BIN_META_WORDS + 3u * (size_t)tiles + 1u, want_records, (size_t)bin_blocks));
     hm::BinArgs b;
     std::memcpy(&b.view, dp, sizeof(hm_params));
     b.bundles = g->bundles;
@@ -813,6 +825,9 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
     b.meta = ctx->bin.meta;
     b.records = ctx->bin.records;
     b.record_capacity = (uint32_t)std::min<size_t>(want_records, 0xffffffffu);
+    b.lists = ctx->bin.lists;
+    b.list_capacity = b.record_capacity;
+    b.wg_list = ctx->bin.wg_list;
     b.long_list = ctx->bin.long_list;
     b.long_capacity = (uint32_t)std::min<size_t>(slots * hm::BIN_STRIDE, 0xffffffffu);
     b.tiles_x = tiles_x;
@@ -823,8 +838,7 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
     b.dom = ctx->dom;
     {
         Timer t(ctx, HM_K_DOM_BIN);
-        HM_HIP(ctx, hm::launch_bin(ctx->stream, b, raster_blocks(ctx, g->capacity, 1u),
-                                   ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu));
+        HM_HIP(ctx, hm::launch_bin(ctx->stream, b, raster_blocks(ctx, g->capacity, 1u), bin_blocks));
     }
     {
         Timer t(ctx, HM_K_DOM_TILES);

@@ -66,6 +66,7 @@ constexpr uint32_t BIN_OK = 1u;          // binned kernels run
 constexpr uint32_t BIN_FALLBACK = 2u;    // direct kernels run
 constexpr uint32_t BIN_LONG = 3u;        // binned and long segments exist
 constexpr uint32_t BIN_LONG_COUNT = 4u;
+constexpr uint32_t BIN_LIST_COUNT = 5u;  // (tile, rank) entries the counting workgroups wrote
 constexpr uint32_t BIN_META_WORDS = 8u;  // then count[tiles], offset[tiles + 1], cursor[tiles]
 constexpr uint32_t BIN_MAX_TILES = 16384u;  // one LDS word per tile in the binning workgroups
 
@@ -83,6 +84,9 @@ struct BinArgs {
     uint32_t* meta;
     uint32_t* records;              // segment ids grouped by tile
     uint32_t record_capacity;
+    uint2* lists;                   // per counting workgroup: (tile, rank within the tile)
+    uint32_t list_capacity;
+    uint2* wg_list;                 // per counting workgroup: (first list entry, entries)
     uint32_t* long_list;            // segments whose box spans more than box_limit tiles
     uint32_t long_capacity;
     uint32_t tiles_x, tiles, tile_shift, box_limit;

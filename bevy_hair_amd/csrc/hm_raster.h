@@ -195,6 +195,16 @@ struct Drawer {
         Projected B = project_world(p, p1);
         if (!A.in_front && !B.in_front) return;
         c.segments++;
+        segment_walk(p0, p1, A.screen, B.screen, t0, t1, uvw0, uvw1, ao0, ao1, beta);
+    }
+
+    // The DDA of raster.wgsl:77-116 between projected end points
+    // a = project(p0).screen and b = project(p1).screen.
+    __device__ void segment_walk(F3 p0, F3 p1, F2 a_screen, F2 b_screen, F3 t0, F3 t1, F3 uvw0, F3 uvw1, float ao0,
+                                 float ao1, float beta) {
+        const struct {
+            F2 screen;
+        } A{a_screen}, B{b_screen};
         F2 delta = B.screen - A.screen;
         int32_t steps = to_i32(ceilf(mx(fabsf(delta.x), fabsf(delta.y))));
         int32_t n = steps < 1 ? 1 : (steps > 96 ? 96 : steps);
