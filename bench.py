@@ -46,6 +46,16 @@ def algorithmic_bytes(kernel: str, w, mesh, counts: dict, hit_pixels: int) -> fl
         return 4.0 * counts["dom1"][2] + groom
     if kernel == "dom_opacity":
         return 8.0 * counts["dom2"][2] + groom
+    if kernel == "cam_bin":
+        # groom in; per control point world position + screen position (20 B)
+        # and world tangent + ao (16 B) out; 16 B per strand; tile box out and
+        # back per segment; one 4 B record per (segment, tile)
+        return (groom + 36.0 * counts["cam_cps"] + 16.0 * N + 8.0 * counts["cam_segments"]
+                + 4.0 * counts["cam_records"])
+    if kernel == "cam_tiles":
+        # the G-buffer written once (centre, conservative, beta); per record its
+        # id, both control points (36 B each) and the strand's uv / beta
+        return 20.0 * P + 92.0 * counts["cam_records"]
     if kernel == "dom_bin":
         # groom in, control points out (12 B), tile box out and back (4 + 4 B
         # per segment), one 4 B record per (segment, tile)
@@ -145,6 +155,13 @@ def main() -> int:
     r.lod(g, sc.params[0], N.PASS_CAMERA, w.control_points, w.strand_fraction)
     r.scan(g, sc.params[0], N.PASS_CAMERA)
     counts = {"cam": r.raster_count(g, sc.params[0], N.PASS_CAMERA)}
+    cam_refs, _ = r.refs(g, N.PASS_CAMERA)
+    cncp = np.clip(cam_refs[:, 3].astype(np.int64), 2, 127)
+    counts["cam_cps"] = int(cncp.sum())
+    counts["cam_segments"] = int((cncp - 1).sum())
+    r.gbuffer_pass(g, sc.params[0])
+    cam_status = r.bin_status(0) if tuning.get("cam_tile") else None
+    counts["cam_records"] = cam_status["records"] if cam_status else 0
     r.hair_pass(g, sc.params, opts)
     r.lod(g, sc.params[1], N.PASS_DOM)
     r.scan(g, sc.params[1], N.PASS_DOM)
@@ -155,7 +172,7 @@ def main() -> int:
     counts["dom_cps"] = int(ncp.sum())
     counts["dom_segments"] = int((ncp - 1).sum())
     r.dom(g, sc.params[1], sc.params[2])
-    bin_status = r.dom_bin_status() if tuning.get("dom_tile") else None
+    bin_status = r.bin_status(1) if tuning.get("dom_tile") else None
     counts["dom_records"] = bin_status["records"] if bin_status else 0
     center, _, _ = r.gbuffer()
     hit = int((center != np.uint64(0xFFFFFFFFFFFFFFFF)).sum())
@@ -199,9 +216,10 @@ def main() -> int:
         "work": {"camera_writes": int(counts["cam"][0]), "camera_center_hits": int(counts["cam"][1]),
                  "camera_atomics": int(atomics_cam), "dom_depth_writes": int(counts["dom1"][2]),
                  "dom_opacity_increments": int(counts["dom2"][2]), "segments": int(counts["cam"][3]),
-                 "hit_pixels": hit, "dom_bin": bin_status},
+                 "hit_pixels": hit, "cam_bin": cam_status, "dom_bin": bin_status},
         "atomic_rates": {
-            "cam_raster_atomics_per_s": round(atomics_cam / (kt["cam_raster"] * 1e-3), 1) if kt["cam_raster"] else None,
+            "cam_raster_atomics_per_s": round(atomics_cam / (kt["cam_raster"] * 1e-3), 1)
+            if kt["cam_raster"] and not cam_status else None,
             "dom_opacity_atomics_per_s": round(float(counts["dom2"][2]) / (kt["dom_opacity"] * 1e-3), 1)
             if kt["dom_opacity"] and not bin_status else None},
     }

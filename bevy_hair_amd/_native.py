@@ -104,7 +104,7 @@ MLP_NW, MLP_NBIAS = 11264, 208
 
 
 class HmTuning(C.Structure):
-    _fields_ = [("raster_flags", u32 * 3), ("raster_blocks_per_cu", u32 * 3), ("dom_tile", u32),
+    _fields_ = [("raster_flags", u32 * 3), ("raster_blocks_per_cu", u32 * 3), ("dom_tile", u32), ("cam_tile", u32),
                 ("bin_box_limit", u32), ("bin_records", u32), ("bin_blocks_per_cu", u32)]
 
 
@@ -130,10 +130,10 @@ BUF_CENTER, BUF_CONSERVATIVE, BUF_BETA, BUF_SHADED, BUF_SHADED_DEPTH = 0, 1, 2, 
 BUF_FILTERED, BUF_FILTERED_DEPTH, BUF_DOM, BUF_SCENE_DEPTH = 5, 6, 7, 8
 BUF_CAMERA_LOD, BUF_CAMERA_REFS, BUF_CAMERA_INDIRECT = 16, 17, 18
 BUF_DOM_LOD, BUF_DOM_REFS, BUF_DOM_INDIRECT = 19, 20, 21
-BUF_DOM_BIN_STATUS = 24
+BUF_CAM_BIN_STATUS, BUF_DOM_BIN_STATUS = 24, 25
 PASS_CAMERA, PASS_DOM = 0, 1
 KERNELS = ["clear", "cam_lod", "cam_scan", "cam_raster", "clear_dom", "dom_lod", "dom_scan",
-           "dom_depth", "dom_opacity", "shade", "filter", "dom_bin", "dom_tiles"]
+           "dom_depth", "dom_opacity", "shade", "filter", "dom_bin", "dom_tiles", "cam_bin", "cam_tiles"]
 
 # The exported C ABI: name -> (restype, argtypes). Every symbol include/hair_mi.h declares.
 P = C.c_void_p
@@ -160,6 +160,7 @@ ABI = {
     "hm_raster": (C.c_int, [P, P, C.POINTER(HmParams), C.c_int]),
     "hm_shade": (C.c_int, [P, C.POINTER(HmParams), f32]),
     "hm_filter": (C.c_int, [P, C.POINTER(HmParams)]),
+    "hm_gbuffer": (C.c_int, [P, P, C.POINTER(HmParams)]),
     "hm_dom": (C.c_int, [P, P, C.POINTER(HmParams), C.POINTER(HmParams)]),
     "hm_raster_count": (C.c_int, [P, P, C.POINTER(HmParams), C.c_int, C.c_uint64 * 4]),
     "hm_frame": (C.c_int, [P, P, C.POINTER(HmParams), C.POINTER(HmFrameOpts)]),

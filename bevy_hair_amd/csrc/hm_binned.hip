@@ -74,6 +74,38 @@ __device__ __forceinline__ uint32_t segment_box(const View& p, F2 a, F2 b, uint3
     return tx0 | (ty0 << 10) | ((bw - 1u) << 20) | ((bh - 1u) << 26);
 }
 
+// Pixels the camera DDA of Drawer::segment_walk can write between projected
+// end points a and b, clipped to the screen, as a tile box. Samples are
+// floor(mix(a, b, t)) for t in [0, 1] (plus the two 4-connected pixels
+// formed from consecutive samples); below 2^21 in magnitude the rounding of
+// mix moves a sample by less than half a pixel, so one pixel of margin covers
+// every sample. Larger or non-finite coordinates are drawn by the long path.
+__device__ __forceinline__ uint32_t camera_box(const View& p, F2 a, F2 b, uint32_t shift, uint32_t box_limit,
+                                               bool& is_long) {
+    is_long = false;
+    const float m = mx(mx(fabsf(a.x), fabsf(a.y)), mx(fabsf(b.x), fabsf(b.y)));
+    if (!(m < 2097152.0f)) {
+        is_long = true;
+        return BIN_NONE;
+    }
+    int64_t lx = (int64_t)floorf(mn(a.x, b.x)) - 1, hx = (int64_t)floorf(mx(a.x, b.x)) + 1;
+    int64_t ly = (int64_t)floorf(mn(a.y, b.y)) - 1, hy = (int64_t)floorf(mx(a.y, b.y)) + 1;
+    const int64_t w = to_i32(p.screen.x), h = to_i32(p.screen.y);
+    lx = mx64(lx, 0);
+    ly = mx64(ly, 0);
+    hx = mn64(hx, w - 1);
+    hy = mn64(hy, h - 1);
+    if (lx > hx || ly > hy) return BIN_NONE;
+    uint32_t tx0 = (uint32_t)lx >> shift, tx1 = (uint32_t)hx >> shift;
+    uint32_t ty0 = (uint32_t)ly >> shift, ty1 = (uint32_t)hy >> shift;
+    uint32_t bw = tx1 - tx0 + 1u, bh = ty1 - ty0 + 1u;
+    if (bw * bh > box_limit) {
+        is_long = true;
+        return BIN_NONE;
+    }
+    return tx0 | (ty0 << 10) | ((bw - 1u) << 20) | ((bh - 1u) << 26);
+}
+
 // Calls f(tile) for every tile of a box, row by row.
 template <class F>
 __device__ __forceinline__ void for_box_tiles(uint32_t box, uint32_t tiles_x, F f) {
@@ -115,6 +147,9 @@ __device__ __forceinline__ Run workgroup_run(uint32_t n) {
 
 }  // namespace
 
+// MODE 0: camera (also stores world tangents, ao and per-strand uv / beta /
+// control-point count for the payload); MODE 1: deep opacity map.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
     const View& p = bin_view(a);
     const int lane = (int)(threadIdx.x & 63u);
@@ -136,6 +171,18 @@ __global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
             a.cps[base + lane + 64] = make_float4(sc.cw[1].x, sc.cw[1].y, sc.cw[1].z, pr1.screen.x);
             a.cpy[base + lane + 64] = pr1.screen.y;
         }
+        if (MODE == 0) {
+            // raster.wgsl:319-337: world tangents, ao; uv and beta per strand.
+            if ((uint32_t)lane < ncp) {
+                F3 wt = dir_world(p, sc.ct[0]);
+                a.cpt[base + lane] = make_float4(wt.x, wt.y, wt.z, sc.ao[0]);
+            }
+            if ((uint32_t)lane + 64u < ncp) {
+                F3 wt = dir_world(p, sc.ct[1]);
+                a.cpt[base + lane + 64] = make_float4(wt.x, wt.y, wt.z, sc.ao[1]);
+            }
+            if (lane == 0) a.sinfo[s] = make_float4(sc.uv.x, sc.uv.y, sc.beta, __uint_as_float(ncp));
+        }
         // Segment idx -> idx + 1: the end point's projection lives in the next lane.
         const int nl = (lane + 1) & 63;
         F2 s0n = {__shfl(pr0.screen.x, nl), __shfl(pr0.screen.y, nl)};
@@ -147,9 +194,11 @@ __global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
         bool long0 = false, long1 = false;
         uint32_t box0 = BIN_NONE, box1 = BIN_NONE;
         if ((uint32_t)lane + 1u < ncp && (pr0.in_front || ef0))
-            box0 = segment_box(p, pr0.screen, e0, a.tile_shift, a.box_limit, long0);
+            box0 = MODE == 0 ? camera_box(p, pr0.screen, e0, a.tile_shift, a.box_limit, long0)
+                             : segment_box(p, pr0.screen, e0, a.tile_shift, a.box_limit, long0);
         if ((uint32_t)lane + 65u < ncp && (pr1.in_front || f1n))
-            box1 = segment_box(p, pr1.screen, s1n, a.tile_shift, a.box_limit, long1);
+            box1 = MODE == 0 ? camera_box(p, pr1.screen, s1n, a.tile_shift, a.box_limit, long1)
+                             : segment_box(p, pr1.screen, s1n, a.tile_shift, a.box_limit, long1);
         // Every slot of the strand is written, so the binning passes can walk
         // segment ids without the strand's control-point count.
         a.seg_box[id0] = box0;
@@ -368,11 +417,12 @@ __global__ __launch_bounds__(256) void k_dom_long(BinArgs a) {
         walk_record(d, a, a.long_list[i]);
 }
 
-hipError_t launch_bin(hipStream_t s, const BinArgs& a, int gen_blocks, int bin_blocks) {
+hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks) {
     if (a.tiles > BIN_MAX_TILES) return hipErrorInvalidValue;
     hipError_t e = launch_fill32(s, a.meta, BIN_META_WORDS + a.tiles, 0u, 64);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_gen, dim3(gen_blocks), dim3(256), 0, s, a);
+    if (mode == 0) hipLaunchKernelGGL(k_bin_gen<0>, dim3(gen_blocks), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_bin_gen<1>, dim3(gen_blocks), dim3(256), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const size_t lds = (size_t)a.tiles * sizeof(uint32_t);
     hipLaunchKernelGGL(k_bin_count, dim3(bin_blocks), dim3(256), lds, s, a);
@@ -399,6 +449,130 @@ hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks) {
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_dom_long<2>, dim3(long_blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---- camera G-buffer ----------------------------------------------------------
+
+// LDS tile of the G-buffer: centre and conservative words, beta words.
+template <int T>
+struct CamTileSink {
+    uint64_t* center;
+    uint64_t* cons;
+    uint32_t* beta;
+    int32_t ox, oy;
+    __device__ __forceinline__ bool accepts(int32_t x, int32_t y) const {
+        return (uint32_t)(x - ox) < (uint32_t)T && (uint32_t)(y - oy) < (uint32_t)T;
+    }
+    __device__ __forceinline__ bool walked_out(int32_t, int32_t, int32_t, int32_t) const { return false; }
+    __device__ __forceinline__ void cam(size_t, int32_t x, int32_t y, uint64_t payload, bool center_hit,
+                                        uint32_t beta_word, Counters&) {
+        const uint32_t i = (uint32_t)((y - oy) * T + (x - ox));
+        atomicMin((unsigned long long*)(cons + i), (unsigned long long)payload);
+        if (center_hit) {
+            atomicMin((unsigned long long*)(center + i), (unsigned long long)payload);
+            atomicMin(beta + i, beta_word);
+        }
+    }
+};
+
+// One camera segment from the stored control points (raster.wgsl:319-337 with
+// the end points' projection, tangents, ao and the strand's uv and beta).
+template <class D>
+__device__ __forceinline__ void walk_cam_record(D& d, const BinArgs& a, uint32_t id) {
+    const float4 c0 = a.cps[id], c1 = a.cps[id + 1u];
+    const float y0 = a.cpy[id], y1 = a.cpy[id + 1u];
+    const float4 t0 = a.cpt[id], t1 = a.cpt[id + 1u];
+    const float4 si = a.sinfo[id / BIN_STRIDE];
+    const uint32_t ncp = __float_as_uint(si.w);
+    const uint32_t idx = id & (BIN_STRIDE - 1u);
+    const float den = u2f(ncp - 1u > 1u ? ncp - 1u : 1u);
+    const float w0 = rdiv(u2f(idx), den), w1 = rdiv(u2f(idx + 1u), den);
+    d.segment_walk(F3{c0.x, c0.y, c0.z}, F3{c1.x, c1.y, c1.z}, F2{c0.w, y0}, F2{c1.w, y1}, F3{t0.x, t0.y, t0.z},
+                   F3{t1.x, t1.y, t1.z}, F3{si.x, si.y, w0}, F3{si.x, si.y, w1}, t0.w, t1.w, si.z);
+}
+
+// One workgroup per screen tile: the tile starts empty (this is the clear),
+// takes every segment binned to it with LDS atomics and is written once.
+template <int T>
+__global__ __launch_bounds__(256) void k_cam_tiles(BinArgs a) {
+    static_assert((T * T) % 256 == 0, "the write-back loop runs whole waves");
+    if (a.meta[BIN_OK] == 0u) return;
+    __shared__ uint64_t center[T * T], cons[T * T];
+    __shared__ uint32_t beta[T * T];
+    const View& p = bin_view(a);
+    const uint32_t tile = blockIdx.x;
+    const int32_t ox = (int32_t)(tile % a.tiles_x) * T, oy = (int32_t)(tile / a.tiles_x) * T;
+    const int32_t W = (int32_t)a.width, H = (int32_t)a.height;
+    for (int i = (int)threadIdx.x; i < T * T; i += 256) {
+        center[i] = ~0ull;
+        cons[i] = ~0ull;
+        beta[i] = ~0u;
+    }
+    __syncthreads();
+    const uint32_t* off = a.meta + BIN_META_WORDS + a.tiles;
+    const uint32_t begin = off[tile], end = off[tile + 1u];
+    CamTileSink<T> sink{center, cons, beta, ox, oy};
+    Drawer<0, CamTileSink<T>> d{p, a.scene_depth, a.scene_clear, sink, {0, 0, 0, 0}};
+    for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) walk_cam_record(d, a, a.records[r]);
+    __syncthreads();
+    const int lane = (int)(threadIdx.x & 63u);
+    for (int i = (int)threadIdx.x; i < T * T; i += 256) {  // T * T is a multiple of 256: whole waves
+        const int32_t x = ox + i % T, y = oy + i / T;
+        const bool in = x < W && y < H;
+        const size_t g = in ? (size_t)y * (size_t)W + (size_t)x : 0u;
+        const uint64_t c = center[i], k = cons[i];
+        if (in) {
+            a.center[g] = c;
+            a.cons[g] = k;
+            a.beta[g] = beta[i];
+        }
+        if (!a.emit_lists) continue;
+        const bool shade = in && !gb_empty(c);
+        const bool filter = in && (shade || !gb_empty(k));
+        if (in && !shade) {
+            a.shaded[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            a.shaded_depth[g] = 0.0f;
+        }
+        if (in && !filter) {
+            a.filtered[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            a.filtered_depth[g] = 0.0f;
+        }
+        const uint64_t ms = __ballot(shade), mf = __ballot(filter);
+        uint32_t bs = 0u, bf = 0u;
+        if (lane == 0) {
+            if (ms) bs = atomicAdd(a.meta + BIN_SHADE_COUNT, (uint32_t)__popcll(ms));
+            if (mf) bf = atomicAdd(a.meta + BIN_FILTER_COUNT, (uint32_t)__popcll(mf));
+        }
+        bs = __shfl(bs, 0);
+        bf = __shfl(bf, 0);
+        if (shade) a.shade_list[bs + lanes_below(ms)] = (uint32_t)g;
+        if (filter) a.filter_list[bf + lanes_below(mf)] = (uint32_t)g;
+    }
+}
+
+// Long segments: global atomics on top of the finished tiles.
+__global__ __launch_bounds__(256) void k_cam_long(BinArgs a) {
+    if (a.meta[BIN_LONG] == 0u) return;
+    const View& p = bin_view(a);
+    const uint32_t n = a.meta[BIN_LONG_COUNT];
+    GlobalSink sink{a.center, a.cons, a.beta, nullptr, 0, true};
+    Drawer<0, GlobalSink> d{p, a.scene_depth, a.scene_clear, sink, {0, 0, 0, 0}};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        walk_cam_record(d, a, a.long_list[i]);
+}
+
+hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int long_blocks) {
+    if (a.tile_shift == 4u) {
+        hipLaunchKernelGGL(k_cam_tiles<16>, dim3(a.tiles), dim3(256), 0, s, a);
+    } else if (a.tile_shift == 5u) {
+        hipLaunchKernelGGL(k_cam_tiles<32>, dim3(a.tiles), dim3(256), 0, s, a);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cam_long, dim3(long_blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

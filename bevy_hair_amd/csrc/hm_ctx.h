@@ -23,11 +23,13 @@ struct hm_groom {
 // Work buffers of the binned DOM (sized for the largest groom and map seen).
 struct hm_bin_state {
     float4* cps = nullptr;          // [slot * 128 + idx]: world xyz, light-space x
-    float* cpy = nullptr;           // light-space y
+    float* cpy = nullptr;           // light-space (or screen) y
+    float4* cpt = nullptr;          // camera: world tangent xyz, ao
+    float4* sinfo = nullptr;        // camera: per strand uv, beta, control points
     uint32_t* seg_box = nullptr;    // per segment id
     uint32_t* long_list = nullptr;
     size_t slots = 0;               // strand slots cps / seg_box / long_list hold
-    uint32_t* meta = nullptr;       // status words, per-tile count / offset / cursor
+    uint32_t* meta[2] = {nullptr, nullptr};  // per pass (camera, DOM): status words, per-tile count / offset / cursor
     size_t meta_words = 0;
     uint32_t* records = nullptr;
     uint2* lists = nullptr;         // same length as records
@@ -52,6 +54,7 @@ struct hm_ctx {
     float* filtered_depth = nullptr;
     uint32_t* dom = nullptr;
     uint32_t* pix_list = nullptr;   // covered-pixel list (shade, then filter)
+    uint32_t* pix_list2 = nullptr;  // filter list when the camera tiles write both lists
     uint32_t* pix_count = nullptr;
     float* scene_depth = nullptr;
     float scene_clear = 0.0f;
@@ -60,7 +63,7 @@ struct hm_ctx {
     hm_directional_light* dirs = nullptr;
     uint32_t dir_count = 0, dir_cap = 0;
     uint64_t* counters = nullptr;
-    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 8u, 0u, 2u};
+    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u};
     hm_bin_state bin;
     // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
     uint32_t slots = 0, frame = 0;

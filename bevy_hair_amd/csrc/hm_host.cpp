@@ -276,6 +276,7 @@ void free_targets(hm_ctx* c) {
     dev_free(c->filtered_depth);
     dev_free(c->dom);
     dev_free(c->pix_list);
+    dev_free(c->pix_list2);
     dev_free(c->pix_count);
     dev_free(c->scene_depth);
     c->width = c->height = c->dom_size = 0;
@@ -285,9 +286,12 @@ void free_bin(hm_ctx* c) {
     hm_bin_state& b = c->bin;
     dev_free(b.cps);
     dev_free(b.cpy);
+    dev_free(b.cpt);
+    dev_free(b.sinfo);
     dev_free(b.seg_box);
     dev_free(b.long_list);
-    dev_free(b.meta);
+    dev_free(b.meta[0]);
+    dev_free(b.meta[1]);
     dev_free(b.records);
     dev_free(b.lists);
     dev_free(b.wg_list);
@@ -305,20 +309,26 @@ hipError_t ensure_bin(hm_ctx* c, size_t slots, size_t meta_words, size_t records
     if (slots > b.slots) {
         dev_free(b.cps);
         dev_free(b.cpy);
+        dev_free(b.cpt);
+        dev_free(b.sinfo);
         dev_free(b.seg_box);
         dev_free(b.long_list);
         b.slots = 0;
         size_t segs = slots * hm::BIN_STRIDE;
         if ((e = dev_alloc(&b.cps, segs * sizeof(float4))) != hipSuccess) return e;
         if ((e = dev_alloc(&b.cpy, segs * sizeof(float))) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.cpt, segs * sizeof(float4))) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.sinfo, slots * sizeof(float4))) != hipSuccess) return e;
         if ((e = dev_alloc(&b.seg_box, segs * 4)) != hipSuccess) return e;
         if ((e = dev_alloc(&b.long_list, segs * 4)) != hipSuccess) return e;
         b.slots = slots;
     }
     if (meta_words > b.meta_words) {
-        dev_free(b.meta);
+        dev_free(b.meta[0]);
+        dev_free(b.meta[1]);
         b.meta_words = 0;
-        if ((e = dev_alloc(&b.meta, meta_words * 4)) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.meta[0], meta_words * 4)) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.meta[1], meta_words * 4)) != hipSuccess) return e;
         b.meta_words = meta_words;
     }
     if (records > b.record_capacity) {
@@ -549,6 +559,7 @@ hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_
     if (e == hipSuccess) e = dev_alloc(&ctx->filtered_depth, px * 4);
     if (e == hipSuccess) e = dev_alloc(&ctx->dom, dom_words * 4);
     if (e == hipSuccess) e = dev_alloc(&ctx->pix_list, px * 4);
+    if (e == hipSuccess) e = dev_alloc(&ctx->pix_list2, px * 4);
     if (e == hipSuccess) e = dev_alloc(&ctx->pix_count, 64);
     if (e == hipSuccess) e = hipMemset(ctx->dom, 0, dom_words * 4);
     if (e != hipSuccess) {
@@ -785,6 +796,106 @@ hm_status hm_raster_count(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass
     return HM_OK;
 }
 
+// Binning work buffers and arguments for one pass over a tiles_x x tiles_y grid.
+static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass, uint32_t T, uint32_t tiles_x,
+                           uint32_t tiles_y, hm::BinArgs& b) {
+    const uint32_t tiles = tiles_x * tiles_y;
+    const size_t slots = g->capacity ? g->capacity : 1u;
+    const size_t want_records = ctx->tuning.bin_records ? ctx->tuning.bin_records : slots * hm::BIN_STRIDE * 2u;
+    const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
+    hipError_t e =
+        ensure_bin(ctx, slots, hm::BIN_META_WORDS + 3u * (size_t)tiles + 1u, want_records, (size_t)bin_blocks);
+    if (e != hipSuccess) return e;
+    std::memset(&b, 0, sizeof(b));
+    std::memcpy(&b.view, p, sizeof(hm_params));
+    b.bundles = g->bundles;
+    b.layers = g->layers;
+    b.style = g->style;
+    b.refs = g->refs[pass];
+    b.indirect = g->indirect[pass];
+    b.capacity = g->capacity;
+    b.cps = ctx->bin.cps;
+    b.cpy = ctx->bin.cpy;
+    b.seg_box = ctx->bin.seg_box;
+    b.meta = ctx->bin.meta[pass];
+    b.records = ctx->bin.records;
+    b.record_capacity = (uint32_t)std::min<size_t>(want_records, 0xffffffffu);
+    b.lists = ctx->bin.lists;
+    b.list_capacity = b.record_capacity;
+    b.wg_list = ctx->bin.wg_list;
+    b.long_list = ctx->bin.long_list;
+    b.long_capacity = (uint32_t)std::min<size_t>(slots * hm::BIN_STRIDE, 0xffffffffu);
+    b.tiles_x = tiles_x;
+    b.tiles = tiles;
+    b.tile_shift = T == 16u ? 4u : 5u;
+    b.box_limit = ctx->tuning.bin_box_limit;
+    b.dom_size = ctx->dom_size;
+    b.dom = ctx->dom;
+    b.cpt = ctx->bin.cpt;
+    b.sinfo = ctx->bin.sinfo;
+    b.width = ctx->width;
+    b.height = ctx->height;
+    b.center = ctx->center;
+    b.cons = ctx->cons;
+    b.beta = ctx->beta;
+    b.scene_depth = ctx->scene_depth;
+    b.scene_clear = ctx->scene_clear;
+    return hipSuccess;
+}
+
+// hm_gbuffer; with emit_lists (a frame's own call) the camera tiles also
+// write the shade and filter pixel lists, and *lists_gate is set to the word
+// that is nonzero when they did not (the direct kernels ran), else null.
+static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool emit_lists,
+                              const uint32_t** lists_gate) {
+    if (lists_gate) *lists_gate = nullptr;
+    hm::RasterArgs a;
+    hm_status st = raster_args(ctx, g, p, HM_PASS_CAMERA, a);
+    if (st != HM_OK) return st;
+    const uint32_t T = ctx->tuning.cam_tile;
+    const uint32_t tiles_x = T ? (ctx->width + T - 1u) / T : 0u, tiles_y = T ? (ctx->height + T - 1u) / T : 0u;
+    const bool binned = T != 0u && tiles_x * tiles_y <= hm::BIN_MAX_TILES &&
+                        (uint64_t)g->capacity * hm::BIN_STRIDE <= 0xffffffffull;
+    if (!binned) {
+        if ((st = hm_clear(ctx)) != HM_OK) return st;
+        return hm_raster(ctx, g, p, HM_PASS_CAMERA);
+    }
+    hm::BinArgs b;
+    HM_HIP(ctx, bin_args(ctx, g, p, HM_PASS_CAMERA, T, tiles_x, tiles_y, b));
+    const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
+    b.emit_lists = emit_lists ? 1u : 0u;
+    b.shade_list = ctx->pix_list;
+    b.filter_list = ctx->pix_list2;
+    b.shaded = ctx->shaded;
+    b.shaded_depth = ctx->shaded_depth;
+    b.filtered = ctx->filtered;
+    b.filtered_depth = ctx->filtered_depth;
+    {
+        Timer t(ctx, HM_K_CAM_BIN);
+        HM_HIP(ctx, hm::launch_bin(ctx->stream, b, 0, raster_blocks(ctx, g->capacity, 0u), bin_blocks));
+    }
+    {
+        Timer t(ctx, HM_K_CAM_TILES);
+        HM_HIP(ctx, hm::launch_cam_tiles(ctx->stream, b, ctx->cu_count));
+    }
+    // Direct kernels, switched on by the device only when the records overflowed.
+    const uint32_t* fallback = ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_FALLBACK;
+    if (emit_lists && lists_gate) *lists_gate = fallback;
+    {
+        Timer t(ctx, HM_K_CLEAR);
+        HM_HIP(ctx, hm::launch_fill32(ctx->stream, static_cast<uint32_t*>(ctx->gb_slab), ctx->gb_words, 0xffffffffu,
+                                      ctx->cu_count * 8, fallback));
+    }
+    a.gate = fallback;
+    {
+        Timer t(ctx, HM_K_CAM_RASTER);
+        HM_HIP(ctx, hm::launch_raster(ctx->stream, a, 0, false, raster_blocks(ctx, g->capacity, 0u)));
+    }
+    return HM_OK;
+}
+
+hm_status hm_gbuffer(hm_ctx* ctx, hm_groom* g, const hm_params* p) { return gbuffer_impl(ctx, g, p, false, nullptr); }
+
 hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params* op) {
     hm::RasterArgs a1, a2;
     hm_status st = raster_args(ctx, g, dp, HM_PASS_DOM, a1);
@@ -806,46 +917,19 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
         if ((st = hm_raster(ctx, g, dp, HM_PASS_DOM)) != HM_OK) return st;
         return hm_raster(ctx, g, op, HM_PASS_DOM);
     }
-    const uint32_t tiles = tiles_x * tiles_x;
-    const size_t slots = g->capacity ? g->capacity : 1u;
-    const size_t want_records = ctx->tuning.bin_records ? ctx->tuning.bin_records : slots * hm::BIN_STRIDE * 2u;
-    const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
-    HM_HIP(ctx, ensure_bin(ctx, slots, hm::BIN_META_WORDS + 3u * (size_t)tiles + 1u, want_records, (size_t)bin_blocks));
     hm::BinArgs b;
-    std::memcpy(&b.view, dp, sizeof(hm_params));
-    b.bundles = g->bundles;
-    b.layers = g->layers;
-    b.style = g->style;
-    b.refs = g->refs[HM_PASS_DOM];
-    b.indirect = g->indirect[HM_PASS_DOM];
-    b.capacity = g->capacity;
-    b.cps = ctx->bin.cps;
-    b.cpy = ctx->bin.cpy;
-    b.seg_box = ctx->bin.seg_box;
-    b.meta = ctx->bin.meta;
-    b.records = ctx->bin.records;
-    b.record_capacity = (uint32_t)std::min<size_t>(want_records, 0xffffffffu);
-    b.lists = ctx->bin.lists;
-    b.list_capacity = b.record_capacity;
-    b.wg_list = ctx->bin.wg_list;
-    b.long_list = ctx->bin.long_list;
-    b.long_capacity = (uint32_t)std::min<size_t>(slots * hm::BIN_STRIDE, 0xffffffffu);
-    b.tiles_x = tiles_x;
-    b.tiles = tiles;
-    b.tile_shift = T == 16u ? 4u : 5u;
-    b.box_limit = ctx->tuning.bin_box_limit;
-    b.dom_size = S;
-    b.dom = ctx->dom;
+    HM_HIP(ctx, bin_args(ctx, g, dp, HM_PASS_DOM, T, tiles_x, tiles_x, b));
+    const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
     {
         Timer t(ctx, HM_K_DOM_BIN);
-        HM_HIP(ctx, hm::launch_bin(ctx->stream, b, raster_blocks(ctx, g->capacity, 1u), bin_blocks));
+        HM_HIP(ctx, hm::launch_bin(ctx->stream, b, 1, raster_blocks(ctx, g->capacity, 1u), bin_blocks));
     }
     {
         Timer t(ctx, HM_K_DOM_TILES);
         HM_HIP(ctx, hm::launch_dom_tiles(ctx->stream, b, ctx->cu_count));
     }
     // Direct kernels, switched on by the device only when the records overflowed.
-    const uint32_t* fallback = ctx->bin.meta + hm::BIN_FALLBACK;
+    const uint32_t* fallback = ctx->bin.meta[HM_PASS_DOM] + hm::BIN_FALLBACK;
     const size_t texels = (size_t)S * S;
     {
         Timer t(ctx, HM_K_CLEAR_DOM);
@@ -866,7 +950,9 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
     return HM_OK;
 }
 
-hm_status hm_shade(hm_ctx* ctx, const hm_params* p, float exposure) {
+// Shading over the covered-pixel list. With lists_gate (see gbuffer_impl) the
+// list is the camera tiles' and the compaction runs only if that word is set.
+static hm_status shade_impl(hm_ctx* ctx, const hm_params* p, float exposure, const uint32_t* lists_gate) {
     hm_status st = check_targets(ctx);
     if (st != HM_OK) return st;
     if (!p || (uint32_t)p->screen[0] != ctx->width || (uint32_t)p->screen[1] != ctx->height)
@@ -887,13 +973,16 @@ hm_status hm_shade(hm_ctx* ctx, const hm_params* p, float exposure) {
     a.shaded_depth = ctx->shaded_depth;
     Timer t(ctx, HM_K_SHADE);
     uint32_t px = ctx->width * ctx->height;
-    HM_HIP(ctx, hm::launch_compact(ctx->stream, 0, ctx->center, ctx->cons, px, ctx->pix_list, ctx->pix_count,
-                                   ctx->shaded, ctx->shaded_depth));
-    HM_HIP(ctx, hm::launch_shade(ctx->stream, a, ctx->pix_list, ctx->pix_count, px, ctx->cu_count * 8));
+    uint32_t* count = lists_gate ? ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_SHADE_COUNT : ctx->pix_count;
+    HM_HIP(ctx, hm::launch_compact(ctx->stream, 0, ctx->center, ctx->cons, px, ctx->pix_list, count, ctx->shaded,
+                                   ctx->shaded_depth, lists_gate));
+    HM_HIP(ctx, hm::launch_shade(ctx->stream, a, ctx->pix_list, count, px, ctx->cu_count * 8));
     return HM_OK;
 }
 
-hm_status hm_filter(hm_ctx* ctx, const hm_params* p) {
+hm_status hm_shade(hm_ctx* ctx, const hm_params* p, float exposure) { return shade_impl(ctx, p, exposure, nullptr); }
+
+static hm_status filter_impl(hm_ctx* ctx, const hm_params* p, const uint32_t* lists_gate) {
     hm_status st = check_targets(ctx);
     if (st != HM_OK) return st;
     if (!p || (uint32_t)p->screen[0] != ctx->width || (uint32_t)p->screen[1] != ctx->height)
@@ -908,11 +997,15 @@ hm_status hm_filter(hm_ctx* ctx, const hm_params* p) {
     a.filtered_depth = ctx->filtered_depth;
     Timer t(ctx, HM_K_FILTER);
     uint32_t px = ctx->width * ctx->height;
-    HM_HIP(ctx, hm::launch_compact(ctx->stream, 1, ctx->center, ctx->cons, px, ctx->pix_list, ctx->pix_count,
-                                   ctx->filtered, ctx->filtered_depth));
-    HM_HIP(ctx, hm::launch_filter(ctx->stream, a, ctx->pix_list, ctx->pix_count, px, ctx->cu_count * 8));
+    uint32_t* list = lists_gate ? ctx->pix_list2 : ctx->pix_list;
+    uint32_t* count = lists_gate ? ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_FILTER_COUNT : ctx->pix_count;
+    HM_HIP(ctx, hm::launch_compact(ctx->stream, 1, ctx->center, ctx->cons, px, list, count, ctx->filtered,
+                                   ctx->filtered_depth, lists_gate));
+    HM_HIP(ctx, hm::launch_filter(ctx->stream, a, list, count, px, ctx->cu_count * 8));
     return HM_OK;
 }
+
+hm_status hm_filter(hm_ctx* ctx, const hm_params* p) { return filter_impl(ctx, p, nullptr); }
 
 hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm_frame_opts* opts) {
     if (!ctx || !g || !params) return fail(ctx, HM_ERR_INVALID, "null argument");
@@ -926,10 +1019,10 @@ hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm
         }
     } slot{ctx};
     const hm_params& cam = params[0];
-    if ((st = hm_clear(ctx)) != HM_OK) return st;
     if ((st = hm_lod(ctx, g, &cam, HM_PASS_CAMERA, ovr)) != HM_OK) return st;
     if ((st = hm_scan(ctx, g, &cam, HM_PASS_CAMERA)) != HM_OK) return st;
-    if ((st = hm_raster(ctx, g, &cam, HM_PASS_CAMERA)) != HM_OK) return st;
+    const uint32_t* lists = nullptr;
+    if ((st = gbuffer_impl(ctx, g, &cam, true, &lists)) != HM_OK) return st;
     if (cam.flags[3]) {
         if ((st = hm_lod(ctx, g, &params[1], HM_PASS_DOM, nullptr)) != HM_OK) return st;
         if ((st = hm_scan(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
@@ -937,11 +1030,11 @@ hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm
     }
     if (opts && opts->shading == HM_SHADING_MLP) {
         if ((st = hm_shade_mlp(ctx, &cam, exposure, opts->mlp_tolerance, nullptr)) != HM_OK) return st;
-    } else if ((st = hm_shade(ctx, &cam, exposure)) != HM_OK) {
+    } else if ((st = shade_impl(ctx, &cam, exposure, lists)) != HM_OK) {
         return st;
     }
     if (cam.flags[1]) {
-        if ((st = hm_filter(ctx, &cam)) != HM_OK) return st;
+        if ((st = filter_impl(ctx, &cam, lists)) != HM_OK) return st;
     }
     return HM_OK;
 }
@@ -959,7 +1052,8 @@ static bool target_buffer(hm_ctx* ctx, hm_groom* g, hm_buffer which, void** ptr,
         case HM_BUF_FILTERED_DEPTH: *ptr = ctx->filtered_depth; *bytes = px * 4; return ctx->filtered_depth != nullptr;
         case HM_BUF_DOM: *ptr = ctx->dom; *bytes = dom; return ctx->dom != nullptr;
         case HM_BUF_SCENE_DEPTH: *ptr = ctx->scene_depth; *bytes = px * 4; return ctx->scene_depth != nullptr;
-        case HM_BUF_DOM_BIN_STATUS: *ptr = ctx->bin.meta; *bytes = hm::BIN_META_WORDS * 4; return ctx->bin.meta != nullptr;
+        case HM_BUF_CAM_BIN_STATUS: *ptr = ctx->bin.meta[0]; *bytes = hm::BIN_META_WORDS * 4; return ctx->bin.meta[0] != nullptr;
+        case HM_BUF_DOM_BIN_STATUS: *ptr = ctx->bin.meta[1]; *bytes = hm::BIN_META_WORDS * 4; return ctx->bin.meta[1] != nullptr;
         default: break;
     }
     if (!g) return false;
@@ -1053,6 +1147,8 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
     }
     if (t->dom_tile != 0u && t->dom_tile != 16u && t->dom_tile != 32u)
         return fail(ctx, HM_ERR_INVALID, "dom_tile must be 0, 16 or 32");
+    if (t->cam_tile != 0u && t->cam_tile != 16u && t->cam_tile != 32u)
+        return fail(ctx, HM_ERR_INVALID, "cam_tile must be 0, 16 or 32");
     if (t->bin_box_limit == 0u || t->bin_box_limit > 64u)
         return fail(ctx, HM_ERR_INVALID, "bin_box_limit must be in [1, 64]");
     if (t->bin_blocks_per_cu == 0u || t->bin_blocks_per_cu > 64u)
@@ -1109,6 +1205,29 @@ hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const
                     out.raster_flags[mode] = flags;
                     out.raster_blocks_per_cu[mode] = grids[gi];
                 }
+            }
+        }
+    }
+    // Camera: direct kernels against binned tiles of either size, whole hm_gbuffer timed.
+    {
+        const uint32_t tile_sizes[3] = {0u, 16u, 32u};
+        float best_cam = 1e30f;
+        for (uint32_t ti = 0; ti < 3; ++ti) {
+            ctx->tuning = out;
+            ctx->tuning.cam_tile = tile_sizes[ti];
+            float total = 0.0f;
+            for (uint32_t r = 0; r <= reps; ++r) {
+                HM_HIP(ctx, hipEventRecord(e0, ctx->stream));
+                if ((st = hm_gbuffer(ctx, g, &params[0])) != HM_OK) return st;
+                HM_HIP(ctx, hipEventRecord(e1, ctx->stream));
+                HM_HIP(ctx, hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) total += ms;
+            }
+            if (total < best_cam) {
+                best_cam = total;
+                out.cam_tile = tile_sizes[ti];
             }
         }
     }

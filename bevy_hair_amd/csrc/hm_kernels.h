@@ -67,6 +67,8 @@ constexpr uint32_t BIN_FALLBACK = 2u;    // direct kernels run
 constexpr uint32_t BIN_LONG = 3u;        // binned and long segments exist
 constexpr uint32_t BIN_LONG_COUNT = 4u;
 constexpr uint32_t BIN_LIST_COUNT = 5u;  // (tile, rank) entries the counting workgroups wrote
+constexpr uint32_t BIN_SHADE_COUNT = 6u;  // camera: pixels with a centre word (shade list)
+constexpr uint32_t BIN_FILTER_COUNT = 7u; // camera: pixels with a centre or conservative word (filter list)
 constexpr uint32_t BIN_META_WORDS = 8u;  // then count[tiles], offset[tiles + 1], cursor[tiles]
 constexpr uint32_t BIN_MAX_TILES = 16384u;  // one LDS word per tile in the binning workgroups
 
@@ -92,10 +94,30 @@ struct BinArgs {
     uint32_t tiles_x, tiles, tile_shift, box_limit;
     uint32_t dom_size;
     uint32_t* dom;                  // fronts S*S, then counts S*S*16
+    // camera pass
+    float4* cpt;                    // per control point: world tangent xyz, ao
+    float4* sinfo;                  // per strand slot: uv.x, uv.y, beta, control points (bits)
+    uint32_t width, height;
+    uint64_t* center;
+    uint64_t* cons;
+    uint32_t* beta;
+    const float* scene_depth;
+    float scene_clear;
+    // With emit_lists the camera tiles also write the shade and filter pixel
+    // lists (counts in meta) and the empty results of uncovered pixels, the
+    // work of k_compact<0> and k_compact<1>.
+    uint32_t emit_lists;
+    uint32_t* shade_list;
+    uint32_t* filter_list;
+    float4* shaded;
+    float* shaded_depth;
+    float4* filtered;
+    float* filtered_depth;
 };
 
-hipError_t launch_bin(hipStream_t s, const BinArgs& a, int gen_blocks, int bin_blocks);
+hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks);
 hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks);
+hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int long_blocks);
 
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes, int blocks);
 hipError_t launch_atomics(hipStream_t s, void* buf, size_t words, uint32_t threads, uint32_t per_thread, uint32_t kind);
@@ -107,7 +129,8 @@ hipError_t launch_scan(hipStream_t s, const hm_lod_record* lod, uint32_t nb, uin
                        uint32_t* indirect, hm_strand_ref* refs);
 hipError_t launch_raster(hipStream_t s, const RasterArgs& a, int mode, bool count, int blocks);
 hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const uint64_t* cons, uint32_t pixels,
-                          uint32_t* list, uint32_t* count, float4* color_out, float* depth_out);
+                          uint32_t* list, uint32_t* count, float4* color_out, float* depth_out,
+                          const uint32_t* gate = nullptr);
 hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, const uint32_t* list, const uint32_t* count,
                         uint32_t pixels, int blocks);
 hipError_t launch_filter(hipStream_t s, const FilterArgs& a, const uint32_t* list, const uint32_t* count,

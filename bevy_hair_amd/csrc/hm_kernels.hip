@@ -259,7 +259,8 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ center, const uint64_t* __restrict__ cons,
                                                  uint32_t pixels, uint32_t* __restrict__ list,
                                                  uint32_t* __restrict__ count, float4* __restrict__ color_out,
-                                                 float* __restrict__ depth_out) {
+                                                 float* __restrict__ depth_out, const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
     uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     bool inb = pix < pixels;
     bool hit = false;
@@ -449,12 +450,14 @@ hipError_t launch_raster(hipStream_t s, const RasterArgs& a, int mode, bool coun
 }
 
 hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const uint64_t* cons, uint32_t pixels,
-                          uint32_t* list, uint32_t* count, float4* color_out, float* depth_out) {
+                          uint32_t* list, uint32_t* count, float4* color_out, float* depth_out, const uint32_t* gate) {
     if (pixels == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u, (const uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_fill32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u, gate);
     dim3 g((pixels + 255) / 256), b(256);
-    if (mode == 0) hipLaunchKernelGGL(k_compact<0>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out);
-    else hipLaunchKernelGGL(k_compact<1>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_compact<0>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out, gate);
+    else
+        hipLaunchKernelGGL(k_compact<1>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out, gate);
     return hipGetLastError();
 }
 

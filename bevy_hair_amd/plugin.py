@@ -182,14 +182,22 @@ class Renderer:
     def raster(self, g: GpuGroom, params, pass_: int):
         self._check(N.lib().hm_raster(self.ctx, g.handle, C.byref(params), pass_), "hm_raster")
 
+    def gbuffer_pass(self, g: GpuGroom, params):
+        """clear + camera raster in one call (binned when tuned so)."""
+        self._check(N.lib().hm_gbuffer(self.ctx, g.handle, C.byref(params)), "hm_gbuffer")
+
     def dom(self, g: GpuGroom, depth_params, opacity_params):
         """clear_dom + DOM depth + DOM opacity in one call (binned when tuned so)."""
         self._check(N.lib().hm_dom(self.ctx, g.handle, C.byref(depth_params), C.byref(opacity_params)), "hm_dom")
 
-    def dom_bin_status(self) -> dict:
-        s = self.read(N.BUF_DOM_BIN_STATUS, np.uint32, 8)
+    def bin_status(self, pass_: int = 0) -> dict:
+        """Status words of the last binned hm_gbuffer (pass 0) or hm_dom (pass 1)."""
+        s = self.read(N.BUF_CAM_BIN_STATUS if pass_ == 0 else N.BUF_DOM_BIN_STATUS, np.uint32, 8)
         return {"records": int(s[0]), "binned": bool(s[1]), "fallback": bool(s[2]), "long": bool(s[3]),
-                "long_count": int(s[4])}
+                "long_count": int(s[4]), "shade_pixels": int(s[6]), "filter_pixels": int(s[7])}
+
+    def dom_bin_status(self) -> dict:
+        return self.bin_status(1)
 
     def raster_count(self, g: GpuGroom, params, pass_: int) -> np.ndarray:
         c = (C.c_uint64 * 4)()
@@ -270,7 +278,7 @@ class Renderer:
     @staticmethod
     def _tuning_dict(t: N.HmTuning) -> dict:
         return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": list(t.raster_blocks_per_cu),
-                "dom_tile": int(t.dom_tile), "bin_box_limit": int(t.bin_box_limit),
+                "dom_tile": int(t.dom_tile), "cam_tile": int(t.cam_tile), "bin_box_limit": int(t.bin_box_limit),
                 "bin_records": int(t.bin_records), "bin_blocks_per_cu": int(t.bin_blocks_per_cu)}
 
     def get_tuning(self) -> dict:
@@ -279,7 +287,7 @@ class Renderer:
         return self._tuning_dict(t)
 
     def set_tuning(self, raster_flags=None, raster_blocks_per_cu=None, dom_tile=None, bin_box_limit=None,
-                   bin_records=None, bin_blocks_per_cu=None):
+                   bin_records=None, bin_blocks_per_cu=None, cam_tile=None):
         """Changes the given launch choices; None keeps the current one."""
         cur = self.get_tuning()
         if raster_flags is not None:
@@ -287,12 +295,14 @@ class Renderer:
         if raster_blocks_per_cu is not None:
             b = raster_blocks_per_cu
             cur["raster_blocks_per_cu"] = [b] * 3 if isinstance(b, int) else list(b)
-        for k, v in (("dom_tile", dom_tile), ("bin_box_limit", bin_box_limit), ("bin_records", bin_records),
+        for k, v in (("dom_tile", dom_tile), ("cam_tile", cam_tile), ("bin_box_limit", bin_box_limit),
+                     ("bin_records", bin_records),
                      ("bin_blocks_per_cu", bin_blocks_per_cu)):
             if v is not None:
                 cur[k] = int(v)
         t = N.HmTuning((C.c_uint32 * 3)(*cur["raster_flags"]), (C.c_uint32 * 3)(*cur["raster_blocks_per_cu"]),
-                       cur["dom_tile"], cur["bin_box_limit"], cur["bin_records"], cur["bin_blocks_per_cu"])
+                       cur["dom_tile"], cur["cam_tile"], cur["bin_box_limit"], cur["bin_records"],
+                       cur["bin_blocks_per_cu"])
         self._check(N.lib().hm_set_tuning(self.ctx, C.byref(t)), "hm_set_tuning")
 
     def autotune(self, g: GpuGroom, params, opts, reps: int = 3) -> dict:

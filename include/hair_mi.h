@@ -193,8 +193,9 @@ typedef enum hm_buffer {
     HM_BUF_DOM_LOD = 19,
     HM_BUF_DOM_REFS = 20,
     HM_BUF_DOM_INDIRECT = 21,
-    HM_BUF_DOM_BIN_STATUS = 24  /* u32[8] of the last hm_dom: records, binned, fallback,
-                                   long segments present, long segment count */
+    HM_BUF_CAM_BIN_STATUS = 24, /* u32[8] of the last binned hm_gbuffer: records, binned, fallback,
+                                   long segments present, long segment count, -, shade / filter list lengths */
+    HM_BUF_DOM_BIN_STATUS = 25  /* u32[8] of the last binned hm_dom, same layout */
 } hm_buffer;
 
 /* Kernel ids for hm_timing_read. */
@@ -212,7 +213,9 @@ typedef enum hm_kernel {
     HM_K_FILTER = 10,
     HM_K_DOM_BIN = 11,          /* binned DOM: control points, tile counts, scan, records */
     HM_K_DOM_TILES = 12,        /* binned DOM: both passes per tile (+ long segments) */
-    HM_K_COUNT = 13
+    HM_K_CAM_BIN = 13,          /* binned camera pass: control points, tile counts, scan, records */
+    HM_K_CAM_TILES = 14,        /* binned camera pass: per tile G-buffer (+ long segments) */
+    HM_K_COUNT = 15
 } hm_kernel;
 
 #define HM_SHADING_PLAIN 0u
@@ -284,6 +287,16 @@ hm_status hm_raster(hm_ctx* ctx, hm_groom* groom, const hm_params* params, int p
 hm_status hm_shade(hm_ctx* ctx, const hm_params* params, float exposure); /* shade.wesl:fs */
 hm_status hm_filter(hm_ctx* ctx, const hm_params* params);              /* filter.wgsl */
 
+/* The G-buffer of one groom in one call: clear (clear.wgsl) and the camera
+ * raster (raster.wgsl, pass_mode.x = 0) over the camera work list that
+ * hm_lod/hm_scan built (pass.rs:379-520 after the lod and scan dispatches).
+ * The result equals hm_clear + hm_raster bit for bit. With hm_tuning.cam_tile
+ * set and at most 16384 tiles it runs binned: one workgroup per screen tile
+ * rasterizes into LDS and writes the tile once; when the tile records do not
+ * fit their buffer the direct kernels run instead, decided on the device
+ * (HM_BUF_CAM_BIN_STATUS says which ran). Asynchronous. */
+hm_status hm_gbuffer(hm_ctx* ctx, hm_groom* groom, const hm_params* camera_params);
+
 /* The deep opacity map of one groom in one call: clear_dom, the depth pass
  * (`depth_params`, pass_mode.x = 1) and the opacity pass (`opacity_params`,
  * pass_mode.x = 2) over the DOM work list that hm_lod/hm_scan built
@@ -306,7 +319,7 @@ hm_status hm_raster_count(hm_ctx* ctx, hm_groom* groom, const hm_params* params,
 
 /* ---- frame driver ------------------------------------------------------- */
 
-/* One groom of hair_pass (pass.rs:379-619): clear, camera lod/scan/raster,
+/* One groom of hair_pass (pass.rs:379-619): camera lod/scan, hm_gbuffer,
  * [DOM lod/scan, hm_dom if flags.w], shade, [filter if flags.y].
  * `params` are hm_make_params's three records. Asynchronous on the stream. */
 hm_status hm_frame(hm_ctx* ctx, hm_groom* groom, const hm_params params[3],
@@ -345,6 +358,7 @@ typedef struct hm_tuning {
     uint32_t raster_flags[3];      /* per pass mode (camera, DOM depth, DOM opacity) */
     uint32_t raster_blocks_per_cu[3]; /* persistent raster grid per pass, 256-thread workgroups per CU */
     uint32_t dom_tile;             /* hm_dom: 0 = direct kernels, 16 or 32 = binned, texels per tile side */
+    uint32_t cam_tile;             /* hm_gbuffer: 0 = direct kernels, 16 or 32 = binned, pixels per tile side */
     uint32_t bin_box_limit;        /* tiles a segment's box may span before it is drawn with
                                       global atomics instead (1..64) */
     uint32_t bin_records;          /* tile record buffer length; 0 = 2 per segment slot */

@@ -242,3 +242,74 @@ def test_binned_dom_bit_exact(renderer, case, variant):
     finally:
         r.set_tuning(**keep)
         g.free()
+
+
+_GB_REF = {}
+
+
+def oracle_gbuffer(name, scene_depth=None):
+    """The oracle's G-buffer after the camera pass over its own camera work list."""
+    key = (name, scene_depth is not None)
+    if key not in _GB_REF:
+        sc = make_scene(name)
+        ref = ref_frame(sc)
+        cap = max(1, sc.mesh.strand_count)
+        lod = ref.lod(sc.params[0], sc.w.control_points, sc.w.strand_fraction)
+        refs, _ = ref.scan(sc.params[0], lod, cap)
+        ref.clear()
+        if scene_depth is not None:
+            ref.scene_depth = scene_depth
+        counts = ref.raster(sc.params[0], refs)
+        _GB_REF[key] = (ref.center.copy(), ref.cons.copy(), ref.beta.copy(), counts)
+    return _GB_REF[key]
+
+
+def depth_checker(w):
+    """A scene depth of 8x8 blocks that alternately keep every hair fragment
+    (0: reverse-Z keeps clip >= scene) and reject every one (1: hair clip depth
+    near / distance is below 1)."""
+    y, x = np.mgrid[0:w.height, 0:w.width]
+    return np.ascontiguousarray(np.where((x // 8 + y // 8) % 2 == 0, 0.0, 1.0).astype(np.float32).ravel())
+
+
+# (tile, box limit, record buffer, binning workgroups per CU, scene depth)
+CAM_VARIANTS = [(16, 8, 0, 2, False), (32, 8, 0, 4, False), (16, 1, 0, 2, False), (32, 64, 0, 1, False),
+                (16, 8, 0, 2, True), (16, 8, 1, 2, False)]
+
+
+@pytest.mark.parametrize("case", ["tiny", "c1", "small_lod"])
+@pytest.mark.parametrize("variant", CAM_VARIANTS,
+                         ids=[f"t{t}-box{b}-rec{n}-wg{w}{'-depth' if d else ''}" for t, b, n, w, d in CAM_VARIANTS])
+def test_binned_gbuffer_bit_exact(renderer, case, variant):
+    tile, box, records, wg, use_depth = variant
+    sc = make_scene(case)
+    w, r = sc.w, renderer
+    print(f"\n[gpu parity] hm_gbuffer case={case} tile={tile} box={box} records={records} wg={wg} "
+          f"depth={use_depth}", flush=True)
+    r.targets(w.width, w.height, w.dom_size)
+    depth = depth_checker(w) if use_depth else None
+    r.set_scene_depth(depth, 0.0)
+    g = r.upload(sc.mesh)
+    keep = r.get_tuning()
+    try:
+        r.set_tuning(cam_tile=tile, bin_box_limit=box, bin_records=records, bin_blocks_per_cu=wg)
+        r.lod(g, sc.params[0], N.PASS_CAMERA, w.control_points, w.strand_fraction)
+        r.scan(g, sc.params[0], N.PASS_CAMERA)
+        want = oracle_gbuffer(case, depth)
+        for _ in range(2):  # the second call must not depend on what the first left behind
+            r.gbuffer_pass(g, sc.params[0])
+            got = r.gbuffer()
+            st = r.bin_status()
+            for a, b, name in zip(got, want[:3], ("centre", "conservative", "beta")):
+                assert np.array_equal(a, b), f"{(a != b).sum()} {name} words differ"
+        assert (got[0] != EMPTY64).sum() > 0
+        assert st["fallback"] == (records == 1) and st["binned"] == (records != 1)
+        if box == 1 and records != 1:
+            assert st["long_count"] > 0
+        if use_depth:
+            full = oracle_gbuffer(case)
+            assert 0 < want[3][0] < full[3][0], "the depth blocks should reject some but not all writes"
+    finally:
+        r.set_tuning(**keep)
+        r.set_scene_depth(None, 0.0)
+        g.free()
