@@ -17,6 +17,9 @@
 //                    pass (LDS add), counts to HBM: the whole DOM is written,
 //                    so no separate clear is needed
 //   k_dom_long<2>    long segments, global atomic add  (same)
+// The camera pass bins the same way into screen tiles (k_bin_gen<0>); its
+// tiles are split into chunks of records so that the few tiles holding most
+// of the hair spread over many workgroups (k_cam_tiles, k_cam_merge).
 // When the records outgrow their buffer the gates switch every binned kernel
 // off and the direct kernels (clear + k_raster<1> + k_raster<2>) run instead.
 // DOM words are built from min and add only, so the result does not depend on
@@ -254,10 +257,14 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
 
 // Exclusive scan of the per-tile counts (one block of 1024 threads, 16 tiles
 // per thread), then the gate words.
+// With chunk > 0 it also splits each tile's records into chunks of at most
+// `chunk` (at least one chunk per tile, so empty tiles are still written):
+// chunk offsets after the cursors, and the tile of every chunk in chunk_tile.
 __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tiles, uint32_t record_capacity,
-                                                   uint32_t long_capacity, uint32_t list_capacity) {
+                                                   uint32_t long_capacity, uint32_t list_capacity, uint32_t chunk,
+                                                   uint32_t* chunk_tile, uint32_t chunk_capacity) {
     constexpr uint32_t PER = BIN_MAX_TILES / 1024u;
-    __shared__ uint32_t wave_total[16];
+    __shared__ uint32_t wave_total[16], wave_chunks[16];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t* cnt = meta + BIN_META_WORDS;
     uint32_t* off = meta + BIN_META_WORDS + tiles;
@@ -270,30 +277,54 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
         v[k] = b + k < tiles ? cnt[b + k] : 0u;
         sum += v[k];
     }
-    uint32_t inc = sum;
+    uint32_t nch[PER];
+    uint32_t csum = 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        nch[k] = b + k < tiles ? (chunk ? (v[k] > chunk ? (v[k] + chunk - 1u) / chunk : 1u) : 1u) : 0u;
+        csum += nch[k];
+    }
+    uint32_t inc = sum, cinc = csum;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(inc, d);
-        if ((int)lane >= d) inc += y;
+        uint32_t y = __shfl_up(inc, d), cy = __shfl_up(cinc, d);
+        if ((int)lane >= d) {
+            inc += y;
+            cinc += cy;
+        }
     }
-    if (lane == 63u) wave_total[wave] = inc;
+    if (lane == 63u) {
+        wave_total[wave] = inc;
+        wave_chunks[wave] = cinc;
+    }
     __syncthreads();
-    uint32_t run = inc - sum;
-    for (uint32_t w = 0; w < wave; ++w) run += wave_total[w];
+    uint32_t run = inc - sum, crun = cinc - csum;
+    for (uint32_t w = 0; w < wave; ++w) {
+        run += wave_total[w];
+        crun += wave_chunks[w];
+    }
+    uint32_t* choff = cur + tiles;
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
         if (b + k < tiles) {
             off[b + k] = run;
             cur[b + k] = run;
+            choff[b + k] = crun;
+            for (uint32_t c = 0; c < nch[k]; ++c)
+                if (crun + c < chunk_capacity) chunk_tile[crun + c] = b + k;
         }
         run += v[k];
+        crun += nch[k];
     }
     if (tid == 1023u) {
         const uint32_t total = run;
         off[tiles] = total;
+        choff[tiles] = crun;
         const uint32_t longs = meta[BIN_LONG_COUNT];
-        const uint32_t ok =
-            (total <= record_capacity && longs <= long_capacity && meta[BIN_LIST_COUNT] <= list_capacity) ? 1u : 0u;
+        const uint32_t ok = (total <= record_capacity && longs <= long_capacity &&
+                             meta[BIN_LIST_COUNT] <= list_capacity && crun <= chunk_capacity)
+                                ? 1u
+                                : 0u;
         meta[BIN_RECORDS] = total;
         meta[BIN_OK] = ok;
         meta[BIN_FALLBACK] = 1u - ok;
@@ -428,7 +459,7 @@ hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks,
     hipLaunchKernelGGL(k_bin_count, dim3(bin_blocks), dim3(256), lds, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, a.meta, a.tiles, a.record_capacity, a.long_capacity,
-                       a.list_capacity);
+                       a.list_capacity, a.chunk, a.chunk_tile, a.chunk_capacity);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin_fill, dim3(bin_blocks), dim3(256), lds, s, a);
     return hipGetLastError();
@@ -492,16 +523,62 @@ __device__ __forceinline__ void walk_cam_record(D& d, const BinArgs& a, uint32_t
                    F3{t1.x, t1.y, t1.z}, F3{si.x, si.y, w0}, F3{si.x, si.y, w1}, t0.w, t1.w, si.z);
 }
 
-// One workgroup per screen tile: the tile starts empty (this is the clear),
-// takes every segment binned to it with LDS atomics and is written once.
+// Final words of one pixel of a tile (every lane of the wave calls it): the
+// G-buffer, and with emit_lists the shade / filter lists and the empty
+// results of uncovered pixels.
+__device__ __forceinline__ void finish_pixel(const BinArgs& a, bool in, size_t g, uint64_t c, uint64_t k, uint32_t b) {
+    const int lane = (int)(threadIdx.x & 63u);
+    if (in) {
+        a.center[g] = c;
+        a.cons[g] = k;
+        a.beta[g] = b;
+    }
+    if (!a.emit_lists) return;
+    const bool shade = in && !gb_empty(c);
+    const bool filter = in && (shade || !gb_empty(k));
+    if (in && !shade) {
+        a.shaded[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.shaded_depth[g] = 0.0f;
+    }
+    if (in && !filter) {
+        a.filtered[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.filtered_depth[g] = 0.0f;
+    }
+    const uint64_t ms = __ballot(shade), mf = __ballot(filter);
+    uint32_t bs = 0u, bf = 0u;
+    if (lane == 0) {
+        if (ms) bs = atomicAdd(a.meta + BIN_SHADE_COUNT, (uint32_t)__popcll(ms));
+        if (mf) bf = atomicAdd(a.meta + BIN_FILTER_COUNT, (uint32_t)__popcll(mf));
+    }
+    bs = __shfl(bs, 0);
+    bf = __shfl(bf, 0);
+    if (shade) a.shade_list[bs + lanes_below(ms)] = (uint32_t)g;
+    if (filter) a.filter_list[bf + lanes_below(mf)] = (uint32_t)g;
+}
+
+// Partial tile of one chunk: centre and conservative words, then beta words.
+template <int T>
+__device__ __forceinline__ uint64_t* partial_tile(const BinArgs& a, uint32_t chunk) {
+    return reinterpret_cast<uint64_t*>(a.partials + (size_t)chunk * (size_t)(T * T * 20));
+}
+
+// One workgroup per chunk of a screen tile's records. The tile starts empty
+// (this is the clear) and takes its segments with LDS atomics. A tile with
+// one chunk is written as it is; the chunks of a larger tile leave partial
+// tiles for k_cam_merge.
 template <int T>
 __global__ __launch_bounds__(256) void k_cam_tiles(BinArgs a) {
     static_assert((T * T) % 256 == 0, "the write-back loop runs whole waves");
     if (a.meta[BIN_OK] == 0u) return;
+    const uint32_t* off = a.meta + BIN_META_WORDS + a.tiles;
+    const uint32_t* choff = off + 2u * a.tiles + 1u;
+    const uint32_t chunk = blockIdx.x;
+    if (chunk >= choff[a.tiles]) return;
     __shared__ uint64_t center[T * T], cons[T * T];
     __shared__ uint32_t beta[T * T];
     const View& p = bin_view(a);
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = a.chunk_tile[chunk];
+    const uint32_t first = choff[tile], nch = choff[tile + 1u] - first;
     const int32_t ox = (int32_t)(tile % a.tiles_x) * T, oy = (int32_t)(tile / a.tiles_x) * T;
     const int32_t W = (int32_t)a.width, H = (int32_t)a.height;
     for (int i = (int)threadIdx.x; i < T * T; i += 256) {
@@ -510,44 +587,54 @@ __global__ __launch_bounds__(256) void k_cam_tiles(BinArgs a) {
         beta[i] = ~0u;
     }
     __syncthreads();
-    const uint32_t* off = a.meta + BIN_META_WORDS + a.tiles;
-    const uint32_t begin = off[tile], end = off[tile + 1u];
+    const uint32_t tile_end = off[tile + 1u];
+    const uint32_t begin = off[tile] + (chunk - first) * (a.chunk ? a.chunk : 0u);
+    const uint32_t end = nch > 1u && begin + a.chunk < tile_end ? begin + a.chunk : tile_end;
     CamTileSink<T> sink{center, cons, beta, ox, oy};
     Drawer<0, CamTileSink<T>> d{p, a.scene_depth, a.scene_clear, sink, {0, 0, 0, 0}};
     for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) walk_cam_record(d, a, a.records[r]);
     __syncthreads();
-    const int lane = (int)(threadIdx.x & 63u);
+    if (nch > 1u) {
+        uint64_t* part = partial_tile<T>(a, chunk);
+        uint32_t* part_beta = reinterpret_cast<uint32_t*>(part + 2 * T * T);
+        for (int i = (int)threadIdx.x; i < T * T; i += 256) {
+            part[i] = center[i];
+            part[T * T + i] = cons[i];
+            part_beta[i] = beta[i];
+        }
+        return;
+    }
     for (int i = (int)threadIdx.x; i < T * T; i += 256) {  // T * T is a multiple of 256: whole waves
         const int32_t x = ox + i % T, y = oy + i / T;
         const bool in = x < W && y < H;
-        const size_t g = in ? (size_t)y * (size_t)W + (size_t)x : 0u;
-        const uint64_t c = center[i], k = cons[i];
-        if (in) {
-            a.center[g] = c;
-            a.cons[g] = k;
-            a.beta[g] = beta[i];
+        finish_pixel(a, in, in ? (size_t)y * (size_t)W + (size_t)x : 0u, center[i], cons[i], beta[i]);
+    }
+}
+
+// Tiles that were split into chunks: the minimum over the partial tiles.
+template <int T>
+__global__ __launch_bounds__(256) void k_cam_merge(BinArgs a) {
+    if (a.meta[BIN_OK] == 0u) return;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t* choff = a.meta + BIN_META_WORDS + 3u * a.tiles + 1u;
+    const uint32_t first = choff[tile], nch = choff[tile + 1u] - first;
+    if (nch <= 1u) return;
+    const int32_t ox = (int32_t)(tile % a.tiles_x) * T, oy = (int32_t)(tile / a.tiles_x) * T;
+    const int32_t W = (int32_t)a.width, H = (int32_t)a.height;
+    for (int i = (int)threadIdx.x; i < T * T; i += 256) {
+        uint64_t c = ~0ull, k = ~0ull;
+        uint32_t b = ~0u;
+        for (uint32_t j = 0; j < nch; ++j) {
+            const uint64_t* part = partial_tile<T>(a, first + j);
+            const uint64_t pc = part[i], pk = part[T * T + i];
+            const uint32_t pb = reinterpret_cast<const uint32_t*>(part + 2 * T * T)[i];
+            c = pc < c ? pc : c;
+            k = pk < k ? pk : k;
+            b = pb < b ? pb : b;
         }
-        if (!a.emit_lists) continue;
-        const bool shade = in && !gb_empty(c);
-        const bool filter = in && (shade || !gb_empty(k));
-        if (in && !shade) {
-            a.shaded[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            a.shaded_depth[g] = 0.0f;
-        }
-        if (in && !filter) {
-            a.filtered[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            a.filtered_depth[g] = 0.0f;
-        }
-        const uint64_t ms = __ballot(shade), mf = __ballot(filter);
-        uint32_t bs = 0u, bf = 0u;
-        if (lane == 0) {
-            if (ms) bs = atomicAdd(a.meta + BIN_SHADE_COUNT, (uint32_t)__popcll(ms));
-            if (mf) bf = atomicAdd(a.meta + BIN_FILTER_COUNT, (uint32_t)__popcll(mf));
-        }
-        bs = __shfl(bs, 0);
-        bf = __shfl(bf, 0);
-        if (shade) a.shade_list[bs + lanes_below(ms)] = (uint32_t)g;
-        if (filter) a.filter_list[bf + lanes_below(mf)] = (uint32_t)g;
+        const int32_t x = ox + i % T, y = oy + i / T;
+        const bool in = x < W && y < H;
+        finish_pixel(a, in, in ? (size_t)y * (size_t)W + (size_t)x : 0u, c, k, b);
     }
 }
 
@@ -563,10 +650,13 @@ __global__ __launch_bounds__(256) void k_cam_long(BinArgs a) {
 }
 
 hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int long_blocks) {
+    // One workgroup per possible chunk; those past the scan's chunk total return.
     if (a.tile_shift == 4u) {
-        hipLaunchKernelGGL(k_cam_tiles<16>, dim3(a.tiles), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_cam_tiles<16>, dim3(a.chunk_capacity), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_cam_merge<16>, dim3(a.tiles), dim3(256), 0, s, a);
     } else if (a.tile_shift == 5u) {
-        hipLaunchKernelGGL(k_cam_tiles<32>, dim3(a.tiles), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_cam_tiles<32>, dim3(a.chunk_capacity), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_cam_merge<32>, dim3(a.tiles), dim3(256), 0, s, a);
     } else {
         return hipErrorInvalidValue;
     }

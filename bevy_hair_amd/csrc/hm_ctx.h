@@ -29,13 +29,20 @@ struct hm_bin_state {
     uint32_t* seg_box = nullptr;    // per segment id
     uint32_t* long_list = nullptr;
     size_t slots = 0;               // strand slots cps / seg_box / long_list hold
-    uint32_t* meta[2] = {nullptr, nullptr};  // per pass (camera, DOM): status words, per-tile count / offset / cursor
-    size_t meta_words = 0;
+    // Per pass (camera, DOM), each grown only by its own pass: status words
+    // (the camera's shade / filter list lengths are read after the DOM pass)
+    // and per-tile count / offset / cursor / chunk offset.
+    uint32_t* meta[2] = {nullptr, nullptr};
+    size_t meta_words[2] = {0, 0};
     uint32_t* records = nullptr;
     uint2* lists = nullptr;         // same length as records
     size_t record_capacity = 0;
     uint2* wg_list = nullptr;       // one per binning workgroup
     size_t wg_capacity = 0;
+    uint32_t* chunk_tile = nullptr; // tile of each tile-workgroup chunk
+    size_t chunk_capacity = 0;
+    uint8_t* partials = nullptr;    // camera: partial tiles of split tiles
+    size_t partial_bytes = 0;
 };
 
 struct hm_ctx {
@@ -63,7 +70,7 @@ struct hm_ctx {
     hm_directional_light* dirs = nullptr;
     uint32_t dir_count = 0, dir_cap = 0;
     uint64_t* counters = nullptr;
-    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u};
+    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u, 1024u};
     hm_bin_state bin;
     // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
     uint32_t slots = 0, frame = 0;

@@ -295,14 +295,20 @@ void free_bin(hm_ctx* c) {
     dev_free(b.records);
     dev_free(b.lists);
     dev_free(b.wg_list);
+    dev_free(b.chunk_tile);
+    dev_free(b.partials);
     b = hm_bin_state();
 }
 
-// Grows the binned-DOM buffers to hold `slots` strand slots, `meta_words`
-// status + per-tile words, `records` tile records and `wgs` binning workgroups.
-hipError_t ensure_bin(hm_ctx* c, size_t slots, size_t meta_words, size_t records, size_t wgs) {
+// Grows the binning buffers to hold `slots` strand slots, `meta_words`
+// status + per-tile words of pass `pass`, `records` tile records, `wgs`
+// binning workgroups, `chunks` tile-workgroup chunks and `partial_bytes` of
+// partial tiles. The other pass's status words are left alone.
+hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size_t records, size_t wgs,
+                      size_t chunks, size_t partial_bytes) {
     hm_bin_state& b = c->bin;
-    if (slots <= b.slots && meta_words <= b.meta_words && records <= b.record_capacity && wgs <= b.wg_capacity)
+    if (slots <= b.slots && meta_words <= b.meta_words[pass] && records <= b.record_capacity &&
+        wgs <= b.wg_capacity && chunks <= b.chunk_capacity && partial_bytes <= b.partial_bytes)
         return hipSuccess;
     hipError_t e = hipStreamSynchronize(c->stream);  // earlier frames may still read them
     if (e != hipSuccess) return e;
@@ -323,13 +329,11 @@ hipError_t ensure_bin(hm_ctx* c, size_t slots, size_t meta_words, size_t records
         if ((e = dev_alloc(&b.long_list, segs * 4)) != hipSuccess) return e;
         b.slots = slots;
     }
-    if (meta_words > b.meta_words) {
-        dev_free(b.meta[0]);
-        dev_free(b.meta[1]);
-        b.meta_words = 0;
-        if ((e = dev_alloc(&b.meta[0], meta_words * 4)) != hipSuccess) return e;
-        if ((e = dev_alloc(&b.meta[1], meta_words * 4)) != hipSuccess) return e;
-        b.meta_words = meta_words;
+    if (meta_words > b.meta_words[pass]) {
+        dev_free(b.meta[pass]);
+        b.meta_words[pass] = 0;
+        if ((e = dev_alloc(&b.meta[pass], meta_words * 4)) != hipSuccess) return e;
+        b.meta_words[pass] = meta_words;
     }
     if (records > b.record_capacity) {
         dev_free(b.records);
@@ -344,6 +348,18 @@ hipError_t ensure_bin(hm_ctx* c, size_t slots, size_t meta_words, size_t records
         b.wg_capacity = 0;
         if ((e = dev_alloc(&b.wg_list, wgs * sizeof(uint2))) != hipSuccess) return e;
         b.wg_capacity = wgs;
+    }
+    if (chunks > b.chunk_capacity) {
+        dev_free(b.chunk_tile);
+        b.chunk_capacity = 0;
+        if ((e = dev_alloc(&b.chunk_tile, chunks * 4)) != hipSuccess) return e;
+        b.chunk_capacity = chunks;
+    }
+    if (partial_bytes > b.partial_bytes) {
+        dev_free(b.partials);
+        b.partial_bytes = 0;
+        if ((e = dev_alloc(&b.partials, partial_bytes)) != hipSuccess) return e;
+        b.partial_bytes = partial_bytes;
     }
     return hipSuccess;
 }
@@ -803,8 +819,13 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     const size_t slots = g->capacity ? g->capacity : 1u;
     const size_t want_records = ctx->tuning.bin_records ? ctx->tuning.bin_records : slots * hm::BIN_STRIDE * 2u;
     const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
-    hipError_t e =
-        ensure_bin(ctx, slots, hm::BIN_META_WORDS + 3u * (size_t)tiles + 1u, want_records, (size_t)bin_blocks);
+    // Camera tiles split into chunks: at most tiles + records / chunk of them.
+    const uint32_t chunk = pass == HM_PASS_CAMERA ? ctx->tuning.cam_chunk : 0u;
+    const size_t records_cap = std::min<size_t>(want_records, 0xffffffffu);
+    const size_t chunks = tiles + (chunk ? records_cap / chunk + 1u : 0u);
+    const size_t partial = pass == HM_PASS_CAMERA && chunk ? chunks * (size_t)T * T * 20u : 0u;
+    hipError_t e = ensure_bin(ctx, pass, slots, hm::BIN_META_WORDS + 4u * (size_t)tiles + 2u, want_records,
+                              (size_t)bin_blocks, chunks, partial);
     if (e != hipSuccess) return e;
     std::memset(&b, 0, sizeof(b));
     std::memcpy(&b.view, p, sizeof(hm_params));
@@ -823,6 +844,10 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     b.lists = ctx->bin.lists;
     b.list_capacity = b.record_capacity;
     b.wg_list = ctx->bin.wg_list;
+    b.chunk = chunk;
+    b.chunk_tile = ctx->bin.chunk_tile;
+    b.chunk_capacity = (uint32_t)std::min<size_t>(chunks, 0xffffffffu);
+    b.partials = ctx->bin.partials;
     b.long_list = ctx->bin.long_list;
     b.long_capacity = (uint32_t)std::min<size_t>(slots * hm::BIN_STRIDE, 0xffffffffu);
     b.tiles_x = tiles_x;
@@ -1149,6 +1174,8 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
         return fail(ctx, HM_ERR_INVALID, "dom_tile must be 0, 16 or 32");
     if (t->cam_tile != 0u && t->cam_tile != 16u && t->cam_tile != 32u)
         return fail(ctx, HM_ERR_INVALID, "cam_tile must be 0, 16 or 32");
+    if (t->cam_chunk != 0u && (t->cam_chunk < 64u || t->cam_chunk > (1u << 20)))
+        return fail(ctx, HM_ERR_INVALID, "cam_chunk must be 0 or in [64, 1048576]");
     if (t->bin_box_limit == 0u || t->bin_box_limit > 64u)
         return fail(ctx, HM_ERR_INVALID, "bin_box_limit must be in [1, 64]");
     if (t->bin_blocks_per_cu == 0u || t->bin_blocks_per_cu > 64u)

@@ -69,7 +69,8 @@ constexpr uint32_t BIN_LONG_COUNT = 4u;
 constexpr uint32_t BIN_LIST_COUNT = 5u;  // (tile, rank) entries the counting workgroups wrote
 constexpr uint32_t BIN_SHADE_COUNT = 6u;  // camera: pixels with a centre word (shade list)
 constexpr uint32_t BIN_FILTER_COUNT = 7u; // camera: pixels with a centre or conservative word (filter list)
-constexpr uint32_t BIN_META_WORDS = 8u;  // then count[tiles], offset[tiles + 1], cursor[tiles]
+constexpr uint32_t BIN_META_WORDS = 8u;  // then count[tiles], offset[tiles + 1], cursor[tiles],
+                                         // chunk offset[tiles + 1]
 constexpr uint32_t BIN_MAX_TILES = 16384u;  // one LDS word per tile in the binning workgroups
 
 struct BinArgs {
@@ -89,6 +90,10 @@ struct BinArgs {
     uint2* lists;                   // per counting workgroup: (tile, rank within the tile)
     uint32_t list_capacity;
     uint2* wg_list;                 // per counting workgroup: (first list entry, entries)
+    uint32_t chunk;                 // records per tile workgroup (0: one workgroup per tile)
+    uint32_t* chunk_tile;           // tile of each chunk
+    uint32_t chunk_capacity;
+    uint8_t* partials;              // camera: one partial tile per chunk of a split tile
     uint32_t* long_list;            // segments whose box spans more than box_limit tiles
     uint32_t long_capacity;
     uint32_t tiles_x, tiles, tile_shift, box_limit;

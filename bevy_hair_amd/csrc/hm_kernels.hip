@@ -286,10 +286,12 @@ __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ ce
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a, const uint32_t* __restrict__ list,
                                                const uint32_t* __restrict__ count) {
     const View& p = *reinterpret_cast<const View*>(&a.view);
+    const uint32_t width = to_u32(p.screen.x), pixels = width * to_u32(p.screen.y);
     uint32_t n = *count;
-    uint32_t width = to_u32(p.screen.x);
+    n = n < pixels ? n : pixels;  // a list never holds more entries than pixels
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t pix = list[i];
+        if (pix >= pixels) continue;
         uint32_t x = pix % width, y = pix / width;
         PixelShade px = pixel_setup(p, a.center[pix], a.beta[pix], x, y);
         F3 color = plain_pixel_color(a, p, px);
@@ -381,10 +383,12 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
 __global__ __launch_bounds__(256) void k_filter(FilterArgs a, const uint32_t* __restrict__ list,
                                                 const uint32_t* __restrict__ count) {
     const View& p = *reinterpret_cast<const View*>(&a.view);
+    const uint32_t width = to_u32(p.screen.x), pixels = width * to_u32(p.screen.y);
     uint32_t n = *count;
-    uint32_t width = to_u32(p.screen.x);
+    n = n < pixels ? n : pixels;  // a list never holds more entries than pixels
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t pix = list[i];
+        if (pix >= pixels) continue;
         filter_pixel(a, p, (int32_t)(pix % width), (int32_t)(pix / width));
     }
 }

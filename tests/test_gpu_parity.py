@@ -272,27 +272,29 @@ def depth_checker(w):
     return np.ascontiguousarray(np.where((x // 8 + y // 8) % 2 == 0, 0.0, 1.0).astype(np.float32).ravel())
 
 
-# (tile, box limit, record buffer, binning workgroups per CU, scene depth)
-CAM_VARIANTS = [(16, 8, 0, 2, False), (32, 8, 0, 4, False), (16, 1, 0, 2, False), (32, 64, 0, 1, False),
-                (16, 8, 0, 2, True), (16, 8, 1, 2, False)]
+# (tile, box limit, record buffer, binning workgroups per CU, records per tile
+# workgroup, scene depth): chunk 64 splits nearly every covered tile
+CAM_VARIANTS = [(16, 8, 0, 2, 1024, False), (32, 8, 0, 4, 1024, False), (16, 1, 0, 2, 64, False),
+                (32, 64, 0, 1, 0, False), (16, 8, 0, 2, 64, True), (16, 8, 1, 2, 1024, False)]
 
 
 @pytest.mark.parametrize("case", ["tiny", "c1", "small_lod"])
 @pytest.mark.parametrize("variant", CAM_VARIANTS,
-                         ids=[f"t{t}-box{b}-rec{n}-wg{w}{'-depth' if d else ''}" for t, b, n, w, d in CAM_VARIANTS])
+                         ids=[f"t{t}-box{b}-rec{n}-wg{w}-ch{c}{'-depth' if d else ''}"
+                              for t, b, n, w, c, d in CAM_VARIANTS])
 def test_binned_gbuffer_bit_exact(renderer, case, variant):
-    tile, box, records, wg, use_depth = variant
+    tile, box, records, wg, chunk, use_depth = variant
     sc = make_scene(case)
     w, r = sc.w, renderer
     print(f"\n[gpu parity] hm_gbuffer case={case} tile={tile} box={box} records={records} wg={wg} "
-          f"depth={use_depth}", flush=True)
+          f"chunk={chunk} depth={use_depth}", flush=True)
     r.targets(w.width, w.height, w.dom_size)
     depth = depth_checker(w) if use_depth else None
     r.set_scene_depth(depth, 0.0)
     g = r.upload(sc.mesh)
     keep = r.get_tuning()
     try:
-        r.set_tuning(cam_tile=tile, bin_box_limit=box, bin_records=records, bin_blocks_per_cu=wg)
+        r.set_tuning(cam_tile=tile, bin_box_limit=box, bin_records=records, bin_blocks_per_cu=wg, cam_chunk=chunk)
         r.lod(g, sc.params[0], N.PASS_CAMERA, w.control_points, w.strand_fraction)
         r.scan(g, sc.params[0], N.PASS_CAMERA)
         want = oracle_gbuffer(case, depth)
