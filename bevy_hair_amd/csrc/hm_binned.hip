@@ -160,12 +160,34 @@ __global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
     total = total < a.capacity ? total : a.capacity;
     const uint32_t wpb = blockDim.x >> 6;
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (threadIdx.x >> 6));
+    uint32_t reuse_total = 0u;
+    if (MODE == 1 && a.reuse_refs) {
+        reuse_total = a.reuse_indirect[0];
+        reuse_total = reuse_total < a.capacity ? reuse_total : a.capacity;
+    }
     for (uint32_t s = w0; s < total; s += gridDim.x * wpb) {
         const hm_strand_ref ref = a.refs[s];
-        StrandCps sc = gen_strand(p, ref, a.bundles, a.layers, a.style, lane);
+        const size_t base = (size_t)s * BIN_STRIDE;
+        // DOM pass: when the camera pass of this frame generated the same strand
+        // (same bundle, strand and control-point count) its world control points
+        // are already in cps; otherwise generate them.
+        bool reuse = false;
+        if (MODE == 1 && s < reuse_total) {
+            const hm_strand_ref c = a.reuse_refs[s];
+            reuse = c.bundle == ref.bundle && c.strand == ref.strand && c.control_points == ref.control_points;
+        }
+        StrandCps sc;
+        if (reuse) {
+            sc.ncp = ref.control_points < 2u ? 2u : (ref.control_points > 127u ? 127u : ref.control_points);
+            const float4 q0 = (uint32_t)lane < sc.ncp ? a.cps[base + lane] : make_float4(0, 0, 0, 0);
+            const float4 q1 = (uint32_t)lane + 64u < sc.ncp ? a.cps[base + lane + 64] : make_float4(0, 0, 0, 0);
+            sc.cw[0] = F3{q0.x, q0.y, q0.z};
+            sc.cw[1] = F3{q1.x, q1.y, q1.z};
+        } else {
+            sc = gen_strand(p, ref, a.bundles, a.layers, a.style, lane);
+        }
         const uint32_t ncp = sc.ncp;
         Projected pr0 = project_world(p, sc.cw[0]), pr1 = project_world(p, sc.cw[1]);
-        const size_t base = (size_t)s * BIN_STRIDE;
         if ((uint32_t)lane < ncp) {
             a.cps[base + lane] = make_float4(sc.cw[0].x, sc.cw[0].y, sc.cw[0].z, pr0.screen.x);
             a.cpy[base + lane] = pr0.screen.y;
@@ -272,9 +294,12 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
     const uint32_t b = tid * PER;
     uint32_t v[PER];
     uint32_t sum = 0u;
+    // Unconditional (clamped) loads, so all sixteen are in flight at once.
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) v[k] = cnt[b + k < tiles ? b + k : tiles - 1u];
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
-        v[k] = b + k < tiles ? cnt[b + k] : 0u;
+        v[k] = b + k < tiles ? v[k] : 0u;
         sum += v[k];
     }
     uint32_t nch[PER];
@@ -624,7 +649,26 @@ __global__ __launch_bounds__(256) void k_cam_merge(BinArgs a) {
     for (int i = (int)threadIdx.x; i < T * T; i += 256) {
         uint64_t c = ~0ull, k = ~0ull;
         uint32_t b = ~0u;
-        for (uint32_t j = 0; j < nch; ++j) {
+        // Four partial tiles per step, so their loads overlap.
+        uint32_t j = 0;
+        for (; j + 4u <= nch; j += 4u) {
+            uint64_t pc[4], pk[4];
+            uint32_t pb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint64_t* part = partial_tile<T>(a, first + j + (uint32_t)u);
+                pc[u] = part[i];
+                pk[u] = part[T * T + i];
+                pb[u] = reinterpret_cast<const uint32_t*>(part + 2 * T * T)[i];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                c = pc[u] < c ? pc[u] : c;
+                k = pk[u] < k ? pk[u] : k;
+                b = pb[u] < b ? pb[u] : b;
+            }
+        }
+        for (; j < nch; ++j) {
             const uint64_t* part = partial_tile<T>(a, first + j);
             const uint64_t pc = part[i], pk = part[T * T + i];
             const uint32_t pb = reinterpret_cast<const uint32_t*>(part + 2 * T * T)[i];

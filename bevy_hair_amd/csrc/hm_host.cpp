@@ -909,19 +909,23 @@ static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool
     {
         Timer t(ctx, HM_K_CLEAR);
         HM_HIP(ctx, hm::launch_fill32(ctx->stream, static_cast<uint32_t*>(ctx->gb_slab), ctx->gb_words, 0xffffffffu,
-                                      ctx->cu_count * 8, fallback));
+                                      ctx->cu_count, fallback));
     }
     a.gate = fallback;
     {
         Timer t(ctx, HM_K_CAM_RASTER);
-        HM_HIP(ctx, hm::launch_raster(ctx->stream, a, 0, false, raster_blocks(ctx, g->capacity, 0u)));
+        HM_HIP(ctx, hm::launch_raster(ctx->stream, a, 0, false, std::min(raster_blocks(ctx, g->capacity, 0u),
+                                                                          ctx->cu_count * 2)));
     }
     return HM_OK;
 }
 
 hm_status hm_gbuffer(hm_ctx* ctx, hm_groom* g, const hm_params* p) { return gbuffer_impl(ctx, g, p, false, nullptr); }
 
-hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params* op) {
+// hm_dom; with reuse_camera (a frame whose camera pass ran binned just before,
+// with the same model transform) the DOM generation takes the world control
+// points of strands the camera work list shares instead of generating them.
+static hm_status dom_impl(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params* op, bool reuse_camera) {
     hm::RasterArgs a1, a2;
     hm_status st = raster_args(ctx, g, dp, HM_PASS_DOM, a1);
     if (st != HM_OK) return st;
@@ -943,7 +947,12 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
         return hm_raster(ctx, g, op, HM_PASS_DOM);
     }
     hm::BinArgs b;
+    const float4* cps_before = ctx->bin.cps;
     HM_HIP(ctx, bin_args(ctx, g, dp, HM_PASS_DOM, T, tiles_x, tiles_x, b));
+    if (reuse_camera && ctx->bin.cps == cps_before) {  // not reallocated since the camera pass
+        b.reuse_refs = g->refs[HM_PASS_CAMERA];
+        b.reuse_indirect = g->indirect[HM_PASS_CAMERA];
+    }
     const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
     {
         Timer t(ctx, HM_K_DOM_BIN);
@@ -958,21 +967,27 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
     const size_t texels = (size_t)S * S;
     {
         Timer t(ctx, HM_K_CLEAR_DOM);
-        HM_HIP(ctx, hm::launch_fill32(ctx->stream, ctx->dom, texels, 0xffffffffu, ctx->cu_count * 8, fallback));
-        HM_HIP(ctx, hm::launch_fill32(ctx->stream, ctx->dom + texels, texels * HM_DOM_LAYERS, 0u, ctx->cu_count * 8,
+        HM_HIP(ctx, hm::launch_fill32(ctx->stream, ctx->dom, texels, 0xffffffffu, ctx->cu_count, fallback));
+        HM_HIP(ctx, hm::launch_fill32(ctx->stream, ctx->dom + texels, texels * HM_DOM_LAYERS, 0u, ctx->cu_count,
                                       fallback));
     }
     a1.gate = fallback;
     a2.gate = fallback;
     {
         Timer t(ctx, HM_K_DOM_DEPTH);
-        HM_HIP(ctx, hm::launch_raster(ctx->stream, a1, 1, false, raster_blocks(ctx, g->capacity, 1u)));
+        HM_HIP(ctx, hm::launch_raster(ctx->stream, a1, 1, false, std::min(raster_blocks(ctx, g->capacity, 1u),
+                                                                           ctx->cu_count * 2)));
     }
     {
         Timer t(ctx, HM_K_DOM_OPACITY);
-        HM_HIP(ctx, hm::launch_raster(ctx->stream, a2, 2, false, raster_blocks(ctx, g->capacity, 2u)));
+        HM_HIP(ctx, hm::launch_raster(ctx->stream, a2, 2, false, std::min(raster_blocks(ctx, g->capacity, 2u),
+                                                                           ctx->cu_count * 2)));
     }
     return HM_OK;
+}
+
+hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params* op) {
+    return dom_impl(ctx, g, dp, op, false);
 }
 
 // Shading over the covered-pixel list. With lists_gate (see gbuffer_impl) the
@@ -1051,7 +1066,8 @@ hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm
     if (cam.flags[3]) {
         if ((st = hm_lod(ctx, g, &params[1], HM_PASS_DOM, nullptr)) != HM_OK) return st;
         if ((st = hm_scan(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
-        if ((st = hm_dom(ctx, g, &params[1], &params[2])) != HM_OK) return st;
+        const bool same_model = std::memcmp(cam.world_from_model, params[1].world_from_model, 64) == 0;
+        if ((st = dom_impl(ctx, g, &params[1], &params[2], lists != nullptr && same_model)) != HM_OK) return st;
     }
     if (opts && opts->shading == HM_SHADING_MLP) {
         if ((st = hm_shade_mlp(ctx, &cam, exposure, opts->mlp_tolerance, nullptr)) != HM_OK) return st;

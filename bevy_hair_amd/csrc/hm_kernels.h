@@ -94,6 +94,8 @@ struct BinArgs {
     uint32_t* chunk_tile;           // tile of each chunk
     uint32_t chunk_capacity;
     uint8_t* partials;              // camera: one partial tile per chunk of a split tile
+    const hm_strand_ref* reuse_refs;  // DOM: camera work list whose control points cps still holds, or null
+    const uint32_t* reuse_indirect;
     uint32_t* long_list;            // segments whose box spans more than box_limit tiles
     uint32_t long_capacity;
     uint32_t tiles_x, tiles, tile_shift, box_limit;

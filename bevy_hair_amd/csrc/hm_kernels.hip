@@ -261,26 +261,29 @@ __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ ce
                                                  uint32_t* __restrict__ count, float4* __restrict__ color_out,
                                                  float* __restrict__ depth_out, const uint32_t* gate) {
     if (gate && *gate == 0u) return;
-    uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    bool inb = pix < pixels;
-    bool hit = false;
-    if (inb) {
-        hit = !gb_empty(center[pix]);
-        if (MODE == 1 && !hit) hit = !gb_empty(cons[pix]);
-        if (!hit) {
-            color_out[pix] = make_float4(0, 0, 0, 0);
-            depth_out[pix] = 0.0f;
+    // Grid-stride over whole blocks, so every wave runs the same trip count.
+    for (uint32_t first = blockIdx.x * blockDim.x; first < pixels; first += gridDim.x * blockDim.x) {
+        uint32_t pix = first + threadIdx.x;
+        bool inb = pix < pixels;
+        bool hit = false;
+        if (inb) {
+            hit = !gb_empty(center[pix]);
+            if (MODE == 1 && !hit) hit = !gb_empty(cons[pix]);
+            if (!hit) {
+                color_out[pix] = make_float4(0, 0, 0, 0);
+                depth_out[pix] = 0.0f;
+            }
         }
+        uint64_t mask = __ballot(hit);
+        uint32_t lane = threadIdx.x & 63u;
+        uint32_t total = (uint32_t)__popcll(mask);
+        uint32_t base = 0;
+        if (total) {
+            if (lane == 0) base = atomicAdd(count, total);
+            base = __shfl(base, 0);
+        }
+        if (hit) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = pix;
     }
-    uint64_t mask = __ballot(hit);
-    uint32_t lane = threadIdx.x & 63u;
-    uint32_t total = (uint32_t)__popcll(mask);
-    uint32_t base = 0;
-    if (total) {
-        if (lane == 0) base = atomicAdd(count, total);
-        base = __shfl(base, 0);
-    }
-    if (hit) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = pix;
 }
 
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a, const uint32_t* __restrict__ list,
@@ -327,7 +330,9 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
     float s_par = mx(p.filter_params.x, 0.05f), s_perp = mx(p.filter_params.y, 0.02f);
     float sigma_par = 5.0f * s_par, sigma_perp = 5.0f * s_perp;
     float spar2 = sigma_par * sigma_par, sperp2 = sigma_perp * sigma_perp;
-    const float sigma_c2 = 0.9f * 0.9f;
+    // Tap weights are continuous: hardware reciprocal and exponential.
+    const float inv_spar2 = frcp(spar2), inv_sperp2 = frcp(sperp2);
+    const float inv_sigma_c2 = 1.0f / (0.9f * 0.9f);
     float p_norm = rdiv(u2f(depth_q), 16777214.0f);
     bool promote = !center_hit;
     bool use_color = !promote && cc.w > 0.0f;
@@ -352,14 +357,13 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
             if (qc_empty) continue;
             float q_norm = rdiv(u2f(gb_depth(q_center)), 16777214.0f);
             if (fabsf(q_norm - p_norm) > 1.45e-3f) continue;
-            float spatial = expf(-d_par * d_par / spar2 - d_perp * d_perp / sperp2);
+            float spatial = fexp(-d_par * d_par * inv_spar2 - d_perp * d_perp * inv_sperp2);
             float4 qc = a.shaded[qpix];
             F3 q_color = {qc.x, qc.y, qc.z};
             float color_w = 1.0f;
             if (use_color) {
                 F3 dc = F3{cc.x, cc.y, cc.z} - q_color;
-                float chroma = len3(dc);
-                color_w = expf(-chroma * chroma / sigma_c2);
+                color_w = fexp(-dot3(dc, dc) * inv_sigma_c2);
             }
             float w = spatial * color_w;
             accum = accum + q_color * w;
@@ -367,7 +371,7 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
         }
     }
     F3 color = {cc.x, cc.y, cc.z};
-    if (weight > 1e-5f) color = accum / weight;
+    if (weight > 1e-5f) color = accum * frcp(weight);
     float coverage = h_center / mx(h_center + h_cons, 1.0f);
     float cons_depth = cons_hit ? rdiv(u2f(gb_depth(cons_word)), 16777214.0f) : 1.0f;
     bool closer = cons_hit && center_hit && (p_norm - cons_depth) >= 5e-4f;
@@ -457,7 +461,8 @@ hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const
                           uint32_t* list, uint32_t* count, float4* color_out, float* depth_out, const uint32_t* gate) {
     if (pixels == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u, gate);
-    dim3 g((pixels + 255) / 256), b(256);
+    const uint32_t want = (pixels + 255) / 256;
+    dim3 g(gate ? (want < 256u ? want : 256u) : want), b(256);  // gated: usually switched off, keep it small
     if (mode == 0)
         hipLaunchKernelGGL(k_compact<0>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out, gate);
     else

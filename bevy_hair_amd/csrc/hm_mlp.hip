@@ -388,7 +388,7 @@ __device__ void mlp_features(const View& p, const PixelShade& px, F3 key_dir, co
 
 // The plain method's values of the 12 outputs for one pixel and key light.
 __device__ void mlp_targets(const PixelShade& px, const DualScatter& ds, F3 key_dir, float depth, float t[12]) {
-    BsdfGeom gk = bsdf_geom(px.tangent, px.view, key_dir);
+    BsdfGeom gk = bsdf_geom_n(px.t_n, px.v_n, key_dir);
     float direct = expf(-depth);
     F3 back = local_back(gk.theta_h, ds.a_back, ds.delta_b, ds.sigma_b2);
     F3 ms = direct * (0.7f * back) + global_spread(gk.theta_h, depth, direct, px.shape, ds);
@@ -449,7 +449,7 @@ __device__ F3 shade_from_mlp(const ShadeArgs& a, const PixelShade& px, const flo
             F3 ltf = point_pos(L) - px.world;
             float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
             if (att == 0.0f) continue;
-            BsdfGeom g = bsdf_geom(px.tangent, px.view, safe_normalize(ltf));
+            BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, safe_normalize(ltf));
             F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back(g.theta_h, a_back, delta_b, sigma_b2);
             F3 c = lobe * point_color(L) * att;
             if (pass == 1) c = c * spot_mask(L, px.world);
@@ -458,7 +458,7 @@ __device__ F3 shade_from_mlp(const ShadeArgs& a, const PixelShade& px, const flo
     }
     for (uint32_t i = 0; i < a.directional_count; ++i) {
         const hm_directional_light& L = a.directionals[i];
-        BsdfGeom g = bsdf_geom(px.tangent, px.view, dir_of(L));
+        BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, dir_of(L));
         F3 single = single_scatter(g, px.sigma_a, px.shape);
         F3 lobe;
         if ((int)i == key) {

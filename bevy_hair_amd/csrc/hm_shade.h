@@ -3,8 +3,11 @@
 //   azimuthal :94-116, lobe_shape :124-128, dual_scatter :153-190,
 //   hair_bsdf :193-244, dom_optical_depth/texel/visibility :246-310,
 //   spot_mask :312-327; Bevy's getDistanceAttenuation restated.
-// Discrete decisions (texel choice, slice index) use the exact arithmetic of
-// hm_device.h; transcendentals use the accurate device library.
+// Shaded RGB is held to a relative tolerance, so the continuous terms below
+// use the hardware approximations (v_exp_f32, v_log_f32, v_sqrt_f32,
+// v_rcp_f32, v_cos_f32; about 1 ulp) rather than the correctly rounded
+// sequences the library is compiled with. The G-buffer decode and every
+// discrete choice (texel, slice) keep the exact arithmetic of hm_device.h.
 #pragma once
 
 #include "hm_device.h"
@@ -12,17 +15,49 @@
 
 namespace hm {
 
+__device__ __forceinline__ float fexp(float x) { return __expf(x); }
+__device__ __forceinline__ float flog(float x) { return __logf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fcos(float x) { return __cosf(x); }
+__device__ __forceinline__ float fpow(float b, float e) { return __expf(e * __logf(b)); }
+// Abramowitz & Stegun 4.4.46 (Hastings): asin x = pi/2 - sqrt(1 - x) P7(x)
+// on [0, 1], |error| <= 2e-8; odd extension for x < 0.
+__device__ __forceinline__ float fasin(float x) {
+    float a = fabsf(x);
+    float p = -0.0012624911f;
+    p = p * a + 0.0066700901f;
+    p = p * a - 0.0170881256f;
+    p = p * a + 0.0308918810f;
+    p = p * a - 0.0501743046f;
+    p = p * a + 0.0889789874f;
+    p = p * a - 0.2145988016f;
+    p = p * a + 1.5707963050f;
+    float r = 1.5707963268f - fsqrt(1.0f - a) * p;
+    return x < 0.0f ? -r : r;
+}
+__device__ __forceinline__ F3 fnormalize(F3 v) {  // safe_normalize with the hardware reciprocal square root
+    float l2 = dot3(v, v);
+    if (l2 < 1e-10f) return {0.0f, 1.0f, 0.0f};
+    float r = __builtin_amdgcn_rsqf(l2);
+    return {v.x * r, v.y * r, v.z * r};
+}
+__device__ __forceinline__ F3 fdiv3(F3 a, F3 b) { return {fdiv(a.x, b.x), fdiv(a.y, b.y), fdiv(a.z, b.z)}; }
+
 struct Lobes { F3 r, tt, trt; };
 struct Shape { F3 alpha, beta; };
 struct DualScatter { Lobes fwd; F3 a_f, beta_f2, a_back, delta_b, sigma_b2; };
 
-__device__ __forceinline__ F3 vexp(F3 a) { return {expf(a.x), expf(a.y), expf(a.z)}; }
-__device__ __forceinline__ F3 vsqrt(F3 a) { return {sqrtf(a.x), sqrtf(a.y), sqrtf(a.z)}; }
+__device__ __forceinline__ F3 vexp(F3 a) { return {fexp(a.x), fexp(a.y), fexp(a.z)}; }
+__device__ __forceinline__ F3 vsqrt(F3 a) { return {fsqrt(a.x), fsqrt(a.y), fsqrt(a.z)}; }
 
 __device__ __forceinline__ F3 gaussian3(F3 b, F3 x) {
     F3 bb = vmax(b, splat3(1e-3f));
-    const float s2pi = sqrtf(2.0f * kPi);
-    return vexp(-0.5f * x * x / (bb * bb)) / (s2pi * bb);
+    const float s2pi = 2.5066282746f;  // sqrt(2 pi)
+    F3 r = {frcp(bb.x), frcp(bb.y), frcp(bb.z)};
+    F3 z = x * r;
+    return vexp(-0.5f * z * z) * r * (1.0f / s2pi);
 }
 
 __device__ __forceinline__ float fresnel(float cos_t) {
@@ -39,36 +74,38 @@ __device__ __forceinline__ F3 absorption_from_color(F3 color, float roughness) {
     float b2 = b * b, b3 = b2 * b;
     float d = 5.969f - 0.215f * b + 2.532f * b2 - 10.73f * b3 + 5.574f * b2 * b2 + 0.245f * b3 * b2;
     F3 c = vclamp(color, 1e-4f, 1.0f);
-    F3 l = {logf(c.x) / d, logf(c.y) / d, logf(c.z) / d};
+    float rd = frcp(d);
+    F3 l = {flog(c.x) * rd, flog(c.y) * rd, flog(c.z) * rd};
     return l * l;
 }
 
 __device__ __forceinline__ Lobes azimuthal(float cos_theta_d, float cos_phi, F3 sigma_a) {
-    float cos_half_phi = sqrtf(clampf(0.5f + 0.5f * cos_phi, 0.0f, 1.0f));
+    float cos_half_phi = fsqrt(clampf(0.5f + 0.5f * cos_phi, 0.0f, 1.0f));
     float n_r = 0.25f * cos_half_phi * fresnel(cos_theta_d * cos_half_phi);
-    float eta = 1.19f / cos_theta_d + 0.36f * cos_theta_d;
-    float a = 1.0f / eta;
+    float inv_cd = frcp(cos_theta_d);
+    float eta = 1.19f * inv_cd + 0.36f * cos_theta_d;
+    float a = frcp(eta);
     float h = cos_half_phi * (1.0f + a * (0.6f - 0.8f * cos_phi));
-    float f_tt = fresnel(cos_theta_d * sqrtf(clampf(1.0f - h * h, 0.0f, 1.0f)));
-    float s_tt = sqrtf(clampf(1.0f - h * h * a * a, 0.0f, 1.0f));
-    F3 t_tt = vexp(sigma_a * -1.0f * 2.0f * s_tt / cos_theta_d);
-    F3 n_tt = t_tt * (expf(-3.65f * cos_phi - 3.98f) * (1.0f - f_tt) * (1.0f - f_tt));
+    float f_tt = fresnel(cos_theta_d * fsqrt(clampf(1.0f - h * h, 0.0f, 1.0f)));
+    float s_tt = fsqrt(clampf(1.0f - h * h * a * a, 0.0f, 1.0f));
+    F3 t_tt = vexp(sigma_a * (-2.0f * s_tt * inv_cd));
+    F3 n_tt = t_tt * (fexp(-3.65f * cos_phi - 3.98f) * (1.0f - f_tt) * (1.0f - f_tt));
     float f_trt = fresnel(cos_theta_d * 0.5f);
-    F3 t_trt = vexp(sigma_a * -1.0f * 3.2f / cos_theta_d);
-    F3 n_trt = t_trt * (expf(17.0f * cos_phi - 16.78f) * (1.0f - f_trt) * (1.0f - f_trt) * f_trt);
+    F3 t_trt = vexp(sigma_a * (-3.2f * inv_cd));
+    F3 n_trt = t_trt * (fexp(17.0f * cos_phi - 16.78f) * (1.0f - f_trt) * (1.0f - f_trt) * f_trt);
     return {splat3(n_r), n_tt, n_trt};
 }
 
 __device__ __forceinline__ F3 lobe_mean(const Lobes& l, F3 total, F3 q) {
-    return (l.r * q.x + l.tt * q.y + l.trt * q.z) / vmax(total, splat3(1e-5f));
+    return fdiv3(l.r * q.x + l.tt * q.y + l.trt * q.z, vmax(total, splat3(1e-5f)));
 }
 
 __device__ DualScatter dual_scatter(float cos_theta_d, F3 sigma_a, const Shape& shape) {
     Lobes fwd = {splat3(0), splat3(0), splat3(0)}, back = fwd;
     for (uint32_t i = 0; i < 8u; ++i) {
-        float u = (u2f(i) + 0.5f) / 8.0f;
-        Lobes b = azimuthal(cos_theta_d, cosf(u * 0.5f * kPi), sigma_a);
-        Lobes f = azimuthal(cos_theta_d, cosf((1.0f + u) * 0.5f * kPi), sigma_a);
+        float u = (u2f(i) + 0.5f) * 0.125f;
+        Lobes b = azimuthal(cos_theta_d, fcos(u * 0.5f * kPi), sigma_a);
+        Lobes f = azimuthal(cos_theta_d, fcos((1.0f + u) * 0.5f * kPi), sigma_a);
         back = {back.r + b.r, back.tt + b.tt, back.trt + b.trt};
         fwd = {fwd.r + f.r, fwd.tt + f.tt, fwd.trt + f.trt};
     }
@@ -85,13 +122,13 @@ __device__ DualScatter dual_scatter(float cos_theta_d, F3 sigma_a, const Shape& 
     F3 rest = splat3(1.0f) - af2;
     F3 ab2 = a_b * a_b, ab3 = ab2 * a_b;
     F3 rest3 = rest * rest * rest;
-    F3 a_back = a_b * af2 / rest + ab3 * af2 / rest3;
-    F3 delta_b = alpha_b * (splat3(1.0f) - 2.0f * ab2 / (rest * rest)) +
-                 alpha_b * (2.0f * rest * rest + 4.0f * af2 * ab2) / rest3;
+    F3 a_back = fdiv3(a_b * af2, rest) + fdiv3(ab3 * af2, rest3);
+    F3 delta_b = alpha_b * (splat3(1.0f) - fdiv3(2.0f * ab2, rest * rest)) +
+                 fdiv3(alpha_b * (2.0f * rest * rest + 4.0f * af2 * ab2), rest3);
     F3 beta_f = vsqrt(beta_f2), beta_b = vsqrt(beta_b2);
-    F3 sigma_b = (splat3(1.0f) + 0.7f * af2) *
-                 (a_b * vsqrt(2.0f * beta_f2 + beta_b2) + ab3 * vsqrt(2.0f * beta_f2 + 3.0f * beta_b2)) /
-                 vmax(a_b + ab3 * (2.0f * beta_f + 3.0f * beta_b), splat3(1e-5f));
+    F3 sigma_b = fdiv3((splat3(1.0f) + 0.7f * af2) *
+                           (a_b * vsqrt(2.0f * beta_f2 + beta_b2) + ab3 * vsqrt(2.0f * beta_f2 + 3.0f * beta_b2)),
+                       vmax(a_b + ab3 * (2.0f * beta_f + 3.0f * beta_b), splat3(1e-5f)));
     return {fwd, a_f, beta_f2, a_back, delta_b, sigma_b * sigma_b};
 }
 
@@ -101,20 +138,22 @@ struct BsdfGeom {
     float theta_h, cos_theta_d, cos_phi;
 };
 
-__device__ __forceinline__ BsdfGeom bsdf_geom(F3 tangent, F3 view, F3 light) {
+// t and v already normalized (safe_normalize of the tangent and view vectors,
+// once per pixel). cos_phi is formed with the exact operations: near
+// cos_phi = -1 the R lobe takes sqrt(0.5 + 0.5 cos_phi), which turns an ulp
+// of cos_phi into a large relative change.
+__device__ __forceinline__ BsdfGeom bsdf_geom_n(F3 t, F3 v, F3 light) {
     BsdfGeom g;
-    F3 t = safe_normalize(tangent);
-    F3 v = safe_normalize(view);
     F3 l = safe_normalize(light);
     float sin_i = clampf(dot3(t, l), -1.0f, 1.0f);
     float sin_o = clampf(dot3(t, v), -1.0f, 1.0f);
-    float theta_i = asinf(sin_i), theta_o = asinf(sin_o);
+    float theta_i = fasin(sin_i), theta_o = fasin(sin_o);
     g.theta_h = theta_i + theta_o;
-    g.cos_theta_d = mx(cosf(0.5f * fabsf(theta_i - theta_o)), 1e-3f);
+    g.cos_theta_d = mx(fcos(0.5f * fabsf(theta_i - theta_o)), 1e-3f);
     F3 t_i = l - t * sin_i;
     F3 t_o = v - t * sin_o;
-    float proj = sqrtf(mx(dot3(t_i, t_i) * dot3(t_o, t_o), 1e-8f));
-    g.cos_phi = clampf(dot3(t_i, t_o) / proj, -1.0f, 1.0f);
+    float proj = __builtin_sqrtf(mx(dot3(t_i, t_i) * dot3(t_o, t_o), 1e-8f));
+    g.cos_phi = clampf(rdiv(dot3(t_i, t_o), proj), -1.0f, 1.0f);
     g.t = t;
     g.l = l;
     return g;
@@ -129,7 +168,7 @@ __device__ __forceinline__ F3 single_scatter(const BsdfGeom& g, F3 sigma_a, cons
 
 // Local backscattering lobe (shade.wesl:220).
 __device__ __forceinline__ F3 local_back(float theta_h, F3 a_back, F3 delta_b, F3 sigma_b2) {
-    return 2.0f * a_back * gaussian3(vsqrt(sigma_b2), splat3(theta_h) - delta_b) / kPi;
+    return 2.0f * a_back * gaussian3(vsqrt(sigma_b2), splat3(theta_h) - delta_b) * (1.0f / kPi);
 }
 
 // Global forward scattering through `depth` of hair (shade.wesl:225-242);
@@ -138,23 +177,23 @@ __device__ __forceinline__ F3 global_spread(float theta_h, float depth, float di
                                             const DualScatter& ds) {
     if (!(depth > 1e-3f)) return splat3(0.0f);
     F3 sigma_f2 = depth * ds.beta_f2;
-    F3 transmit = {powf(ds.a_f.x, depth), powf(ds.a_f.y, depth), powf(ds.a_f.z, depth)};
+    F3 transmit = {fpow(ds.a_f.x, depth), fpow(ds.a_f.y, depth), fpow(ds.a_f.z, depth)};
     F3 beta2 = shape.beta * shape.beta;
     F3 x = splat3(theta_h);
     F3 spread = gaussian3(vsqrt(splat3(beta2.x) + sigma_f2), x - splat3(shape.alpha.x)) * ds.fwd.r +
                 gaussian3(vsqrt(splat3(beta2.y) + sigma_f2), x - splat3(shape.alpha.y)) * ds.fwd.tt +
                 gaussian3(vsqrt(splat3(beta2.z) + sigma_f2), x - splat3(shape.alpha.z)) * ds.fwd.trt;
-    F3 back_spread = 2.0f * ds.a_back * gaussian3(vsqrt(ds.sigma_b2 + sigma_f2), x - ds.delta_b) / kPi;
-    return (1.0f - direct) * transmit * 0.7f * (spread / kPi + kPi * 0.7f * back_spread);
+    F3 back_spread = 2.0f * ds.a_back * gaussian3(vsqrt(ds.sigma_b2 + sigma_f2), x - ds.delta_b) * (1.0f / kPi);
+    return (1.0f - direct) * transmit * 0.7f * (spread * (1.0f / kPi) + kPi * 0.7f * back_spread);
 }
 
-// hair_bsdf (shade.wesl:193-244) with shadow = 1.
-__device__ F3 hair_bsdf(F3 tangent, F3 view, F3 light, F3 sigma_a, const Shape& shape, const DualScatter& ds,
+// hair_bsdf (shade.wesl:193-244) with shadow = 1; t_n, v_n normalized.
+__device__ F3 hair_bsdf(F3 t_n, F3 v_n, F3 light, F3 sigma_a, const Shape& shape, const DualScatter& ds,
                         float depth) {
-    BsdfGeom g = bsdf_geom(tangent, view, light);
+    BsdfGeom g = bsdf_geom_n(t_n, v_n, light);
     F3 single = single_scatter(g, sigma_a, shape);
     F3 back = local_back(g.theta_h, ds.a_back, ds.delta_b, ds.sigma_b2);
-    float direct = expf(-depth);
+    float direct = fexp(-depth);
     F3 color = direct * (single + 0.7f * back);
     if (depth > 1e-3f) color = color + global_spread(g.theta_h, depth, direct, shape, ds);
     return color;
@@ -185,7 +224,7 @@ __device__ __forceinline__ float dom_texel_visibility(const View& p, const uint3
             else if (i < t) occ = occ + count * (t - i);
         }
     }
-    return expf(-occ * p.filter_params.w);
+    return fexp(-occ * p.filter_params.w);
 }
 
 __device__ float dom_visibility(const View& p, const uint32_t* __restrict__ dom, F3 world, float beta) {
@@ -200,7 +239,7 @@ __device__ float dom_visibility(const View& p, const uint32_t* __restrict__ dom,
     if (ux < 0.0f || uy < 0.0f || ux > 1.0f || uy > 1.0f) return 1.0f;
     uint32_t dom_w = to_u32(p.dom_info.x), dom_h = to_u32(p.dom_info.y);
     float near_ = p.dom_info.z, far_ = p.dom_info.w;
-    float dist = dot3(world - xyz(p.light_eye), xyz(p.light_forward)) - (-logf(mx(beta, 1e-4f)) * p.filter_params.z);
+    float dist = dot3(world - xyz(p.light_eye), xyz(p.light_forward)) - (-flog(mx(beta, 1e-4f)) * p.filter_params.z);
     float fx = ux * u2f(dom_w) - 0.5f;
     float fy = uy * u2f(dom_h) - 0.5f;
     float flx = floorf(fx), fly = floorf(fy);
@@ -216,15 +255,15 @@ __device__ float dom_visibility(const View& p, const uint32_t* __restrict__ dom,
 __device__ __forceinline__ float distance_attenuation(float dist2, float inv_range2) {
     float factor = dist2 * inv_range2;
     float smooth = clampf(1.0f - factor * factor, 0.0f, 1.0f);
-    return smooth * smooth * 1.0f / mx(dist2, 1e-4f);
+    return smooth * smooth * frcp(mx(dist2, 1e-4f));
 }
 
 __device__ __forceinline__ float spot_mask(const hm_point_light& L, F3 world) {
     F3 sd = {L.light_custom_data[0], 0.0f, L.light_custom_data[1]};
-    sd.y = sqrtf(mx(0.0f, 1.0f - sd.x * sd.x - sd.z * sd.z));
+    sd.y = fsqrt(mx(0.0f, 1.0f - sd.x * sd.x - sd.z * sd.z));
     if (L.flags & HM_LIGHT_FLAG_SPOT_Y_NEGATIVE) sd.y = -sd.y;
     F3 ltf = F3{L.position_radius[0], L.position_radius[1], L.position_radius[2]} - world;
-    float cd = dot3(sd * -1.0f, safe_normalize(ltf));
+    float cd = dot3(sd * -1.0f, fnormalize(ltf));
     float att = clampf(cd * L.light_custom_data[2] + L.light_custom_data[3], 0.0f, 1.0f);
     return att * att;
 }
@@ -233,6 +272,7 @@ __device__ __forceinline__ float spot_mask(const hm_point_light& L, F3 world) {
 // Per-pixel inputs of shade.wesl:fs (shade.wesl:539-571, 442-446).
 struct PixelShade {
     F3 world, tangent, view, albedo, sigma_a;
+    F3 t_n, v_n;  // safe_normalize(tangent), safe_normalize(view)
     float view_dist, beta, roughness, cos_theta_o;
     Shape shape;
 };
@@ -254,6 +294,8 @@ __device__ __forceinline__ PixelShade pixel_setup(const View& p, uint64_t word, 
     s.view = ortho ? safe_normalize(F3{p.clip[2], p.clip[6], p.clip[10]}) : safe_normalize(xyz(p.camera_pos) - s.world);
     float bb = clampf(s.roughness * s.roughness, 0.02f, 1.0f);
     s.shape = {{tilt, -0.5f * tilt, -1.5f * tilt}, {bb, 0.5f * bb, 2.0f * bb}};
+    s.t_n = safe_normalize(s.tangent);
+    s.v_n = safe_normalize(s.view);
     float tv = dot3(s.tangent, s.view);
     s.cos_theta_o = sqrtf(mx(1.0f - tv * tv, 1e-6f));
     s.sigma_a = absorption_from_color(s.albedo, s.roughness);
@@ -282,7 +324,7 @@ __device__ F3 plain_pixel_color(const ShadeArgs& a, const View& p, const PixelSh
             float att = distance_attenuation(dist2, L.color_inverse_square_range[3]);
             if (att == 0.0f) continue;  // outside the light's range: the cluster would not list it
             F3 incident = safe_normalize(ltf);
-            F3 lobe = hair_bsdf(px.tangent, px.view, incident, px.sigma_a, px.shape, ds, 0.0f);
+            F3 lobe = hair_bsdf(px.t_n, px.v_n, incident, px.sigma_a, px.shape, ds, 0.0f);
             F3 c = lobe * point_color(L) * att;
             if (pass == 1) c = c * spot_mask(L, px.world);
             color = color + c;
@@ -294,10 +336,10 @@ __device__ F3 plain_pixel_color(const ShadeArgs& a, const View& p, const PixelSh
         bool casts = (L.flags & HM_LIGHT_FLAG_SHADOWS) != 0u;
         float depth = 0.0f;
         if (dom_left && casts) {
-            depth = -logf(mx(dom_visibility(p, a.dom, px.world, px.beta), 1e-6f));
+            depth = -flog(mx(dom_visibility(p, a.dom, px.world, px.beta), 1e-6f));
             dom_left = false;
         }
-        F3 lobe = hair_bsdf(px.tangent, px.view, F3{L.direction_to_light[0], L.direction_to_light[1],
+        F3 lobe = hair_bsdf(px.t_n, px.v_n, F3{L.direction_to_light[0], L.direction_to_light[1],
                                                      L.direction_to_light[2]},
                             px.sigma_a, px.shape, ds, depth);
         color = color + lobe * F3{L.color[0], L.color[1], L.color[2]};
