@@ -19,7 +19,7 @@
 //   k_dom_long<2>    long segments, global atomic add  (same)
 // The camera pass bins the same way into screen tiles (k_bin_gen<0>); its
 // tiles are split into chunks of records so that the few tiles holding most
-// of the hair spread over many workgroups (k_cam_tiles, k_cam_merge).
+// of the hair spread over many workgroups (k_cam_tiles, k_cam_finish).
 // When the records outgrow their buffer the gates switch every binned kernel
 // off and the direct kernels (clear + k_raster<1> + k_raster<2>) run instead.
 // DOM words are built from min and add only, so the result does not depend on
@@ -28,6 +28,7 @@
 #include "hm_kernels.h"
 
 #include "hm_device.h"
+#include "hm_lists.h"
 #include "hm_raster.h"
 
 namespace hm {
@@ -251,11 +252,20 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
     total = total < a.capacity ? total : a.capacity;
     const Run run = workgroup_run(total * BIN_STRIDE);
     const uint4* boxes = reinterpret_cast<const uint4*>(a.seg_box);
-    for (uint32_t q = run.first + threadIdx.x; q < run.end; q += blockDim.x) {
-        const uint4 b = boxes[q];
-        const uint32_t bx[4] = {b.x, b.y, b.z, b.w};
+    const uint4 none = make_uint4(BIN_NONE, BIN_NONE, BIN_NONE, BIN_NONE);
+    for (uint32_t q0 = run.first + threadIdx.x; q0 < run.end; q0 += 4u * blockDim.x) {
+        uint4 bq[4];  // four loads in flight per thread
 #pragma unroll
-        for (int k = 0; k < 4; ++k) for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { atomicAdd(hist + t, 1u); });
+        for (uint32_t u = 0; u < 4u; ++u) {
+            const uint32_t q = q0 + u * blockDim.x;
+            bq[u] = q < run.end ? boxes[q] : none;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; ++u) {
+            const uint32_t bx[4] = {bq[u].x, bq[u].y, bq[u].z, bq[u].w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { atomicAdd(hist + t, 1u); });
+        }
     }
     __syncthreads();
     uint32_t mine = 0u;
@@ -278,35 +288,41 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
 }
 
 // Exclusive scan of the per-tile counts (one block of 1024 threads, 16 tiles
-// per thread), then the gate words.
+// per thread), then the gate words. Counts and results pass through LDS so
+// that every global load and store is coalesced.
 // With chunk > 0 it also splits each tile's records into chunks of at most
 // `chunk` (at least one chunk per tile, so empty tiles are still written):
-// chunk offsets after the cursors, and the tile of every chunk in chunk_tile.
+// chunk offsets after the cursors, the tile of every chunk in chunk_tile,
+// and BIN_SPLIT set when some tile has more than one chunk.
 __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tiles, uint32_t record_capacity,
                                                    uint32_t long_capacity, uint32_t list_capacity, uint32_t chunk,
                                                    uint32_t* chunk_tile, uint32_t chunk_capacity) {
     constexpr uint32_t PER = BIN_MAX_TILES / 1024u;
-    __shared__ uint32_t wave_total[16], wave_chunks[16];
+    __shared__ __align__(16) uint32_t stage[BIN_MAX_TILES];
+    __shared__ uint32_t wave_total[16], wave_chunks[16], split, ok_s;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t* cnt = meta + BIN_META_WORDS;
     uint32_t* off = meta + BIN_META_WORDS + tiles;
     uint32_t* cur = off + tiles + 1u;
+    uint32_t* choff = cur + tiles;
+    if (tid == 0) split = 0u;
+    for (uint32_t i = tid; i < BIN_MAX_TILES; i += 1024u) stage[i] = i < tiles ? cnt[i] : 0u;
+    __syncthreads();
     const uint32_t b = tid * PER;
     uint32_t v[PER];
-    uint32_t sum = 0u;
-    // Unconditional (clamped) loads, so all sixteen are in flight at once.
 #pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) v[k] = cnt[b + k < tiles ? b + k : tiles - 1u];
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-        v[k] = b + k < tiles ? v[k] : 0u;
-        sum += v[k];
+    for (uint32_t q = 0; q < PER / 4u; ++q) {
+        const uint4 w = reinterpret_cast<const uint4*>(stage)[tid * (PER / 4u) + q];
+        v[4 * q] = w.x;
+        v[4 * q + 1] = w.y;
+        v[4 * q + 2] = w.z;
+        v[4 * q + 3] = w.w;
     }
-    uint32_t nch[PER];
-    uint32_t csum = 0u;
+    uint32_t sum = 0u, csum = 0u, nch[PER];
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
-        nch[k] = b + k < tiles ? (chunk ? (v[k] > chunk ? (v[k] + chunk - 1u) / chunk : 1u) : 1u) : 0u;
+        sum += v[k];
+        nch[k] = b + k < tiles ? (chunk && v[k] > chunk ? (v[k] + chunk - 1u) / chunk : 1u) : 0u;
         csum += nch[k];
     }
     uint32_t inc = sum, cinc = csum;
@@ -328,33 +344,63 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
         run += wave_total[w];
         crun += wave_chunks[w];
     }
-    uint32_t* choff = cur + tiles;
+    // Record offsets through LDS, then out (offset and cursor arrays).
+    uint32_t r = run, cr = crun;
+    bool any_split = false;
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
-        if (b + k < tiles) {
-            off[b + k] = run;
-            cur[b + k] = run;
-            choff[b + k] = crun;
-            for (uint32_t c = 0; c < nch[k]; ++c)
-                if (crun + c < chunk_capacity) chunk_tile[crun + c] = b + k;
+        stage[b + k] = r;
+        r += v[k];
+        any_split |= nch[k] > 1u;
+    }
+    if (any_split) split = 1u;
+    __syncthreads();
+    for (uint32_t i = tid; i < tiles; i += 1024u) {
+        const uint32_t o = stage[i];
+        off[i] = o;
+        cur[i] = o;
+    }
+    __syncthreads();
+    // Chunk offsets the same way; then each chunk finds its tile by binary
+    // search over them (coalesced chunk_tile stores).
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        stage[b + k] = cr;
+        cr += nch[k];
+    }
+    __syncthreads();
+    const uint32_t chunks = wave_chunks[0] + wave_chunks[1] + wave_chunks[2] + wave_chunks[3] + wave_chunks[4] +
+                            wave_chunks[5] + wave_chunks[6] + wave_chunks[7] + wave_chunks[8] + wave_chunks[9] +
+                            wave_chunks[10] + wave_chunks[11] + wave_chunks[12] + wave_chunks[13] + wave_chunks[14] +
+                            wave_chunks[15];
+    for (uint32_t i = tid; i < tiles; i += 1024u) choff[i] = stage[i];
+    const uint32_t nmap = !chunk_tile ? 0u : (chunks < chunk_capacity ? chunks : chunk_capacity);
+    for (uint32_t c = tid; c < nmap; c += 1024u) {
+        uint32_t lo = 0u, hi = tiles;  // last tile whose first chunk <= c
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (stage[mid] <= c) lo = mid;
+            else hi = mid;
         }
-        run += v[k];
-        crun += nch[k];
+        chunk_tile[c] = lo;
     }
     if (tid == 1023u) {
-        const uint32_t total = run;
+        const uint32_t total = r;
         off[tiles] = total;
-        choff[tiles] = crun;
+        choff[tiles] = chunks;
         const uint32_t longs = meta[BIN_LONG_COUNT];
         const uint32_t ok = (total <= record_capacity && longs <= long_capacity &&
-                             meta[BIN_LIST_COUNT] <= list_capacity && crun <= chunk_capacity)
+                             meta[BIN_LIST_COUNT] <= list_capacity && chunks <= chunk_capacity)
                                 ? 1u
                                 : 0u;
         meta[BIN_RECORDS] = total;
         meta[BIN_OK] = ok;
         meta[BIN_FALLBACK] = 1u - ok;
         meta[BIN_LONG] = (ok && longs) ? 1u : 0u;
+        ok_s = ok;
     }
+    __syncthreads();
+    if (tid == 0) meta[BIN_SPLIT] = (split && ok_s) ? 1u : 0u;
 }
 
 // Same runs as k_bin_count: the workgroup's (tile, rank) list gives the first
@@ -373,14 +419,24 @@ __global__ __launch_bounds__(256) void k_bin_fill(BinArgs a) {
     total = total < a.capacity ? total : a.capacity;
     const Run run = workgroup_run(total * BIN_STRIDE);
     const uint4* boxes = reinterpret_cast<const uint4*>(a.seg_box);
+    const uint4 none = make_uint4(BIN_NONE, BIN_NONE, BIN_NONE, BIN_NONE);
     uint32_t* records = a.records;
-    for (uint32_t q = run.first + threadIdx.x; q < run.end; q += blockDim.x) {
-        const uint4 b = boxes[q];
-        const uint32_t bx[4] = {b.x, b.y, b.z, b.w};
+    for (uint32_t q0 = run.first + threadIdx.x; q0 < run.end; q0 += 4u * blockDim.x) {
+        uint4 bq[4];  // four loads in flight per thread
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t id = q * 4u + (uint32_t)k;
-            for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { records[atomicAdd(hist + t, 1u)] = id; });
+        for (uint32_t u = 0; u < 4u; ++u) {
+            const uint32_t q = q0 + u * blockDim.x;
+            bq[u] = q < run.end ? boxes[q] : none;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; ++u) {
+            const uint32_t bx[4] = {bq[u].x, bq[u].y, bq[u].z, bq[u].w};
+            const uint32_t q = q0 + u * blockDim.x;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t id = q * 4u + (uint32_t)k;
+                for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { records[atomicAdd(hist + t, 1u)] = id; });
+            }
         }
     }
 }
@@ -548,12 +604,12 @@ __device__ __forceinline__ void walk_cam_record(D& d, const BinArgs& a, uint32_t
                    F3{t1.x, t1.y, t1.z}, F3{si.x, si.y, w0}, F3{si.x, si.y, w1}, t0.w, t1.w, si.z);
 }
 
-// Final words of one pixel of a tile (every lane of the wave calls it): the
-// G-buffer, and with emit_lists the shade / filter lists and the empty
-// results of uncovered pixels.
-__device__ __forceinline__ void finish_pixel(const BinArgs& a, bool in, size_t g, uint64_t c, uint64_t k, uint32_t b) {
-    const int lane = (int)(threadIdx.x & 63u);
-    if (in) {
+// Final words of one pixel of a tile (every thread of the workgroup calls it):
+// the G-buffer, and with emit_lists the pixel's entries in the shade / filter
+// sub-lists of its tile and the empty results of uncovered pixels.
+__device__ __forceinline__ void finish_pixel(const BinArgs& a, uint32_t tile, bool in, size_t g, uint64_t c,
+                                             uint64_t k, uint32_t b, bool write_gbuffer, uint32_t* scratch) {
+    if (in && write_gbuffer) {
         a.center[g] = c;
         a.cons[g] = k;
         a.beta[g] = b;
@@ -569,116 +625,82 @@ __device__ __forceinline__ void finish_pixel(const BinArgs& a, bool in, size_t g
         a.filtered[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         a.filtered_depth[g] = 0.0f;
     }
-    const uint64_t ms = __ballot(shade), mf = __ballot(filter);
-    uint32_t bs = 0u, bf = 0u;
-    if (lane == 0) {
-        if (ms) bs = atomicAdd(a.meta + BIN_SHADE_COUNT, (uint32_t)__popcll(ms));
-        if (mf) bf = atomicAdd(a.meta + BIN_FILTER_COUNT, (uint32_t)__popcll(mf));
-    }
-    bs = __shfl(bs, 0);
-    bf = __shfl(bf, 0);
-    if (shade) a.shade_list[bs + lanes_below(ms)] = (uint32_t)g;
-    if (filter) a.filter_list[bf + lanes_below(mf)] = (uint32_t)g;
-}
-
-// Partial tile of one chunk: centre and conservative words, then beta words.
-template <int T>
-__device__ __forceinline__ uint64_t* partial_tile(const BinArgs& a, uint32_t chunk) {
-    return reinterpret_cast<uint64_t*>(a.partials + (size_t)chunk * (size_t)(T * T * 20));
+    const uint32_t sub = tile % LIST_SUBS;
+    block_append(PixelList{a.shade_list, a.meta + BIN_SHADE_SUBS, a.list_cap}, sub, shade, (uint32_t)g, scratch);
+    block_append(PixelList{a.filter_list, a.meta + BIN_FILTER_SUBS, a.list_cap}, sub, filter, (uint32_t)g, scratch);
 }
 
 // One workgroup per chunk of a screen tile's records. The tile starts empty
-// (this is the clear) and takes its segments with LDS atomics. A tile with
-// one chunk is written as it is; the chunks of a larger tile leave partial
-// tiles for k_cam_merge.
+// and takes its segments with LDS atomics. A tile with one chunk is written as
+// it is (this is its clear); the chunks of a larger tile take the minimum into
+// the G-buffer, cleared beforehand, with global atomics, and k_cam_finish
+// then emits that tile's pixel lists.
 template <int T>
 __global__ __launch_bounds__(256) void k_cam_tiles(BinArgs a) {
     static_assert((T * T) % 256 == 0, "the write-back loop runs whole waves");
     if (a.meta[BIN_OK] == 0u) return;
     const uint32_t* off = a.meta + BIN_META_WORDS + a.tiles;
     const uint32_t* choff = off + 2u * a.tiles + 1u;
-    const uint32_t chunk = blockIdx.x;
-    if (chunk >= choff[a.tiles]) return;
+    const uint32_t nchunks = choff[a.tiles];
     __shared__ uint64_t center[T * T], cons[T * T];
     __shared__ uint32_t beta[T * T];
+    __shared__ uint32_t scratch[5];
     const View& p = bin_view(a);
-    const uint32_t tile = a.chunk_tile[chunk];
-    const uint32_t first = choff[tile], nch = choff[tile + 1u] - first;
-    const int32_t ox = (int32_t)(tile % a.tiles_x) * T, oy = (int32_t)(tile / a.tiles_x) * T;
     const int32_t W = (int32_t)a.width, H = (int32_t)a.height;
-    for (int i = (int)threadIdx.x; i < T * T; i += 256) {
-        center[i] = ~0ull;
-        cons[i] = ~0ull;
-        beta[i] = ~0u;
-    }
-    __syncthreads();
-    const uint32_t tile_end = off[tile + 1u];
-    const uint32_t begin = off[tile] + (chunk - first) * (a.chunk ? a.chunk : 0u);
-    const uint32_t end = nch > 1u && begin + a.chunk < tile_end ? begin + a.chunk : tile_end;
-    CamTileSink<T> sink{center, cons, beta, ox, oy};
-    Drawer<0, CamTileSink<T>> d{p, a.scene_depth, a.scene_clear, sink, {0, 0, 0, 0}};
-    for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) walk_cam_record(d, a, a.records[r]);
-    __syncthreads();
-    if (nch > 1u) {
-        uint64_t* part = partial_tile<T>(a, chunk);
-        uint32_t* part_beta = reinterpret_cast<uint32_t*>(part + 2 * T * T);
+    // The grid covers the usual number of chunks with one workgroup each (the
+    // hardware dispatcher balances them); a grid-stride loop takes any rest.
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const uint32_t tile = a.chunk_tile[chunk];
+        const uint32_t first = choff[tile], nch = choff[tile + 1u] - first;
+        const int32_t ox = (int32_t)(tile % a.tiles_x) * T, oy = (int32_t)(tile / a.tiles_x) * T;
         for (int i = (int)threadIdx.x; i < T * T; i += 256) {
-            part[i] = center[i];
-            part[T * T + i] = cons[i];
-            part_beta[i] = beta[i];
+            center[i] = ~0ull;
+            cons[i] = ~0ull;
+            beta[i] = ~0u;
         }
-        return;
-    }
-    for (int i = (int)threadIdx.x; i < T * T; i += 256) {  // T * T is a multiple of 256: whole waves
-        const int32_t x = ox + i % T, y = oy + i / T;
-        const bool in = x < W && y < H;
-        finish_pixel(a, in, in ? (size_t)y * (size_t)W + (size_t)x : 0u, center[i], cons[i], beta[i]);
+        __syncthreads();
+        const uint32_t tile_end = off[tile + 1u];
+        const uint32_t begin = off[tile] + (chunk - first) * (a.chunk ? a.chunk : 0u);
+        const uint32_t end = nch > 1u && begin + a.chunk < tile_end ? begin + a.chunk : tile_end;
+        CamTileSink<T> sink{center, cons, beta, ox, oy};
+        Drawer<0, CamTileSink<T>> d{p, a.scene_depth, a.scene_clear, sink, {0, 0, 0, 0}};
+        for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) walk_cam_record(d, a, a.records[r]);
+        __syncthreads();
+        for (int i = (int)threadIdx.x; i < T * T; i += 256) {  // T * T is a multiple of 256: whole waves
+            const int32_t x = ox + i % T, y = oy + i / T;
+            const bool in = x < W && y < H;
+            const size_t g = in ? (size_t)y * (size_t)W + (size_t)x : 0u;
+            if (nch > 1u) {
+                if (in) {
+                    if (cons[i] != ~0ull) atomicMin((unsigned long long*)a.cons + g, (unsigned long long)cons[i]);
+                    if (center[i] != ~0ull) atomicMin((unsigned long long*)a.center + g, (unsigned long long)center[i]);
+                    if (beta[i] != ~0u) atomicMin(a.beta + g, beta[i]);
+                }
+            } else {
+                finish_pixel(a, tile, in, g, center[i], cons[i], beta[i], true, scratch);
+            }
+        }
+        __syncthreads();  // the next chunk reuses the LDS tile
     }
 }
 
-// Tiles that were split into chunks: the minimum over the partial tiles.
+// Tiles that were split into chunks: their finished words are in the
+// G-buffer; emit the pixel lists.
 template <int T>
-__global__ __launch_bounds__(256) void k_cam_merge(BinArgs a) {
-    if (a.meta[BIN_OK] == 0u) return;
+__global__ __launch_bounds__(256) void k_cam_finish(BinArgs a) {
+    if (a.meta[BIN_SPLIT] == 0u || !a.emit_lists) return;
     const uint32_t tile = blockIdx.x;
     const uint32_t* choff = a.meta + BIN_META_WORDS + 3u * a.tiles + 1u;
-    const uint32_t first = choff[tile], nch = choff[tile + 1u] - first;
-    if (nch <= 1u) return;
+    if (choff[tile + 1u] - choff[tile] <= 1u) return;
+    __shared__ uint32_t scratch[5];
     const int32_t ox = (int32_t)(tile % a.tiles_x) * T, oy = (int32_t)(tile / a.tiles_x) * T;
     const int32_t W = (int32_t)a.width, H = (int32_t)a.height;
     for (int i = (int)threadIdx.x; i < T * T; i += 256) {
-        uint64_t c = ~0ull, k = ~0ull;
-        uint32_t b = ~0u;
-        // Four partial tiles per step, so their loads overlap.
-        uint32_t j = 0;
-        for (; j + 4u <= nch; j += 4u) {
-            uint64_t pc[4], pk[4];
-            uint32_t pb[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint64_t* part = partial_tile<T>(a, first + j + (uint32_t)u);
-                pc[u] = part[i];
-                pk[u] = part[T * T + i];
-                pb[u] = reinterpret_cast<const uint32_t*>(part + 2 * T * T)[i];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                c = pc[u] < c ? pc[u] : c;
-                k = pk[u] < k ? pk[u] : k;
-                b = pb[u] < b ? pb[u] : b;
-            }
-        }
-        for (; j < nch; ++j) {
-            const uint64_t* part = partial_tile<T>(a, first + j);
-            const uint64_t pc = part[i], pk = part[T * T + i];
-            const uint32_t pb = reinterpret_cast<const uint32_t*>(part + 2 * T * T)[i];
-            c = pc < c ? pc : c;
-            k = pk < k ? pk : k;
-            b = pb < b ? pb : b;
-        }
         const int32_t x = ox + i % T, y = oy + i / T;
         const bool in = x < W && y < H;
-        finish_pixel(a, in, in ? (size_t)y * (size_t)W + (size_t)x : 0u, c, k, b);
+        const size_t g = in ? (size_t)y * (size_t)W + (size_t)x : 0u;
+        const uint64_t c = in ? a.center[g] : ~0ull, k = in ? a.cons[g] : ~0ull;
+        finish_pixel(a, tile, in, g, c, k, 0u, false, scratch);
     }
 }
 
@@ -693,19 +715,20 @@ __global__ __launch_bounds__(256) void k_cam_long(BinArgs a) {
         walk_cam_record(d, a, a.long_list[i]);
 }
 
-hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int long_blocks) {
-    // One workgroup per possible chunk; those past the scan's chunk total return.
-    if (a.tile_shift == 4u) {
-        hipLaunchKernelGGL(k_cam_tiles<16>, dim3(a.chunk_capacity), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_cam_merge<16>, dim3(a.tiles), dim3(256), 0, s, a);
-    } else if (a.tile_shift == 5u) {
-        hipLaunchKernelGGL(k_cam_tiles<32>, dim3(a.chunk_capacity), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_cam_merge<32>, dim3(a.tiles), dim3(256), 0, s, a);
-    } else {
-        return hipErrorInvalidValue;
-    }
-    hipError_t e = hipGetLastError();
+hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int tile_blocks, int long_blocks) {
+    if (a.tile_shift != 4u && a.tile_shift != 5u) return hipErrorInvalidValue;
+    // Split tiles take atomic minima into the G-buffer: clear it first (only then).
+    hipError_t e = launch_fill32(s, a.gb_slab, (size_t)a.gb_words, 0xffffffffu, long_blocks, a.meta + BIN_SPLIT);
     if (e != hipSuccess) return e;
+    const uint32_t grid = a.chunk_capacity < (uint32_t)tile_blocks ? a.chunk_capacity : (uint32_t)tile_blocks;
+    if (a.tile_shift == 4u) {
+        hipLaunchKernelGGL(k_cam_tiles<16>, dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_cam_finish<16>, dim3(a.tiles), dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_cam_tiles<32>, dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_cam_finish<32>, dim3(a.tiles), dim3(256), 0, s, a);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_cam_long, dim3(long_blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }

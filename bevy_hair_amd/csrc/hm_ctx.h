@@ -35,14 +35,13 @@ struct hm_bin_state {
     uint32_t* meta[2] = {nullptr, nullptr};
     size_t meta_words[2] = {0, 0};
     uint32_t* records = nullptr;
-    uint2* lists = nullptr;         // same length as records
     size_t record_capacity = 0;
+    uint2* lists = nullptr;         // (tile, rank) entries of the counting workgroups
+    size_t list_capacity = 0;
     uint2* wg_list = nullptr;       // one per binning workgroup
     size_t wg_capacity = 0;
     uint32_t* chunk_tile = nullptr; // tile of each tile-workgroup chunk
     size_t chunk_capacity = 0;
-    uint8_t* partials = nullptr;    // camera: partial tiles of split tiles
-    size_t partial_bytes = 0;
 };
 
 struct hm_ctx {
@@ -62,7 +61,8 @@ struct hm_ctx {
     uint32_t* dom = nullptr;
     uint32_t* pix_list = nullptr;   // covered-pixel list (shade, then filter)
     uint32_t* pix_list2 = nullptr;  // filter list when the camera tiles write both lists
-    uint32_t* pix_count = nullptr;
+    size_t list_len = 0;            // entries in pix_list / pix_list2
+    uint32_t* pix_count = nullptr;  // sub-list counters: shade, then filter
     float* scene_depth = nullptr;
     float scene_clear = 0.0f;
     hm_point_light* points = nullptr;

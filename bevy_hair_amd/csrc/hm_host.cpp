@@ -296,19 +296,18 @@ void free_bin(hm_ctx* c) {
     dev_free(b.lists);
     dev_free(b.wg_list);
     dev_free(b.chunk_tile);
-    dev_free(b.partials);
     b = hm_bin_state();
 }
 
 // Grows the binning buffers to hold `slots` strand slots, `meta_words`
-// status + per-tile words of pass `pass`, `records` tile records, `wgs`
-// binning workgroups, `chunks` tile-workgroup chunks and `partial_bytes` of
-// partial tiles. The other pass's status words are left alone.
-hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size_t records, size_t wgs,
-                      size_t chunks, size_t partial_bytes) {
+// status + per-tile words of pass `pass`, `records` tile records, `lists`
+// (tile, rank) entries, `wgs` binning workgroups and `chunks` tile-workgroup
+// chunks. The other pass's status words are left alone.
+hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size_t records, size_t lists,
+                      size_t wgs, size_t chunks) {
     hm_bin_state& b = c->bin;
     if (slots <= b.slots && meta_words <= b.meta_words[pass] && records <= b.record_capacity &&
-        wgs <= b.wg_capacity && chunks <= b.chunk_capacity && partial_bytes <= b.partial_bytes)
+        lists <= b.list_capacity && wgs <= b.wg_capacity && chunks <= b.chunk_capacity)
         return hipSuccess;
     hipError_t e = hipStreamSynchronize(c->stream);  // earlier frames may still read them
     if (e != hipSuccess) return e;
@@ -337,11 +336,15 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
     }
     if (records > b.record_capacity) {
         dev_free(b.records);
-        dev_free(b.lists);
         b.record_capacity = 0;
         if ((e = dev_alloc(&b.records, records * 4)) != hipSuccess) return e;
-        if ((e = dev_alloc(&b.lists, records * sizeof(uint2))) != hipSuccess) return e;
         b.record_capacity = records;
+    }
+    if (lists > b.list_capacity) {
+        dev_free(b.lists);
+        b.list_capacity = 0;
+        if ((e = dev_alloc(&b.lists, lists * sizeof(uint2))) != hipSuccess) return e;
+        b.list_capacity = lists;
     }
     if (wgs > b.wg_capacity) {
         dev_free(b.wg_list);
@@ -354,12 +357,6 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
         b.chunk_capacity = 0;
         if ((e = dev_alloc(&b.chunk_tile, chunks * 4)) != hipSuccess) return e;
         b.chunk_capacity = chunks;
-    }
-    if (partial_bytes > b.partial_bytes) {
-        dev_free(b.partials);
-        b.partial_bytes = 0;
-        if ((e = dev_alloc(&b.partials, partial_bytes)) != hipSuccess) return e;
-        b.partial_bytes = partial_bytes;
     }
     return hipSuccess;
 }
@@ -574,9 +571,12 @@ hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_
     if (e == hipSuccess) e = dev_alloc(&ctx->filtered, px * 16);
     if (e == hipSuccess) e = dev_alloc(&ctx->filtered_depth, px * 4);
     if (e == hipSuccess) e = dev_alloc(&ctx->dom, dom_words * 4);
-    if (e == hipSuccess) e = dev_alloc(&ctx->pix_list, px * 4);
-    if (e == hipSuccess) e = dev_alloc(&ctx->pix_list2, px * 4);
-    if (e == hipSuccess) e = dev_alloc(&ctx->pix_count, 64);
+    // Covered-pixel sub-lists: room for every pixel of every (partial) tile
+    // up to 32 x 32 plus one tile's slack per sub-list.
+    ctx->list_len = ((size_t)width + 32u) * ((size_t)height + 32u) + (size_t)hm::LIST_SUBS * 1024u;
+    if (e == hipSuccess) e = dev_alloc(&ctx->pix_list, ctx->list_len * 4);
+    if (e == hipSuccess) e = dev_alloc(&ctx->pix_list2, ctx->list_len * 4);
+    if (e == hipSuccess) e = dev_alloc(&ctx->pix_count, 2 * hm::LIST_SUBS * 4);
     if (e == hipSuccess) e = hipMemset(ctx->dom, 0, dom_words * 4);
     if (e != hipSuccess) {
         free_targets(ctx);
@@ -817,15 +817,20 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
                            uint32_t tiles_y, hm::BinArgs& b) {
     const uint32_t tiles = tiles_x * tiles_y;
     const size_t slots = g->capacity ? g->capacity : 1u;
-    const size_t want_records = ctx->tuning.bin_records ? ctx->tuning.bin_records : slots * hm::BIN_STRIDE * 2u;
+    // Every segment (126 per strand at most) spans at most box_limit tiles or
+    // goes to the long list, so this many records always fit. The default
+    // record buffer is that size; a smaller bin_records can overflow, and then
+    // the device switches to the direct kernels.
+    const size_t worst = std::min<size_t>(slots * 126u * ctx->tuning.bin_box_limit, 0xffffffffu);
+    const size_t want_records = ctx->tuning.bin_records ? std::min<size_t>(ctx->tuning.bin_records, worst) : worst;
     const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
+    // (tile, rank) list entries: at most one per (counting workgroup, tile) and per record.
+    const size_t lists = std::min<size_t>(want_records, (size_t)bin_blocks * tiles);
     // Camera tiles split into chunks: at most tiles + records / chunk of them.
     const uint32_t chunk = pass == HM_PASS_CAMERA ? ctx->tuning.cam_chunk : 0u;
-    const size_t records_cap = std::min<size_t>(want_records, 0xffffffffu);
-    const size_t chunks = tiles + (chunk ? records_cap / chunk + 1u : 0u);
-    const size_t partial = pass == HM_PASS_CAMERA && chunk ? chunks * (size_t)T * T * 20u : 0u;
-    hipError_t e = ensure_bin(ctx, pass, slots, hm::BIN_META_WORDS + 4u * (size_t)tiles + 2u, want_records,
-                              (size_t)bin_blocks, chunks, partial);
+    const size_t chunks = tiles + (chunk ? want_records / chunk + 1u : 0u);
+    hipError_t e = ensure_bin(ctx, pass, slots, hm::BIN_META_WORDS + 4u * (size_t)tiles + 2u, want_records, lists,
+                              (size_t)bin_blocks, chunks);
     if (e != hipSuccess) return e;
     std::memset(&b, 0, sizeof(b));
     std::memcpy(&b.view, p, sizeof(hm_params));
@@ -840,14 +845,16 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     b.seg_box = ctx->bin.seg_box;
     b.meta = ctx->bin.meta[pass];
     b.records = ctx->bin.records;
-    b.record_capacity = (uint32_t)std::min<size_t>(want_records, 0xffffffffu);
+    b.record_capacity = (uint32_t)want_records;
+    b.can_overflow = want_records < worst ? 1u : 0u;
     b.lists = ctx->bin.lists;
-    b.list_capacity = b.record_capacity;
+    b.list_capacity = (uint32_t)std::min<size_t>(lists, 0xffffffffu);
     b.wg_list = ctx->bin.wg_list;
     b.chunk = chunk;
-    b.chunk_tile = ctx->bin.chunk_tile;
+    b.chunk_tile = pass == HM_PASS_CAMERA ? ctx->bin.chunk_tile : nullptr;  // DOM tiles are never split
     b.chunk_capacity = (uint32_t)std::min<size_t>(chunks, 0xffffffffu);
-    b.partials = ctx->bin.partials;
+    b.gb_slab = static_cast<uint32_t*>(ctx->gb_slab);
+    b.gb_words = ctx->gb_words;
     b.long_list = ctx->bin.long_list;
     b.long_capacity = (uint32_t)std::min<size_t>(slots * hm::BIN_STRIDE, 0xffffffffu);
     b.tiles_x = tiles_x;
@@ -868,12 +875,18 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     return hipSuccess;
 }
 
+// Where shade and filter find their covered-pixel lists after gbuffer_impl.
+struct TileLists {
+    bool from_tiles = false;          // the camera tiles wrote them (sub-list counters in the camera status words)
+    uint32_t cap = 0;                 // their entries per sub-list
+    const uint32_t* gate = nullptr;   // with from_tiles: nonzero when the direct kernels ran instead;
+                                      // null when the records cannot overflow
+};
+
 // hm_gbuffer; with emit_lists (a frame's own call) the camera tiles also
-// write the shade and filter pixel lists, and *lists_gate is set to the word
-// that is nonzero when they did not (the direct kernels ran), else null.
-static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool emit_lists,
-                              const uint32_t** lists_gate) {
-    if (lists_gate) *lists_gate = nullptr;
+// write the shade and filter pixel lists, described in *lists.
+static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool emit_lists, TileLists* lists) {
+    if (lists) *lists = TileLists();
     hm::RasterArgs a;
     hm_status st = raster_args(ctx, g, p, HM_PASS_CAMERA, a);
     if (st != HM_OK) return st;
@@ -891,6 +904,12 @@ static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool
     b.emit_lists = emit_lists ? 1u : 0u;
     b.shade_list = ctx->pix_list;
     b.filter_list = ctx->pix_list2;
+    // Tile t writes sub-list t % LIST_SUBS; the fallback compaction, if it
+    // runs instead, writes the same layout, so the capacity covers both.
+    b.list_cap = std::max((b.tiles + hm::LIST_SUBS - 1u) / hm::LIST_SUBS * T * T,
+                          hm::compact_list_cap(ctx->width * ctx->height, true));
+    if ((size_t)b.list_cap * hm::LIST_SUBS > ctx->list_len)
+        return fail(ctx, HM_ERR_INVALID, "pixel lists too small for the camera tiles");
     b.shaded = ctx->shaded;
     b.shaded_depth = ctx->shaded_depth;
     b.filtered = ctx->filtered;
@@ -901,11 +920,19 @@ static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool
     }
     {
         Timer t(ctx, HM_K_CAM_TILES);
-        HM_HIP(ctx, hm::launch_cam_tiles(ctx->stream, b, ctx->cu_count));
+        // One workgroup per chunk for up to 2 records per segment slot.
+        const uint64_t usual = (uint64_t)b.tiles + (b.chunk ? (uint64_t)g->capacity * 256u / b.chunk : 0u);
+        const int tile_blocks = (int)std::min<uint64_t>(std::max<uint64_t>(usual, 1u), 1u << 20);
+        HM_HIP(ctx, hm::launch_cam_tiles(ctx->stream, b, tile_blocks, ctx->cu_count));
     }
-    // Direct kernels, switched on by the device only when the records overflowed.
     const uint32_t* fallback = ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_FALLBACK;
-    if (emit_lists && lists_gate) *lists_gate = fallback;
+    if (emit_lists && lists) {
+        lists->from_tiles = true;
+        lists->cap = b.list_cap;
+        lists->gate = b.can_overflow ? fallback : nullptr;
+    }
+    if (!b.can_overflow) return HM_OK;
+    // Direct kernels, switched on by the device only when the records overflowed.
     {
         Timer t(ctx, HM_K_CLEAR);
         HM_HIP(ctx, hm::launch_fill32(ctx->stream, static_cast<uint32_t*>(ctx->gb_slab), ctx->gb_words, 0xffffffffu,
@@ -962,6 +989,7 @@ static hm_status dom_impl(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const h
         Timer t(ctx, HM_K_DOM_TILES);
         HM_HIP(ctx, hm::launch_dom_tiles(ctx->stream, b, ctx->cu_count));
     }
+    if (!b.can_overflow) return HM_OK;
     // Direct kernels, switched on by the device only when the records overflowed.
     const uint32_t* fallback = ctx->bin.meta[HM_PASS_DOM] + hm::BIN_FALLBACK;
     const size_t texels = (size_t)S * S;
@@ -990,9 +1018,10 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
     return dom_impl(ctx, g, dp, op, false);
 }
 
-// Shading over the covered-pixel list. With lists_gate (see gbuffer_impl) the
-// list is the camera tiles' and the compaction runs only if that word is set.
-static hm_status shade_impl(hm_ctx* ctx, const hm_params* p, float exposure, const uint32_t* lists_gate) {
+// Shading over the covered-pixel list: the camera tiles' (then the
+// compaction runs only if the direct kernels may have run instead) or one
+// compacted here.
+static hm_status shade_impl(hm_ctx* ctx, const hm_params* p, float exposure, const TileLists& lists) {
     hm_status st = check_targets(ctx);
     if (st != HM_OK) return st;
     if (!p || (uint32_t)p->screen[0] != ctx->width || (uint32_t)p->screen[1] != ctx->height)
@@ -1013,16 +1042,24 @@ static hm_status shade_impl(hm_ctx* ctx, const hm_params* p, float exposure, con
     a.shaded_depth = ctx->shaded_depth;
     Timer t(ctx, HM_K_SHADE);
     uint32_t px = ctx->width * ctx->height;
-    uint32_t* count = lists_gate ? ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_SHADE_COUNT : ctx->pix_count;
-    HM_HIP(ctx, hm::launch_compact(ctx->stream, 0, ctx->center, ctx->cons, px, ctx->pix_list, count, ctx->shaded,
-                                   ctx->shaded_depth, lists_gate));
-    HM_HIP(ctx, hm::launch_shade(ctx->stream, a, ctx->pix_list, count, px, ctx->cu_count * 8));
+    hm::PixelList L{ctx->pix_list, ctx->pix_count, hm::compact_list_cap(px, lists.gate != nullptr)};
+    if (lists.from_tiles) L = {ctx->pix_list, ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_SHADE_SUBS, lists.cap};
+    if (!lists.from_tiles || lists.gate) {
+        const hm::PixelList& C = L;  // same layout as the camera tiles' when it replaces them
+        if ((size_t)C.cap * hm::LIST_SUBS > ctx->list_len || C.cap < hm::compact_list_cap(px, lists.gate != nullptr))
+            return fail(ctx, HM_ERR_INVALID, "pixel lists too small for the compaction");
+        HM_HIP(ctx, hm::launch_compact(ctx->stream, 0, ctx->center, ctx->cons, px, C, ctx->shaded, ctx->shaded_depth,
+                                       lists.gate));
+    }
+    HM_HIP(ctx, hm::launch_shade(ctx->stream, a, L, px, ctx->cu_count * 8));
     return HM_OK;
 }
 
-hm_status hm_shade(hm_ctx* ctx, const hm_params* p, float exposure) { return shade_impl(ctx, p, exposure, nullptr); }
+hm_status hm_shade(hm_ctx* ctx, const hm_params* p, float exposure) {
+    return shade_impl(ctx, p, exposure, TileLists());
+}
 
-static hm_status filter_impl(hm_ctx* ctx, const hm_params* p, const uint32_t* lists_gate) {
+static hm_status filter_impl(hm_ctx* ctx, const hm_params* p, const TileLists& lists) {
     hm_status st = check_targets(ctx);
     if (st != HM_OK) return st;
     if (!p || (uint32_t)p->screen[0] != ctx->width || (uint32_t)p->screen[1] != ctx->height)
@@ -1037,15 +1074,20 @@ static hm_status filter_impl(hm_ctx* ctx, const hm_params* p, const uint32_t* li
     a.filtered_depth = ctx->filtered_depth;
     Timer t(ctx, HM_K_FILTER);
     uint32_t px = ctx->width * ctx->height;
-    uint32_t* list = lists_gate ? ctx->pix_list2 : ctx->pix_list;
-    uint32_t* count = lists_gate ? ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_FILTER_COUNT : ctx->pix_count;
-    HM_HIP(ctx, hm::launch_compact(ctx->stream, 1, ctx->center, ctx->cons, px, list, count, ctx->filtered,
-                                   ctx->filtered_depth, lists_gate));
-    HM_HIP(ctx, hm::launch_filter(ctx->stream, a, list, count, px, ctx->cu_count * 8));
+    hm::PixelList L{ctx->pix_list, ctx->pix_count + hm::LIST_SUBS, hm::compact_list_cap(px, lists.gate != nullptr)};
+    if (lists.from_tiles) L = {ctx->pix_list2, ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_FILTER_SUBS, lists.cap};
+    if (!lists.from_tiles || lists.gate) {
+        const hm::PixelList& C = L;  // same layout as the camera tiles' when it replaces them
+        if ((size_t)C.cap * hm::LIST_SUBS > ctx->list_len || C.cap < hm::compact_list_cap(px, lists.gate != nullptr))
+            return fail(ctx, HM_ERR_INVALID, "pixel lists too small for the compaction");
+        HM_HIP(ctx, hm::launch_compact(ctx->stream, 1, ctx->center, ctx->cons, px, C, ctx->filtered,
+                                       ctx->filtered_depth, lists.gate));
+    }
+    HM_HIP(ctx, hm::launch_filter(ctx->stream, a, L, px, ctx->cu_count * 8));
     return HM_OK;
 }
 
-hm_status hm_filter(hm_ctx* ctx, const hm_params* p) { return filter_impl(ctx, p, nullptr); }
+hm_status hm_filter(hm_ctx* ctx, const hm_params* p) { return filter_impl(ctx, p, TileLists()); }
 
 hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm_frame_opts* opts) {
     if (!ctx || !g || !params) return fail(ctx, HM_ERR_INVALID, "null argument");
@@ -1061,13 +1103,13 @@ hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm
     const hm_params& cam = params[0];
     if ((st = hm_lod(ctx, g, &cam, HM_PASS_CAMERA, ovr)) != HM_OK) return st;
     if ((st = hm_scan(ctx, g, &cam, HM_PASS_CAMERA)) != HM_OK) return st;
-    const uint32_t* lists = nullptr;
+    TileLists lists;
     if ((st = gbuffer_impl(ctx, g, &cam, true, &lists)) != HM_OK) return st;
     if (cam.flags[3]) {
         if ((st = hm_lod(ctx, g, &params[1], HM_PASS_DOM, nullptr)) != HM_OK) return st;
         if ((st = hm_scan(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
         const bool same_model = std::memcmp(cam.world_from_model, params[1].world_from_model, 64) == 0;
-        if ((st = dom_impl(ctx, g, &params[1], &params[2], lists != nullptr && same_model)) != HM_OK) return st;
+        if ((st = dom_impl(ctx, g, &params[1], &params[2], lists.from_tiles && same_model)) != HM_OK) return st;
     }
     if (opts && opts->shading == HM_SHADING_MLP) {
         if ((st = hm_shade_mlp(ctx, &cam, exposure, opts->mlp_tolerance, nullptr)) != HM_OK) return st;

@@ -32,6 +32,16 @@ struct RasterArgs {
     const uint32_t* gate;           // when set and zero, the launch does nothing
 };
 
+// Covered-pixel list split into LIST_SUBS sub-lists with a counter each, so
+// that writers spread their atomics: entry j of sub-list s is
+// list[s * cap + j], j < counts[s].
+constexpr uint32_t LIST_SUBS = 64u;
+struct PixelList {
+    uint32_t* list;
+    uint32_t* counts;  // LIST_SUBS words
+    uint32_t cap;      // entries per sub-list
+};
+
 struct ShadeArgs {
     hm_params view;
     const uint64_t* center;
@@ -67,9 +77,10 @@ constexpr uint32_t BIN_FALLBACK = 2u;    // direct kernels run
 constexpr uint32_t BIN_LONG = 3u;        // binned and long segments exist
 constexpr uint32_t BIN_LONG_COUNT = 4u;
 constexpr uint32_t BIN_LIST_COUNT = 5u;  // (tile, rank) entries the counting workgroups wrote
-constexpr uint32_t BIN_SHADE_COUNT = 6u;  // camera: pixels with a centre word (shade list)
-constexpr uint32_t BIN_FILTER_COUNT = 7u; // camera: pixels with a centre or conservative word (filter list)
-constexpr uint32_t BIN_META_WORDS = 8u;  // then count[tiles], offset[tiles + 1], cursor[tiles],
+constexpr uint32_t BIN_SPLIT = 8u;       // binned and some tile has more than one chunk
+constexpr uint32_t BIN_SHADE_SUBS = 16u;                        // camera: shade sub-list counters
+constexpr uint32_t BIN_FILTER_SUBS = BIN_SHADE_SUBS + LIST_SUBS; // camera: filter sub-list counters
+constexpr uint32_t BIN_META_WORDS = BIN_FILTER_SUBS + LIST_SUBS; // then count[tiles], offset[tiles + 1], cursor[tiles],
                                          // chunk offset[tiles + 1]
 constexpr uint32_t BIN_MAX_TILES = 16384u;  // one LDS word per tile in the binning workgroups
 
@@ -87,13 +98,15 @@ struct BinArgs {
     uint32_t* meta;
     uint32_t* records;              // segment ids grouped by tile
     uint32_t record_capacity;
+    uint32_t can_overflow;          // record_capacity is below the worst case: the fallback may run
     uint2* lists;                   // per counting workgroup: (tile, rank within the tile)
     uint32_t list_capacity;
     uint2* wg_list;                 // per counting workgroup: (first list entry, entries)
     uint32_t chunk;                 // records per tile workgroup (0: one workgroup per tile)
     uint32_t* chunk_tile;           // tile of each chunk
     uint32_t chunk_capacity;
-    uint8_t* partials;              // camera: one partial tile per chunk of a split tile
+    uint32_t* gb_slab;              // camera: centre | conservative | beta words, cleared when tiles split
+    uint64_t gb_words;
     const hm_strand_ref* reuse_refs;  // DOM: camera work list whose control points cps still holds, or null
     const uint32_t* reuse_indirect;
     uint32_t* long_list;            // segments whose box spans more than box_limit tiles
@@ -111,11 +124,12 @@ struct BinArgs {
     const float* scene_depth;
     float scene_clear;
     // With emit_lists the camera tiles also write the shade and filter pixel
-    // lists (counts in meta) and the empty results of uncovered pixels, the
-    // work of k_compact<0> and k_compact<1>.
+    // lists (sub-list of a tile: tile % LIST_SUBS; counters in meta) and the
+    // empty results of uncovered pixels, the work of k_compact<0> and <1>.
     uint32_t emit_lists;
     uint32_t* shade_list;
     uint32_t* filter_list;
+    uint32_t list_cap;              // entries per sub-list
     float4* shaded;
     float* shaded_depth;
     float4* filtered;
@@ -124,7 +138,7 @@ struct BinArgs {
 
 hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks);
 hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks);
-hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int long_blocks);
+hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int tile_blocks, int long_blocks);
 
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes, int blocks);
 hipError_t launch_atomics(hipStream_t s, void* buf, size_t words, uint32_t threads, uint32_t per_thread, uint32_t kind);
@@ -135,12 +149,11 @@ hipError_t launch_lod(hipStream_t s, const hm_params& prm, const hm_bundle* bund
 hipError_t launch_scan(hipStream_t s, const hm_lod_record* lod, uint32_t nb, uint32_t capacity, uint32_t* prefix,
                        uint32_t* indirect, hm_strand_ref* refs);
 hipError_t launch_raster(hipStream_t s, const RasterArgs& a, int mode, bool count, int blocks);
+// Sub-list capacity launch_compact needs for `pixels` (gated or not).
+uint32_t compact_list_cap(uint32_t pixels, bool gated);
 hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const uint64_t* cons, uint32_t pixels,
-                          uint32_t* list, uint32_t* count, float4* color_out, float* depth_out,
-                          const uint32_t* gate = nullptr);
-hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, const uint32_t* list, const uint32_t* count,
-                        uint32_t pixels, int blocks);
-hipError_t launch_filter(hipStream_t s, const FilterArgs& a, const uint32_t* list, const uint32_t* count,
-                         uint32_t pixels, int blocks);
+                          const PixelList& out, float4* color_out, float* depth_out, const uint32_t* gate = nullptr);
+hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, const PixelList& list, uint32_t pixels, int blocks);
+hipError_t launch_filter(hipStream_t s, const FilterArgs& a, const PixelList& list, uint32_t pixels, int blocks);
 
 }  // namespace hm

@@ -12,6 +12,7 @@
 #include <cstring>
 
 #include "hm_device.h"
+#include "hm_lists.h"
 #include "hm_raster.h"
 #include "hm_shade.h"
 
@@ -257,11 +258,12 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs a) {
 // conservative word present).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ center, const uint64_t* __restrict__ cons,
-                                                 uint32_t pixels, uint32_t* __restrict__ list,
-                                                 uint32_t* __restrict__ count, float4* __restrict__ color_out,
+                                                 uint32_t pixels, PixelList out, float4* __restrict__ color_out,
                                                  float* __restrict__ depth_out, const uint32_t* gate) {
     if (gate && *gate == 0u) return;
-    // Grid-stride over whole blocks, so every wave runs the same trip count.
+    __shared__ uint32_t scratch[5];
+    const uint32_t sub = blockIdx.x % LIST_SUBS;
+    // Grid-stride over whole blocks, so every thread runs the same trip count.
     for (uint32_t first = blockIdx.x * blockDim.x; first < pixels; first += gridDim.x * blockDim.x) {
         uint32_t pix = first + threadIdx.x;
         bool inb = pix < pixels;
@@ -274,26 +276,18 @@ __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ ce
                 depth_out[pix] = 0.0f;
             }
         }
-        uint64_t mask = __ballot(hit);
-        uint32_t lane = threadIdx.x & 63u;
-        uint32_t total = (uint32_t)__popcll(mask);
-        uint32_t base = 0;
-        if (total) {
-            if (lane == 0) base = atomicAdd(count, total);
-            base = __shfl(base, 0);
-        }
-        if (hit) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = pix;
+        block_append(out, sub, hit, pix, scratch);
     }
 }
 
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs a, const uint32_t* __restrict__ list,
-                                               const uint32_t* __restrict__ count) {
+__global__ __launch_bounds__(256) void k_shade(ShadeArgs a, PixelList L) {
+    __shared__ uint32_t pre[LIST_SUBS + 1];
+    list_prefix(L, pre);
     const View& p = *reinterpret_cast<const View*>(&a.view);
     const uint32_t width = to_u32(p.screen.x), pixels = width * to_u32(p.screen.y);
-    uint32_t n = *count;
-    n = n < pixels ? n : pixels;  // a list never holds more entries than pixels
+    const uint32_t n = pre[LIST_SUBS];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t pix = list[i];
+        uint32_t pix = list_entry(L, pre, i);
         if (pix >= pixels) continue;
         uint32_t x = pix % width, y = pix / width;
         PixelShade px = pixel_setup(p, a.center[pix], a.beta[pix], x, y);
@@ -384,14 +378,14 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
     a.filtered_depth[pix] = view_dist;
 }
 
-__global__ __launch_bounds__(256) void k_filter(FilterArgs a, const uint32_t* __restrict__ list,
-                                                const uint32_t* __restrict__ count) {
+__global__ __launch_bounds__(256) void k_filter(FilterArgs a, PixelList L) {
+    __shared__ uint32_t pre[LIST_SUBS + 1];
+    list_prefix(L, pre);
     const View& p = *reinterpret_cast<const View*>(&a.view);
     const uint32_t width = to_u32(p.screen.x), pixels = width * to_u32(p.screen.y);
-    uint32_t n = *count;
-    n = n < pixels ? n : pixels;  // a list never holds more entries than pixels
+    const uint32_t n = pre[LIST_SUBS];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t pix = list[i];
+        uint32_t pix = list_entry(L, pre, i);
         if (pix >= pixels) continue;
         filter_pixel(a, p, (int32_t)(pix % width), (int32_t)(pix / width));
     }
@@ -457,33 +451,40 @@ hipError_t launch_raster(hipStream_t s, const RasterArgs& a, int mode, bool coun
     return hipGetLastError();
 }
 
-hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const uint64_t* cons, uint32_t pixels,
-                          uint32_t* list, uint32_t* count, float4* color_out, float* depth_out, const uint32_t* gate) {
-    if (pixels == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u, gate);
+static uint32_t compact_blocks(uint32_t pixels, bool gated) {
     const uint32_t want = (pixels + 255) / 256;
-    dim3 g(gate ? (want < 256u ? want : 256u) : want), b(256);  // gated: usually switched off, keep it small
+    return gated ? (want < 256u ? want : 256u) : want;  // gated: usually switched off, keep it small
+}
+
+uint32_t compact_list_cap(uint32_t pixels, bool gated) {
+    const uint32_t blocks = compact_blocks(pixels, gated);
+    const uint32_t iters = blocks ? (pixels + blocks * 256u - 1u) / (blocks * 256u) : 0u;
+    return ((blocks + LIST_SUBS - 1u) / LIST_SUBS) * iters * 256u;  // sub-list of block b: b % LIST_SUBS
+}
+
+hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const uint64_t* cons, uint32_t pixels,
+                          const PixelList& out, float4* color_out, float* depth_out, const uint32_t* gate) {
+    if (pixels == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill32, dim3(1), dim3(64), 0, s, out.counts, (size_t)LIST_SUBS, 0u, gate);
+    dim3 g(compact_blocks(pixels, gate != nullptr)), b(256);
     if (mode == 0)
-        hipLaunchKernelGGL(k_compact<0>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out, gate);
+        hipLaunchKernelGGL(k_compact<0>, g, b, 0, s, center, cons, pixels, out, color_out, depth_out, gate);
     else
-        hipLaunchKernelGGL(k_compact<1>, g, b, 0, s, center, cons, pixels, list, count, color_out, depth_out, gate);
+        hipLaunchKernelGGL(k_compact<1>, g, b, 0, s, center, cons, pixels, out, color_out, depth_out, gate);
     return hipGetLastError();
 }
 
-hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, const uint32_t* list, const uint32_t* count,
-                        uint32_t pixels, int blocks) {
+hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, const PixelList& list, uint32_t pixels, int blocks) {
     if (pixels == 0) return hipSuccess;
     uint32_t want = (pixels + 255) / 256;
-    hipLaunchKernelGGL(k_shade, dim3(want < (uint32_t)blocks ? want : (uint32_t)blocks), dim3(256), 0, s, a, list, count);
+    hipLaunchKernelGGL(k_shade, dim3(want < (uint32_t)blocks ? want : (uint32_t)blocks), dim3(256), 0, s, a, list);
     return hipGetLastError();
 }
 
-hipError_t launch_filter(hipStream_t s, const FilterArgs& a, const uint32_t* list, const uint32_t* count,
-                         uint32_t pixels, int blocks) {
+hipError_t launch_filter(hipStream_t s, const FilterArgs& a, const PixelList& list, uint32_t pixels, int blocks) {
     if (pixels == 0) return hipSuccess;
     uint32_t want = (pixels + 255) / 256;
-    hipLaunchKernelGGL(k_filter, dim3(want < (uint32_t)blocks ? want : (uint32_t)blocks), dim3(256), 0, s, a, list,
-                       count);
+    hipLaunchKernelGGL(k_filter, dim3(want < (uint32_t)blocks ? want : (uint32_t)blocks), dim3(256), 0, s, a, list);
     return hipGetLastError();
 }
 

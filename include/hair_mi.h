@@ -361,11 +361,12 @@ typedef struct hm_tuning {
     uint32_t cam_tile;             /* hm_gbuffer: 0 = direct kernels, 16 or 32 = binned, pixels per tile side */
     uint32_t bin_box_limit;        /* tiles a segment's box may span before it is drawn with
                                       global atomics instead (1..64) */
-    uint32_t bin_records;          /* tile record buffer length; 0 = 2 per segment slot */
+    uint32_t bin_records;          /* tile record buffer length; 0 = the worst case (126 segments per
+                                      strand x bin_box_limit tiles), which cannot overflow */
     uint32_t bin_blocks_per_cu;    /* binning grid: 256-thread workgroups per CU, each counting
                                       one contiguous run of segments in LDS (1..64) */
     uint32_t cam_chunk;            /* hm_gbuffer: records per tile workgroup; larger tiles are split
-                                      and merged (0 = one workgroup per tile, else 64..1048576) */
+                                      over several workgroups (0 = one per tile, else 64..1048576) */
 } hm_tuning;
 
 hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);
