@@ -506,3 +506,53 @@ def test_filter_coverage_is_center_over_conservative_in_strip():
     assert abs(alpha - 6.0 / 11.0) < 1e-7
     # an empty pixel stays empty
     assert (f.filtered[0] == 0).all()
+
+
+def test_fp32_against_fp64_error_statement(capsys):
+    """The float restatement (what the GPU path must match) against its float64
+    instantiation, stage by stage on the same inputs: the stated error of each
+    output of the path."""
+    from tests.scenes import make_scene, rel_err
+    sc = make_scene("small_lod")
+    w = sc.w
+    f32 = oracle.Frame(sc.mesh, w.width, w.height, w.dom_size)
+    f64 = oracle.Frame(sc.mesh, w.width, w.height, w.dom_size)
+    cap = max(1, sc.mesh.strand_count)
+    cam, dom1, dom2 = sc.params
+    lod = f32.lod(cam, w.control_points, w.strand_fraction)
+    lod64 = f64.lod(cam, w.control_points, w.strand_fraction, f64=True)
+    refs, _ = f32.scan(cam, lod, cap)
+    f32.clear(); f64.clear()
+    f32.raster(cam, refs)
+    f64.raster(cam, refs, f64=True)
+    depth_step = np.abs((f32.center >> np.uint64(40)).astype(np.int64) - (f64.center >> np.uint64(40)).astype(np.int64))
+    both = (f32.center != np.uint64(EMPTY64)) & (f64.center != np.uint64(EMPTY64))
+    gb_diff = float((f32.center != f64.center).mean())
+    dlod = f32.lod(dom1)
+    drefs, _ = f32.scan(dom1, dlod, cap)
+    f32.clear_dom(); f64.clear_dom()
+    for p in (dom1, dom2):
+        f32.raster(p, drefs)
+        f64.raster(p, drefs, f64=True)
+    dom_diff = float((f32.dom != f64.dom).mean())
+    # shading and filtering on identical (float-path) inputs
+    f64.center[:], f64.cons[:], f64.beta[:], f64.dom[:] = f32.center, f32.cons, f32.beta, f32.dom
+    pa, n_p, da, n_d = sc.light_arrays()
+    f32.shade(cam, pa, n_p, da, n_d)
+    f64.shade(cam, pa, n_p, da, n_d, f64=True)
+    e_shade = rel_err(f32.shaded, f64.shaded)
+    f64.shaded[:], f64.shaded_depth[:] = f32.shaded, f32.shaded_depth
+    f32.filter(cam)
+    f64.filter(cam, f64=True)
+    e_filter = rel_err(f32.filtered, f64.filtered)
+    with capsys.disabled():
+        print(f"\n[fp32 vs fp64] lod records equal: {np.array_equal(lod, lod64)}; "
+              f"G-buffer words differing: {gb_diff:.2e} (max depth difference {int(depth_step[both].max()) if both.any() else 0} "
+              f"quantization steps); DOM words differing: {dom_diff:.2e}; shaded RGB max rel {e_shade.max():.2e}; "
+              f"filtered max rel {e_filter.max():.2e}")
+    # sanity bounds of the statement (small_lod, seeded): float rounding moves a
+    # few percent of the words' low bits and the depth key by at most a couple
+    # of quantization steps
+    assert np.array_equal(lod, lod64)
+    assert gb_diff < 0.1 and (not both.any() or depth_step[both].max() <= 4) and dom_diff < 0.02
+    assert e_shade.max() < 1e-2 and e_filter.max() < 1e-3
