@@ -101,6 +101,7 @@ class HmFrameOpts(C.Structure):
 
 SHADING_PLAIN, SHADING_MLP = 0, 1
 MLP_NW, MLP_NBIAS = 11264, 208
+MLP_NOUT = 16  # network outputs (the dual-scattering terms)
 
 
 class HmTuning(C.Structure):

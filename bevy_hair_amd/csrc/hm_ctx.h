@@ -82,7 +82,21 @@ struct hm_ctx {
 };
 
 
+// Where the covered-pixel lists of a frame are: written by the camera tiles
+// (sub-list counters in the camera status words, `cap` entries per sub-list;
+// `gate` nonzero on the device when the direct kernels ran instead, null when
+// that cannot happen), or compacted by the shading call itself.
+struct hm_lists {
+    bool from_tiles = false;
+    uint32_t cap = 0;
+    const uint32_t* gate = nullptr;
+};
+
 // Shared helpers of the host translation units.
 hm_status hm_fail(hm_ctx* ctx, hm_status st, const std::string& msg);
 hm_status hm_hip_fail(hm_ctx* ctx, hipError_t e, const char* what);
 void hm_mlp_release(hm_ctx* ctx);
+// The shade list for `lists`, compacting it first where needed (enqueued on the stream).
+hm_status hm_shade_list(hm_ctx* ctx, const hm_lists& lists, hm::PixelList* out);
+hm_status hm_shade_mlp_impl(hm_ctx* ctx, const hm_params* p, float exposure, float tolerance, uint32_t* fallback,
+                            const hm_lists& lists);

@@ -875,13 +875,7 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     return hipSuccess;
 }
 
-// Where shade and filter find their covered-pixel lists after gbuffer_impl.
-struct TileLists {
-    bool from_tiles = false;          // the camera tiles wrote them (sub-list counters in the camera status words)
-    uint32_t cap = 0;                 // their entries per sub-list
-    const uint32_t* gate = nullptr;   // with from_tiles: nonzero when the direct kernels ran instead;
-                                      // null when the records cannot overflow
-};
+using TileLists = hm_lists;
 
 // hm_gbuffer; with emit_lists (a frame's own call) the camera tiles also
 // write the shade and filter pixel lists, described in *lists.
@@ -1018,6 +1012,25 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const hm_params*
     return dom_impl(ctx, g, dp, op, false);
 }
 
+}  // extern "C"
+
+hm_status hm_shade_list(hm_ctx* ctx, const hm_lists& lists, hm::PixelList* out) {
+    const uint32_t px = ctx->width * ctx->height;
+    hm::PixelList L{ctx->pix_list, ctx->pix_count, hm::compact_list_cap(px, lists.gate != nullptr)};
+    if (lists.from_tiles) L = {ctx->pix_list, ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_SHADE_SUBS, lists.cap};
+    if (!lists.from_tiles || lists.gate) {
+        // same layout as the camera tiles' when it replaces them
+        if ((size_t)L.cap * hm::LIST_SUBS > ctx->list_len || L.cap < hm::compact_list_cap(px, lists.gate != nullptr))
+            return fail(ctx, HM_ERR_INVALID, "pixel lists too small for the compaction");
+        HM_HIP(ctx, hm::launch_compact(ctx->stream, 0, ctx->center, ctx->cons, px, L, ctx->shaded, ctx->shaded_depth,
+                                       lists.gate));
+    }
+    *out = L;
+    return HM_OK;
+}
+
+extern "C" {
+
 // Shading over the covered-pixel list: the camera tiles' (then the
 // compaction runs only if the direct kernels may have run instead) or one
 // compacted here.
@@ -1042,15 +1055,8 @@ static hm_status shade_impl(hm_ctx* ctx, const hm_params* p, float exposure, con
     a.shaded_depth = ctx->shaded_depth;
     Timer t(ctx, HM_K_SHADE);
     uint32_t px = ctx->width * ctx->height;
-    hm::PixelList L{ctx->pix_list, ctx->pix_count, hm::compact_list_cap(px, lists.gate != nullptr)};
-    if (lists.from_tiles) L = {ctx->pix_list, ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_SHADE_SUBS, lists.cap};
-    if (!lists.from_tiles || lists.gate) {
-        const hm::PixelList& C = L;  // same layout as the camera tiles' when it replaces them
-        if ((size_t)C.cap * hm::LIST_SUBS > ctx->list_len || C.cap < hm::compact_list_cap(px, lists.gate != nullptr))
-            return fail(ctx, HM_ERR_INVALID, "pixel lists too small for the compaction");
-        HM_HIP(ctx, hm::launch_compact(ctx->stream, 0, ctx->center, ctx->cons, px, C, ctx->shaded, ctx->shaded_depth,
-                                       lists.gate));
-    }
+    hm::PixelList L;
+    if ((st = hm_shade_list(ctx, lists, &L)) != HM_OK) return st;
     HM_HIP(ctx, hm::launch_shade(ctx->stream, a, L, px, ctx->cu_count * 8));
     return HM_OK;
 }
@@ -1112,7 +1118,7 @@ hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm
         if ((st = dom_impl(ctx, g, &params[1], &params[2], lists.from_tiles && same_model)) != HM_OK) return st;
     }
     if (opts && opts->shading == HM_SHADING_MLP) {
-        if ((st = hm_shade_mlp(ctx, &cam, exposure, opts->mlp_tolerance, nullptr)) != HM_OK) return st;
+        if ((st = hm_shade_mlp_impl(ctx, &cam, exposure, opts->mlp_tolerance, nullptr, lists)) != HM_OK) return st;
     } else if ((st = shade_impl(ctx, &cam, exposure, lists)) != HM_OK) {
         return st;
     }

@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ ce
     }
 }
 
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs a, PixelList L) {
+__global__ __launch_bounds__(256, 3) void k_shade(ShadeArgs a, PixelList L) {
     __shared__ uint32_t pre[LIST_SUBS + 1];
     list_prefix(L, pre);
     const View& p = *reinterpret_cast<const View*>(&a.view);

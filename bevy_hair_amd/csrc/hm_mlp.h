@@ -8,15 +8,29 @@
 
 namespace hm {
 
-constexpr int MLP_IN = 32;      // 29 features + 3 zero pads
-constexpr int MLP_NFEAT = 29;
+// Features (all 32 used): tangent 3, view 3, key-light direction 3, the 16
+// DOM layer transmittances, sigma_a * MLP_SIGMA_SCALE 3, roughness, then the
+// key light's theta_h / pi, cos(theta_o) and optical depth * MLP_DEPTH_SCALE.
+constexpr int MLP_IN = 32;
+constexpr int MLP_NFEAT = 32;
 constexpr int MLP_H = 64;
-constexpr int MLP_OUT = 16;     // 12 outputs + 4 zero pads
-constexpr int MLP_NOUT = 12;
+constexpr int MLP_OUT = 16;
+constexpr int MLP_NOUT = 16;    // all outputs used
 constexpr int MLP_TILES = 22;   // 16x32 bf16 operand tiles over the four layers
 constexpr int MLP_NW = 64 * 32 + 64 * 64 + 64 * 64 + 16 * 64;  // 11264 weights
 constexpr int MLP_NBIAS = 64 + 64 + 64 + 16;                    // 208 biases
 constexpr float MLP_SIGMA_SCALE = 0.25f;
+constexpr float MLP_DEPTH_SCALE = 0.125f;
+
+// Outputs: the dual-scattering terms of dual_scatter (shade.wesl:153-190),
+//   0       forward R lobe sum (the same in every channel)
+//   1-3     forward TT lobe sum, 4-6 forward TRT lobe sum
+//   7-9     a_back, 10-12 delta_b, 13-15 sigma_b^2.
+// The non-negative ones (all but delta_b, a signed angle) span orders of
+// magnitude: the network predicts log(value + MLP_LOG_EPS), so that its error
+// is relative.
+constexpr float MLP_LOG_EPS = 1e-6f;
+__host__ __device__ constexpr bool mlp_log_output(int k) { return k < 10 || k >= 13; }
 
 // Canonical float32 layout: W_L is (out x in) row-major, layers back to back.
 __host__ __device__ constexpr int mlp_w_offset(int L) {
@@ -26,7 +40,7 @@ __host__ __device__ constexpr int mlp_b_offset(int L) { return 64 * L; }
 
 struct TrainArgs {
     const float* x;        // [n][32] features
-    const float* t;        // [n][12] targets
+    const float* t;        // [n][MLP_NOUT] targets
     uint32_t n;            // samples held
     uint32_t first;        // index of the batch's first sample in the epoch order
     uint32_t batch;        // samples in this step
@@ -40,7 +54,7 @@ struct TrainArgs {
     __bf16* act[3];        // [batch][64] inputs of layers 1..3
     float* dz[4];          // [batch][64 or 16] dL/dz per layer
     float scale[MLP_NOUT]; // loss weight per output
-    float inv_count;       // 1 / (batch * 12)
+    float inv_count;       // 1 / (batch * MLP_NOUT)
     float* loss;
 };
 
@@ -56,10 +70,11 @@ struct SampleArgs {
 struct MlpShadeArgs {
     const __bf16* packed;
     const float* bias;
-    uint32_t* tile_flags;  // per 16x4 tile: 0 network, 1 out of range, 2 probe error; bit 2 once re-shaded plain
-    uint32_t tiles_x, tiles_y;
+    uint32_t* group_flags; // per 64-pixel group of the shade list: 0 network, 1 out of range (plain),
+                           // 2 probe error (plain)
+    uint32_t groups_cap;   // group_flags length
     float lo[32], hi[32];  // trained feature range
-    float tol;             // probe relative error bound
+    float tol;             // probe relative error bound (<= 0: no probe)
     float floor;           // absolute floor of the probe's relative error
 };
 
@@ -71,6 +86,6 @@ hipError_t launch_mlp_train_step(hipStream_t s, const TrainArgs& a, float* w, fl
 hipError_t launch_mlp_grads(hipStream_t s, const TrainArgs& a);
 hipError_t launch_mlp_samples(hipStream_t s, const ShadeArgs& a, const SampleArgs& sa, uint32_t width,
                               uint32_t height);
-hipError_t launch_shade_mlp(hipStream_t s, const ShadeArgs& a, const MlpShadeArgs& m, uint32_t width, uint32_t height);
+hipError_t launch_shade_mlp(hipStream_t s, const ShadeArgs& a, const MlpShadeArgs& m, const PixelList& L, int blocks);
 
 }  // namespace hm

@@ -1,10 +1,10 @@
-// Learned multiple-scattering term: a 29 -> 64 -> 64 -> 64 -> 12 MLP on the
-// gfx950 matrix cores (v_mfma_f32_16x16x32_bf16: bf16 operands, f32
-// accumulation). It stands in, per pixel, for dual_scatter (shade.wesl:153-190)
-// and for the key light's multiple-scattering part of hair_bsdf
-// (shade.wesl:220-242): outputs 0-2 are that key-light term per unit light
-// colour, 3-11 the dual-scattering local terms a_back, delta_b, sigma_b^2
-// that the other lights' back lobe needs. Single scattering stays analytic.
+// Learned dual scattering: a 32 -> 64 -> 64 -> 64 -> 16 MLP on the gfx950
+// matrix cores (v_mfma_f32_16x16x32_bf16: bf16 operands, f32 accumulation)
+// stands in, per pixel, for dual_scatter (shade.wesl:153-190), the 16-point
+// azimuthal quadrature behind every light's back lobe and the key light's
+// global multiple scattering and DOM forward-scattering spread
+// (shade.wesl:220-242), which are then evaluated from the predicted terms.
+// Single scattering stays analytic.
 //
 // Operand layout (one wave, 16 pixels per column block):
 //   A (weights, 16 rows x 32 k): lane l holds row l%16, k = 8*(l/16) + j
@@ -17,6 +17,7 @@
 
 #include "hm_device.h"
 #include "hm_kernels.h"
+#include "hm_lists.h"
 #include "hm_mlp.h"
 #include "hm_shade.h"
 
@@ -96,6 +97,66 @@ __device__ __forceinline__ f32x4 mlp_fwd16(const bf16x8* T, const float* bias, b
     return y;
 }
 
+// Forward pass of four 16-pixel column blocks at once, layer by layer: each
+// weight tile is read from `T` once per layer and feeds four independent
+// matrix-core chains; the outputs of block cb are returned in y[cb].
+__device__ __forceinline__ void mlp_fwd64(const bf16x8* T, const float* bias, const bf16x8 b0[4], int lane,
+                                          f32x4 y[4]) {
+    const int g = lane >> 4;
+    const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const bf16x8 w = T[(tile_base(0) + r) * 64 + lane];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w, b0[cb], z);
+    }
+    bf16x8 B[4][2];
+#pragma unroll
+    for (int L = 1; L <= 3; ++L) {
+        const float4* bl = reinterpret_cast<const float4*>(bias + 64 * (L - 1));
+        const float4 bv[4] = {bl[g], bl[4 + g], bl[8 + g], bl[12 + g]};  // rows 4g.., 16+4g.., 32+4g.., 48+4g..
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const float lo4[4] = {bv[2 * c].x, bv[2 * c].y, bv[2 * c].z, bv[2 * c].w};
+                const float hi4[4] = {bv[2 * c + 1].x, bv[2 * c + 1].y, bv[2 * c + 1].z, bv[2 * c + 1].w};
+                bf16x8 v;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float lo = acc[cb][2 * c][i] + lo4[i];
+                    const float hi = acc[cb][2 * c + 1][i] + hi4[i];
+                    v[i] = (__bf16)(lo > 0.0f ? lo : 0.0f);
+                    v[4 + i] = (__bf16)(hi > 0.0f ? hi : 0.0f);
+                }
+                B[cb][c] = v;
+            }
+        }
+        if (L < 3) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const bf16x8 w0 = T[(tile_base(L) + r * 2) * 64 + lane];
+                const bf16x8 w1 = T[(tile_base(L) + r * 2 + 1) * 64 + lane];
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w0, B[cb][0], z);
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w1, B[cb][1], acc[cb][r]);
+            }
+        }
+    }
+    const bf16x8 w0 = T[tile_base(3) * 64 + lane];
+    const bf16x8 w1 = T[(tile_base(3) + 1) * 64 + lane];
+    const float4 bo = reinterpret_cast<const float4*>(bias + 192)[g];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        f32x4 v = mfma(w0, B[cb][0], z);
+        v = mfma(w1, B[cb][1], v);
+        v[0] += bo.x; v[1] += bo.y; v[2] += bo.z; v[3] += bo.w;
+        y[cb] = v;
+    }
+}
+
 __device__ __forceinline__ bf16x8 load_x8(const float* row8) {
     float4 a = *reinterpret_cast<const float4*>(row8);
     float4 b = *reinterpret_cast<const float4*>(row8 + 4);
@@ -105,21 +166,28 @@ __device__ __forceinline__ bf16x8 load_x8(const float* row8) {
     return v;
 }
 
-// Batch inference: Y[n][16] = mlp(X[n][32]).
+// Batch inference: Y[n][16] = mlp(X[n][32]), 64 samples per wave step
+// through the same four-block pass as the shading kernel.
 __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X, uint32_t n, const __bf16* packed,
                                                      const float* __restrict__ bias, float* __restrict__ Y) {
     const int lane = threadIdx.x & 63, g = lane >> 4, q = lane & 15;
     const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * 4u;
     const bf16x8* T = reinterpret_cast<const bf16x8*>(packed);
-    for (uint32_t base = wave * 16u; base < n; base += nw * 16u) {
-        uint32_t s = base + (uint32_t)q;
-        bf16x8 b0 = {};
-        if (s < n) b0 = load_x8(X + (size_t)s * 32 + 8 * g);
-        f32x4 y = mlp_fwd16(T, bias, b0, lane, nullptr);
-        if (s < n) {
-            float4 o = make_float4(y[0], y[1], y[2], y[3]);
-            *reinterpret_cast<float4*>(Y + (size_t)s * 16 + 4 * g) = o;
+    for (uint32_t base = wave * 64u; base < n; base += nw * 64u) {
+        bf16x8 b0[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const uint32_t s = base + 16u * cb + (uint32_t)q;
+            b0[cb] = bf16x8{};
+            if (s < n) b0[cb] = load_x8(X + (size_t)s * 32 + 8 * g);
+        }
+        f32x4 y[4];
+        mlp_fwd64(T, bias, b0, lane, y);
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const uint32_t s = base + 16u * cb + (uint32_t)q;
+            if (s < n) *reinterpret_cast<float4*>(Y + (size_t)s * 16 + 4 * g) = make_float4(y[cb][0], y[cb][1], y[cb][2], y[cb][3]);
         }
     }
 }
@@ -295,13 +363,15 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ p, float* 
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// Keeps the zero padding of the weights at zero (inputs 29..31 of layer 0,
-// outputs 12..15 of layer 3) after an update.
+// Keeps the zero padding of the weights at zero (unused inputs of layer 0,
+// padded outputs of layer 3) after an update.
 __global__ void k_mlp_mask(float* __restrict__ w, float* __restrict__ b) {
     uint32_t i = threadIdx.x;
-    if (i < 64 * (MLP_IN - MLP_NFEAT)) {
-        uint32_t row = i / (MLP_IN - MLP_NFEAT), col = MLP_NFEAT + i % (MLP_IN - MLP_NFEAT);
-        w[mlp_w_offset(0) + row * MLP_IN + col] = 0.0f;
+    if constexpr (MLP_IN > MLP_NFEAT) {
+        if (i < 64 * (MLP_IN - MLP_NFEAT)) {
+            uint32_t row = i / (MLP_IN - MLP_NFEAT), col = MLP_NFEAT + i % (MLP_IN - MLP_NFEAT);
+            w[mlp_w_offset(0) + row * MLP_IN + col] = 0.0f;
+        }
     }
     if (i < (MLP_OUT - MLP_NOUT) * MLP_H) w[mlp_w_offset(3) + MLP_NOUT * MLP_H + i] = 0.0f;
     if (i < MLP_OUT - MLP_NOUT) b[mlp_b_offset(3) + MLP_NOUT + i] = 0.0f;
@@ -309,103 +379,79 @@ __global__ void k_mlp_mask(float* __restrict__ w, float* __restrict__ b) {
 
 // ---- features and targets from the plain path ----------------------------------------
 
-// The 16 DOM transmittance samples: visibility (shade.wesl:279-310) at the
-// pixel's Eq.-8 light depth shifted by (k - 8) / 4 opacity layers. Sample 8
-// is the plain method's own visibility.
-__device__ void dom_samples(const View& p, const uint32_t* __restrict__ dom, F3 world, float beta, float out[16]) {
-    if (p.flags[3] == 0u) {
-        for (int k = 0; k < 16; ++k) out[k] = 1.0f;
-        return;
-    }
-    float cx = mat_row(p.lclip, 0, world.x, world.y, world.z, 1.0f);
-    float cy = mat_row(p.lclip, 1, world.x, world.y, world.z, 1.0f);
-    float cw = mat_row(p.lclip, 3, world.x, world.y, world.z, 1.0f);
-    float ux = cx / cw * 0.5f + 0.5f, uy = 1.0f - (cy / cw * 0.5f + 0.5f);
-    if (cw <= 0.0f || ux < 0.0f || uy < 0.0f || ux > 1.0f || uy > 1.0f) {
-        for (int k = 0; k < 16; ++k) out[k] = 1.0f;
-        return;
-    }
-    uint32_t S = to_u32(p.dom_info.x);
-    float near_ = p.dom_info.z, far_ = p.dom_info.w;
-    float thickness = mx(far_ - near_, 1e-3f) / 16.0f;
-    float dist = dot3(world - xyz(p.light_eye), xyz(p.light_forward)) - (-logf(mx(beta, 1e-4f)) * p.filter_params.z);
-    float fx = ux * (float)S - 0.5f, fy = uy * (float)S - 0.5f;
-    float flx = floorf(fx), fly = floorf(fy);
-    int32_t x0 = to_i32(flx), y0 = to_i32(fly);
-    float tx = fx - flx, ty = fy - fly;
-    float wts[4] = {(1.0f - tx) * (1.0f - ty), tx * (1.0f - ty), (1.0f - tx) * ty, tx * ty};
-    // per texel: prefix occupancy; visibility blended bilinearly like dom_visibility
-    float vis[16];
-    for (int k = 0; k < 16; ++k) vis[k] = 0.0f;
-    for (int c = 0; c < 4; ++c) {
-        int32_t x = x0 + (c & 1), y = y0 + (c >> 1);
-        bool inside = x >= 0 && y >= 0 && x < (int32_t)S && y < (int32_t)S;
-        uint32_t front = inside ? dom[(size_t)y * S + x] : HM_DOM_EMPTY;
-        if (front == HM_DOM_EMPTY) {
-            for (int k = 0; k < 16; ++k) vis[k] += wts[c];
-            continue;
-        }
-        float z_front = dequantize_depth(front, near_, far_);
-        float pre[17];
-        pre[0] = 0.0f;
-        const uint4* c4 = reinterpret_cast<const uint4*>(dom + (size_t)S * S + ((size_t)y * S + x) * HM_DOM_LAYERS);
-        for (int qd = 0; qd < 4; ++qd) {
-            uint4 v = c4[qd];
-            pre[4 * qd + 1] = pre[4 * qd] + (float)v.x;
-            pre[4 * qd + 2] = pre[4 * qd + 1] + (float)v.y;
-            pre[4 * qd + 3] = pre[4 * qd + 2] + (float)v.z;
-            pre[4 * qd + 4] = pre[4 * qd + 3] + (float)v.w;
-        }
-        float rel0 = (dist - z_front - thickness * 0.05f) / thickness;
-        for (int k = 0; k < 16; ++k) {
-            float t = mx(rel0 + 0.25f * (float)(k - 8), 0.0f);
-            float occ;
-            if (t >= 16.0f) {
-                occ = pre[16];
-            } else {
-                int s = (int)t;
-                occ = pre[s] + (pre[s + 1] - pre[s]) * (t - (float)s);
+// The DOM inputs of one pixel from dom_visibility's four taps
+// (shade.wesl:279-310): returns that visibility, bit for bit, and fills
+// `layers` with the transmittance through layers 0..k of the bilinearly
+// weighted layer counts (k = 0..15).
+__device__ float dom_features(const View& p, const uint32_t* __restrict__ dom, F3 world, float beta,
+                              float layers[16]) {
+    for (int k = 0; k < 16; ++k) layers[k] = 1.0f;
+    DomTaps T;
+    if (!dom_taps(p, dom, world, beta, T)) return 1.0f;
+    const float wts[4] = {(1.0f - T.tx) * (1.0f - T.ty), T.tx * (1.0f - T.ty), (1.0f - T.tx) * T.ty, T.tx * T.ty};
+    float pre = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float c = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const uint4 v = T.counts[t][q];
+                const uint32_t cs[4] = {v.x, v.y, v.z, v.w};
+                if (T.front[t] != HM_DOM_EMPTY) c += wts[t] * u2f(cs[k]);
             }
-            vis[k] += wts[c] * expf(-occ * p.filter_params.w);
+            pre += c;
+            layers[4 * q + k] = fexp(-pre * p.filter_params.w);
         }
     }
-    for (int k = 0; k < 16; ++k) out[k] = vis[k];
+    return dom_taps_visibility(p, T);
 }
 
-// Features of one pixel (see hm_mlp.h for the order and scaling).
-__device__ void mlp_features(const View& p, const PixelShade& px, F3 key_dir, const uint32_t* dom, float f[32]) {
+// Features of one pixel (order and scaling in hm_mlp.h); returns the key
+// light's optical depth -log(visibility), as the plain method forms it.
+__device__ float mlp_features(const View& p, const PixelShade& px, F3 key_dir, const uint32_t* dom, float f[32]) {
     f[0] = px.tangent.x; f[1] = px.tangent.y; f[2] = px.tangent.z;
     f[3] = px.view.x; f[4] = px.view.y; f[5] = px.view.z;
     F3 l = safe_normalize(key_dir);
     f[6] = l.x; f[7] = l.y; f[8] = l.z;
-    dom_samples(p, dom, px.world, px.beta, f + 9);
+    const float vis = dom_features(p, dom, px.world, px.beta, f + 9);
+    const float depth = -flog(mx(vis, 1e-6f));
     f[25] = px.sigma_a.x * MLP_SIGMA_SCALE;
     f[26] = px.sigma_a.y * MLP_SIGMA_SCALE;
     f[27] = px.sigma_a.z * MLP_SIGMA_SCALE;
     f[28] = px.roughness;
-    f[29] = f[30] = f[31] = 0.0f;
+    const float sin_i = clampf(dot3(px.t_n, l), -1.0f, 1.0f);
+    const float sin_o = clampf(dot3(px.t_n, px.v_n), -1.0f, 1.0f);
+    f[29] = (fasin(sin_i) + fasin(sin_o)) * (1.0f / kPi);  // theta_h / pi
+    f[30] = px.cos_theta_o;
+    f[31] = depth * MLP_DEPTH_SCALE;
+    return depth;
 }
 
-// The plain method's values of the 12 outputs for one pixel and key light.
-__device__ void mlp_targets(const PixelShade& px, const DualScatter& ds, F3 key_dir, float depth, float t[12]) {
-    BsdfGeom gk = bsdf_geom_n(px.t_n, px.v_n, key_dir);
-    float direct = expf(-depth);
-    F3 back = local_back(gk.theta_h, ds.a_back, ds.delta_b, ds.sigma_b2);
-    F3 ms = direct * (0.7f * back) + global_spread(gk.theta_h, depth, direct, px.shape, ds);
-    t[0] = ms.x; t[1] = ms.y; t[2] = ms.z;
-    t[3] = ds.a_back.x; t[4] = ds.a_back.y; t[5] = ds.a_back.z;
-    t[6] = ds.delta_b.x; t[7] = ds.delta_b.y; t[8] = ds.delta_b.z;
-    t[9] = ds.sigma_b2.x; t[10] = ds.sigma_b2.y; t[11] = ds.sigma_b2.z;
+// The plain method's values of the outputs (hm_mlp.h) for one pixel.
+__device__ void mlp_targets(const DualScatter& ds, float t[MLP_NOUT]) {
+    t[0] = ds.fwd.r.x;
+    t[1] = ds.fwd.tt.x; t[2] = ds.fwd.tt.y; t[3] = ds.fwd.tt.z;
+    t[4] = ds.fwd.trt.x; t[5] = ds.fwd.trt.y; t[6] = ds.fwd.trt.z;
+    t[7] = ds.a_back.x; t[8] = ds.a_back.y; t[9] = ds.a_back.z;
+    t[10] = ds.delta_b.x; t[11] = ds.delta_b.y; t[12] = ds.delta_b.z;
+    t[13] = ds.sigma_b2.x; t[14] = ds.sigma_b2.y; t[15] = ds.sigma_b2.z;
+    for (int k = 0; k < MLP_NOUT; ++k)
+        if (mlp_log_output(k)) t[k] = logf(mx(t[k], 0.0f) + MLP_LOG_EPS);
 }
 
-__device__ __forceinline__ int key_light(const ShadeArgs& a) {
-    for (uint32_t i = 0; i < a.directional_count; ++i)
-        if (a.directionals[i].flags & HM_LIGHT_FLAG_SHADOWS) return (int)i;
-    return -1;
-}
-
-__device__ __forceinline__ F3 dir_of(const hm_directional_light& L) {
-    return {L.direction_to_light[0], L.direction_to_light[1], L.direction_to_light[2]};
+// The dual-scattering terms from the decoded outputs; a_f and beta_f^2 follow
+// from the forward lobes as in dual_scatter.
+__device__ __forceinline__ DualScatter network_dual_scatter(const float o[MLP_NOUT], const Shape& shape) {
+    DualScatter ds;
+    ds.fwd = {splat3(o[0]), {o[1], o[2], o[3]}, {o[4], o[5], o[6]}};
+    ds.a_f = vmin(ds.fwd.r + ds.fwd.tt + ds.fwd.trt, splat3(0.95f));
+    ds.beta_f2 = lobe_mean(ds.fwd, ds.a_f, shape.beta * shape.beta);
+    ds.a_back = {o[7], o[8], o[9]};
+    ds.delta_b = {o[10], o[11], o[12]};
+    ds.sigma_b2 = vmax(F3{o[13], o[14], o[15]}, splat3(1e-6f));
+    return ds;
 }
 
 // Appends (features, targets) of the hit pixels kept by a hash test.
@@ -422,170 +468,223 @@ __global__ __launch_bounds__(256) void k_mlp_samples(ShadeArgs a, SampleArgs s) 
     if ((float)pcg((uint32_t)pix ^ s.seed) * (1.0f / 4294967296.0f) >= s.keep) return;
     PixelShade px = pixel_setup(p, word, a.beta[pix], x, y);
     DualScatter ds = dual_scatter(px.cos_theta_o, px.sigma_a, px.shape);
-    float depth = -logf(mx(dom_visibility(p, a.dom, px.world, px.beta), 1e-6f));
     F3 kd = dir_of(a.directionals[key]);
-    float f[32], t[12];
-    mlp_features(p, px, kd, a.dom, f);
-    mlp_targets(px, ds, kd, depth, t);
+    float f[32], t[MLP_NOUT];
+    (void)mlp_features(p, px, kd, a.dom, f);
+    mlp_targets(ds, t);
     uint32_t slot = atomicAdd(s.count, 1u);
     if (slot >= s.capacity) return;
     for (int k = 0; k < 32; ++k) s.x[(size_t)slot * 32 + k] = f[k];
-    for (int k = 0; k < 12; ++k) s.t[(size_t)slot * 12 + k] = t[k];
+    for (int k = 0; k < MLP_NOUT; ++k) s.t[(size_t)slot * MLP_NOUT + k] = t[k];
 }
 
 // ---- MLP shading ---------------------------------------------------------------------
 
-// Colour of one pixel from the network's 12 outputs: analytic single
-// scattering for every light, the predicted local terms for the back lobe of
-// the non-key lights, the predicted key-light multiple scattering.
-__device__ F3 shade_from_mlp(const ShadeArgs& a, const PixelShade& px, const float o[12], int key, float depth) {
-    F3 a_back = {o[3], o[4], o[5]}, delta_b = {o[6], o[7], o[8]};
-    F3 sigma_b2 = vmax(F3{o[9], o[10], o[11]}, splat3(1e-6f));
-    F3 color = splat3(0.0f);
-    for (int pass = 0; pass < 2; ++pass) {
-        for (uint32_t i = 0; i < a.point_count; ++i) {
-            const hm_point_light& L = a.points[i];
-            if ((pass == 0) != (L.kind == HM_LIGHT_POINT)) continue;
-            F3 ltf = point_pos(L) - px.world;
-            float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
-            if (att == 0.0f) continue;
-            BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, safe_normalize(ltf));
-            F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back(g.theta_h, a_back, delta_b, sigma_b2);
-            F3 c = lobe * point_color(L) * att;
-            if (pass == 1) c = c * spot_mask(L, px.world);
-            color = color + c;
-        }
+// Sum of v over the 64 lanes of the wave, in every lane.
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Sum of v over the aligned group of 8 lanes that holds this lane.
+__device__ __forceinline__ float sum8(float v) {
+#pragma unroll
+    for (int m = 4; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__device__ __forceinline__ F3 sum8_3(F3 v) { return {sum8(v.x), sum8(v.y), sum8(v.z)}; }
+__device__ __forceinline__ F3 lane_read3(F3 v, int lane) {
+    return {__shfl(v.x, lane, 64), __shfl(v.y, lane, 64), __shfl(v.z, lane, 64)};
+}
+
+// pixel_color of one pixel (the same in every lane) with two sets of terms,
+// `net` and `ref`, evaluated by the whole wave: lane i takes the point lights
+// i, i + 64, ... and the directional lights i, i + 64, ...; only the
+// summation order differs from pixel_color. The geometry and single
+// scattering are shared by the two colours.
+__device__ void color_pair_by_wave(const ShadeArgs& a, const PixelShade& px, const LocalBack& net, F3 net_ms,
+                                   const LocalBack& ref, F3 ref_ms, int key, float depth, int lane, F3& got,
+                                   F3& want) {
+    F3 cn = splat3(0.0f), cr = splat3(0.0f);
+    for (uint32_t i = (uint32_t)lane; i < a.point_count; i += 64u) {
+        const hm_point_light& L = a.points[i];
+        F3 ltf = point_pos(L) - px.world;
+        float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
+        if (att == 0.0f) continue;
+        BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, safe_normalize(ltf));
+        F3 single = single_scatter(g, px.sigma_a, px.shape);
+        F3 w = point_color(L) * att;
+        if (L.kind != HM_LIGHT_POINT) w = w * spot_mask(L, px.world);
+        cn = cn + (single + 0.7f * local_back(g.theta_h, net.a_back, net.delta_b, net.sigma_b2)) * w;
+        cr = cr + (single + 0.7f * local_back(g.theta_h, ref.a_back, ref.delta_b, ref.sigma_b2)) * w;
     }
-    for (uint32_t i = 0; i < a.directional_count; ++i) {
+    for (uint32_t i = (uint32_t)lane; i < a.directional_count; i += 64u) {
         const hm_directional_light& L = a.directionals[i];
         BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, dir_of(L));
         F3 single = single_scatter(g, px.sigma_a, px.shape);
-        F3 lobe;
+        F3 w = F3{L.color[0], L.color[1], L.color[2]};
         if ((int)i == key) {
-            lobe = expf(-depth) * single + F3{o[0], o[1], o[2]};
+            single = fexp(-depth) * single;
+            cn = cn + (single + net_ms) * w;
+            cr = cr + (single + ref_ms) * w;
         } else {
-            lobe = single + 0.7f * local_back(g.theta_h, a_back, delta_b, sigma_b2);
+            cn = cn + (single + 0.7f * local_back(g.theta_h, net.a_back, net.delta_b, net.sigma_b2)) * w;
+            cr = cr + (single + 0.7f * local_back(g.theta_h, ref.a_back, ref.delta_b, ref.sigma_b2)) * w;
         }
-        color = color + lobe * F3{L.color[0], L.color[1], L.color[2]};
     }
-    return color * a.exposure;
+    got = F3{wave_sum(cn.x), wave_sum(cn.y), wave_sum(cn.z)} * a.exposure;
+    want = F3{wave_sum(cr.x), wave_sum(cr.y), wave_sum(cr.z)} * a.exposure;
 }
 
-// One wave shades a 16x4 tile: features -> LDS -> MFMA forward -> colour.
-// Tiles whose inputs leave the trained range are flagged for the plain path.
-__global__ __launch_bounds__(256) void k_shade_mlp(ShadeArgs a, MlpShadeArgs m) {
+// dual_scatter of one pixel (the same in every lane) by the whole wave:
+// lanes 0-15 take its 16 azimuthal samples.
+__device__ DualScatter dual_scatter_by_wave(const PixelShade& px, int lane) {
+    Lobes mine = {splat3(0.0f), splat3(0.0f), splat3(0.0f)};
+    if (lane < 16) mine = dual_sample((uint32_t)lane, px.cos_theta_o, px.sigma_a);
+    mine = {sum8_3(mine.r), sum8_3(mine.tt), sum8_3(mine.trt)};
+    const Lobes back = {lane_read3(mine.r, 0), lane_read3(mine.tt, 0), lane_read3(mine.trt, 0)};
+    const Lobes fwd = {lane_read3(mine.r, 8), lane_read3(mine.tt, 8), lane_read3(mine.trt, 8)};
+    return dual_finish(fwd, back, px.shape);
+}
+
+// Network shading, one wave per group of 64 consecutive covered pixels, one
+// lane per pixel. Per group: features -> LDS (bf16) -> MFMA forward -> LDS ->
+// dual-scattering terms -> colour (plain_pixel_color with the network's terms
+// in place of dual_scatter's). A group with a feature outside the trained range is shaded by the
+// plain method instead, decided before the network runs. When `tol` > 0,
+// every PROBE_STRIDE-th group also shades one probe pixel by the plain method
+// (all 64 lanes on that pixel) and is re-shaded by the plain method when the
+// network colour misses it by more than `tol`. The pixel set-up is redone
+// after the network (from the G-buffer word held in registers) so that its
+// 30 values are not held through it.
+constexpr uint32_t PROBE_STRIDE = 4;
+
+__global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs m, PixelList L) {
     __shared__ bf16x8 s_tiles[MLP_TILES * 64];
-    __shared__ float s_bias[MLP_NBIAS];
-    __shared__ __bf16 s_x[4][64 * 32];
-    __shared__ float s_y[4][64 * 16];
-    const View& p = *reinterpret_cast<const View*>(&a.view);
+    __shared__ __align__(16) float s_bias[MLP_NBIAS];
+    __shared__ float s_lo[MLP_IN], s_hi[MLP_IN];
+    __shared__ float4 s_io[4][64 * 4];  // per wave: 64 x 32 bf16 features, then 64 x 16 outputs
+    __shared__ uint32_t pre[LIST_SUBS + 1];
+    list_prefix(L, pre);  // synchronizes the block
+    const uint32_t n = pre[LIST_SUBS], groups = (n + 63u) / 64u;
+    if (blockIdx.x * 4u >= groups) return;  // the grid is sized for the list's capacity
     {
         const uint4* src = reinterpret_cast<const uint4*>(m.packed);
         uint4* dst = reinterpret_cast<uint4*>(s_tiles);
         for (int i = threadIdx.x; i < MLP_TILES * 64; i += 256) dst[i] = src[i];
         for (int i = threadIdx.x; i < MLP_NBIAS; i += 256) s_bias[i] = m.bias[i];
+        if (threadIdx.x < MLP_IN) {
+            s_lo[threadIdx.x] = m.lo[threadIdx.x];
+            s_hi[threadIdx.x] = m.hi[threadIdx.x];
+        }
     }
     __syncthreads();
+    const View& p = *reinterpret_cast<const View*>(&a.view);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, q = lane & 15;
-    uint32_t x = blockIdx.x * 16u + (uint32_t)(lane & 15);
-    uint32_t y = blockIdx.y * 16u + (uint32_t)(wv * 4 + (lane >> 4));
-    uint32_t width = to_u32(p.screen.x), height = to_u32(p.screen.y);
-    bool inb = x < width && y < height;
-    size_t pix = inb ? (size_t)y * width + x : 0;
-    uint64_t word = inb ? a.center[pix] : HM_GB_EMPTY;
-    bool hit = !gb_empty(word);
-    int key = key_light(a);
-    PixelShade px;
-    float depth = 0.0f;
-    bool outside = false;
-    float f[32];
-    for (int k = 0; k < 32; ++k) f[k] = 0.0f;
-    if (hit) {
-        px = pixel_setup(p, word, a.beta[pix], x, y);
-        F3 kd = key >= 0 ? dir_of(a.directionals[key]) : F3{0.0f, 1.0f, 0.0f};
-        mlp_features(p, px, kd, a.dom, f);
-        depth = -logf(mx(f[9 + 8], 1e-6f));
-        for (int k = 9; k < MLP_NFEAT; ++k) outside |= f[k] < m.lo[k] || f[k] > m.hi[k];
-    }
-    // stage features for the matrix cores (row = pixel lane)
-    for (int k = 0; k < 32; ++k) s_x[wv][lane * 32 + k] = (__bf16)f[k];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const bf16x8* X = reinterpret_cast<const bf16x8*>(s_x[wv]);
+    const uint32_t width = to_u32(p.screen.x), pixels = width * to_u32(p.screen.y);
+    const int key = key_light(a);
+    const F3 kd = key >= 0 ? dir_of(a.directionals[key]) : F3{0.0f, 1.0f, 0.0f};
+    float4* io = s_io[wv];
+    for (uint32_t grp = blockIdx.x * 4u + (uint32_t)wv; grp < groups && grp < m.groups_cap;
+         grp += gridDim.x * 4u) {
+        const uint32_t i = grp * 64u + (uint32_t)lane;
+        const uint32_t pix = i < n ? list_entry(L, pre, i) : pixels;
+        const bool valid = pix < pixels;
+        float depth = 0.0f;
+        bool outside = false;
+        // the pixel's G-buffer word and coverage, kept to redo the set-up
+        // after the network without reading them again
+        uint64_t word = 0;
+        uint32_t beta_word = 0;
+        {
+            float f[32];
+            for (int k = 0; k < 32; ++k) f[k] = 0.0f;
+            if (valid) {
+                word = a.center[pix];
+                beta_word = a.beta[pix];
+                const PixelShade px = pixel_setup(p, word, beta_word, pix % width, pix / width);
+                depth = mlp_features(p, px, kd, a.dom, f);
+                for (int k = 9; k < MLP_NFEAT; ++k) outside |= f[k] < s_lo[k] || f[k] > s_hi[k];
+            }
+            // row `lane`: 32 bf16 = four 16-byte vectors
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-        bf16x8 b0 = X[(cb * 16 + q) * 4 + g];
-        f32x4 yv = mlp_fwd16(s_tiles, s_bias, b0, lane, nullptr);
-        *reinterpret_cast<float4*>(&s_y[wv][(cb * 16 + q) * 16 + 4 * g]) = make_float4(yv[0], yv[1], yv[2], yv[3]);
+            for (int c = 0; c < 4; ++c) {
+                bf16x8 v;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v[k] = (__bf16)f[8 * c + k];
+                io[lane * 4 + c] = __builtin_bit_cast(float4, v);
+            }
+        }
+        uint32_t flag = __any(outside) ? 1u : 0u;
+        float o[MLP_NOUT] = {};
+        if (flag == 0u) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            bf16x8 b0[4];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) b0[cb] = __builtin_bit_cast(bf16x8, io[(cb * 16 + q) * 4 + g]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();  // operands read before the outputs overwrite them
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            f32x4 yv[4];
+            mlp_fwd64(s_tiles, s_bias, b0, lane, yv);
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) io[(cb * 16 + q) * 4 + g] = make_float4(yv[cb][0], yv[cb][1], yv[cb][2], yv[cb][3]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const float* y = reinterpret_cast<const float*>(io + lane * 4);
+#pragma unroll
+            for (int k = 0; k < MLP_NOUT; ++k) o[k] = mlp_log_output(k) ? mx(fexp(y[k]) - MLP_LOG_EPS, 0.0f) : y[k];
+        }
+        if (flag == 0u && m.tol > 0.0f && grp % PROBE_STRIDE == 0u) {
+            // probe: one pixel of the group, the same in every lane, shaded
+            // with the network's terms and with the plain method's
+            const uint32_t in_group = n - grp * 64u < 64u ? n - grp * 64u : 64u;
+            const int r = (int)(pcg(grp) % in_group);
+            const uint32_t ppix = __shfl(pix, r, 64);
+            if (ppix < pixels) {
+                const PixelShade pp = pixel_setup(p, a.center[ppix], a.beta[ppix], ppix % width, ppix / width);
+                const float pd = __shfl(depth, r, 64);
+                float po[MLP_NOUT];
+                for (int k = 0; k < MLP_NOUT; ++k) po[k] = __shfl(o[k], r, 64);
+                const F3 key_dir_p = kd;
+                const DualScatter dn = network_dual_scatter(po, pp.shape);
+                const DualScatter ds = dual_scatter_by_wave(pp, lane);
+                const float th = key >= 0 ? key_theta_h(pp, key_dir_p) : 0.0f;
+                const F3 net_ms = key >= 0 ? key_multiple_scattering(th, pd, pp.shape, dn) : splat3(0.0f);
+                const F3 key_ms = key >= 0 ? key_multiple_scattering(th, pd, pp.shape, ds) : splat3(0.0f);
+                F3 got, ref;
+                color_pair_by_wave(a, pp, local_terms(dn), net_ms, local_terms(ds), key_ms, key, pd, lane, got, ref);
+                const float err = fabsf(got.x - ref.x) + fabsf(got.y - ref.y) + fabsf(got.z - ref.z);
+                const float mag = fabsf(ref.x) + fabsf(ref.y) + fabsf(ref.z);
+                if (err > m.tol * mx(mag, m.floor)) flag = 2u;
+            }
+        }
+        F3 c = splat3(0.0f);
+        PixelShade px;
+        if (valid) {
+            px = pixel_setup(p, word, beta_word, pix % width, pix / width);
+            // the network's dual-scattering terms, or the plain method's
+            // (plain_pixel_color; `depth` is its key-light depth)
+            const DualScatter ds = flag == 0u ? network_dual_scatter(o, px.shape)
+                                              : dual_scatter(px.cos_theta_o, px.sigma_a, px.shape);
+            const LocalBack lb = local_terms(ds);
+            const F3 key_ms = key >= 0 ? key_multiple_scattering(key_theta_h(px, kd), depth, px.shape, ds) : splat3(0.0f);
+            c = pixel_color(a, px, lb, key, depth, key_ms);
+        }
+        if (valid) {
+            a.shaded[pix] = make_float4(c.x, c.y, c.z, 1.0f);
+            a.shaded_depth[pix] = px.view_dist;
+        }
+        if (lane == 0) m.group_flags[grp] = flag;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();  // the LDS rows are rewritten by the next group
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    bool flag = __any(hit && outside);
-    if (lane == 0) {
-        uint32_t ty = blockIdx.y * 4u + (uint32_t)wv;
-        if (ty < m.tiles_y) m.tile_flags[ty * m.tiles_x + blockIdx.x] = flag ? 1u : 0u;
-    }
-    if (!inb) return;
-    if (!hit) {
-        a.shaded[pix] = make_float4(0, 0, 0, 0);
-        a.shaded_depth[pix] = 0.0f;
-        return;
-    }
-    if (flag) return;  // the plain path shades this tile
-    float o[12];
-    for (int k = 0; k < 12; ++k) o[k] = s_y[wv][lane * 16 + k];
-    F3 c = shade_from_mlp(a, px, o, key, depth);
-    a.shaded[pix] = make_float4(c.x, c.y, c.z, 1.0f);
-    a.shaded_depth[pix] = px.view_dist;
-}
-
-// One probe pixel per network-shaded tile is shaded by the plain method; a
-// tile whose probe misses by more than the bound is flagged for fallback.
-__global__ __launch_bounds__(256) void k_mlp_check(ShadeArgs a, MlpShadeArgs m) {
-    const View& p = *reinterpret_cast<const View*>(&a.view);
-    uint32_t tile = blockIdx.x * 256u + threadIdx.x;
-    if (tile >= m.tiles_x * m.tiles_y || m.tile_flags[tile] != 0u) return;
-    uint32_t width = to_u32(p.screen.x), height = to_u32(p.screen.y);
-    uint32_t x0 = (tile % m.tiles_x) * 16u, y0 = (tile / m.tiles_x) * 4u;
-    uint32_t start = pcg(tile) & 63u;
-    for (uint32_t k = 0; k < 64u; ++k) {
-        uint32_t idx = (start + k) & 63u;
-        uint32_t x = x0 + (idx & 15u), y = y0 + (idx >> 4);
-        if (x >= width || y >= height) continue;
-        size_t pix = (size_t)y * width + x;
-        uint64_t word = a.center[pix];
-        if (gb_empty(word)) continue;
-        PixelShade px = pixel_setup(p, word, a.beta[pix], x, y);
-        F3 ref = plain_pixel_color(a, p, px);
-        float4 got = a.shaded[pix];
-        float err = fabsf(got.x - ref.x) + fabsf(got.y - ref.y) + fabsf(got.z - ref.z);
-        float mag = fabsf(ref.x) + fabsf(ref.y) + fabsf(ref.z);
-        if (err > m.tol * mx(mag, m.floor)) m.tile_flags[tile] = 2u;
-        return;
-    }
-}
-
-// Plain shading of every flagged tile (one wave per 16x4 tile).
-__global__ __launch_bounds__(256) void k_shade_tiles(ShadeArgs a, MlpShadeArgs m) {
-    const View& p = *reinterpret_cast<const View*>(&a.view);
-    uint32_t tile = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6));
-    if (tile >= m.tiles_x * m.tiles_y) return;
-    uint32_t f = m.tile_flags[tile];
-    if (f == 0u) return;
-    uint32_t lane = threadIdx.x & 63u;
-    uint32_t width = to_u32(p.screen.x), height = to_u32(p.screen.y);
-    uint32_t x = (tile % m.tiles_x) * 16u + (lane & 15u), y = (tile / m.tiles_x) * 4u + (lane >> 4);
-    if (lane == 0) m.tile_flags[tile] = f | 4u;  // remembers that the plain path ran
-    if (x >= width || y >= height) return;
-    size_t pix = (size_t)y * width + x;
-    uint64_t word = a.center[pix];
-    if (gb_empty(word)) return;
-    PixelShade px = pixel_setup(p, word, a.beta[pix], x, y);
-    F3 c = plain_pixel_color(a, p, px);
-    a.shaded[pix] = make_float4(c.x, c.y, c.z, 1.0f);
-    a.shaded_depth[pix] = px.view_dist;
 }
 
 // ---- launchers --------------------------------------------------------------------------
@@ -598,7 +697,7 @@ hipError_t launch_mlp_pack(hipStream_t s, const float* w, __bf16* packed) {
 hipError_t launch_mlp_forward(hipStream_t s, const float* x, uint32_t n, const __bf16* packed, const float* bias,
                               float* y) {
     if (n == 0) return hipSuccess;
-    uint32_t blocks = (n + 63) / 64;
+    uint32_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(256), 0, s, x, n, packed, bias, y);
     return hipGetLastError();
@@ -640,13 +739,12 @@ hipError_t launch_mlp_samples(hipStream_t s, const ShadeArgs& a, const SampleArg
     return hipGetLastError();
 }
 
-hipError_t launch_shade_mlp(hipStream_t s, const ShadeArgs& a, const MlpShadeArgs& m, uint32_t width,
-                            uint32_t height) {
-    if (width == 0 || height == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_shade_mlp, dim3((width + 15) / 16, (height + 15) / 16), dim3(256), 0, s, a, m);
-    uint32_t tiles = m.tiles_x * m.tiles_y;
-    hipLaunchKernelGGL(k_mlp_check, dim3((tiles + 255) / 256), dim3(256), 0, s, a, m);
-    hipLaunchKernelGGL(k_shade_tiles, dim3((tiles + 3) / 4), dim3(256), 0, s, a, m);
+hipError_t launch_shade_mlp(hipStream_t s, const ShadeArgs& a, const MlpShadeArgs& m, const PixelList& L,
+                            int blocks) {
+    if (m.groups_cap == 0) return hipSuccess;
+    const uint32_t want = (m.groups_cap + 3u) / 4u;
+    const uint32_t grid = want < (uint32_t)blocks ? want : (uint32_t)blocks;
+    hipLaunchKernelGGL(k_shade_mlp, dim3(grid), dim3(256), 0, s, a, m, L);
     return hipGetLastError();
 }
 

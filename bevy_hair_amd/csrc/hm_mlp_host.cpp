@@ -98,7 +98,7 @@ hm_status ensure_batch(hm_ctx* ctx, hm_mlp_state* s, uint32_t batch) {
     return HM_OK;
 }
 
-hm::TrainArgs train_args(hm_mlp_state* s, const float* x, const float* t, uint32_t n, const float scale[12]) {
+hm::TrainArgs train_args(hm_mlp_state* s, const float* x, const float* t, uint32_t n, const float scale[MLP_NOUT]) {
     hm::TrainArgs a;
     std::memset(&a, 0, sizeof(a));
     a.x = x;
@@ -206,7 +206,7 @@ hm_status hm_mlp_forward(hm_ctx* ctx, const float* x, uint32_t n, float* y) {
     return HM_OK;
 }
 
-hm_status hm_mlp_gradients(hm_ctx* ctx, const float* x, const float* t, uint32_t n, const float scale[12],
+hm_status hm_mlp_gradients(hm_ctx* ctx, const float* x, const float* t, uint32_t n, const float scale[MLP_NOUT],
                            float* loss, float* dw, float* db) {
     hm_mlp_state* s;
     hm_status st = state(ctx, &s);
@@ -216,9 +216,9 @@ hm_status hm_mlp_gradients(hm_ctx* ctx, const float* x, const float* t, uint32_t
     if ((st = ensure_batch(ctx, s, n)) != HM_OK) return st;
     float *dx = nullptr, *dt = nullptr;
     MLP_HIP(ctx, alloc(&dx, (size_t)n * 32 * 4));
-    hipError_t e = alloc(&dt, (size_t)n * 12 * 4);
+    hipError_t e = alloc(&dt, (size_t)n * MLP_NOUT * 4);
     if (e == hipSuccess) e = hipMemcpy(dx, x, (size_t)n * 32 * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dt, t, (size_t)n * 12 * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dt, t, (size_t)n * MLP_NOUT * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(s->gw, 0, MLP_NW * 4);
     if (e == hipSuccess) e = hipMemset(s->gb, 0, MLP_NBIAS * 4);
     if (e == hipSuccess) e = hipMemset(s->loss, 0, 4);
@@ -249,7 +249,7 @@ hm_status hm_mlp_reserve_samples(hm_ctx* ctx, uint32_t capacity) {
         release(s->st);
         s->scap = 0;
         MLP_HIP(ctx, alloc(&s->sx, (size_t)capacity * 32 * 4));
-        MLP_HIP(ctx, alloc(&s->st, (size_t)capacity * 12 * 4));
+        MLP_HIP(ctx, alloc(&s->st, (size_t)capacity * MLP_NOUT * 4));
         s->scap = capacity;
     }
     MLP_HIP(ctx, hipMemset(s->scount, 0, 4));
@@ -296,7 +296,7 @@ hm_status hm_mlp_read_samples(hm_ctx* ctx, float* x, float* t, uint32_t n) {
     if (n > s->scap) return hm_fail(ctx, HM_ERR_INVALID, "more samples than held");
     MLP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (x) MLP_HIP(ctx, hipMemcpy(x, s->sx, (size_t)n * 32 * 4, hipMemcpyDeviceToHost));
-    if (t) MLP_HIP(ctx, hipMemcpy(t, s->st, (size_t)n * 12 * 4, hipMemcpyDeviceToHost));
+    if (t) MLP_HIP(ctx, hipMemcpy(t, s->st, (size_t)n * MLP_NOUT * 4, hipMemcpyDeviceToHost));
     return HM_OK;
 }
 
@@ -308,7 +308,7 @@ hm_status hm_mlp_set_samples(hm_ctx* ctx, const float* x, const float* t, uint32
     if (n > s->scap) return hm_fail(ctx, HM_ERR_CAPACITY, "more samples than the reserved capacity");
     MLP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     MLP_HIP(ctx, hipMemcpy(s->sx, x, (size_t)n * 32 * 4, hipMemcpyHostToDevice));
-    MLP_HIP(ctx, hipMemcpy(s->st, t, (size_t)n * 12 * 4, hipMemcpyHostToDevice));
+    MLP_HIP(ctx, hipMemcpy(s->st, t, (size_t)n * MLP_NOUT * 4, hipMemcpyHostToDevice));
     MLP_HIP(ctx, hipMemcpy(s->scount, &n, 4, hipMemcpyHostToDevice));
     return HM_OK;
 }
@@ -327,7 +327,7 @@ hm_status hm_mlp_set_range(hm_ctx* ctx, const float lo[32], const float hi[32]) 
     return HM_OK;
 }
 
-hm_status hm_mlp_train(hm_ctx* ctx, uint32_t epochs, uint32_t batch, float lr, const float scale[12], uint32_t seed,
+hm_status hm_mlp_train(hm_ctx* ctx, uint32_t epochs, uint32_t batch, float lr, const float scale[MLP_NOUT], uint32_t seed,
                        float* final_loss) {
     hm_mlp_state* s;
     hm_status st = state(ctx, &s);
@@ -368,6 +368,13 @@ hm_status hm_mlp_train(hm_ctx* ctx, uint32_t epochs, uint32_t batch, float lr, c
 }
 
 hm_status hm_shade_mlp(hm_ctx* ctx, const hm_params* p, float exposure, float tolerance, uint32_t* fallback_tiles) {
+    return hm_shade_mlp_impl(ctx, p, exposure, tolerance, fallback_tiles, hm_lists());
+}
+
+}  // extern "C"
+
+hm_status hm_shade_mlp_impl(hm_ctx* ctx, const hm_params* p, float exposure, float tolerance, uint32_t* fallback,
+                            const hm_lists& lists) {
     hm_mlp_state* s;
     hm_status st = state(ctx, &s);
     if (st != HM_OK) return st;
@@ -376,55 +383,55 @@ hm_status hm_shade_mlp(hm_ctx* ctx, const hm_params* p, float exposure, float to
     if (!p || (uint32_t)p->screen[0] != ctx->width || (uint32_t)p->screen[1] != ctx->height ||
         (uint32_t)p->dom_info[0] != ctx->dom_size)
         return hm_fail(ctx, HM_ERR_INVALID, "shade params differ from the targets");
-    uint32_t tx = (ctx->width + 15) / 16, ty = (ctx->height + 3) / 4;
-    if (tx * ty > s->flags_cap) {
-        release(s->flags);
-        s->flags_cap = 0;
-        MLP_HIP(ctx, alloc(&s->flags, (size_t)tx * ty * 4));
-        s->flags_cap = tx * ty;
-    }
     hm::ShadeArgs a;
     shade_args(ctx, p, exposure, a);
+    struct T {
+        hm_ctx* c;
+        size_t base;
+        bool on;
+        T(hm_ctx* ctx_) : c(ctx_) {
+            on = c->slots && c->frame < c->slots;
+            base = ((size_t)c->frame * HM_K_COUNT + HM_K_SHADE) * 2;
+            if (on) (void)hipEventRecord(c->ev[base], c->stream);
+        }
+        ~T() {
+            if (!on) return;
+            (void)hipEventRecord(c->ev[base + 1], c->stream);
+            c->ran[base / 2] = 1;
+        }
+    } timer(ctx);
+    hm::PixelList L;
+    if ((st = hm_shade_list(ctx, lists, &L)) != HM_OK) return st;
+    // At most LIST_SUBS * cap listed pixels: at most cap groups of 64.
+    const uint32_t groups_cap = L.cap;
+    if (groups_cap > s->flags_cap) {
+        release(s->flags);
+        s->flags_cap = 0;
+        MLP_HIP(ctx, alloc(&s->flags, (size_t)groups_cap * 4));
+        s->flags_cap = groups_cap;
+    }
     hm::MlpShadeArgs m;
     std::memset(&m, 0, sizeof(m));
     m.packed = s->packed;
     m.bias = s->b;
-    m.tile_flags = s->flags;
-    m.tiles_x = tx;
-    m.tiles_y = ty;
+    m.group_flags = s->flags;
+    m.groups_cap = groups_cap;
     for (int k = 0; k < MLP_IN; ++k) {
         m.lo[k] = s->range ? s->lo[k] : -3.4e38f;
         m.hi[k] = s->range ? s->hi[k] : 3.4e38f;
     }
     m.tol = tolerance;
     m.floor = 1e-4f;
-    {
-        struct T {
-            hm_ctx* c;
-            size_t base;
-            bool on;
-            T(hm_ctx* ctx_) : c(ctx_) {
-                on = c->slots && c->frame < c->slots;
-                base = ((size_t)c->frame * HM_K_COUNT + HM_K_SHADE) * 2;
-                if (on) (void)hipEventRecord(c->ev[base], c->stream);
-            }
-            ~T() {
-                if (!on) return;
-                (void)hipEventRecord(c->ev[base + 1], c->stream);
-                c->ran[base / 2] = 1;
-            }
-        } timer(ctx);
-        MLP_HIP(ctx, hm::launch_shade_mlp(ctx->stream, a, m, ctx->width, ctx->height));
-    }
-    if (fallback_tiles) {
-        std::vector<uint32_t> f((size_t)tx * ty);
+    if (fallback)  // flags of groups past this frame's list would be stale: clear them for the count
+        MLP_HIP(ctx, hm::launch_fill32(ctx->stream, s->flags, groups_cap, 0u, ctx->cu_count));
+    MLP_HIP(ctx, hm::launch_shade_mlp(ctx->stream, a, m, L, ctx->cu_count * 8));
+    if (fallback) {
+        std::vector<uint32_t> f(groups_cap);
         MLP_HIP(ctx, hipMemcpyAsync(f.data(), s->flags, f.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
         MLP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         uint32_t c = 0;
         for (uint32_t v : f) c += (v & 3u) != 0u;
-        *fallback_tiles = c;
+        *fallback = c;
     }
     return HM_OK;
 }
-
-}  // extern "C"

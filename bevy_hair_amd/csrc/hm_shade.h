@@ -100,15 +100,31 @@ __device__ __forceinline__ F3 lobe_mean(const Lobes& l, F3 total, F3 q) {
     return fdiv3(l.r * q.x + l.tt * q.y + l.trt * q.z, vmax(total, splat3(1e-5f)));
 }
 
+// Azimuthal sample i of dual_scatter's 16-point quadrature: i < 8 backward
+// (phi = (i + 0.5) pi / 16), i >= 8 forward (phi = pi/2 + (i - 7.5) pi / 16).
+__device__ __forceinline__ Lobes dual_sample(uint32_t i, float cos_theta_d, F3 sigma_a) {
+    float u = (u2f(i & 7u) + 0.5f) * 0.125f;
+    float phi = i < 8u ? u * 0.5f * kPi : (1.0f + u) * 0.5f * kPi;
+    return azimuthal(cos_theta_d, fcos(phi), sigma_a);
+}
+
+__device__ __forceinline__ Lobes lobes_add(const Lobes& a, const Lobes& b) {
+    return {a.r + b.r, a.tt + b.tt, a.trt + b.trt};
+}
+
+// dual_scatter's closing part from the forward and backward quadrature sums.
+__device__ DualScatter dual_finish(Lobes fwd, Lobes back, const Shape& shape);
+
 __device__ DualScatter dual_scatter(float cos_theta_d, F3 sigma_a, const Shape& shape) {
     Lobes fwd = {splat3(0), splat3(0), splat3(0)}, back = fwd;
     for (uint32_t i = 0; i < 8u; ++i) {
-        float u = (u2f(i) + 0.5f) * 0.125f;
-        Lobes b = azimuthal(cos_theta_d, fcos(u * 0.5f * kPi), sigma_a);
-        Lobes f = azimuthal(cos_theta_d, fcos((1.0f + u) * 0.5f * kPi), sigma_a);
-        back = {back.r + b.r, back.tt + b.tt, back.trt + b.trt};
-        fwd = {fwd.r + f.r, fwd.tt + f.tt, fwd.trt + f.trt};
+        back = lobes_add(back, dual_sample(i, cos_theta_d, sigma_a));
+        fwd = lobes_add(fwd, dual_sample(8u + i, cos_theta_d, sigma_a));
     }
+    return dual_finish(fwd, back, shape);
+}
+
+__device__ DualScatter dual_finish(Lobes fwd, Lobes back, const Shape& shape) {
     float w = kPi / 8.0f;
     fwd = {fwd.r * w, fwd.tt * w, fwd.trt * w};
     back = {back.r * w, back.tt * w, back.trt * w};
@@ -187,35 +203,63 @@ __device__ __forceinline__ F3 global_spread(float theta_h, float depth, float di
     return (1.0f - direct) * transmit * 0.7f * (spread * (1.0f / kPi) + kPi * 0.7f * back_spread);
 }
 
-// hair_bsdf (shade.wesl:193-244) with shadow = 1; t_n, v_n normalized.
-__device__ F3 hair_bsdf(F3 t_n, F3 v_n, F3 light, F3 sigma_a, const Shape& shape, const DualScatter& ds,
-                        float depth) {
-    BsdfGeom g = bsdf_geom_n(t_n, v_n, light);
-    F3 single = single_scatter(g, sigma_a, shape);
-    F3 back = local_back(g.theta_h, ds.a_back, ds.delta_b, ds.sigma_b2);
-    float direct = fexp(-depth);
-    F3 color = direct * (single + 0.7f * back);
-    if (depth > 1e-3f) color = color + global_spread(g.theta_h, depth, direct, shape, ds);
-    return color;
+
+// The four bilinear taps of one DOM lookup (shade.wesl:279-310): every load
+// is issued before any is used, so the lookup costs one memory round trip. A
+// tap outside the map reads texel 0 and is marked empty.
+struct DomTaps {
+    uint32_t front[4];
+    uint4 counts[4][4];
+    float tx, ty, dist;
+};
+
+// False when the pixel projects outside the light's clip volume or the DOM
+// is off (visibility 1).
+__device__ __forceinline__ bool dom_taps(const View& p, const uint32_t* __restrict__ dom, F3 world, float beta,
+                                         DomTaps& T) {
+    if (p.flags[3] == 0u) return false;
+    float cx = mat_row(p.lclip, 0, world.x, world.y, world.z, 1.0f);
+    float cy = mat_row(p.lclip, 1, world.x, world.y, world.z, 1.0f);
+    float cw = mat_row(p.lclip, 3, world.x, world.y, world.z, 1.0f);
+    if (cw <= 0.0f) return false;
+    float ndc_x = rdiv(cx, cw), ndc_y = rdiv(cy, cw);
+    float ux = ndc_x * 0.5f + 0.5f;
+    float uy = 1.0f - (ndc_y * 0.5f + 0.5f);
+    if (ux < 0.0f || uy < 0.0f || ux > 1.0f || uy > 1.0f) return false;
+    uint32_t dom_w = to_u32(p.dom_info.x), dom_h = to_u32(p.dom_info.y);
+    T.dist = dot3(world - xyz(p.light_eye), xyz(p.light_forward)) - (-flog(mx(beta, 1e-4f)) * p.filter_params.z);
+    float fx = ux * u2f(dom_w) - 0.5f;
+    float fy = uy * u2f(dom_h) - 0.5f;
+    float flx = floorf(fx), fly = floorf(fy);
+    int32_t x0 = to_i32(flx), y0 = to_i32(fly);
+    T.tx = fx - flx;
+    T.ty = fy - fly;
+    const uint4* counts = reinterpret_cast<const uint4*>(dom + (size_t)dom_w * dom_h);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        int32_t x = x0 + (c & 1), y = y0 + (c >> 1);
+        bool inside = x >= 0 && y >= 0 && x < (int32_t)dom_w && y < (int32_t)dom_h;
+        uint32_t pix = inside ? (uint32_t)y * dom_w + (uint32_t)x : 0u;
+        uint32_t f = dom[pix];
+        T.front[c] = inside ? f : HM_DOM_EMPTY;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) T.counts[c][q] = counts[(size_t)pix * (HM_DOM_LAYERS / 4) + q];
+    }
+    return true;
 }
 
-__device__ __forceinline__ float dom_texel_visibility(const View& p, const uint32_t* __restrict__ dom, int32_t x,
-                                                      int32_t y, float dist, float near_, float far_, uint32_t dom_w,
-                                                      uint32_t dom_h) {
-    if (x < 0 || y < 0 || x >= (int32_t)dom_w || y >= (int32_t)dom_h) return 1.0f;
-    uint32_t pix = (uint32_t)y * dom_w + (uint32_t)x;
-    uint32_t front_q = dom[pix];
+// Visibility through one tap at light-space distance `dist`.
+__device__ __forceinline__ float dom_tap_visibility(const View& p, uint32_t front_q, const uint4 counts[4], float dist) {
     if (front_q == HM_DOM_EMPTY) return 1.0f;
+    float near_ = p.dom_info.z, far_ = p.dom_info.w;
     float z_front = dequantize_depth(front_q, near_, far_);
     float thickness = rdiv(mx(far_ - near_, 1e-3f), 16.0f);
     float rel = mx(dist - z_front - thickness * 0.05f, 0.0f);
     float t = rdiv(rel, thickness);
-    const uint4* c4 = reinterpret_cast<const uint4*>(dom + (size_t)dom_w * dom_h + (size_t)pix * HM_DOM_LAYERS);
     float occ = 0.0f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        uint4 c = c4[q];
-        uint32_t cs[4] = {c.x, c.y, c.z, c.w};
+        uint32_t cs[4] = {counts[q].x, counts[q].y, counts[q].z, counts[q].w};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             float i = (float)(q * 4 + k);
@@ -227,29 +271,18 @@ __device__ __forceinline__ float dom_texel_visibility(const View& p, const uint3
     return fexp(-occ * p.filter_params.w);
 }
 
+__device__ __forceinline__ float dom_taps_visibility(const View& p, const DomTaps& T) {
+    float v00 = dom_tap_visibility(p, T.front[0], T.counts[0], T.dist);
+    float v10 = dom_tap_visibility(p, T.front[1], T.counts[1], T.dist);
+    float v01 = dom_tap_visibility(p, T.front[2], T.counts[2], T.dist);
+    float v11 = dom_tap_visibility(p, T.front[3], T.counts[3], T.dist);
+    return mixs(mixs(v00, v10, T.tx), mixs(v01, v11, T.tx), T.ty);
+}
+
 __device__ float dom_visibility(const View& p, const uint32_t* __restrict__ dom, F3 world, float beta) {
-    if (p.flags[3] == 0u) return 1.0f;
-    float cx = mat_row(p.lclip, 0, world.x, world.y, world.z, 1.0f);
-    float cy = mat_row(p.lclip, 1, world.x, world.y, world.z, 1.0f);
-    float cw = mat_row(p.lclip, 3, world.x, world.y, world.z, 1.0f);
-    if (cw <= 0.0f) return 1.0f;
-    float ndc_x = rdiv(cx, cw), ndc_y = rdiv(cy, cw);
-    float ux = ndc_x * 0.5f + 0.5f;
-    float uy = 1.0f - (ndc_y * 0.5f + 0.5f);
-    if (ux < 0.0f || uy < 0.0f || ux > 1.0f || uy > 1.0f) return 1.0f;
-    uint32_t dom_w = to_u32(p.dom_info.x), dom_h = to_u32(p.dom_info.y);
-    float near_ = p.dom_info.z, far_ = p.dom_info.w;
-    float dist = dot3(world - xyz(p.light_eye), xyz(p.light_forward)) - (-flog(mx(beta, 1e-4f)) * p.filter_params.z);
-    float fx = ux * u2f(dom_w) - 0.5f;
-    float fy = uy * u2f(dom_h) - 0.5f;
-    float flx = floorf(fx), fly = floorf(fy);
-    int32_t x0 = to_i32(flx), y0 = to_i32(fly);
-    float tx = fx - flx, ty = fy - fly;
-    float v00 = dom_texel_visibility(p, dom, x0, y0, dist, near_, far_, dom_w, dom_h);
-    float v10 = dom_texel_visibility(p, dom, x0 + 1, y0, dist, near_, far_, dom_w, dom_h);
-    float v01 = dom_texel_visibility(p, dom, x0, y0 + 1, dist, near_, far_, dom_w, dom_h);
-    float v11 = dom_texel_visibility(p, dom, x0 + 1, y0 + 1, dist, near_, far_, dom_w, dom_h);
-    return mixs(mixs(v00, v10, tx), mixs(v01, v11, tx), ty);
+    DomTaps T;
+    if (!dom_taps(p, dom, world, beta, T)) return 1.0f;
+    return dom_taps_visibility(p, T);
 }
 
 __device__ __forceinline__ float distance_attenuation(float dist2, float inv_range2) {
@@ -309,42 +342,96 @@ __device__ __forceinline__ F3 point_pos(const hm_point_light& L) {
     return {L.position_radius[0], L.position_radius[1], L.position_radius[2]};
 }
 
-// The plain method's colour of one hit pixel (shade.wesl:442-521, 586-588),
-// probe radiance zero (no environment map) and shadow factors 1.
-__device__ F3 plain_pixel_color(const ShadeArgs& a, const View& p, const PixelShade& px) {
-    DualScatter ds = dual_scatter(px.cos_theta_o, px.sigma_a, px.shape);
+__device__ __forceinline__ int key_light(const ShadeArgs& a) {
+    for (uint32_t i = 0; i < a.directional_count; ++i)
+        if (a.directionals[i].flags & HM_LIGHT_FLAG_SHADOWS) return (int)i;
+    return -1;
+}
+
+__device__ __forceinline__ F3 dir_of(const hm_directional_light& L) {
+    return {L.direction_to_light[0], L.direction_to_light[1], L.direction_to_light[2]};
+}
+
+// The dual-scattering terms of the local back lobe that every light uses.
+struct LocalBack {
+    F3 a_back, delta_b, sigma_b2;
+};
+
+__device__ __forceinline__ LocalBack local_terms(const DualScatter& ds) { return {ds.a_back, ds.delta_b, ds.sigma_b2}; }
+
+// The key light's multiple scattering per unit light colour: hair_bsdf
+// (shade.wesl:193-244) minus its direct single scattering,
+// direct * 0.7 * back + [depth > 1e-3] * global spread.
+__device__ __forceinline__ F3 key_multiple_scattering(float theta_h, float depth, const Shape& shape,
+                                                      const DualScatter& ds) {
+    const float direct = fexp(-depth);
+    F3 ms = direct * (0.7f * local_back(theta_h, ds.a_back, ds.delta_b, ds.sigma_b2));
+    if (depth > 1e-3f) ms = ms + global_spread(theta_h, depth, direct, shape, ds);
+    return ms;
+}
+
+// Point light L's contribution (shade.wesl:449-505) with back-lobe terms
+// `lb`; zero where the light's range does not reach the pixel, as the
+// cluster would not list the light.
+__device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelShade& px, const LocalBack& lb) {
+    F3 ltf = point_pos(L) - px.world;
+    float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
+    if (att == 0.0f) return splat3(0.0f);
+    BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, safe_normalize(ltf));
+    F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back(g.theta_h, lb.a_back, lb.delta_b, lb.sigma_b2);
+    F3 c = lobe * point_color(L) * att;
+    if (L.kind != HM_LIGHT_POINT) c = c * spot_mask(L, px.world);
+    return c;
+}
+
+// Directional light i's contribution: for the key light (optical depth
+// `depth`) exp(-depth) * single + key_ms, for the others single + 0.7 * back.
+__device__ __forceinline__ F3 directional_term(const hm_directional_light& L, bool is_key, const PixelShade& px,
+                                               const LocalBack& lb, float depth, F3 key_ms) {
+    BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, dir_of(L));
+    F3 single = single_scatter(g, px.sigma_a, px.shape);
+    F3 lobe = is_key ? fexp(-depth) * single + key_ms
+                     : single + 0.7f * local_back(g.theta_h, lb.a_back, lb.delta_b, lb.sigma_b2);
+    return lobe * F3{L.color[0], L.color[1], L.color[2]};
+}
+
+// Colour of one hit pixel (shade.wesl:442-521, 586-588), probe radiance zero
+// (no environment map) and shadow factors 1, from the back-lobe terms `lb`
+// and the key light's multiple scattering `key_ms`. Point lights, then spot
+// lights (the order of Bevy's cluster ranges), then directional lights.
+__device__ F3 pixel_color(const ShadeArgs& a, const PixelShade& px, const LocalBack& lb, int key, float depth,
+                          F3 key_ms) {
     F3 color = splat3(0.0f);
-    // Point lights, then spot lights (the order of Bevy's cluster ranges).
     for (int pass = 0; pass < 2; ++pass) {
         for (uint32_t i = 0; i < a.point_count; ++i) {
             const hm_point_light& L = a.points[i];
             if ((pass == 0) != (L.kind == HM_LIGHT_POINT)) continue;
-            F3 ltf = point_pos(L) - px.world;
-            float dist2 = dot3(ltf, ltf);
-            float att = distance_attenuation(dist2, L.color_inverse_square_range[3]);
-            if (att == 0.0f) continue;  // outside the light's range: the cluster would not list it
-            F3 incident = safe_normalize(ltf);
-            F3 lobe = hair_bsdf(px.t_n, px.v_n, incident, px.sigma_a, px.shape, ds, 0.0f);
-            F3 c = lobe * point_color(L) * att;
-            if (pass == 1) c = c * spot_mask(L, px.world);
-            color = color + c;
+            color = color + point_term(L, px, lb);
         }
     }
-    bool dom_left = true;
-    for (uint32_t i = 0; i < a.directional_count; ++i) {
-        const hm_directional_light& L = a.directionals[i];
-        bool casts = (L.flags & HM_LIGHT_FLAG_SHADOWS) != 0u;
-        float depth = 0.0f;
-        if (dom_left && casts) {
-            depth = -flog(mx(dom_visibility(p, a.dom, px.world, px.beta), 1e-6f));
-            dom_left = false;
-        }
-        F3 lobe = hair_bsdf(px.t_n, px.v_n, F3{L.direction_to_light[0], L.direction_to_light[1],
-                                                     L.direction_to_light[2]},
-                            px.sigma_a, px.shape, ds, depth);
-        color = color + lobe * F3{L.color[0], L.color[1], L.color[2]};
-    }
+    for (uint32_t i = 0; i < a.directional_count; ++i)
+        color = color + directional_term(a.directionals[i], (int)i == key, px, lb, depth, key_ms);
     return color * a.exposure;
+}
+
+// theta_h of the key light at this pixel (bsdf_geom_n's).
+__device__ __forceinline__ float key_theta_h(const PixelShade& px, F3 key_dir) {
+    return bsdf_geom_n(px.t_n, px.v_n, key_dir).theta_h;
+}
+
+// The plain method's colour of one hit pixel: dual scattering evaluated
+// (shade.wesl:153-190), the DOM visibility for the first shadow-casting
+// directional light (the key light).
+__device__ F3 plain_pixel_color(const ShadeArgs& a, const View& p, const PixelShade& px) {
+    const DualScatter ds = dual_scatter(px.cos_theta_o, px.sigma_a, px.shape);
+    const int key = key_light(a);
+    float depth = 0.0f;
+    F3 key_ms = splat3(0.0f);
+    if (key >= 0) {
+        depth = -flog(mx(dom_visibility(p, a.dom, px.world, px.beta), 1e-6f));
+        key_ms = key_multiple_scattering(key_theta_h(px, dir_of(a.directionals[key])), depth, px.shape, ds);
+    }
+    return pixel_color(a, px, local_terms(ds), key, depth, key_ms);
 }
 
 }  // namespace hm

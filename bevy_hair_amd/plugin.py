@@ -365,14 +365,14 @@ class Renderer:
 
     def mlp_read_samples(self, n: int):
         x = np.empty((n, 32), np.float32)
-        t = np.empty((n, 12), np.float32)
+        t = np.empty((n, N.MLP_NOUT), np.float32)
         self._check(N.lib().hm_mlp_read_samples(self.ctx, N.ptr(x), N.ptr(t), n), "hm_mlp_read_samples")
         return x, t
 
     def mlp_set_samples(self, x: np.ndarray, t: np.ndarray):
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(t, np.float32)
-        assert x.shape[1] == 32 and t.shape[1] == 12 and x.shape[0] == t.shape[0]
+        assert x.shape[1] == 32 and t.shape[1] == N.MLP_NOUT and x.shape[0] == t.shape[0]
         self._check(N.lib().hm_mlp_set_samples(self.ctx, N.ptr(x), N.ptr(t), x.shape[0]), "hm_mlp_set_samples")
 
     def mlp_set_range(self, lo, hi):

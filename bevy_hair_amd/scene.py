@@ -122,6 +122,8 @@ WORKLOADS = {
     "c2": Workload("c2_wavy_100000x64_1920x1080", "wavy", 100_000, 2, 1920, 1080, 2048, 65, 1.0, 32),
     # 60k curly strands x 126 segments (ncp 127 is the maximum), 2160p, key + 128 lights.
     "c3": Workload("c3_curly_60000x126_3840x2160", "curly", 60_000, 3, 3840, 2160, 2048, 127, 1.0, 128),
+    # The learned shading term's evaluation: the c1 groom rendered at 1080p.
+    "c4": Workload("c4_straight_4096x16_1920x1080", "straight", 4096, 1, 1920, 1080, 512, 17, 1.0, 0),
     # Small parity cases.
     "tiny": Workload("tiny_straight_512x16_128x96", "straight", 512, 7, 128, 96, 128, 17, 0.0, 2),
     "small_lod": Workload("small_wavy_2048_lod_256x160", "wavy", 2048, 11, 256, 160, 256, 0, 0.0, 4),

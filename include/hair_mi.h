@@ -388,14 +388,17 @@ hm_status hm_bench_copy(hm_ctx* ctx, size_t bytes, uint32_t reps, float* gbps);
 hm_status hm_bench_atomics(hm_ctx* ctx, size_t target_bytes, uint32_t kind, uint64_t count, float* per_second);
 
 /* ---- learned shading (no counterpart in the reference) -------------------
- * A 29 -> 64 -> 64 -> 64 -> 12 MLP (bf16 weights on the matrix cores, f32
- * accumulation) stands in for dual_scatter (shade.wesl:153-190) and the key
- * light's multiple-scattering part of hair_bsdf (shade.wesl:220-242); single
- * scattering stays analytic. Features per pixel (float[32]): tangent[3],
- * view[3], key light direction[3], 16 DOM visibility samples at the pixel's
- * light depth shifted by (k - 8) / 4 opacity layers, sigma_a * 0.25 [3],
- * roughness, 3 zeros. Outputs (float[16], 12 used): key-light multiple
- * scattering per unit light colour [3], a_back[3], delta_b[3], sigma_b^2[3].
+ * A 32 -> 64 -> 64 -> 64 -> 16 MLP (bf16 weights on the matrix cores, f32
+ * accumulation) stands in for dual_scatter (shade.wesl:153-190): its outputs
+ * are the dual-scattering terms from which every light's back lobe and the
+ * key light's global multiple scattering and DOM forward-scattering spread
+ * (shade.wesl:220-242) are evaluated; single scattering stays analytic.
+ * Features per pixel (float[32]): tangent[3], view[3], key light
+ * direction[3], the 16 DOM layer transmittances (bilinear over the pixel's
+ * four DOM taps), sigma_a * 0.25 [3], roughness, the key light's theta_h / pi,
+ * cos(theta_o), the key light's optical depth * 0.125. Outputs (float[16]):
+ * log(v + 1e-6) of the forward R lobe sum, forward TT[3], forward TRT[3],
+ * a_back[3], then delta_b[3] (linear), then log(sigma_b^2 + 1e-6)[3].
  * Weights: float[11264] (W0 64x32, W1 64x64, W2 64x64, W3 16x64, out x in
  * row-major), biases float[208]. */
 
@@ -405,9 +408,9 @@ hm_status hm_mlp_set_weights(hm_ctx* ctx, const float* weights, const float* bia
 hm_status hm_mlp_get_weights(hm_ctx* ctx, float* weights, float* biases);
 /* y[n][16] = network(x[n][32]); host arrays, synchronous. */
 hm_status hm_mlp_forward(hm_ctx* ctx, const float* x, uint32_t n, float* y);
-/* Loss sum_o ((y_o - t_o) * scale_o)^2 / (12 n) over the 12 outputs and its
+/* Loss sum_o ((y_o - t_o) * scale_o)^2 / (16 n) over the 16 outputs and its
  * gradients (float[11264], float[208]) for one batch; synchronous. */
-hm_status hm_mlp_gradients(hm_ctx* ctx, const float* x, const float* t, uint32_t n, const float scale[12],
+hm_status hm_mlp_gradients(hm_ctx* ctx, const float* x, const float* t, uint32_t n, const float scale[16],
                            float* loss, float* grad_weights, float* grad_biases);
 /* Sample buffer for training (features + plain-method targets); resets the count. */
 hm_status hm_mlp_reserve_samples(hm_ctx* ctx, uint32_t capacity);
@@ -415,20 +418,23 @@ hm_status hm_mlp_reserve_samples(hm_ctx* ctx, uint32_t capacity);
  * a frame): each hit pixel is kept with probability `keep`. */
 hm_status hm_mlp_collect(hm_ctx* ctx, const hm_params* camera_params, float keep, uint32_t seed, uint32_t* total);
 hm_status hm_mlp_read_samples(hm_ctx* ctx, float* x, float* t, uint32_t n);
-/* Replaces the held samples with host ones (x[n][32], t[n][12]); n <= capacity. */
+/* Replaces the held samples with host ones (x[n][32], t[n][16]); n <= capacity. */
 hm_status hm_mlp_set_samples(hm_ctx* ctx, const float* x, const float* t, uint32_t n);
 /* Feature range the network was trained on (NULL clears it); tiles with a
  * pixel outside it are shaded by the plain method. */
 hm_status hm_mlp_set_range(hm_ctx* ctx, const float lo[32], const float hi[32]);
 /* Adam over the collected samples for `epochs` passes; returns the last
  * epoch's mean loss. */
-hm_status hm_mlp_train(hm_ctx* ctx, uint32_t epochs, uint32_t batch, float learning_rate, const float scale[12],
+hm_status hm_mlp_train(hm_ctx* ctx, uint32_t epochs, uint32_t batch, float learning_rate, const float scale[16],
                        uint32_t seed, float* final_loss);
-/* Network shading of the frame's G-buffer into HM_BUF_SHADED. Per 16x4 tile,
- * one probe pixel is shaded by the plain method; tiles whose probe misses by
- * more than `tolerance` (relative, L1 over RGB) or that leave the trained
- * range are re-shaded by the plain method. `fallback_tiles` (may be NULL,
- * then the call is asynchronous) receives their count. */
+/* Network shading of the frame's G-buffer into HM_BUF_SHADED. The covered
+ * pixels are shaded in tiles of 64 consecutive covered pixels (one wave
+ * each). Tiles with a pixel whose features leave the trained range are shaded
+ * by the plain method. With `tolerance` > 0, every 4th tile also shades one
+ * probe pixel by the plain method and is re-shaded by it when the network
+ * colour misses by more than `tolerance` (relative, L1 over RGB).
+ * `fallback_tiles` (may be NULL, then the call is asynchronous) receives the
+ * count of tiles the plain method shaded. */
 hm_status hm_shade_mlp(hm_ctx* ctx, const hm_params* params, float exposure, float tolerance,
                        uint32_t* fallback_tiles);
 

@@ -1,7 +1,7 @@
 """THE ORACLE for the learned shading term (test infrastructure).
 
 A float64 restatement of the surrogate network the HIP kernels train and run:
-  x (32: 29 features, zero-padded) -> 64 -> 64 -> 64 (ReLU) -> 16 (12 used).
+  x (32 features) -> 64 -> 64 -> 64 (ReLU) -> 16 outputs.
 Weights are stored out x in (row-major), biases per output. With
 `bf16_operands=True` the operands are rounded to bfloat16 (round to nearest
 even) exactly where the kernels feed the matrix cores - weights, inputs and
@@ -9,7 +9,7 @@ every hidden activation, and the backward pass's output errors - while
 accumulation stays in float64, which
 isolates the kernels' float32 accumulation error.
 
-The loss is the mean over samples and the 12 used outputs of
+The loss is the mean over samples and the N_OUT used outputs of
 ((y - t) * s)^2, with per-output scales s.
 """
 from __future__ import annotations
@@ -17,8 +17,8 @@ from __future__ import annotations
 import numpy as np
 
 DIMS = (32, 64, 64, 64, 16)
-N_OUT = 12
-N_IN = 29
+N_OUT = 16
+N_IN = 32  # features used (inputs past N_IN are zero-padded)
 
 
 def bf16(x: np.ndarray) -> np.ndarray:
@@ -37,9 +37,9 @@ def init(seed: int = 0):
         fan_in = DIMS[i] if i else N_IN
         lim = np.sqrt(6.0 / fan_in)
         w = rng.uniform(-lim, lim, (DIMS[i + 1], DIMS[i]))
-        if i == 0:
+        if i == 0 and N_IN < DIMS[0]:
             w[:, N_IN:] = 0.0
-        if i == 3:
+        if i == 3 and N_OUT < DIMS[4]:
             w[N_OUT:, :] = 0.0
         ws.append(w.astype(np.float32))
         bs.append(np.zeros(DIMS[i + 1], np.float32))
