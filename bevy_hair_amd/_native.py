@@ -104,6 +104,11 @@ MLP_NW, MLP_NBIAS = 11264, 208
 MLP_NOUT = 16  # network outputs (the dual-scattering terms)
 
 
+class HmBatchView(C.Structure):
+    _fields_ = [("groom", C.c_void_p), ("params", HmParams * 3), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("dom_size", C.c_uint32), ("opts", HmFrameOpts)]
+
+
 class HmTuning(C.Structure):
     _fields_ = [("raster_flags", u32 * 3), ("raster_blocks_per_cu", u32 * 3), ("dom_tile", u32), ("cam_tile", u32),
                 ("bin_box_limit", u32), ("bin_records", u32), ("bin_blocks_per_cu", u32),
@@ -189,6 +194,11 @@ ABI = {
     "hm_mlp_set_range": (C.c_int, [P, P, P]),
     "hm_mlp_train": (C.c_int, [P, u32, u32, f32, P, u32, C.POINTER(f32)]),
     "hm_shade_mlp": (C.c_int, [P, C.POINTER(HmParams), f32, f32, C.POINTER(u32)]),
+    "hm_batch_create": (C.c_int, [P, C.POINTER(HmBatchView), u32, C.POINTER(C.c_void_p)]),
+    "hm_batch_run": (C.c_int, [P, P, u32]),
+    "hm_batch_time": (C.c_int, [P, P, u32, C.POINTER(f32)]),
+    "hm_batch_read": (C.c_int, [P, P, u32, P, P]),
+    "hm_batch_free": (None, [P, P]),
 }
 
 _lib = None

@@ -1,0 +1,163 @@
+// Batched multi-view frames: a fixed list of views rendered back to back on
+// the context's stream and replayed from one captured HIP graph. The views'
+// buffers are grown by one eager pass before the capture, so the captured
+// sequence holds kernel launches only (no allocation, no synchronization, no
+// memset nodes: every clear is a fill kernel).
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hm_ctx.h"
+#include "hm_kernels.h"
+
+struct hm_batch {
+    std::vector<hm_batch_view> views;
+    std::vector<void*> out;  // per view: rgba float4[px], then depth float[px]
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    // the context's buffers the captured launches read: their allocation
+    // generation, and the light counts baked into the launches
+    uint64_t alloc_gen = 0;
+    uint32_t point_count = 0, dir_count = 0;
+};
+
+namespace {
+
+#define BATCH_HIP(ctx, call)                                            \
+    do {                                                                \
+        hipError_t e_ = (call);                                         \
+        if (e_ != hipSuccess) return hm_hip_fail((ctx), e_, #call);     \
+    } while (0)
+
+void release(hm_batch* b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (hipEvent_t& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (void* p : b->out)
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+// Every view once: targets, frame, copy of the final colour and depth.
+hm_status record(hm_ctx* ctx, hm_batch* b) {
+    for (size_t i = 0; i < b->views.size(); ++i) {
+        const hm_batch_view& v = b->views[i];
+        hm_status st = hm_targets(ctx, v.width, v.height, v.dom_size);
+        if (st != HM_OK) return st;
+        if ((st = hm_frame(ctx, v.groom, v.params, &v.opts)) != HM_OK) return st;
+        const size_t px = (size_t)v.width * v.height;
+        const bool filtered = v.params[0].flags[1] != 0u;
+        const void* rgba = filtered ? (const void*)ctx->filtered : (const void*)ctx->shaded;
+        const void* depth = filtered ? (const void*)ctx->filtered_depth : (const void*)ctx->shaded_depth;
+        char* dst = static_cast<char*>(b->out[i]);
+        BATCH_HIP(ctx, hm::launch_copy32(ctx->stream, rgba, dst, px * 4, ctx->cu_count * 8));
+        BATCH_HIP(ctx, hm::launch_copy32(ctx->stream, depth, dst + px * 16, px, ctx->cu_count * 8));
+    }
+    return HM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hm_status hm_batch_create(hm_ctx* ctx, const hm_batch_view* views, uint32_t count, hm_batch** out) {
+    if (!ctx || !views || !out || count == 0) return hm_fail(ctx, HM_ERR_INVALID, "null argument or empty batch");
+    *out = nullptr;
+    for (uint32_t i = 0; i < count; ++i) {
+        const hm_batch_view& v = views[i];
+        if (!v.groom || v.width == 0 || v.height == 0)
+            return hm_fail(ctx, HM_ERR_INVALID, "batch view " + std::to_string(i) + ": no groom or zero size");
+        if ((uint32_t)v.params[0].screen[0] != v.width || (uint32_t)v.params[0].screen[1] != v.height ||
+            (uint32_t)v.params[0].dom_info[0] != v.dom_size)
+            return hm_fail(ctx, HM_ERR_INVALID, "batch view " + std::to_string(i) + ": params differ from its size");
+    }
+    (void)hipSetDevice(ctx->device);
+    hm_batch* b = new hm_batch();
+    b->views.assign(views, views + count);
+    b->out.assign(count, nullptr);
+    hipError_t e = hipSuccess;
+    for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+        e = hipMalloc(&b->out[i], (size_t)views[i].width * views[i].height * 20);
+    for (hipEvent_t& ev : b->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+        release(b);
+        return hm_hip_fail(ctx, e, "hm_batch_create");
+    }
+    // Per-kernel timing records events on the stream: keep it out of the graph.
+    const uint32_t slots = ctx->slots;
+    ctx->slots = 0;
+    hm_status st = record(ctx, b);  // eager pass: grows every buffer the views need
+    if (st == HM_OK) {
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = hm_hip_fail(ctx, e, "hm_batch_create (eager pass)");
+    }
+    if (st == HM_OK) {
+        e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            st = hm_hip_fail(ctx, e, "hipStreamBeginCapture");
+        } else {
+            st = record(ctx, b);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(ctx->stream, &g);
+            b->graph = g;
+            if (st == HM_OK && e != hipSuccess) st = hm_hip_fail(ctx, e, "hipStreamEndCapture");
+        }
+    }
+    if (st == HM_OK) {
+        e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) st = hm_hip_fail(ctx, e, "hipGraphInstantiate");
+    }
+    ctx->slots = slots;
+    if (st != HM_OK) {
+        release(b);
+        return st;
+    }
+    b->alloc_gen = ctx->alloc_gen;
+    b->point_count = ctx->point_count;
+    b->dir_count = ctx->dir_count;
+    *out = b;
+    return HM_OK;
+}
+
+hm_status hm_batch_run(hm_ctx* ctx, hm_batch* b, uint32_t reps) {
+    if (!ctx || !b) return hm_fail(ctx, HM_ERR_INVALID, "null argument");
+    if (b->alloc_gen != ctx->alloc_gen)
+        return hm_fail(ctx, HM_ERR_NOT_READY, "context buffers were reallocated since the batch was created; create it again");
+    if (b->point_count != ctx->point_count || b->dir_count != ctx->dir_count)
+        return hm_fail(ctx, HM_ERR_NOT_READY, "the light counts changed since the batch was created; create it again");
+    for (uint32_t r = 0; r < reps; ++r) BATCH_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    return HM_OK;
+}
+
+hm_status hm_batch_time(hm_ctx* ctx, hm_batch* b, uint32_t reps, float* total_ms) {
+    if (!ctx || !b || !total_ms) return hm_fail(ctx, HM_ERR_INVALID, "null argument");
+    BATCH_HIP(ctx, hipEventRecord(b->ev[0], ctx->stream));
+    hm_status st = hm_batch_run(ctx, b, reps);
+    if (st != HM_OK) return st;
+    BATCH_HIP(ctx, hipEventRecord(b->ev[1], ctx->stream));
+    BATCH_HIP(ctx, hipEventSynchronize(b->ev[1]));
+    BATCH_HIP(ctx, hipEventElapsedTime(total_ms, b->ev[0], b->ev[1]));
+    return HM_OK;
+}
+
+hm_status hm_batch_read(hm_ctx* ctx, hm_batch* b, uint32_t view, float* rgba, float* depth) {
+    if (!ctx || !b || view >= b->views.size()) return hm_fail(ctx, HM_ERR_INVALID, "bad batch view");
+    const size_t px = (size_t)b->views[view].width * b->views[view].height;
+    const char* src = static_cast<const char*>(b->out[view]);
+    if (rgba) BATCH_HIP(ctx, hipMemcpyAsync(rgba, src, px * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (depth) BATCH_HIP(ctx, hipMemcpyAsync(depth, src + px * 16, px * 4, hipMemcpyDeviceToHost, ctx->stream));
+    BATCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HM_OK;
+}
+
+void hm_batch_free(hm_ctx* ctx, hm_batch* b) {
+    if (!b) return;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    release(b);
+}
+
+}  // extern "C"

@@ -49,6 +49,10 @@ struct hm_ctx {
     int cu_count = 256;
     hipStream_t stream = nullptr;
     uint32_t width = 0, height = 0, dom_size = 0;
+    // Target allocations only grow: a smaller size reuses them (the layouts
+    // below follow the current size), so views of several sizes can follow
+    // one another without allocating.
+    size_t cap_gb_words = 0, cap_px = 0, cap_dom_words = 0;
     void* gb_slab = nullptr;  // center | conservative | beta, cleared by one fill
     size_t gb_words = 0;
     uint64_t* center = nullptr;
@@ -61,7 +65,7 @@ struct hm_ctx {
     uint32_t* dom = nullptr;
     uint32_t* pix_list = nullptr;   // covered-pixel list (shade, then filter)
     uint32_t* pix_list2 = nullptr;  // filter list when the camera tiles write both lists
-    size_t list_len = 0;            // entries in pix_list / pix_list2
+    size_t list_len = 0;            // entries in pix_list / pix_list2 (capacity)
     uint32_t* pix_count = nullptr;  // sub-list counters: shade, then filter
     float* scene_depth = nullptr;
     float scene_clear = 0.0f;
@@ -78,6 +82,9 @@ struct hm_ctx {
     std::vector<uint8_t> ran;
     hipEvent_t frame_ev[2] = {nullptr, nullptr};
     hm_mlp_state* mlp = nullptr;
+    // Bumped whenever a context-owned device buffer is (re)allocated or freed:
+    // a captured batch graph holds their addresses.
+    uint64_t alloc_gen = 0;
     std::string err;
 };
 

@@ -280,6 +280,8 @@ void free_targets(hm_ctx* c) {
     dev_free(c->pix_count);
     dev_free(c->scene_depth);
     c->width = c->height = c->dom_size = 0;
+    c->cap_gb_words = c->cap_px = c->cap_dom_words = 0;
+    c->list_len = 0;
 }
 
 void free_bin(hm_ctx* c) {
@@ -311,6 +313,7 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
         return hipSuccess;
     hipError_t e = hipStreamSynchronize(c->stream);  // earlier frames may still read them
     if (e != hipSuccess) return e;
+    c->alloc_gen++;
     if (slots > b.slots) {
         dev_free(b.cps);
         dev_free(b.cpy);
@@ -557,30 +560,47 @@ hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_
     if ((uint64_t)width * height > (1ull << 31)) return fail(ctx, HM_ERR_CAPACITY, "target larger than 2^31 pixels");
     if (ctx->gb_slab && ctx->width == width && ctx->height == height && ctx->dom_size == dom_size) return HM_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    float keep_clear = ctx->scene_clear;
-    free_targets(ctx);
-    size_t px = (size_t)width * height;
-    size_t dom_words = (size_t)dom_size * dom_size * (1 + HM_DOM_LAYERS);
+    const size_t px = (size_t)width * height;
+    const size_t dom_words = (size_t)dom_size * dom_size * (1 + HM_DOM_LAYERS);
     // center (8 B) | conservative (8 B) | beta (4 B, padded to 16 B)
-    size_t beta_words = (px + 3) & ~(size_t)3;
-    size_t gb_words = px * 2 + px * 2 + beta_words;
-    hipError_t e = dev_alloc(&ctx->gb_slab, gb_words * 4);
-    if (e == hipSuccess) e = dev_alloc(&ctx->shaded, px * 16);
-    if (e == hipSuccess) e = dev_alloc(&ctx->shaded_depth, px * 4);
-    if (e == hipSuccess) e = dev_alloc(&ctx->filtered, px * 16);
-    if (e == hipSuccess) e = dev_alloc(&ctx->filtered_depth, px * 4);
-    if (e == hipSuccess) e = dev_alloc(&ctx->dom, dom_words * 4);
+    const size_t beta_words = (px + 3) & ~(size_t)3;
+    const size_t gb_words = px * 2 + px * 2 + beta_words;
     // Covered-pixel sub-lists: room for every pixel of every (partial) tile
     // up to 32 x 32 plus one tile's slack per sub-list.
-    ctx->list_len = ((size_t)width + 32u) * ((size_t)height + 32u) + (size_t)hm::LIST_SUBS * 1024u;
-    if (e == hipSuccess) e = dev_alloc(&ctx->pix_list, ctx->list_len * 4);
-    if (e == hipSuccess) e = dev_alloc(&ctx->pix_list2, ctx->list_len * 4);
-    if (e == hipSuccess) e = dev_alloc(&ctx->pix_count, 2 * hm::LIST_SUBS * 4);
-    if (e == hipSuccess) e = hipMemset(ctx->dom, 0, dom_words * 4);
-    if (e != hipSuccess) {
+    const size_t list_len = ((size_t)width + 32u) * ((size_t)height + 32u) + (size_t)hm::LIST_SUBS * 1024u;
+    const bool fits = ctx->gb_slab && gb_words <= ctx->cap_gb_words && px <= ctx->cap_px &&
+                      dom_words <= ctx->cap_dom_words && list_len <= ctx->list_len;
+    if (!fits) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->alloc_gen++;
+        float keep_clear = ctx->scene_clear;
+        const size_t c_gb = std::max(gb_words, ctx->cap_gb_words), c_px = std::max(px, ctx->cap_px);
+        const size_t c_dom = std::max(dom_words, ctx->cap_dom_words), c_list = std::max(list_len, ctx->list_len);
         free_targets(ctx);
-        return hip_fail(ctx, e, "hm_targets");
+        hipError_t e = dev_alloc(&ctx->gb_slab, c_gb * 4);
+        if (e == hipSuccess) e = dev_alloc(&ctx->shaded, c_px * 16);
+        if (e == hipSuccess) e = dev_alloc(&ctx->shaded_depth, c_px * 4);
+        if (e == hipSuccess) e = dev_alloc(&ctx->filtered, c_px * 16);
+        if (e == hipSuccess) e = dev_alloc(&ctx->filtered_depth, c_px * 4);
+        if (e == hipSuccess) e = dev_alloc(&ctx->dom, c_dom * 4);
+        if (e == hipSuccess) e = dev_alloc(&ctx->pix_list, c_list * 4);
+        if (e == hipSuccess) e = dev_alloc(&ctx->pix_list2, c_list * 4);
+        if (e == hipSuccess) e = dev_alloc(&ctx->pix_count, 2 * hm::LIST_SUBS * 4);
+        if (e == hipSuccess) e = hipMemset(ctx->dom, 0, c_dom * 4);
+        if (e != hipSuccess) {
+            free_targets(ctx);
+            return hip_fail(ctx, e, "hm_targets");
+        }
+        ctx->cap_gb_words = c_gb;
+        ctx->cap_px = c_px;
+        ctx->cap_dom_words = c_dom;
+        ctx->list_len = c_list;
+        ctx->scene_clear = keep_clear;
+    } else if (ctx->scene_depth && (width != ctx->width || height != ctx->height)) {
+        // the scene depth belongs to the previous size
+        (void)hipStreamSynchronize(ctx->stream);
+        dev_free(ctx->scene_depth);
+        ctx->alloc_gen++;
     }
     uint32_t* slab = static_cast<uint32_t*>(ctx->gb_slab);
     ctx->center = reinterpret_cast<uint64_t*>(slab);
@@ -590,7 +610,6 @@ hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_
     ctx->width = width;
     ctx->height = height;
     ctx->dom_size = dom_size;
-    ctx->scene_clear = keep_clear;
     return HM_OK;
 }
 
@@ -600,13 +619,17 @@ hm_status hm_set_scene_depth(hm_ctx* ctx, const float* host_depth, float clear_v
     ctx->scene_clear = clear_value;
     if (!host_depth) {
         (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->scene_depth) ctx->alloc_gen++;
         dev_free(ctx->scene_depth);
         return HM_OK;
     }
     hm_status st = check_targets(ctx);
     if (st != HM_OK) return st;
     size_t bytes = (size_t)ctx->width * ctx->height * 4;
-    if (!ctx->scene_depth) HM_HIP(ctx, dev_alloc(&ctx->scene_depth, bytes));
+    if (!ctx->scene_depth) {
+        ctx->alloc_gen++;
+        HM_HIP(ctx, dev_alloc(&ctx->scene_depth, bytes));
+    }
     HM_HIP(ctx, hipMemcpyAsync(ctx->scene_depth, host_depth, bytes, hipMemcpyHostToDevice, ctx->stream));
     HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return HM_OK;
@@ -620,12 +643,14 @@ hm_status hm_set_lights(hm_ctx* ctx, const hm_point_light* points, uint32_t np, 
     if (np > ctx->point_cap) {
         dev_free(ctx->points);
         ctx->point_cap = 0;
+        ctx->alloc_gen++;
         HM_HIP(ctx, dev_alloc(&ctx->points, (size_t)np * sizeof(hm_point_light)));
         ctx->point_cap = np;
     }
     if (nd > ctx->dir_cap) {
         dev_free(ctx->dirs);
         ctx->dir_cap = 0;
+        ctx->alloc_gen++;
         HM_HIP(ctx, dev_alloc(&ctx->dirs, (size_t)nd * sizeof(hm_directional_light)));
         ctx->dir_cap = nd;
     }

@@ -141,6 +141,8 @@ hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks);
 hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int tile_blocks, int long_blocks);
 
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes, int blocks);
+// Copies `words` 32-bit words (a graph-safe device copy).
+hipError_t launch_copy32(hipStream_t s, const void* src, void* dst, size_t words, int max_blocks);
 hipError_t launch_atomics(hipStream_t s, void* buf, size_t words, uint32_t threads, uint32_t per_thread, uint32_t kind);
 hipError_t launch_fill32(hipStream_t s, uint32_t* dst, size_t words, uint32_t value, int max_blocks,
                          const uint32_t* gate = nullptr);

@@ -43,6 +43,12 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+__global__ __launch_bounds__(256) void k_copy32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                size_t n) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // Random-address atomics over `words` targets: kind 0 u64 min, 1 u32 min, 2 u32 add.
 __global__ __launch_bounds__(256) void k_atomics(void* __restrict__ buf, size_t words, uint32_t per_thread,
                                                  uint32_t kind, uint32_t seed) {
@@ -410,6 +416,14 @@ hipError_t launch_fill32(hipStream_t s, uint32_t* dst, size_t words, uint32_t va
 
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes, int blocks) {
     hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, bytes / 16);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy32(hipStream_t s, const void* src, void* dst, size_t words, int max_blocks) {
+    if (words == 0) return hipSuccess;
+    size_t blocks = (words + 255) / 256;
+    if (blocks > (size_t)max_blocks) blocks = (size_t)max_blocks;
+    hipLaunchKernelGGL(k_copy32, dim3((unsigned)blocks), dim3(256), 0, s, (const uint32_t*)src, (uint32_t*)dst, words);
     return hipGetLastError();
 }
 

@@ -61,6 +61,7 @@ hm_status state(hm_ctx* ctx, hm_mlp_state** out) {
     if (!ctx) return HM_ERR_INVALID;
     if (!ctx->mlp) {
         (void)hipSetDevice(ctx->device);
+        ctx->alloc_gen++;
         hm_mlp_state* s = new hm_mlp_state();
         hipError_t e = alloc(&s->w, MLP_NW * 4);
         if (e == hipSuccess) e = alloc(&s->b, MLP_NBIAS * 4);
@@ -405,6 +406,7 @@ hm_status hm_shade_mlp_impl(hm_ctx* ctx, const hm_params* p, float exposure, flo
     // At most LIST_SUBS * cap listed pixels: at most cap groups of 64.
     const uint32_t groups_cap = L.cap;
     if (groups_cap > s->flags_cap) {
+        ctx->alloc_gen++;
         release(s->flags);
         s->flags_cap = 0;
         MLP_HIP(ctx, alloc(&s->flags, (size_t)groups_cap * 4));

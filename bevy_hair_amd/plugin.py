@@ -103,6 +103,34 @@ class GpuGroom:
             self.handle = None
 
 
+class Batch:
+    """A captured multi-view frame owned by a Renderer (hm_batch_*)."""
+
+    def __init__(self, renderer: "Renderer", handle, sizes):
+        self.renderer, self.handle, self.sizes = renderer, handle, sizes
+
+    def run(self, reps: int = 1):
+        self.renderer._check(N.lib().hm_batch_run(self.renderer.ctx, self.handle, reps), "hm_batch_run")
+
+    def time(self, reps: int) -> float:
+        ms = C.c_float()
+        self.renderer._check(N.lib().hm_batch_time(self.renderer.ctx, self.handle, reps, C.byref(ms)), "hm_batch_time")
+        return float(ms.value)
+
+    def read(self, view: int):
+        w, h = self.sizes[view]
+        rgba = np.empty((w * h, 4), np.float32)
+        depth = np.empty(w * h, np.float32)
+        self.renderer._check(N.lib().hm_batch_read(self.renderer.ctx, self.handle, view, N.ptr(rgba), N.ptr(depth)),
+                             "hm_batch_read")
+        return rgba, depth
+
+    def free(self):
+        if self.handle:
+            N.lib().hm_batch_free(self.renderer.ctx, self.handle)
+            self.handle = None
+
+
 class Renderer:
     """One device context (hm_ctx): targets, lights, grooms, frames."""
 
@@ -220,6 +248,20 @@ class Renderer:
     def hair_pass(self, g: GpuGroom, params, opts: N.HmFrameOpts | None = None):
         opts = opts or self.frame_opts()
         self._check(N.lib().hm_frame(self.ctx, g.handle, params, C.byref(opts)), "hm_frame")
+
+    def batch(self, views) -> "Batch":
+        """A captured multi-view frame (hm_batch_create): `views` are
+        (GpuGroom, params[3], width, height, dom_size, HmFrameOpts) tuples."""
+        arr = (N.HmBatchView * len(views))()
+        for i, (g, params, w, h, dom, opts) in enumerate(views):
+            arr[i].groom = g.handle
+            for k in range(3):
+                arr[i].params[k] = params[k]
+            arr[i].width, arr[i].height, arr[i].dom_size = w, h, dom
+            arr[i].opts = opts
+        out = C.c_void_p()
+        self._check(N.lib().hm_batch_create(self.ctx, arr, len(views), C.byref(out)), "hm_batch_create")
+        return Batch(self, out.value, [(w, h) for (_, _, w, h, _, _) in views])
 
     def time_frames(self, g: GpuGroom, params, opts, frames: int) -> float:
         ms = C.c_float()

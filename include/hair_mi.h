@@ -246,7 +246,10 @@ uint32_t hm_groom_bundle_count(const hm_groom* groom);
 uint32_t hm_groom_strand_count(const hm_groom* groom);
 
 /* Replaces ensure_targets (pass.rs:789-815) with a runtime DOM size (the
- * reference hard-codes 512, pass.rs:22). Reallocates only when sizes change. */
+ * reference hard-codes 512, pass.rs:22). Allocates only when a size exceeds
+ * what was allocated before: the allocations only grow, so target sizes can
+ * alternate without allocating or synchronizing. A change of the target size
+ * drops the scene depth (hm_set_scene_depth). */
 hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_size);
 
 /* The scene depth texture raster tests against (raster.wgsl:93). NULL means
@@ -437,6 +440,36 @@ hm_status hm_mlp_train(hm_ctx* ctx, uint32_t epochs, uint32_t batch, float learn
  * count of tiles the plain method shaded. */
 hm_status hm_shade_mlp(hm_ctx* ctx, const hm_params* params, float exposure, float tolerance,
                        uint32_t* fallback_tiles);
+
+/* ---- batched multi-view frames (no counterpart in the reference) ----------
+ * A batch renders a fixed list of views - groom, its three HairParams, target
+ * and DOM size, frame options - back to back on the context's stream, each
+ * view's final colour (filtered when the groom's filter flag is set, else
+ * shaded) and depth copied into the batch's own output slot. hm_batch_create
+ * runs the views once eagerly (which grows every buffer to what the views
+ * need), then captures the whole sequence into one HIP graph; hm_batch_run
+ * replays it with one launch. The graph holds the addresses of the context's
+ * buffers and the light counts: hm_batch_run fails when a buffer was
+ * reallocated since (a larger target, more lights, a scene depth...) or the
+ * light counts changed - create the batch again. Light values and network
+ * weights are read at each replay. The grooms must outlive the batch. Calls
+ * that overlap a running batch on the same context race. */
+typedef struct hm_batch hm_batch;
+typedef struct hm_batch_view {
+    hm_groom* groom;
+    hm_params params[3];            /* camera, DOM depth, DOM opacity (hm_make_params) */
+    uint32_t width, height, dom_size;
+    hm_frame_opts opts;
+} hm_batch_view;
+hm_status hm_batch_create(hm_ctx* ctx, const hm_batch_view* views, uint32_t count, hm_batch** out);
+/* Enqueues `reps` replays of the captured frame (asynchronous). */
+hm_status hm_batch_run(hm_ctx* ctx, hm_batch* batch, uint32_t reps);
+/* Wall time of `reps` replays in milliseconds (synchronous). */
+hm_status hm_batch_time(hm_ctx* ctx, hm_batch* batch, uint32_t reps, float* total_ms);
+/* One view's output: rgba float[width*height*4], depth float[width*height]
+ * (either may be NULL); synchronizes the stream. */
+hm_status hm_batch_read(hm_ctx* ctx, hm_batch* batch, uint32_t view, float* rgba, float* depth);
+void hm_batch_free(hm_ctx* ctx, hm_batch* batch);
 
 #ifdef __cplusplus
 }  /* extern "C" */
