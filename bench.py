@@ -3,7 +3,11 @@
 A step is one full hair_pass of one groom over resident inputs: clear, camera
 LOD/scan/raster, DOM clear/LOD/scan/raster x2, shade, filter. The default
 workload is the 1080p, 100k-strand x 64-segment wavy groom with a 2048^2 x 16
-deep opacity map and 1 directional + 32 point lights.
+deep opacity map and 1 directional + 32 point lights. `value` is the plain
+method's frames/s; the "mlp" entry times the same frames with the learned
+dual-scattering term and gives its error against the plain method, and the
+"batch" entry the views/s of a 24-view, three-groom batch replayed from one
+captured HIP graph.
 
 Prints one JSON line (rank 0). Per-kernel HIP events are recorded on the
 library's own stream across the timed frames; the dominant kernel's mean
@@ -112,6 +116,68 @@ def cpu_baseline(sc, w, threads: int, fraction: int) -> dict:
             "seconds_per_frame": {k: round(v, 3) for k, v in t.items()}}
 
 
+def rel_err(got: np.ndarray, ref: np.ndarray, floor: float = 1e-3) -> float:
+    """Mean over covered pixels of |got - ref|_1 / max(|ref|_1, floor) over RGB."""
+    cov = ref[:, 3] > 0
+    d = np.abs(got[cov, :3] - ref[cov, :3]).sum(1)
+    return float((d / np.maximum(np.abs(ref[cov, :3]).sum(1), floor)).mean())
+
+
+def mlp_leg_run(r, g, sc, w, args) -> dict:
+    """The same frames with the learned dual-scattering term (committed
+    weights): frames/s, shade time against the plain shade, and the mean
+    relative RGB error of the shaded image against the plain method's."""
+    from bevy_hair_amd import mlp_train
+
+    info = mlp_train.install(r)
+    mopts = r.frame_opts(control_points=w.control_points, strand_fraction=w.strand_fraction, mlp=True,
+                         mlp_tolerance=args.mlp_tolerance)
+    popts = r.frame_opts(control_points=w.control_points, strand_fraction=w.strand_fraction)
+    for _ in range(args.warmup):
+        r.hair_pass(g, sc.params, mopts)
+    r.timing(args.steps)
+    r.sync()
+    ms = r.time_frames(g, sc.params, mopts, args.steps) / args.steps
+    kt = r.kernel_times()
+    r.timing(0)
+    r.hair_pass(g, sc.params, popts)
+    r.shade(sc.params[0])
+    plain = r.shaded()[0].copy()
+    fallback = r.shade_mlp(sc.params[0], 1.0, args.mlp_tolerance)
+    err = rel_err(r.shaded()[0], plain)
+    return {"frames_per_s": round(1000.0 / ms, 2), "ms_per_step": round(ms, 4), "shade_ms": round(kt["shade"], 5),
+            "mean_rel_rgb_err": round(err, 5), "fallback_tiles": fallback, "tolerance": args.mlp_tolerance,
+            "dtype": "bf16 MFMA, f32 accumulate", "training_samples": int(info.get("samples", 0))}
+
+
+def batch_leg_run(r, args) -> dict:
+    """24 views over three grooms (20k x 16, 100k x 64, 150k x 126) at
+    1280x720 / 1920x1080 / 2560x1440, learned shading with the trained-range
+    fallback, replayed from one captured HIP graph: views/s."""
+    from bevy_hair_amd import mlp_train
+    from bevy_hair_amd import workload as W
+
+    bs = W.make_batch_scene()
+    mlp_train.install(r)
+    r.set_lights(bs.points, bs.dirs)
+    gpu = [r.upload(m) for m in bs.meshes]
+    views = [(gpu[gi], params, wd, ht, W.BATCH_DOM,
+              r.frame_opts(control_points=ncp, strand_fraction=1.0, mlp=True, mlp_tolerance=args.mlp_tolerance))
+             for gi, wd, ht, _, params, ncp in bs.views]
+    b = r.batch(views)
+    try:
+        b.run(2)
+        reps = max(3, min(20, args.steps // 5))
+        ms = b.time(reps) / reps
+    finally:
+        b.free()
+        for gg in gpu:
+            gg.free()
+    return {"views": len(views), "views_per_s": round(len(views) * 1000.0 / ms, 1), "ms_per_batch": round(ms, 3),
+            "replays": reps, "grooms": [f"{k} {n}x{ncp - 1}" for k, n, _, ncp in W.BATCH_GROOMS],
+            "sizes": [f"{a}x{b_}" for a, b_ in W.BATCH_SIZES], "dom_size": W.BATCH_DOM, "shading": "mlp"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +188,10 @@ def main() -> int:
     ap.add_argument("--cpu-fraction", type=int, default=2)
     ap.add_argument("--no-probes", action="store_true", help="skip the copy/atomic microbenchmarks")
     ap.add_argument("--no-tune", action="store_true", help="keep the default raster launch choices")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the learned-shading leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view leg")
+    ap.add_argument("--mlp-tolerance", type=float, default=0.0,
+                    help="per-tile probe bound of the learned shading (0: trained-range fallback only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this path runs on a single MI355X (asked for --gpus {args.gpus})",
@@ -187,6 +257,7 @@ def main() -> int:
     ms = ms_total / args.steps
     fps = 1000.0 / ms
 
+    mlp_leg = None if args.no_mlp else mlp_leg_run(r, g, sc, w, args)
     dom_k = max(kt, key=kt.get)
     alg = algorithmic_bytes(dom_k, w, sc.mesh, counts, hit)
     achieved = alg / (kt[dom_k] * 1e-3) / 1e9
@@ -223,6 +294,7 @@ def main() -> int:
             "dom_opacity_atomics_per_s": round(float(counts["dom2"][2]) / (kt["dom_opacity"] * 1e-3), 1)
             if kt["dom_opacity"] and not bin_status else None},
     }
+    result["mlp"] = mlp_leg
     if not args.no_probes:
         # Measured denominators: STREAM copy and random-address atomics over the
         # footprints the raster passes hit (G-buffer words, DOM fronts, DOM counts).
@@ -238,6 +310,7 @@ def main() -> int:
         if rates["dom_opacity_atomics_per_s"]:
             rates["dom_opacity_frac_of_u32_add_peak"] = round(rates["dom_opacity_atomics_per_s"] / peaks["u32_add_per_s"], 4)
     g.free()
+    result["batch"] = None if args.no_batch else batch_leg_run(r, args)
     r.close()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

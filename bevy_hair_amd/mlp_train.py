@@ -140,6 +140,8 @@ def install(r: plugin.Renderer, path: str = DEFAULT_WEIGHTS) -> dict:
     """Loads trained weights and their feature range into the renderer."""
     with np.load(path, allow_pickle=False) as z:
         d = {k: z[k] for k in z.files}
+    if d["scale"].shape != (N.MLP_NOUT,):
+        raise ValueError(f"{path}: weights for {d['scale'].shape[0]} outputs, the network has {N.MLP_NOUT}; retrain")
     r.mlp_set_weights(d["weights"], d["biases"])
     r.mlp_set_range(d["lo"], d["hi"])
     return d
