@@ -116,6 +116,36 @@ def cpu_baseline(sc, w, threads: int, fraction: int) -> dict:
             "seconds_per_frame": {k: round(v, 3) for k, v in t.items()}}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_traffic_pmc.txt")
+
+
+def measured_traffic(kernel: str, tuning: dict, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by
+    what the counter reads for a 1 GiB copy kernel); None when the summary
+    does not cover this kernel on this workload."""
+    names = {"cam_tiles": f"k_cam_tiles<{tuning.get('cam_tile')}>", "dom_tiles": f"k_dom_tiles<{tuning.get('dom_tile')}>",
+             "shade": "k_shade", "filter": "k_filter"}
+    if kernel not in names or not os.path.exists(TRAFFIC_FILE):
+        return None
+    rows, wl = {}, None
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) >= 2 and f[0] == "workload":
+            wl = f[1]
+        elif len(f) == 3 and not line.startswith("#"):
+            try:
+                rows[f[0]] = (float(f[1]), float(f[2]))
+            except ValueError:
+                pass
+    if wl != workload or "copy_1GiB" not in rows or names[kernel] not in rows:
+        return None
+    cf, cw = rows["copy_1GiB"]
+    sf, sw = 1048576.0 / cf, 1048576.0 / cw  # the copy reads and writes 1 GiB
+    fk, wk = rows[names[kernel]]
+    return round((fk * sf + wk * sw) * 1024.0)
+
+
 def rel_err(got: np.ndarray, ref: np.ndarray, floor: float = 1e-3) -> float:
     """Mean over covered pixels of |got - ref|_1 / max(|ref|_1, floor) over RGB."""
     cov = ref[:, 3] > 0
@@ -188,6 +218,8 @@ def main() -> int:
     ap.add_argument("--cpu-fraction", type=int, default=2)
     ap.add_argument("--no-probes", action="store_true", help="skip the copy/atomic microbenchmarks")
     ap.add_argument("--no-tune", action="store_true", help="keep the default raster launch choices")
+    ap.add_argument("--tuning", default=None,
+                    help="launch choices as JSON (hm_tuning fields) instead of autotuning, e.g. for profiling")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-shading leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view leg")
     ap.add_argument("--mlp-tolerance", type=float, default=0.0,
@@ -217,7 +249,11 @@ def main() -> int:
         r.hair_pass(g, sc.params, opts)
     r.sync()
     # Launch choices of the raster passes picked on this groom and view (untimed).
-    tuning = r.autotune(g, sc.params, opts, reps=3) if not args.no_tune else r.get_tuning()
+    if args.tuning:
+        r.set_tuning(**json.loads(args.tuning))
+        tuning = r.get_tuning()
+    else:
+        tuning = r.autotune(g, sc.params, opts, reps=3) if not args.no_tune else r.get_tuning()
     r.hair_pass(g, sc.params, opts)
     r.sync()
     # Work counts of each raster pass (untimed counting launches).
@@ -281,7 +317,8 @@ def main() -> int:
                    "directional_lights": len(sc.dirs), "shading": "plain", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "algorithmic_bytes": alg, "mean_ms": round(kt[dom_k], 5), "traffic": None},
+                     "algorithmic_bytes": alg, "mean_ms": round(kt[dom_k], 5),
+                     "traffic": measured_traffic(dom_k, tuning, w.name)},
         "kernels_ms": {k: round(v, 5) for k, v in kt.items()},
         "tuning": tuning,
         "work": {"camera_writes": int(counts["cam"][0]), "camera_center_hits": int(counts["cam"][1]),
