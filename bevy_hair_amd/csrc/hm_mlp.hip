@@ -552,13 +552,15 @@ __device__ DualScatter dual_scatter_by_wave(const PixelShade& px, int lane) {
 // Network shading, one wave per group of 64 consecutive covered pixels, one
 // lane per pixel. Per group: features -> LDS (bf16) -> MFMA forward -> LDS ->
 // dual-scattering terms -> colour (plain_pixel_color with the network's terms
-// in place of dual_scatter's). A group with a feature outside the trained range is shaded by the
-// plain method instead, decided before the network runs. When `tol` > 0,
-// every PROBE_STRIDE-th group also shades one probe pixel by the plain method
-// (all 64 lanes on that pixel) and is re-shaded by the plain method when the
-// network colour misses it by more than `tol`. The pixel set-up is redone
-// after the network (from the G-buffer word held in registers) so that its
-// 30 values are not held through it.
+// in place of dual_scatter's). A pixel with a feature outside the trained
+// range takes the plain method's terms instead (per pixel: its colour does not
+// depend on the order of the covered-pixel list). When `tol` > 0, every
+// PROBE_STRIDE-th range-clean group also shades one probe pixel by the plain
+// method (all 64 lanes on that pixel) and the group is re-shaded by the plain
+// method when the network colour misses it by more than `tol`; which pixels
+// share a group follows the list order, so that re-shading does not. The
+// pixel set-up is redone after the network (from the G-buffer word held in
+// registers) so that its 30 values are not held through it.
 constexpr uint32_t PROBE_STRIDE = 4;
 
 __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs m, PixelList L) {
@@ -617,9 +619,11 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
                 io[lane * 4 + c] = __builtin_bit_cast(float4, v);
             }
         }
+        // flag (per group, for the count): 1 when a pixel is outside the
+        // trained range; that pixel alone takes the plain method
         uint32_t flag = __any(outside) ? 1u : 0u;
         float o[MLP_NOUT] = {};
-        if (flag == 0u) {
+        if (!__all(outside || !valid)) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -670,8 +674,8 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
             px = pixel_setup(p, word, beta_word, pix % width, pix / width);
             // the network's dual-scattering terms, or the plain method's
             // (plain_pixel_color; `depth` is its key-light depth)
-            const DualScatter ds = flag == 0u ? network_dual_scatter(o, px.shape)
-                                              : dual_scatter(px.cos_theta_o, px.sigma_a, px.shape);
+            const DualScatter ds = (flag == 2u || outside) ? dual_scatter(px.cos_theta_o, px.sigma_a, px.shape)
+                                                           : network_dual_scatter(o, px.shape);
             const LocalBack lb = local_terms(ds);
             const F3 key_ms = key >= 0 ? key_multiple_scattering(key_theta_h(px, kd), depth, px.shape, ds) : splat3(0.0f);
             c = pixel_color(a, px, lb, key, depth, key_ms);

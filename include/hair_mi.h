@@ -432,12 +432,13 @@ hm_status hm_mlp_train(hm_ctx* ctx, uint32_t epochs, uint32_t batch, float learn
                        uint32_t seed, float* final_loss);
 /* Network shading of the frame's G-buffer into HM_BUF_SHADED. The covered
  * pixels are shaded in tiles of 64 consecutive covered pixels (one wave
- * each). Tiles with a pixel whose features leave the trained range are shaded
- * by the plain method. With `tolerance` > 0, every 4th tile also shades one
- * probe pixel by the plain method and is re-shaded by it when the network
- * colour misses by more than `tolerance` (relative, L1 over RGB).
- * `fallback_tiles` (may be NULL, then the call is asynchronous) receives the
- * count of tiles the plain method shaded. */
+ * each). A pixel whose features leave the trained range is shaded by the plain
+ * method. With `tolerance` > 0, every 4th tile free of such pixels also shades
+ * one probe pixel by the plain method and is re-shaded by it when the network
+ * colour misses by more than `tolerance` (relative, L1 over RGB); the tiles
+ * follow the list order, which varies between runs, so only tolerance 0 gives
+ * run-to-run identical images. `fallback_tiles` (may be NULL, then the call
+ * is asynchronous) receives the count of tiles with plain-shaded pixels. */
 hm_status hm_shade_mlp(hm_ctx* ctx, const hm_params* params, float exposure, float tolerance,
                        uint32_t* fallback_tiles);
 

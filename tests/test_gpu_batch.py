@@ -25,7 +25,7 @@ def _views(renderer, meshes, grooms, dirs, mlp):
         dom = 512 if v % 2 else 256
         params = plugin.Renderer.make_params(cam, grooms[g], meshes[g], dom, dirs)
         opts = renderer.frame_opts(control_points=GROOMS[g][3], strand_fraction=1.0, mlp=mlp and v % 3 != 1,
-                                   mlp_tolerance=0.05 if v == 4 else 0.0)
+                                   mlp_tolerance=0.0)
         views.append((gpu[g], params, w, h, dom, opts))
     return gpu, views
 
