@@ -1324,13 +1324,16 @@ hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const
             }
         }
     }
-    // Camera: direct kernels against binned tiles of either size, whole hm_gbuffer timed.
+    // Camera: direct kernels against binned tiles of either size with 128 to
+    // 1024 records (segments) per tile workgroup, whole hm_gbuffer timed.
     {
-        const uint32_t tile_sizes[3] = {0u, 16u, 32u};
+        const uint32_t variants[9][2] = {{0u, 0u},     {16u, 128u}, {16u, 256u}, {16u, 512u}, {16u, 1024u},
+                                         {32u, 128u}, {32u, 256u}, {32u, 512u}, {32u, 1024u}};
         float best_cam = 1e30f;
-        for (uint32_t ti = 0; ti < 3; ++ti) {
+        for (uint32_t ti = 0; ti < 9; ++ti) {
             ctx->tuning = out;
-            ctx->tuning.cam_tile = tile_sizes[ti];
+            ctx->tuning.cam_tile = variants[ti][0];
+            if (variants[ti][1]) ctx->tuning.cam_chunk = variants[ti][1];
             float total = 0.0f;
             for (uint32_t r = 0; r <= reps; ++r) {
                 HM_HIP(ctx, hipEventRecord(e0, ctx->stream));
@@ -1343,7 +1346,8 @@ hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const
             }
             if (total < best_cam) {
                 best_cam = total;
-                out.cam_tile = tile_sizes[ti];
+                out.cam_tile = ctx->tuning.cam_tile;
+                out.cam_chunk = ctx->tuning.cam_chunk;
             }
         }
     }
