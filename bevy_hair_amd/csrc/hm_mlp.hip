@@ -517,8 +517,8 @@ __device__ void color_pair_by_wave(const ShadeArgs& a, const PixelShade& px, con
         F3 single = single_scatter(g, px.sigma_a, px.shape);
         F3 w = point_color(L) * att;
         if (L.kind != HM_LIGHT_POINT) w = w * spot_mask(L, px.world);
-        cn = cn + (single + 0.7f * local_back(g.theta_h, net.a_back, net.delta_b, net.sigma_b2)) * w;
-        cr = cr + (single + 0.7f * local_back(g.theta_h, ref.a_back, ref.delta_b, ref.sigma_b2)) * w;
+        cn = cn + (single + 0.7f * local_back_r(g.theta_h, net.a_back, net.delta_b, net.inv_sb)) * w;
+        cr = cr + (single + 0.7f * local_back_r(g.theta_h, ref.a_back, ref.delta_b, ref.inv_sb)) * w;
     }
     for (uint32_t i = (uint32_t)lane; i < a.directional_count; i += 64u) {
         const hm_directional_light& L = a.directionals[i];
@@ -530,8 +530,8 @@ __device__ void color_pair_by_wave(const ShadeArgs& a, const PixelShade& px, con
             cn = cn + (single + net_ms) * w;
             cr = cr + (single + ref_ms) * w;
         } else {
-            cn = cn + (single + 0.7f * local_back(g.theta_h, net.a_back, net.delta_b, net.sigma_b2)) * w;
-            cr = cr + (single + 0.7f * local_back(g.theta_h, ref.a_back, ref.delta_b, ref.sigma_b2)) * w;
+            cn = cn + (single + 0.7f * local_back_r(g.theta_h, net.a_back, net.delta_b, net.inv_sb)) * w;
+            cr = cr + (single + 0.7f * local_back_r(g.theta_h, ref.a_back, ref.delta_b, ref.inv_sb)) * w;
         }
     }
     got = F3{wave_sum(cn.x), wave_sum(cn.y), wave_sum(cn.z)} * a.exposure;

@@ -46,19 +46,27 @@ __device__ __forceinline__ F3 fnormalize(F3 v) {  // safe_normalize with the har
 __device__ __forceinline__ F3 fdiv3(F3 a, F3 b) { return {fdiv(a.x, b.x), fdiv(a.y, b.y), fdiv(a.z, b.z)}; }
 
 struct Lobes { F3 r, tt, trt; };
-struct Shape { F3 alpha, beta; };
+// inv_beta: the reciprocal gaussian3 takes of the widths, formed once per pixel.
+struct Shape { F3 alpha, beta, inv_beta; };
 struct DualScatter { Lobes fwd; F3 a_f, beta_f2, a_back, delta_b, sigma_b2; };
 
 __device__ __forceinline__ F3 vexp(F3 a) { return {fexp(a.x), fexp(a.y), fexp(a.z)}; }
 __device__ __forceinline__ F3 vsqrt(F3 a) { return {fsqrt(a.x), fsqrt(a.y), fsqrt(a.z)}; }
 
-__device__ __forceinline__ F3 gaussian3(F3 b, F3 x) {
+// 1 / max(b, 1e-3) per channel: the reciprocal width of gaussian3.
+__device__ __forceinline__ F3 gaussian_rcp(F3 b) {
     F3 bb = vmax(b, splat3(1e-3f));
+    return {frcp(bb.x), frcp(bb.y), frcp(bb.z)};
+}
+
+// gaussian3 from the reciprocal width r = gaussian_rcp(b).
+__device__ __forceinline__ F3 gaussian3_r(F3 r, F3 x) {
     const float s2pi = 2.5066282746f;  // sqrt(2 pi)
-    F3 r = {frcp(bb.x), frcp(bb.y), frcp(bb.z)};
     F3 z = x * r;
     return vexp(-0.5f * z * z) * r * (1.0f / s2pi);
 }
+
+__device__ __forceinline__ F3 gaussian3(F3 b, F3 x) { return gaussian3_r(gaussian_rcp(b), x); }
 
 __device__ __forceinline__ float fresnel(float cos_t) {
     const float ior = 1.55f;
@@ -178,13 +186,17 @@ __device__ __forceinline__ BsdfGeom bsdf_geom_n(F3 t, F3 v, F3 light) {
 // Single scattering M_p * N_p summed over R, TT, TRT (shade.wesl:217-219).
 __device__ __forceinline__ F3 single_scatter(const BsdfGeom& g, F3 sigma_a, const Shape& shape) {
     Lobes n = azimuthal(g.cos_theta_d, g.cos_phi, sigma_a);
-    F3 m = gaussian3(shape.beta, splat3(g.theta_h) - shape.alpha);
+    F3 m = gaussian3_r(shape.inv_beta, splat3(g.theta_h) - shape.alpha);
     return m.x * n.r + m.y * n.tt + m.z * n.trt;
 }
 
-// Local backscattering lobe (shade.wesl:220).
+// Local backscattering lobe (shade.wesl:220); inv_sb = gaussian_rcp(sqrt(sigma_b2)).
+__device__ __forceinline__ F3 local_back_r(float theta_h, F3 a_back, F3 delta_b, F3 inv_sb) {
+    return 2.0f * a_back * gaussian3_r(inv_sb, splat3(theta_h) - delta_b) * (1.0f / kPi);
+}
+
 __device__ __forceinline__ F3 local_back(float theta_h, F3 a_back, F3 delta_b, F3 sigma_b2) {
-    return 2.0f * a_back * gaussian3(vsqrt(sigma_b2), splat3(theta_h) - delta_b) * (1.0f / kPi);
+    return local_back_r(theta_h, a_back, delta_b, gaussian_rcp(vsqrt(sigma_b2)));
 }
 
 // Global forward scattering through `depth` of hair (shade.wesl:225-242);
@@ -326,7 +338,9 @@ __device__ __forceinline__ PixelShade pixel_setup(const View& p, uint64_t word, 
     bool ortho = p.clip[3] == 0.0f && p.clip[7] == 0.0f && p.clip[11] == 0.0f && p.clip[15] == 1.0f;
     s.view = ortho ? safe_normalize(F3{p.clip[2], p.clip[6], p.clip[10]}) : safe_normalize(xyz(p.camera_pos) - s.world);
     float bb = clampf(s.roughness * s.roughness, 0.02f, 1.0f);
-    s.shape = {{tilt, -0.5f * tilt, -1.5f * tilt}, {bb, 0.5f * bb, 2.0f * bb}};
+    s.shape.alpha = {tilt, -0.5f * tilt, -1.5f * tilt};
+    s.shape.beta = {bb, 0.5f * bb, 2.0f * bb};
+    s.shape.inv_beta = gaussian_rcp(s.shape.beta);
     s.t_n = safe_normalize(s.tangent);
     s.v_n = safe_normalize(s.view);
     float tv = dot3(s.tangent, s.view);
@@ -354,10 +368,12 @@ __device__ __forceinline__ F3 dir_of(const hm_directional_light& L) {
 
 // The dual-scattering terms of the local back lobe that every light uses.
 struct LocalBack {
-    F3 a_back, delta_b, sigma_b2;
+    F3 a_back, delta_b, inv_sb;  // inv_sb = gaussian_rcp(sqrt(sigma_b^2)), once per pixel
 };
 
-__device__ __forceinline__ LocalBack local_terms(const DualScatter& ds) { return {ds.a_back, ds.delta_b, ds.sigma_b2}; }
+__device__ __forceinline__ LocalBack local_terms(const DualScatter& ds) {
+    return {ds.a_back, ds.delta_b, gaussian_rcp(vsqrt(ds.sigma_b2))};
+}
 
 // The key light's multiple scattering per unit light colour: hair_bsdf
 // (shade.wesl:193-244) minus its direct single scattering,
@@ -378,7 +394,7 @@ __device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelSha
     float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
     if (att == 0.0f) return splat3(0.0f);
     BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, safe_normalize(ltf));
-    F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back(g.theta_h, lb.a_back, lb.delta_b, lb.sigma_b2);
+    F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
     F3 c = lobe * point_color(L) * att;
     if (L.kind != HM_LIGHT_POINT) c = c * spot_mask(L, px.world);
     return c;
@@ -391,7 +407,7 @@ __device__ __forceinline__ F3 directional_term(const hm_directional_light& L, bo
     BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, dir_of(L));
     F3 single = single_scatter(g, px.sigma_a, px.shape);
     F3 lobe = is_key ? fexp(-depth) * single + key_ms
-                     : single + 0.7f * local_back(g.theta_h, lb.a_back, lb.delta_b, lb.sigma_b2);
+                     : single + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
     return lobe * F3{L.color[0], L.color[1], L.color[2]};
 }
 
