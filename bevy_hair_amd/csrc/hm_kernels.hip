@@ -334,6 +334,28 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
     const float inv_spar2 = frcp(spar2), inv_sperp2 = frcp(sperp2);
     const float inv_sigma_c2 = 1.0f / (0.9f * 0.9f);
     float p_norm = rdiv(u2f(depth_q), 16777214.0f);
+    // filter.wgsl's tap depth test |rdiv(q, 16777214) - p_norm| <= 1.45e-3 as
+    // an interval of depth codes: the quotient and the difference are
+    // monotone in q, and the pixel's own code passes. The bounds start from a
+    // float estimate and move to the exact edge with the test itself.
+    uint32_t q_lo, q_hi;
+    {
+        auto pass = [&](uint32_t q) { return !(fabsf(rdiv(u2f(q), 16777214.0f) - p_norm) > 1.45e-3f); };
+        q_lo = to_u32((p_norm - 1.45e-3f) * 16777214.0f);
+        q_lo = q_lo < depth_q ? q_lo : depth_q;
+        if (pass(q_lo)) {
+            while (q_lo > 0u && pass(q_lo - 1u)) --q_lo;
+        } else {
+            do { ++q_lo; } while (!pass(q_lo));  // ends at depth_q at the latest
+        }
+        q_hi = to_u32((p_norm + 1.45e-3f) * 16777214.0f);
+        q_hi = q_hi > depth_q ? (q_hi < 0xFFFFFEu ? q_hi : 0xFFFFFEu) : depth_q;
+        if (pass(q_hi)) {
+            while (q_hi < 0xFFFFFEu && pass(q_hi + 1u)) ++q_hi;
+        } else {
+            do { --q_hi; } while (!pass(q_hi));
+        }
+    }
     bool promote = !center_hit;
     bool use_color = !promote && cc.w > 0.0f;
     F3 accum = splat3(0.0f);
@@ -347,17 +369,20 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
             size_t qpix = (size_t)qy * (uint32_t)width + (uint32_t)qx;
             F2 d = {i2f(dx), i2f(dy)};
             float d_par = dot2(d, ts);
-            float d_perp = len2(d - ts * d_par);
+            F2 perp = d - ts * d_par;
+            // d_perp = sqrt(perp2); correctly rounded sqrt(x) <= 0.5 exactly when
+            // x <= 0x3E800001 (0.25 + 2^-25)
+            float perp2 = dot2(perp, perp);
             uint64_t q_center = a.center[qpix];
             bool qc_empty = gb_empty(q_center);
-            if (d_perp <= 0.5f) {
+            if (perp2 <= __uint_as_float(0x3E800001u)) {
                 if (!qc_empty) h_center = h_center + 1.0f;
                 else if (!gb_empty(a.cons[qpix])) h_cons = h_cons + 1.0f;
             }
             if (qc_empty) continue;
-            float q_norm = rdiv(u2f(gb_depth(q_center)), 16777214.0f);
-            if (fabsf(q_norm - p_norm) > 1.45e-3f) continue;
-            float spatial = fexp(-d_par * d_par * inv_spar2 - d_perp * d_perp * inv_sperp2);
+            const uint32_t qd = gb_depth(q_center);
+            if (qd < q_lo || qd > q_hi) continue;
+            float spatial = fexp(-d_par * d_par * inv_spar2 - perp2 * inv_sperp2);
             float4 qc = a.shaded[qpix];
             F3 q_color = {qc.x, qc.y, qc.z};
             float color_w = 1.0f;
