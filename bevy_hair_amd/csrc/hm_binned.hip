@@ -598,8 +598,8 @@ __device__ __forceinline__ void walk_cam_record(D& d, const BinArgs& a, uint32_t
     const float4 si = a.sinfo[id / BIN_STRIDE];
     const uint32_t ncp = __float_as_uint(si.w);
     const uint32_t idx = id & (BIN_STRIDE - 1u);
-    const float den = u2f(ncp - 1u > 1u ? ncp - 1u : 1u);
-    const float w0 = rdiv(u2f(idx), den), w1 = rdiv(u2f(idx + 1u), den);
+    const float den = u2f(ncp - 1u > 1u ? ncp - 1u : 1u), inv_den = rdiv(1.0f, den);
+    const float w0 = div_by_recip(u2f(idx), den, inv_den), w1 = div_by_recip(u2f(idx + 1u), den, inv_den);
     d.segment_walk(F3{c0.x, c0.y, c0.z}, F3{c1.x, c1.y, c1.z}, F2{c0.w, y0}, F2{c1.w, y1}, F3{t0.x, t0.y, t0.z},
                    F3{t1.x, t1.y, t1.z}, F3{si.x, si.y, w0}, F3{si.x, si.y, w1}, t0.w, t1.w, si.z);
 }

@@ -9,12 +9,15 @@
 //   inverseSqrt(x) = 1/sqrt(x); normalize(v) = v * inverseSqrt(dot(v,v));
 //   mix(a,b,t) = a*(1-t) + b*t; f32->u32/i32 truncate and saturate to
 //   [0, 4294967040] / [-2^31, 2147483520]; i32 add/sub/abs wrap.
+// Where a quotient or root has a cheaper form that equals it for every input
+// of its domain (hm_exact.h), that form is used instead.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/hair_mi.h"
+#include "hm_exact.h"
 
 namespace hm {
 
@@ -54,22 +57,29 @@ __device__ __forceinline__ float len3(F3 a) { return __builtin_sqrtf(dot3(a, a))
 __device__ __forceinline__ float rdiv(float a, float b) { return a / b; }
 __device__ __forceinline__ float inv_sqrt(float x) { return 1.0f / __builtin_sqrtf(x); }
 __device__ __forceinline__ F3 normalize3(F3 v) { return v * inv_sqrt(dot3(v, v)); }
+// normalize3 for a vector expected to be unit length to rounding already
+// (equal to normalize3 for every input; cheaper when the length is near 1).
+__device__ __forceinline__ F3 normalize3_unit(F3 v) { return v * inv_sqrt_rn(dot3(v, v)); }
 __device__ __forceinline__ float mixs(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 __device__ __forceinline__ F2 mix2(F2 a, F2 b, float t) { return {mixs(a.x, b.x, t), mixs(a.y, b.y, t)}; }
 __device__ __forceinline__ F3 mix3(F3 a, F3 b, float t) {
     return {mixs(a.x, b.x, t), mixs(a.y, b.y, t), mixs(a.z, b.z, t)};
 }
 
+// Saturating float -> integer conversions (NaN -> 0, out-of-range -> the
+// nearest representable bound, else truncation). The input is clamped with
+// selects first, so the conversion itself is always in range and the code is
+// branch-free.
 __device__ __forceinline__ uint32_t to_u32(float x) {
-    if (!(x > 0.0f)) return 0u;
-    if (x >= 4294967040.0f) return 4294967040u;
-    return (uint32_t)x;
+    float c = x > 0.0f ? x : 0.0f;  // NaN -> 0
+    c = c < 4294967040.0f ? c : 4294967040.0f;
+    return (uint32_t)c;
 }
 __device__ __forceinline__ int32_t to_i32(float x) {
-    if (x != x) return 0;
-    if (x <= -2147483648.0f) return INT32_MIN;
-    if (x >= 2147483520.0f) return 2147483520;
-    return (int32_t)x;
+    float c = x != x ? 0.0f : x;
+    c = c > -2147483648.0f ? c : -2147483648.0f;
+    c = c < 2147483520.0f ? c : 2147483520.0f;
+    return (int32_t)c;
 }
 __device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
 __device__ __forceinline__ int32_t wsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
@@ -123,11 +133,11 @@ __device__ __forceinline__ uint32_t quantize_depth(float dist, float near_, floa
 }
 
 __device__ __forceinline__ float dequantize_depth(uint32_t depth, float near_, float far_) {
-    return mixs(near_, far_, rdiv(u2f(depth), 16777214.0f));
+    return mixs(near_, far_, depth_code_fraction(depth));
 }
 
 __device__ __forceinline__ void oct_encode(F3 n_in, uint32_t& ox, uint32_t& oy) {
-    F3 n = normalize3(n_in);
+    F3 n = normalize3_unit(n_in);  // callers pass normalized tangents
     float denom = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
     float m = mx(denom, 1e-6f);
     float px = rdiv(n.x, m), py = rdiv(n.y, m);
@@ -159,6 +169,12 @@ __device__ __forceinline__ F3 safe_normalize(F3 v) {
     float l2 = dot3(v, v);
     if (l2 < 1e-10f) return {0.0f, 1.0f, 0.0f};
     return v * inv_sqrt(l2);
+}
+// safe_normalize for a vector expected to be near unit length (same results).
+__device__ __forceinline__ F3 safe_normalize_unit(F3 v) {
+    float l2 = dot3(v, v);
+    if (l2 < 1e-10f) return {0.0f, 1.0f, 0.0f};
+    return v * inv_sqrt_rn(l2);
 }
 
 __device__ __forceinline__ F3 calc_tangent(uint32_t idx, uint32_t count, F3 prev, F3 curr, F3 next) {
@@ -213,8 +229,10 @@ __device__ __forceinline__ F3 to_world(const View& p, F3 q) {
             mat_row(p.wfm, 2, q.x, q.y, q.z, 1.0f)};
 }
 
+// d is a normalized model-space direction: with a world_from_model of unit
+// scale its image is unit length to rounding.
 __device__ __forceinline__ F3 dir_world(const View& p, F3 d) {
-    return safe_normalize(F3{mat_row(p.wfm, 0, d.x, d.y, d.z, 0.0f), mat_row(p.wfm, 1, d.x, d.y, d.z, 0.0f),
+    return safe_normalize_unit(F3{mat_row(p.wfm, 0, d.x, d.y, d.z, 0.0f), mat_row(p.wfm, 1, d.x, d.y, d.z, 0.0f),
                              mat_row(p.wfm, 2, d.x, d.y, d.z, 0.0f)});
 }
 

@@ -333,14 +333,14 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
     // Tap weights are continuous: hardware reciprocal and exponential.
     const float inv_spar2 = frcp(spar2), inv_sperp2 = frcp(sperp2);
     const float inv_sigma_c2 = 1.0f / (0.9f * 0.9f);
-    float p_norm = rdiv(u2f(depth_q), 16777214.0f);
+    float p_norm = depth_code_fraction(depth_q);
     // filter.wgsl's tap depth test |rdiv(q, 16777214) - p_norm| <= 1.45e-3 as
     // an interval of depth codes: the quotient and the difference are
     // monotone in q, and the pixel's own code passes. The bounds start from a
     // float estimate and move to the exact edge with the test itself.
     uint32_t q_lo, q_hi;
     {
-        auto pass = [&](uint32_t q) { return !(fabsf(rdiv(u2f(q), 16777214.0f) - p_norm) > 1.45e-3f); };
+        auto pass = [&](uint32_t q) { return !(fabsf(depth_code_fraction(q) - p_norm) > 1.45e-3f); };
         q_lo = to_u32((p_norm - 1.45e-3f) * 16777214.0f);
         q_lo = q_lo < depth_q ? q_lo : depth_q;
         if (pass(q_lo)) {
@@ -398,7 +398,7 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
     F3 color = {cc.x, cc.y, cc.z};
     if (weight > 1e-5f) color = accum * frcp(weight);
     float coverage = h_center / mx(h_center + h_cons, 1.0f);
-    float cons_depth = cons_hit ? rdiv(u2f(gb_depth(cons_word)), 16777214.0f) : 1.0f;
+    float cons_depth = cons_hit ? depth_code_fraction(gb_depth(cons_word)) : 1.0f;
     bool closer = cons_hit && center_hit && (p_norm - cons_depth) >= 5e-4f;
     float alpha = closer ? 1.0f : clampf(coverage, 0.0f, 1.0f);
     if (promote && weight <= 1e-5f) {

@@ -67,12 +67,12 @@ __device__ __forceinline__ StrandCps gen_strand(const View& p, const hm_strand_r
 
     // Control points (raster.wgsl:270-284, eval_cp raster.wgsl:62-75).
     const uint32_t ncp = ref.control_points < 2u ? 2u : (ref.control_points > 127u ? 127u : ref.control_points);
-    const float den = u2f(ncp - 1u > 1u ? ncp - 1u : 1u);
+    const float den = u2f(ncp - 1u > 1u ? ncp - 1u : 1u), inv_den = rdiv(1.0f, den);
     F3 cp[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         uint32_t idx = (uint32_t)lane + 64u * (uint32_t)j;
-        float w = rdiv(u2f(idx), den);
+        float w = div_by_recip(u2f(idx), den, inv_den);  // idx / den (exact for idx <= den; other lanes are unused)
         float layers_f = w * 15.0f;
         uint32_t i0 = to_u32(floorf(layers_f));
         i0 = i0 < 15u ? i0 : 15u;
@@ -171,21 +171,31 @@ struct Drawer {
     float scene_clear;
     Sink& sink;
     Counters c;
+    // The centre-hit test dist <= diameter / 2 of raster.wgsl as
+    // dist^2 <= hit_d2 (dist = sqrt(dist^2) rounded); once per drawer.
+    float hit_d2 = sqrt_le_bound(mx(p.appearance.w, 0.05f) * 0.5f);
 
     __device__ __forceinline__ float scene_at(int32_t x, int32_t y) const {
         if (!scene_depth) return scene_clear;
         return scene_depth[(size_t)(uint32_t)y * (uint32_t)to_i32(p.screen.x) + (uint32_t)x];
     }
 
+    // With no scene depth buffer and a zero clear value the scene test of
+    // raster.wgsl:93-98 depends only on the sign of the clip depth, which is
+    // decided once per sample (no division); otherwise put() tests each pixel.
+    __device__ __forceinline__ bool scene_sign_only() const { return !scene_depth && scene_clear == 0.0f; }
+
     // One camera write of a payload (raster.wgsl:93-115 once the payload is
     // built: it depends only on t, not on the pixel).
     __device__ __forceinline__ void put(int32_t x, int32_t y, int32_t width, float clip_depth, uint64_t payload,
                                         bool center_hit, uint32_t beta_word) {
-        float scene = scene_at(x, y);
-        if (p.pass_mode[2] == 1u) {
-            if (clip_depth < scene) return;
-        } else if (clip_depth > scene) {
-            return;
+        if (!scene_sign_only()) {
+            float scene = scene_at(x, y);
+            if (p.pass_mode[2] == 1u) {
+                if (clip_depth < scene) return;
+            } else if (clip_depth > scene) {
+                return;
+            }
         }
         sink.cam((size_t)(uint32_t)y * (uint32_t)width + (uint32_t)x, x, y, payload, center_hit, beta_word, c);
     }
@@ -208,14 +218,15 @@ struct Drawer {
         F2 delta = B.screen - A.screen;
         int32_t steps = to_i32(ceilf(mx(fabsf(delta.x), fabsf(delta.y))));
         int32_t n = steps < 1 ? 1 : (steps > 96 ? 96 : steps);
-        float diameter = mx(p.appearance.w, 0.05f);
         const int32_t width = to_i32(p.screen.x), height = to_i32(p.screen.y);
         F2 seg = B.screen - A.screen;
         float denom = mx(dot2(seg, seg), 1e-6f);
         uint32_t beta_q = to_u32(clampf(beta, 0.0f, 1.0f) * 255.0f + 0.5f);
+        const float fn = i2f(n), inv_n = rdiv(1.0f, fn);  // t = i / n below, exactly (i <= n <= 96)
+        const bool sign_only = scene_sign_only();
         int32_t ppx = 0, ppy = 0;
         for (int32_t i = 0; i <= n; ++i) {
-            float t = rdiv(i2f(i), i2f(n));
+            float t = div_by_recip(i2f(i), fn, inv_n);
             F2 screen = mix2(A.screen, B.screen, t);
             int32_t px = to_i32(floorf(screen.x)), py = to_i32(floorf(screen.y));
             bool diag = i > 0 && ppx != px && ppy != py;
@@ -225,10 +236,15 @@ struct Drawer {
             if (in0 || in1 || in2) {
                 F3 world = mix3(p0, p1, t);
                 float vd = dot3(world - xyz(p.camera_pos), xyz(p.camera_forward));
-                if (vd > p.screen.z) {
-                    float cz = mat_row(p.clip, 2, world.x, world.y, world.z, 1.0f);
-                    float cw = mat_row(p.clip, 3, world.x, world.y, world.z, 1.0f);
-                    float clip_depth = rdiv(cz, mx(cw, 1e-5f));
+                float cz = mat_row(p.clip, 2, world.x, world.y, world.z, 1.0f);
+                float cw = mx(mat_row(p.clip, 3, world.x, world.y, world.z, 1.0f), 1e-5f);
+                float clip_depth = 0.0f;  // cz / cw, formed only when put() compares it per pixel
+                bool visible = vd > p.screen.z;
+                if (visible && sign_only)
+                    visible = !(p.pass_mode[2] == 1u ? div_below_zero(cz, cw) : div_above_zero(cz, cw));
+                else if (visible)
+                    clip_depth = rdiv(cz, cw);
+                if (visible) {
                     F3 tangent = safe_normalize(mix3(t0, t1, t));
                     F3 uvw = mix3(uvw0, uvw1, t);
                     float ao = mixs(ao0, ao1, t);
@@ -242,8 +258,8 @@ struct Drawer {
                     if (in0) {
                         F2 center = {i2f(px) + 0.5f, i2f(py) + 0.5f};
                         float closest = clampf(rdiv(dot2(center - A.screen, seg), denom), 0.0f, 1.0f);
-                        float dist = len2(center - (A.screen + seg * closest));
-                        bool hit = dist <= diameter * 0.5f;
+                        F2 off = center - (A.screen + seg * closest);
+                        bool hit = dot2(off, off) <= hit_d2;
                         put(px, py, width, clip_depth, payload, hit, beta_word);
                     }
                     if (in1) put(ppx, py, width, clip_depth, payload, false, beta_word);
@@ -278,9 +294,8 @@ struct Drawer {
         if (front_q == HM_DOM_EMPTY) return;
         float z_front = dequantize_depth(front_q, near_, far_);
         float thickness = rdiv(mx(far_ - near_, 1e-3f), 16.0f);
-        float t = rdiv(mx(dist - z_front, 0.0f), thickness);
-        uint32_t slice = to_u32(t);
-        slice = slice < 15u ? slice : 15u;
+        // min(to_u32(max(dist - z_front, 0) / thickness), 15), exactly
+        uint32_t slice = slice_of(mx(dist - z_front, 0.0f), thickness, rdiv(1.0f, thickness));
         sink.dom_add(pix, x, y, slice, c);
     }
 

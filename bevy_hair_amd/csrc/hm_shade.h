@@ -341,7 +341,7 @@ __device__ __forceinline__ PixelShade pixel_setup(const View& p, uint64_t word, 
     s.shape.alpha = {tilt, -0.5f * tilt, -1.5f * tilt};
     s.shape.beta = {bb, 0.5f * bb, 2.0f * bb};
     s.shape.inv_beta = gaussian_rcp(s.shape.beta);
-    s.t_n = safe_normalize(s.tangent);
+    s.t_n = safe_normalize_unit(s.tangent);  // the decoded tangent is normalized
     s.v_n = safe_normalize(s.view);
     float tv = dot3(s.tangent, s.view);
     s.cos_theta_o = sqrtf(mx(1.0f - tv * tv, 1e-6f));
