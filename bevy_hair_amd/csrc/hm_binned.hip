@@ -704,13 +704,56 @@ __global__ __launch_bounds__(256) void k_cam_finish(BinArgs a) {
     }
 }
 
-// Long segments: global atomics on top of the finished tiles.
+// Long camera segments go to the G-buffer after the tiles have written it
+// and emitted the pixel lists. The write that first covers a pixel (its
+// atomic returns the empty word) appends the pixel to its tile's sub-list,
+// so pixels that only long segments reach are shaded and filtered as well.
+// Every centre write comes with a conservative write of the same payload,
+// so "conservative word set" is the filter list's condition.
+struct LongCamSink {
+    uint64_t* center;
+    uint64_t* cons;
+    uint32_t* beta;
+    PixelList shade, filter;
+    bool emit;
+    uint32_t tile_shift, tiles_x;
+    __device__ __forceinline__ bool accepts(int32_t, int32_t) const { return true; }
+    __device__ __forceinline__ bool walked_out(int32_t, int32_t, int32_t, int32_t) const { return false; }
+    __device__ __forceinline__ void append(const PixelList& L, int32_t x, int32_t y, size_t pix) const {
+        const uint32_t sub = ((((uint32_t)y >> tile_shift) * tiles_x) + ((uint32_t)x >> tile_shift)) % LIST_SUBS;
+        const uint32_t pos = atomicAdd(L.counts + sub, 1u);
+        if (pos < L.cap) L.list[(size_t)sub * L.cap + pos] = (uint32_t)pix;
+    }
+    __device__ __forceinline__ void cam(size_t pix, int32_t x, int32_t y, uint64_t payload, bool center_hit,
+                                        uint32_t beta_word, Counters&) {
+        // a loaded word at or below the payload is set already (the words only decrease)
+        if (payload < cons[pix]) {
+            const uint64_t old = atomicMin((unsigned long long*)(cons + pix), (unsigned long long)payload);
+            if (emit && gb_empty(old)) append(filter, x, y, pix);
+        }
+        if (center_hit) {
+            if (payload < center[pix]) {
+                const uint64_t old = atomicMin((unsigned long long*)(center + pix), (unsigned long long)payload);
+                if (emit && gb_empty(old)) append(shade, x, y, pix);
+            }
+            if (beta_word < beta[pix]) atomicMin(beta + pix, beta_word);
+        }
+    }
+};
+
 __global__ __launch_bounds__(256) void k_cam_long(BinArgs a) {
     if (a.meta[BIN_LONG] == 0u) return;
     const View& p = bin_view(a);
     const uint32_t n = a.meta[BIN_LONG_COUNT];
-    GlobalSink sink{a.center, a.cons, a.beta, nullptr, 0, true};
-    Drawer<0, GlobalSink> d{p, a.scene_depth, a.scene_clear, sink, {0, 0, 0, 0}};
+    LongCamSink sink{a.center,
+                     a.cons,
+                     a.beta,
+                     PixelList{a.shade_list, a.meta + BIN_SHADE_SUBS, a.list_cap},
+                     PixelList{a.filter_list, a.meta + BIN_FILTER_SUBS, a.list_cap},
+                     a.emit_lists != 0u,
+                     a.tile_shift,
+                     a.tiles_x};
+    Drawer<0, LongCamSink> d{p, a.scene_depth, a.scene_clear, sink, {0, 0, 0, 0}};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         walk_cam_record(d, a, a.long_list[i]);
 }

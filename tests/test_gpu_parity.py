@@ -161,6 +161,38 @@ def test_frame_matches_oracle_and_is_deterministic(renderer):
     assert rel_err(first[4], ref.filtered).max() <= RGB_TOL
 
 
+@pytest.mark.parametrize("cam_tile,chunk", [(16, 1024), (32, 64)])
+def test_frame_with_long_camera_segments(renderer, cam_tile, chunk):
+    """A box limit of one tile sends most segments down the long path, which
+    draws after the tiles have emitted the pixel lists: pixels only long
+    segments cover must still be shaded and filtered like the oracle's."""
+    sc = make_scene("c1")
+    w, r = sc.w, renderer
+    print(f"\n[gpu parity] frame with long segments cam_tile={cam_tile} chunk={chunk}", flush=True)
+    r.targets(w.width, w.height, w.dom_size)
+    r.set_scene_depth(None, 0.0)
+    r.set_lights(sc.points, sc.dirs)
+    g = r.upload(sc.mesh)
+    keep = r.get_tuning()
+    try:
+        r.set_tuning(cam_tile=cam_tile, bin_box_limit=1, cam_chunk=chunk)
+        r.hair_pass(g, sc.params, r.frame_opts(control_points=w.control_points))
+        st = r.bin_status()
+        gb = r.gbuffer()
+        shaded, filtered = r.shaded()[0], r.filtered()[0]
+    finally:
+        r.set_tuning(**keep)
+        g.free()
+    assert st["binned"] and st["long"] and st["long_count"] > 0
+    ref = ref_frame(sc)
+    pa, n_p, da, n_d = sc.light_arrays()
+    ref.hair_pass(sc.params, pa, n_p, da, n_d, control_points=w.control_points)
+    assert np.array_equal(gb[0], ref.center) and np.array_equal(gb[1], ref.cons)
+    # every covered pixel shaded and filtered, every other one cleared
+    assert rel_err(shaded, ref.shaded).max() <= RGB_TOL
+    assert rel_err(filtered, ref.filtered).max() <= RGB_TOL
+
+
 @pytest.mark.parametrize("flags", [0, 1, 2, 3])
 def test_raster_variants_give_identical_bytes(renderer, flags):
     """Every launch choice the autotuner may pick writes the same G-buffer and DOM."""
