@@ -151,12 +151,116 @@ __device__ __forceinline__ Run workgroup_run(uint32_t n) {
 
 }  // namespace
 
+// Two strands per wave: the half-wave at lane gb (0 or 32) takes strand
+// s = 2 * pair + gb / 32, and lane gb + r holds its control points
+// idx = r + 32 * j for the S slots j < S = ceil(max ncp / 32) of the pair, so
+// a pair of 65-point strands takes 3 slot passes where one strand per wave
+// took 2 each. Neighbouring control points are in lane r +- 1 of the same
+// slot, or in lane 31 of slot j - 1 / lane 0 of slot j + 1 of the same half.
 // MODE 0: camera (also stores world tangents, ao and per-strand uv / beta /
 // control-point count for the payload); MODE 1: deep opacity map.
+template <int MODE, int S>
+__device__ __forceinline__ void gen_pair(const BinArgs& a, const View& p, uint32_t s, bool live, uint32_t reuse_total,
+                                         int r, int gb) {
+    const int lane = gb + r;
+    const hm_strand_ref ref = a.refs[s];
+    const size_t base = (size_t)s * BIN_STRIDE;
+    const uint32_t ncp_live = ref.control_points < 2u ? 2u : (ref.control_points > 127u ? 127u : ref.control_points);
+    const uint32_t ncp = live ? ncp_live : 0u;  // a missing second strand stores nothing
+    // DOM pass: when the camera pass of this frame generated the same strand
+    // (same bundle, strand and control-point count) its world control points
+    // are already in cps; otherwise generate them.
+    bool reuse = false;
+    if (MODE == 1 && s < reuse_total) {
+        const hm_strand_ref c = a.reuse_refs[s];
+        reuse = c.bundle == ref.bundle && c.strand == ref.strand && c.control_points == ref.control_points;
+    }
+    F3 cw[S], ct[S];
+    float ao[S];
+    StrandLayers L;
+    if (reuse) {
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const uint32_t idx = (uint32_t)r + 32u * (uint32_t)j;
+            const float4 q = idx < ncp ? a.cps[base + idx] : make_float4(0, 0, 0, 0);
+            cw[j] = F3{q.x, q.y, q.z};
+        }
+    } else {
+        L = strand_layers(ref, a.bundles, a.layers, a.style, r, gb);
+        F3 cp[S];
+        float w;
+#pragma unroll
+        for (int j = 0; j < S; ++j) cp[j] = strand_cp(L, (uint32_t)r + 32u * (uint32_t)j, gb, w, ao[j]);
+        if (MODE == 0) {
+            // Control-point tangents (raster.wgsl:286-317).
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                const uint32_t idx = (uint32_t)r + 32u * (uint32_t)j;
+                const F3 up = shfl3(cp[j], r > 0 ? lane - 1 : lane);
+                const F3 dn = shfl3(cp[j], r < 31 ? lane + 1 : lane);
+                const F3 up_prev = j > 0 ? shfl3(cp[j > 0 ? j - 1 : 0], gb + 31) : cp[j];
+                const F3 dn_next = j + 1 < S ? shfl3(cp[j + 1 < S ? j + 1 : j], gb) : cp[j];
+                const F3 prev = r > 0 ? up : up_prev;
+                const F3 next = idx + 1u < ncp_live ? (r < 31 ? dn : dn_next) : cp[j];
+                ct[j] = safe_normalize(calc_tangent(idx, ncp_live, prev, cp[j], next));
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < S; ++j) cw[j] = to_world(p, cp[j]);
+    }
+    Projected pr[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const uint32_t idx = (uint32_t)r + 32u * (uint32_t)j;
+        pr[j] = project_world(p, cw[j]);
+        if (idx < ncp) {
+            a.cps[base + idx] = make_float4(cw[j].x, cw[j].y, cw[j].z, pr[j].screen.x);
+            a.cpy[base + idx] = pr[j].screen.y;
+            if (MODE == 0) {
+                // raster.wgsl:319-337: world tangents, ao; uv and beta per strand.
+                const F3 wt = dir_world(p, ct[j]);
+                a.cpt[base + idx] = make_float4(wt.x, wt.y, wt.z, ao[j]);
+            }
+        }
+    }
+    if (MODE == 0 && r == 0 && live) a.sinfo[s] = make_float4(L.uv.x, L.uv.y, L.beta, __uint_as_float(ncp));
+    // Segment idx -> idx + 1: the end point's projection is in the next lane,
+    // or in lane 0 of the next slot.
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const uint32_t idx = (uint32_t)r + 32u * (uint32_t)j;
+        const int nl = r < 31 ? lane + 1 : lane;  // r = 31 of the last slot: idx + 1 >= ncp, unused
+        F2 e = {__shfl(pr[j].screen.x, nl), __shfl(pr[j].screen.y, nl)};
+        bool ef = __shfl((int)pr[j].in_front, nl) != 0;
+        if (j + 1 < S) {
+            const Projected& q = pr[j + 1 < S ? j + 1 : j];
+            const F2 e1 = {__shfl(q.screen.x, gb), __shfl(q.screen.y, gb)};
+            const bool ef1 = __shfl((int)q.in_front, gb) != 0;
+            if (r == 31) {
+                e = e1;
+                ef = ef1;
+            }
+        }
+        const uint32_t id = s * BIN_STRIDE + idx;
+        bool is_long = false;
+        uint32_t box = BIN_NONE;
+        if (idx + 1u < ncp && (pr[j].in_front || ef))
+            box = MODE == 0 ? camera_box(p, pr[j].screen, e, a.tile_shift, a.box_limit, is_long)
+                            : segment_box(p, pr[j].screen, e, a.tile_shift, a.box_limit, is_long);
+        // Every slot of the strand is written, so the binning passes can walk
+        // segment ids without the strand's control-point count.
+        if (live) a.seg_box[id] = box;
+        append_long(is_long, id, a.meta + BIN_LONG_COUNT, a.long_list, a.long_capacity);
+    }
+#pragma unroll
+    for (int j = S; j < (int)(BIN_STRIDE / 32u); ++j)
+        if (live) a.seg_box[s * BIN_STRIDE + (uint32_t)r + 32u * (uint32_t)j] = BIN_NONE;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
     const View& p = bin_view(a);
-    const int lane = (int)(threadIdx.x & 63u);
+    const int lane = (int)(threadIdx.x & 63u), r = lane & 31, gb = lane & 32;
     uint32_t total = a.indirect[0];
     total = total < a.capacity ? total : a.capacity;
     const uint32_t wpb = blockDim.x >> 6;
@@ -166,71 +270,20 @@ __global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
         reuse_total = a.reuse_indirect[0];
         reuse_total = reuse_total < a.capacity ? reuse_total : a.capacity;
     }
-    for (uint32_t s = w0; s < total; s += gridDim.x * wpb) {
-        const hm_strand_ref ref = a.refs[s];
-        const size_t base = (size_t)s * BIN_STRIDE;
-        // DOM pass: when the camera pass of this frame generated the same strand
-        // (same bundle, strand and control-point count) its world control points
-        // are already in cps; otherwise generate them.
-        bool reuse = false;
-        if (MODE == 1 && s < reuse_total) {
-            const hm_strand_ref c = a.reuse_refs[s];
-            reuse = c.bundle == ref.bundle && c.strand == ref.strand && c.control_points == ref.control_points;
-        }
-        StrandCps sc;
-        if (reuse) {
-            sc.ncp = ref.control_points < 2u ? 2u : (ref.control_points > 127u ? 127u : ref.control_points);
-            const float4 q0 = (uint32_t)lane < sc.ncp ? a.cps[base + lane] : make_float4(0, 0, 0, 0);
-            const float4 q1 = (uint32_t)lane + 64u < sc.ncp ? a.cps[base + lane + 64] : make_float4(0, 0, 0, 0);
-            sc.cw[0] = F3{q0.x, q0.y, q0.z};
-            sc.cw[1] = F3{q1.x, q1.y, q1.z};
-        } else {
-            sc = gen_strand(p, ref, a.bundles, a.layers, a.style, lane);
-        }
-        const uint32_t ncp = sc.ncp;
-        Projected pr0 = project_world(p, sc.cw[0]), pr1 = project_world(p, sc.cw[1]);
-        if ((uint32_t)lane < ncp) {
-            a.cps[base + lane] = make_float4(sc.cw[0].x, sc.cw[0].y, sc.cw[0].z, pr0.screen.x);
-            a.cpy[base + lane] = pr0.screen.y;
-        }
-        if ((uint32_t)lane + 64u < ncp) {
-            a.cps[base + lane + 64] = make_float4(sc.cw[1].x, sc.cw[1].y, sc.cw[1].z, pr1.screen.x);
-            a.cpy[base + lane + 64] = pr1.screen.y;
-        }
-        if (MODE == 0) {
-            // raster.wgsl:319-337: world tangents, ao; uv and beta per strand.
-            if ((uint32_t)lane < ncp) {
-                F3 wt = dir_world(p, sc.ct[0]);
-                a.cpt[base + lane] = make_float4(wt.x, wt.y, wt.z, sc.ao[0]);
-            }
-            if ((uint32_t)lane + 64u < ncp) {
-                F3 wt = dir_world(p, sc.ct[1]);
-                a.cpt[base + lane + 64] = make_float4(wt.x, wt.y, wt.z, sc.ao[1]);
-            }
-            if (lane == 0) a.sinfo[s] = make_float4(sc.uv.x, sc.uv.y, sc.beta, __uint_as_float(ncp));
-        }
-        // Segment idx -> idx + 1: the end point's projection lives in the next lane.
-        const int nl = (lane + 1) & 63;
-        F2 s0n = {__shfl(pr0.screen.x, nl), __shfl(pr0.screen.y, nl)};
-        F2 s1n = {__shfl(pr1.screen.x, nl), __shfl(pr1.screen.y, nl)};
-        bool f0n = __shfl((int)pr0.in_front, nl) != 0, f1n = __shfl((int)pr1.in_front, nl) != 0;
-        F2 e0 = lane < 63 ? s0n : s1n;
-        bool ef0 = lane < 63 ? f0n : f1n;
-        const uint32_t id0 = s * BIN_STRIDE + (uint32_t)lane, id1 = id0 + 64u;
-        bool long0 = false, long1 = false;
-        uint32_t box0 = BIN_NONE, box1 = BIN_NONE;
-        if ((uint32_t)lane + 1u < ncp && (pr0.in_front || ef0))
-            box0 = MODE == 0 ? camera_box(p, pr0.screen, e0, a.tile_shift, a.box_limit, long0)
-                             : segment_box(p, pr0.screen, e0, a.tile_shift, a.box_limit, long0);
-        if ((uint32_t)lane + 65u < ncp && (pr1.in_front || f1n))
-            box1 = MODE == 0 ? camera_box(p, pr1.screen, s1n, a.tile_shift, a.box_limit, long1)
-                             : segment_box(p, pr1.screen, s1n, a.tile_shift, a.box_limit, long1);
-        // Every slot of the strand is written, so the binning passes can walk
-        // segment ids without the strand's control-point count.
-        a.seg_box[id0] = box0;
-        a.seg_box[id1] = box1;
-        append_long(long0, id0, a.meta + BIN_LONG_COUNT, a.long_list, a.long_capacity);
-        append_long(long1, id1, a.meta + BIN_LONG_COUNT, a.long_list, a.long_capacity);
+    for (uint32_t pair = w0; 2u * pair < total; pair += gridDim.x * wpb) {
+        const uint32_t s_want = 2u * pair + (gb ? 1u : 0u);
+        const bool live = s_want < total;
+        const uint32_t s = live ? s_want : 2u * pair;  // the missing second strand of an odd total
+        // slots for the longer strand of the pair (wave-uniform)
+        const uint32_t c0 = a.refs[2u * pair].control_points;
+        const uint32_t c1 = 2u * pair + 1u < total ? a.refs[2u * pair + 1u].control_points : 0u;
+        uint32_t need = c0 > c1 ? c0 : c1;
+        need = need < 2u ? 2u : (need > 127u ? 127u : need);
+        const uint32_t slots = __builtin_amdgcn_readfirstlane((need + 31u) / 32u);
+        if (slots <= 1u) gen_pair<MODE, 1>(a, p, s, live, reuse_total, r, gb);
+        else if (slots == 2u) gen_pair<MODE, 2>(a, p, s, live, reuse_total, r, gb);
+        else if (slots == 3u) gen_pair<MODE, 3>(a, p, s, live, reuse_total, r, gb);
+        else gen_pair<MODE, 4>(a, p, s, live, reuse_total, r, gb);
     }
 }
 

@@ -7,6 +7,8 @@
 //               style levels at the strand's uv (the u/v half of
 //               raster.wgsl:34-60, shared by every control point of the
 //               strand), control point idx lives in lane idx % 64 (slot idx/64).
+//               strand_layers / strand_cp are its two halves, also used by
+//               the binned generation with two strands per wave.
 // Drawer:       draw_segment/write_pixel (raster.wgsl:77-116, 192-223) and
 //               draw_dom_segment/write_dom_pixel (raster.wgsl:118-190). The
 //               write target is a Sink: global atomics, counters, or an LDS tile.
@@ -22,28 +24,33 @@ __device__ __forceinline__ F4 shfl4(F4 v, int src) {
     return {__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src), __shfl(v.w, src)};
 }
 
-// Per-lane result of gen_strand: control points idx = lane + 64 * j.
-struct StrandCps {
-    F3 cw[2];       // world position (to_world)
-    F3 ct[2];       // control-point tangent, model space, normalized
-    float w[2], ao[2];
+// Layer and style samples of one strand, held by a group of lanes starting
+// at lane gb (r = lane - gb): layer k (position + ao, and its tangent) and
+// style level k both sit in lane gb + k for k < 16 (raster.wgsl:22-60,
+// 244-267). Lanes r < 16 sample the layers, lanes 16 <= r < 32 the style
+// levels; other lanes of the group hold unused values.
+struct StrandLayers {
+    F4 lay;         // layer position (xyz) and ao (w)
+    F3 ltan;        // layer tangent
+    F3 sty;         // style offset of level r
     F2 uv;
     float beta;     // n_lod / max(N_b, 1)
     uint32_t ncp;   // clamp(control_points, 2, 127)
+    float den, inv_den;  // max(ncp - 1, 1) and its reciprocal
 };
 
-__device__ __forceinline__ StrandCps gen_strand(const View& p, const hm_strand_ref& ref, const hm_bundle* bundles,
-                                                const hm_layer* layers, const float* style, int lane) {
-    StrandCps o;
+__device__ __forceinline__ StrandLayers strand_layers(const hm_strand_ref& ref, const hm_bundle* bundles,
+                                                      const hm_layer* layers, const float* style, int r, int gb) {
+    StrandLayers o;
     const hm_bundle bundle = bundles[ref.bundle];
     const F2 uv = pcg_uv(ref.strand ^ (ref.bundle * 0x85ebca6bu));
     F4 lay = {0, 0, 0, 0};
     F3 sty = {0, 0, 0};
-    if (lane < 16) {
-        const hm_layer* L = layers + bundle.layer_offset + (uint32_t)lane * 4u;
+    if (r < 16) {
+        const hm_layer* L = layers + bundle.layer_offset + (uint32_t)r * 4u;
         lay = bilinear4(ld4(L[0].pos_ao), ld4(L[1].pos_ao), ld4(L[2].pos_ao), ld4(L[3].pos_ao), uv);
-    } else if (lane < 32) {
-        uint32_t k = (uint32_t)lane - 16u;
+    } else if (r < 32) {
+        uint32_t k = (uint32_t)r - 16u;
         float u = clampf(uv.x, 0.0f, 1.0f) * 7.0f;
         float v = clampf(uv.y, 0.0f, 1.0f) * 7.0f;
         uint32_t u0 = to_u32(floorf(u)), v0 = to_u32(floorf(v));
@@ -57,37 +64,60 @@ __device__ __forceinline__ StrandCps gen_strand(const View& p, const hm_strand_r
         F3 c110 = xyz(ld4(style + 4u * (off + v1 * 8u + u1)));
         sty = mix3(mix3(c000, c100, tu), mix3(c010, c110, tu), tv);
     }
-    sty = shfl3(sty, (lane + 16) & 63);  // style level k now sits in lane k
+    o.sty = shfl3(sty, gb + ((r + 16) & 31));  // style level k now sits in lane gb + k
 
     // Layer tangents (raster.wgsl:244-267).
     F3 lp = xyz(lay);
-    F3 prev_l = shfl3(lp, lane > 0 ? lane - 1 : 0);
-    F3 next_l = shfl3(lp, lane < 15 ? lane + 1 : 15);
-    F3 ltan = calc_tangent((uint32_t)lane, HM_LAYER_COUNT, prev_l, lp, next_l);
+    F3 prev_l = shfl3(lp, gb + (r > 0 ? r - 1 : 0));
+    F3 next_l = shfl3(lp, gb + (r < 15 ? r + 1 : 15));
+    o.ltan = calc_tangent((uint32_t)r, HM_LAYER_COUNT, prev_l, lp, next_l);
+    o.lay = lay;
+    o.uv = uv;
+    o.ncp = ref.control_points < 2u ? 2u : (ref.control_points > 127u ? 127u : ref.control_points);
+    o.den = u2f(o.ncp - 1u > 1u ? o.ncp - 1u : 1u);
+    o.inv_den = rdiv(1.0f, o.den);
+    o.beta = rdiv(u2f(ref.n_lod), u2f(bundle.strand_count > 1u ? bundle.strand_count : 1u));
+    return o;
+}
 
-    // Control points (raster.wgsl:270-284, eval_cp raster.wgsl:62-75).
-    const uint32_t ncp = ref.control_points < 2u ? 2u : (ref.control_points > 127u ? 127u : ref.control_points);
-    const float den = u2f(ncp - 1u > 1u ? ncp - 1u : 1u), inv_den = rdiv(1.0f, den);
+// Control point idx of the strand whose samples the group at lane gb holds
+// (raster.wgsl:270-284, eval_cp raster.wgsl:62-75): model-space position,
+// ao and the parameter w = idx / (ncp - 1).
+__device__ __forceinline__ F3 strand_cp(const StrandLayers& L, uint32_t idx, int gb, float& w, float& ao) {
+    w = div_by_recip(u2f(idx), L.den, L.inv_den);  // exact for idx <= den; other lanes are unused
+    float layers_f = w * 15.0f;
+    uint32_t i0 = to_u32(floorf(layers_f));
+    i0 = i0 < 15u ? i0 : 15u;
+    uint32_t i1 = i0 + 1u < 15u ? i0 + 1u : 15u;
+    float t = layers_f - u2f(i0);
+    F4 a0 = shfl4(L.lay, gb + (int)i0), a1 = shfl4(L.lay, gb + (int)i1);
+    F3 m0 = shfl3(L.ltan, gb + (int)i0), m1 = shfl3(L.ltan, gb + (int)i1);
+    F3 base = hermite(xyz(a0), xyz(a1), m0, m1, t);
+    ao = mixs(a0.w, a1.w, t);
+    // w is in [0, 1], so the style level index and blend equal i0/i1/t.
+    F3 s0 = shfl3(L.sty, gb + (int)i0), s1 = shfl3(L.sty, gb + (int)i1);
+    return base + mix3(s0, s1, t);
+}
+
+// Per-lane result of gen_strand: control points idx = lane + 64 * j.
+struct StrandCps {
+    F3 cw[2];       // world position (to_world)
+    F3 ct[2];       // control-point tangent, model space, normalized
+    float w[2], ao[2];
+    F2 uv;
+    float beta;     // n_lod / max(N_b, 1)
+    uint32_t ncp;   // clamp(control_points, 2, 127)
+};
+
+// One strand per wave (raster.wgsl:231-317).
+__device__ __forceinline__ StrandCps gen_strand(const View& p, const hm_strand_ref& ref, const hm_bundle* bundles,
+                                                const hm_layer* layers, const float* style, int lane) {
+    StrandCps o;
+    const StrandLayers L = strand_layers(ref, bundles, layers, style, lane, 0);
+    const uint32_t ncp = L.ncp;
     F3 cp[2];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        uint32_t idx = (uint32_t)lane + 64u * (uint32_t)j;
-        float w = div_by_recip(u2f(idx), den, inv_den);  // idx / den (exact for idx <= den; other lanes are unused)
-        float layers_f = w * 15.0f;
-        uint32_t i0 = to_u32(floorf(layers_f));
-        i0 = i0 < 15u ? i0 : 15u;
-        uint32_t i1 = i0 + 1u < 15u ? i0 + 1u : 15u;
-        float t = layers_f - u2f(i0);
-        F4 a0 = shfl4(lay, (int)i0), a1 = shfl4(lay, (int)i1);
-        F3 m0 = shfl3(ltan, (int)i0), m1 = shfl3(ltan, (int)i1);
-        F3 base = hermite(xyz(a0), xyz(a1), m0, m1, t);
-        float ao = mixs(a0.w, a1.w, t);
-        // w is in [0, 1], so the style level index and blend equal i0/i1/t.
-        F3 s0 = shfl3(sty, (int)i0), s1 = shfl3(sty, (int)i1);
-        cp[j] = base + mix3(s0, s1, t);
-        o.w[j] = w;
-        o.ao[j] = ao;
-    }
+    for (int j = 0; j < 2; ++j) cp[j] = strand_cp(L, (uint32_t)lane + 64u * (uint32_t)j, 0, o.w[j], o.ao[j]);
     // Control-point tangents (raster.wgsl:286-317).
     F3 up0 = shfl3(cp[0], (lane + 63) & 63), up1 = shfl3(cp[1], (lane + 63) & 63);
     F3 dn0 = shfl3(cp[0], (lane + 1) & 63), dn1 = shfl3(cp[1], (lane + 1) & 63);
@@ -101,10 +131,10 @@ __device__ __forceinline__ StrandCps gen_strand(const View& p, const hm_strand_r
         next = idx + 1u < ncp ? dn1 : cp[1];
         o.ct[1] = safe_normalize(calc_tangent(idx, ncp, prev, cp[1], next));
     }
-    o.beta = rdiv(u2f(ref.n_lod), u2f(bundle.strand_count > 1u ? bundle.strand_count : 1u));
+    o.beta = L.beta;
 #pragma unroll
     for (int j = 0; j < 2; ++j) o.cw[j] = to_world(p, cp[j]);
-    o.uv = uv;
+    o.uv = L.uv;
     o.ncp = ncp;
     return o;
 }
