@@ -73,6 +73,19 @@ HM_EXACT_FN uint32_t slice_of(float x, float b, float y) {
     return q > 0.0f ? (q < 15.0f ? (uint32_t)q : 15u) : 0u;
 }
 
+// True when the 6-bit code trunc(clamp(v, 0, 1) * 63 + 0.5) is the same
+// for every v = c * (1 - t) + c * t, t in [0, 1] (both products and the sum
+// rounded): such a v is within 3 ulps of c, which moves clamp(v) * 63 + 0.5
+// by less than 2^-16 for |c| <= 2, so the code is c's own whenever
+// clamp(c) * 63 + 0.5 lies at least 2^-12 from an integer.
+HM_EXACT_FN bool quant63_fixed_under_mix(float c) {
+    if (!(c >= -2.0f && c <= 2.0f)) return false;  // also NaN
+    const float a = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+    const float x = a * 63.0f + 0.5f;
+    const float f = x - __builtin_floorf(x);
+    return f >= 0x1p-12f && f <= 1.0f - 0x1p-12f;
+}
+
 // The largest float x with sqrt(x) <= h (sqrt correctly rounded), so that
 // sqrt(d) <= h  <=>  d <= sqrt_le_bound(h) for every float d >= 0 (rounded
 // sqrt is monotone). Domain: h = 0, h >= 2^-60 (h * h normal), +inf, or NaN

@@ -103,30 +103,36 @@ __device__ __forceinline__ f32x4 mlp_fwd16(const bf16x8* T, const float* bias, b
 __device__ __forceinline__ void mlp_fwd64(const bf16x8* T, const float* bias, const bf16x8 b0[4], int lane,
                                           f32x4 y[4]) {
     const int g = lane >> 4;
-    const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-    f32x4 acc[4][4];
+    // The biases enter as the accumulator input of each layer's first
+    // product: lane rows 16 r + 4 g .. + 3 of output tile r.
+    auto bias_tiles = [&](int layer, f32x4 bt[4]) {
+        const float4* bl = reinterpret_cast<const float4*>(bias + 64 * layer);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float4 v = bl[4 * r + g];
+            bt[r] = f32x4{v.x, v.y, v.z, v.w};
+        }
+    };
+    f32x4 acc[4][4], bt[4];
+    bias_tiles(0, bt);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const bf16x8 w = T[(tile_base(0) + r) * 64 + lane];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w, b0[cb], z);
+        for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w, b0[cb], bt[r]);
     }
     bf16x8 B[4][2];
 #pragma unroll
     for (int L = 1; L <= 3; ++L) {
-        const float4* bl = reinterpret_cast<const float4*>(bias + 64 * (L - 1));
-        const float4 bv[4] = {bl[g], bl[4 + g], bl[8 + g], bl[12 + g]};  // rows 4g.., 16+4g.., 32+4g.., 48+4g..
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) {
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                const float lo4[4] = {bv[2 * c].x, bv[2 * c].y, bv[2 * c].z, bv[2 * c].w};
-                const float hi4[4] = {bv[2 * c + 1].x, bv[2 * c + 1].y, bv[2 * c + 1].z, bv[2 * c + 1].w};
                 bf16x8 v;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const float lo = acc[cb][2 * c][i] + lo4[i];
-                    const float hi = acc[cb][2 * c + 1][i] + hi4[i];
+                    const float lo = acc[cb][2 * c][i];
+                    const float hi = acc[cb][2 * c + 1][i];
                     v[i] = (__bf16)(lo > 0.0f ? lo : 0.0f);
                     v[4 + i] = (__bf16)(hi > 0.0f ? hi : 0.0f);
                 }
@@ -134,12 +140,13 @@ __device__ __forceinline__ void mlp_fwd64(const bf16x8* T, const float* bias, co
             }
         }
         if (L < 3) {
+            bias_tiles(L, bt);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const bf16x8 w0 = T[(tile_base(L) + r * 2) * 64 + lane];
                 const bf16x8 w1 = T[(tile_base(L) + r * 2 + 1) * 64 + lane];
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w0, B[cb][0], z);
+                for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w0, B[cb][0], bt[r]);
 #pragma unroll
                 for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w1, B[cb][1], acc[cb][r]);
             }
@@ -147,13 +154,12 @@ __device__ __forceinline__ void mlp_fwd64(const bf16x8* T, const float* bias, co
     }
     const bf16x8 w0 = T[tile_base(3) * 64 + lane];
     const bf16x8 w1 = T[(tile_base(3) + 1) * 64 + lane];
-    const float4 bo = reinterpret_cast<const float4*>(bias + 192)[g];
+    const float4 bo4 = reinterpret_cast<const float4*>(bias + 192)[g];
+    const f32x4 bo = {bo4.x, bo4.y, bo4.z, bo4.w};
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
-        f32x4 v = mfma(w0, B[cb][0], z);
-        v = mfma(w1, B[cb][1], v);
-        v[0] += bo.x; v[1] += bo.y; v[2] += bo.z; v[3] += bo.w;
-        y[cb] = v;
+        f32x4 v = mfma(w0, B[cb][0], bo);
+        y[cb] = mfma(w1, B[cb][1], v);
     }
 }
 

@@ -252,6 +252,11 @@ struct Drawer {
         F2 seg = B.screen - A.screen;
         float denom = mx(dot2(seg, seg), 1e-6f);
         uint32_t beta_q = to_u32(clampf(beta, 0.0f, 1.0f) * 255.0f + 0.5f);
+        // A strand's u and v are the same at both ends: their codes are the
+        // ends' own unless a mix could cross a code boundary.
+        const bool uv_fixed = uvw0.x == uvw1.x && uvw0.y == uvw1.y && quant63_fixed_under_mix(uvw0.x) &&
+                              quant63_fixed_under_mix(uvw0.y);
+        const uint32_t qu_end = quant63(uvw0.x), qv_end = quant63(uvw0.y);
         const float fn = i2f(n), inv_n = rdiv(1.0f, fn);  // t = i / n below, exactly (i <= n <= 96)
         const bool sign_only = scene_sign_only();
         int32_t ppx = 0, ppy = 0;
@@ -276,14 +281,19 @@ struct Drawer {
                     clip_depth = rdiv(cz, cw);
                 if (visible) {
                     F3 tangent = safe_normalize(mix3(t0, t1, t));
-                    F3 uvw = mix3(uvw0, uvw1, t);
+                    uint32_t qu = qu_end, qv = qv_end;
+                    if (!uv_fixed) {
+                        qu = quant63(mixs(uvw0.x, uvw1.x, t));
+                        qv = quant63(mixs(uvw0.y, uvw1.y, t));
+                    }
+                    const float uvw_z = mixs(uvw0.z, uvw1.z, t);
                     float ao = mixs(ao0, ao1, t);
                     uint32_t depth = quantize_depth(vd, p.screen.z, p.screen.w);
                     uint32_t ox, oy;
                     oct_encode(tangent, ox, oy);
                     uint32_t ao_q = to_u32(clampf(ao, 0.0f, 1.0f) * 63.0f + 0.5f);
                     if (p.flags[2] == 0u) ao_q = 63u;
-                    uint64_t payload = pack_gb(depth, ox, oy, quant63(uvw.x), quant63(uvw.y), quant63(uvw.z), ao_q);
+                    uint64_t payload = pack_gb(depth, ox, oy, qu, qv, quant63(uvw_z), ao_q);
                     uint32_t beta_word = (depth << 8u) | beta_q;
                     if (in0) {
                         F2 center = {i2f(px) + 0.5f, i2f(py) + 0.5f};

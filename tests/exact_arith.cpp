@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <initializer_list>
+#include <vector>
 
 #include "hm_exact.h"
 
@@ -47,6 +48,37 @@ int main() {
             if (exact_f_bits(hm::inv_sqrt_rn(d)) != exact_f_bits(want)) return fail("inv_sqrt_rn", d, want);
         }
         std::printf("inverse sqrt: %ld values\n", checked);
+    }
+
+    // 6-bit codes of a strand constant mixed with itself: where
+    // quant63_fixed_under_mix(c) holds, every t = i / n (n <= 96) gives c's code.
+    {
+        auto code = [](float v) -> uint32_t {
+            const float a = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);  // clampf on non-NaN input
+            const float x = a * 63.0f + 0.5f;
+            const uint32_t q = x > 0.0f ? (uint32_t)x : 0u;
+            return q < 63u ? q : 63u;
+        };
+        long checked = 0, fixed = 0;
+        std::vector<float> cs;
+        for (int k = 0; k <= 64; ++k) {  // around every code boundary (k - 0.5) / 63, and dense elsewhere
+            const float edge = ((float)k - 0.5f) / 63.0f;
+            for (int d = -2000; d <= 2000; d += 3) cs.push_back(exact_bits_f(exact_f_bits(edge < 0 ? 0.001f : edge) + (uint32_t)d));
+        }
+        for (uint32_t u = exact_f_bits(1e-6f); u < exact_f_bits(1.0f); u += 65537u) cs.push_back(exact_bits_f(u));
+        for (float c : {0.0f, 1.0f, -0.5f, 1.5f, 2.0f, -2.0f}) cs.push_back(c);
+        for (float c : cs) {
+            if (!hm::quant63_fixed_under_mix(c)) continue;
+            ++fixed;
+            const uint32_t want = code(c);
+            for (int n = 1; n <= 96; ++n)
+                for (int i = 0; i <= n; ++i, ++checked) {
+                    const float t = ieee_div((float)i, (float)n);
+                    const float v = c * (1.0f - t) + c * t;
+                    if (code(v) != want) return fail("quant63_fixed_under_mix", c, t);
+                }
+        }
+        std::printf("mixed codes: %ld fixed constants, %ld pairs\n", fixed, checked);
     }
 
     // DOM depth codes / 16777214.
