@@ -153,6 +153,23 @@ def rel_err(got: np.ndarray, ref: np.ndarray, floor: float = 1e-3) -> float:
     return float((d / np.maximum(np.abs(ref[cov, :3]).sum(1), floor)).mean())
 
 
+def timed_frames(r, g, params, opts, steps: int):
+    """K frames between one event pair with nothing else on the stream (the
+    throughput), then K more with an event pair around every kernel group
+    (the per-kernel durations, in ms per frame): the per-kernel events
+    themselves cost a few microseconds per frame, so they stay out of the
+    first measurement. Returns (ms per frame, ms per frame with the kernel
+    events, kernel times)."""
+    r.sync()
+    ms = r.time_frames(g, params, opts, steps) / steps
+    r.timing(steps)
+    r.sync()
+    ms_events = r.time_frames(g, params, opts, steps) / steps
+    kt = r.kernel_times()
+    r.timing(0)
+    return ms, ms_events, kt
+
+
 def mlp_leg_run(r, g, sc, w, args) -> dict:
     """The same frames with the learned dual-scattering term (committed
     weights): frames/s, shade time against the plain shade, and the mean
@@ -165,11 +182,7 @@ def mlp_leg_run(r, g, sc, w, args) -> dict:
     popts = r.frame_opts(control_points=w.control_points, strand_fraction=w.strand_fraction)
     for _ in range(args.warmup):
         r.hair_pass(g, sc.params, mopts)
-    r.timing(args.steps)
-    r.sync()
-    ms = r.time_frames(g, sc.params, mopts, args.steps) / args.steps
-    kt = r.kernel_times()
-    r.timing(0)
+    ms, _, kt = timed_frames(r, g, sc.params, mopts, args.steps)
     r.hair_pass(g, sc.params, popts)
     r.shade(sc.params[0])
     plain = r.shaded()[0].copy()
@@ -227,11 +240,7 @@ def heavy_leg_run(r, args) -> dict:
             opts = r.frame_opts(control_points=w.control_points, strand_fraction=frac)
             for _ in range(2):
                 r.hair_pass(g, sc.params, opts)
-            r.timing(steps)
-            r.sync()
-            ms = r.time_frames(g, sc.params, opts, steps) / steps
-            kt = r.kernel_times()
-            r.timing(0)
+            ms, _, kt = timed_frames(r, g, sc.params, opts, steps)
             top = sorted(((k, v) for k, v in kt.items() if v), key=lambda kv: -kv[1])[:4]
             out["fractions"][str(frac)] = {"frames_per_s": round(1000.0 / ms, 2), "ms_per_step": round(ms, 4),
                                            "top_kernels_ms": {k: round(v, 4) for k, v in top}}
@@ -317,13 +326,9 @@ def main() -> int:
     hit = int((center != np.uint64(0xFFFFFFFFFFFFFFFF)).sum())
     r.sync()
 
-    # Timed region: K frames between two events, per-kernel events inside.
-    r.timing(args.steps)
-    r.sync()
-    ms_total = r.time_frames(g, sc.params, opts, args.steps)
-    kt = r.kernel_times()
-    r.timing(0)
-    ms = ms_total / args.steps
+    # Timed region: K frames between two events; then K frames with the
+    # per-kernel events that give the kernel breakdown and the roofline.
+    ms, ms_events, kt = timed_frames(r, g, sc.params, opts, args.steps)
     fps = 1000.0 / ms
 
     mlp_leg = None if args.no_mlp else mlp_leg_run(r, g, sc, w, args)
@@ -353,6 +358,7 @@ def main() -> int:
                      "algorithmic_bytes": alg, "mean_ms": round(kt[dom_k], 5),
                      "traffic": measured_traffic(dom_k, tuning, w.name)},
         "kernels_ms": {k: round(v, 5) for k, v in kt.items()},
+        "ms_per_step_with_kernel_events": round(ms_events, 4),
         "tuning": tuning,
         "work": {"camera_writes": int(counts["cam"][0]), "camera_center_hits": int(counts["cam"][1]),
                  "camera_atomics": int(atomics_cam), "dom_depth_writes": int(counts["dom1"][2]),

@@ -133,6 +133,12 @@ __device__ __forceinline__ void append_long(bool flag, uint32_t id, uint32_t* co
     if (flag && pos < cap) list[pos] = id;
 }
 
+// Tiles (or list entries) per thread whose global round trips the binning
+// kernels keep in flight together.
+constexpr uint32_t BIN_BATCH = 8u;
+// Segment boxes per thread per round of k_bin_count / k_bin_fill.
+constexpr uint32_t BIN_SEGS = 16u;
+
 // Workgroup b takes the contiguous run [first, end) of the n segment-box
 // words, in whole uint4 groups (n is a multiple of 4: 128 slots per strand);
 // its threads interleave within the run.
@@ -304,21 +310,20 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
     uint32_t total = a.indirect[0];
     total = total < a.capacity ? total : a.capacity;
     const Run run = workgroup_run(total * BIN_STRIDE);
-    const uint4* boxes = reinterpret_cast<const uint4*>(a.seg_box);
-    const uint4 none = make_uint4(BIN_NONE, BIN_NONE, BIN_NONE, BIN_NONE);
-    for (uint32_t q0 = run.first + threadIdx.x; q0 < run.end; q0 += 4u * blockDim.x) {
-        uint4 bq[4];  // four loads in flight per thread
+    // Thread i takes segment ids first + i + k * blockDim.x: a wave's lanes
+    // hold consecutive segments, so the loads are coalesced dwords and, in
+    // k_bin_fill, a tile's slots go to consecutive segments in lane order.
+    const uint32_t wend = run.end * 4u;
+    for (uint32_t r0 = run.first * 4u; r0 < wend; r0 += BIN_SEGS * blockDim.x) {
+        uint32_t bx[BIN_SEGS];  // all loads in flight
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; ++u) {
-            const uint32_t q = q0 + u * blockDim.x;
-            bq[u] = q < run.end ? boxes[q] : none;
+        for (uint32_t k = 0; k < BIN_SEGS; ++k) {
+            const uint32_t w = r0 + threadIdx.x + k * blockDim.x;
+            bx[k] = w < wend ? a.seg_box[w] : BIN_NONE;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; ++u) {
-            const uint32_t bx[4] = {bq[u].x, bq[u].y, bq[u].z, bq[u].w};
-#pragma unroll
-            for (int k = 0; k < 4; ++k) for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { atomicAdd(hist + t, 1u); });
-        }
+        for (uint32_t k = 0; k < BIN_SEGS; ++k)
+            for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { atomicAdd(hist + t, 1u); });
     }
     __syncthreads();
     uint32_t mine = 0u;
@@ -331,12 +336,23 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
     }
     __syncthreads();
     uint32_t* count = a.meta + BIN_META_WORDS;
-    for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x) {
-        const uint32_t h = hist[t];
-        if (!h) continue;
-        const uint32_t rank = atomicAdd(count + t, h);
-        const uint32_t pos = list_base + atomicAdd(&list_pos, 1u);
-        if (pos < a.list_capacity) a.lists[pos] = make_uint2(t, rank);
+    // BIN_BATCH tiles per thread at a time: their global atomics are all in
+    // flight before the first result is used.
+    for (uint32_t t0 = threadIdx.x; t0 < a.tiles; t0 += BIN_BATCH * blockDim.x) {
+        uint32_t h[BIN_BATCH], rank[BIN_BATCH];
+#pragma unroll
+        for (uint32_t k = 0; k < BIN_BATCH; ++k) {
+            const uint32_t t = t0 + k * blockDim.x;
+            h[k] = t < a.tiles ? hist[t] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < BIN_BATCH; ++k) rank[k] = h[k] ? atomicAdd(count + t0 + k * blockDim.x, h[k]) : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < BIN_BATCH; ++k) {
+            if (!h[k]) continue;
+            const uint32_t pos = list_base + atomicAdd(&list_pos, 1u);
+            if (pos < a.list_capacity) a.lists[pos] = make_uint2(t0 + k * blockDim.x, rank[k]);
+        }
     }
 }
 
@@ -359,7 +375,11 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
     uint32_t* cur = off + tiles + 1u;
     uint32_t* choff = cur + tiles;
     if (tid == 0) split = 0u;
-    for (uint32_t i = tid; i < BIN_MAX_TILES; i += 1024u) stage[i] = i < tiles ? cnt[i] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {  // all loads in flight at once
+        const uint32_t i = tid + 1024u * k;
+        stage[i] = i < tiles ? cnt[i] : 0u;
+    }
     __syncthreads();
     const uint32_t b = tid * PER;
     uint32_t v[PER];
@@ -463,33 +483,37 @@ __global__ __launch_bounds__(256) void k_bin_fill(BinArgs a) {
     extern __shared__ uint32_t hist[];
     const uint32_t* off = a.meta + BIN_META_WORDS + a.tiles;
     const uint2 wl = a.wg_list[blockIdx.x];
-    for (uint32_t i = threadIdx.x; i < wl.y; i += blockDim.x) {
-        const uint2 e = a.lists[wl.x + i];
-        hist[e.x] = off[e.x] + e.y;
+    for (uint32_t i0 = threadIdx.x; i0 < wl.y; i0 += BIN_BATCH * blockDim.x) {
+        uint2 e[BIN_BATCH];
+        uint32_t o[BIN_BATCH];
+#pragma unroll
+        for (uint32_t k = 0; k < BIN_BATCH; ++k) {
+            const uint32_t i = i0 + k * blockDim.x;
+            e[k] = i < wl.y ? a.lists[wl.x + i] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < BIN_BATCH; ++k) o[k] = i0 + k * blockDim.x < wl.y ? off[e[k].x] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < BIN_BATCH; ++k)
+            if (i0 + k * blockDim.x < wl.y) hist[e[k].x] = o[k] + e[k].y;
     }
     __syncthreads();
     uint32_t total = a.indirect[0];
     total = total < a.capacity ? total : a.capacity;
     const Run run = workgroup_run(total * BIN_STRIDE);
-    const uint4* boxes = reinterpret_cast<const uint4*>(a.seg_box);
-    const uint4 none = make_uint4(BIN_NONE, BIN_NONE, BIN_NONE, BIN_NONE);
     uint32_t* records = a.records;
-    for (uint32_t q0 = run.first + threadIdx.x; q0 < run.end; q0 += 4u * blockDim.x) {
-        uint4 bq[4];  // four loads in flight per thread
+    const uint32_t wend = run.end * 4u;
+    for (uint32_t r0 = run.first * 4u; r0 < wend; r0 += BIN_SEGS * blockDim.x) {
+        uint32_t bx[BIN_SEGS];  // all loads in flight
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; ++u) {
-            const uint32_t q = q0 + u * blockDim.x;
-            bq[u] = q < run.end ? boxes[q] : none;
+        for (uint32_t k = 0; k < BIN_SEGS; ++k) {
+            const uint32_t w = r0 + threadIdx.x + k * blockDim.x;
+            bx[k] = w < wend ? a.seg_box[w] : BIN_NONE;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; ++u) {
-            const uint32_t bx[4] = {bq[u].x, bq[u].y, bq[u].z, bq[u].w};
-            const uint32_t q = q0 + u * blockDim.x;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t id = q * 4u + (uint32_t)k;
-                for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { records[atomicAdd(hist + t, 1u)] = id; });
-            }
+        for (uint32_t k = 0; k < BIN_SEGS; ++k) {
+            const uint32_t id = r0 + threadIdx.x + k * blockDim.x;
+            for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { records[atomicAdd(hist + t, 1u)] = id; });
         }
     }
 }
