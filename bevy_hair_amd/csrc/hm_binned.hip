@@ -530,7 +530,10 @@ struct DomTileSink {
     // The walk is monotone in x and y: once past the tile in its direction of
     // travel it cannot come back.
     __device__ __forceinline__ bool walked_out(int32_t x, int32_t y, int32_t sx, int32_t sy) const {
-        return (sx > 0 ? x >= ox + T : x < ox) || (sy > 0 ? y >= oy + T : y < oy);
+        // bitwise, not short-circuit: selects rather than branches per step
+        const bool out_x = (sx > 0) ? (x >= ox + T) : (x < ox);
+        const bool out_y = (sy > 0) ? (y >= oy + T) : (y < oy);
+        return out_x | out_y;
     }
     __device__ __forceinline__ uint32_t at(int32_t x, int32_t y) const { return (uint32_t)((y - oy) * T + (x - ox)); }
     __device__ __forceinline__ void dom_min(size_t, int32_t x, int32_t y, uint32_t q, Counters&) {
