@@ -138,6 +138,8 @@ __device__ __forceinline__ void append_long(bool flag, uint32_t id, uint32_t* co
 constexpr uint32_t BIN_BATCH = 8u;
 // Segment boxes per thread per round of k_bin_count / k_bin_fill.
 constexpr uint32_t BIN_SEGS = 16u;
+// Records per thread a DOM tile keeps in registers between its two passes.
+constexpr uint32_t DOM_KEEP = 2u;
 
 // Workgroup b takes the contiguous run [first, end) of the n segment-box
 // words, in whole uint4 groups (n is a multiple of 4: 128 slots per strand);
@@ -547,11 +549,24 @@ struct DomTileSink {
 
 // One segment of the map from the stored control points (world position and
 // light-space projection of both ends, raster.wgsl:150-156).
+struct RecordEnds {
+    float4 c0, c1;  // world xyz, projected x
+    float y0, y1;   // projected y
+    bool first;     // segment 0 of its strand
+};
+
+__device__ __forceinline__ RecordEnds load_record(const BinArgs& a, uint32_t id) {
+    return {a.cps[id], a.cps[id + 1u], a.cpy[id], a.cpy[id + 1u], (id & (BIN_STRIDE - 1u)) == 0u};
+}
+
+template <class D>
+__device__ __forceinline__ void walk_ends(D& d, const RecordEnds& e) {
+    d.dom_walk(F3{e.c0.x, e.c0.y, e.c0.z}, F3{e.c1.x, e.c1.y, e.c1.z}, F2{e.c0.w, e.y0}, F2{e.c1.w, e.y1}, e.first);
+}
+
 template <class D>
 __device__ __forceinline__ void walk_record(D& d, const BinArgs& a, uint32_t id) {
-    const float4 c0 = a.cps[id], c1 = a.cps[id + 1u];
-    const float y0 = a.cpy[id], y1 = a.cpy[id + 1u];
-    d.dom_walk(F3{c0.x, c0.y, c0.z}, F3{c1.x, c1.y, c1.z}, F2{c0.w, y0}, F2{c1.w, y1}, (id & (BIN_STRIDE - 1u)) == 0u);
+    walk_ends(d, load_record(a, id));
 }
 
 template <int T>
@@ -576,9 +591,19 @@ __global__ __launch_bounds__(256) void k_dom_tiles(BinArgs a) {
     const uint32_t* off = a.meta + BIN_META_WORDS + a.tiles;
     const uint32_t begin = off[tile], end = off[tile + 1u];
     DomTileSink<T> sink{front, counts, ox, oy};
+    // Each thread's first DOM_KEEP records stay in registers for the opacity
+    // pass, which then reads only a crowded tile's further records again.
+    RecordEnds keep[DOM_KEEP];
     {
         Drawer<1, DomTileSink<T>> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
-        for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) walk_record(d, a, a.records[r]);
+        uint32_t r = begin + threadIdx.x;
+#pragma unroll
+        for (uint32_t j = 0; j < DOM_KEEP; ++j, r += 256u)
+            if (r < end) {
+                keep[j] = load_record(a, a.records[r]);
+                walk_ends(d, keep[j]);
+            }
+        for (; r < end; r += 256u) walk_record(d, a, a.records[r]);
     }
     __syncthreads();
     for (int i = (int)threadIdx.x; i < T * T; i += 256) {
@@ -587,7 +612,11 @@ __global__ __launch_bounds__(256) void k_dom_tiles(BinArgs a) {
     }
     {
         Drawer<2, DomTileSink<T>> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
-        for (uint32_t r = begin + threadIdx.x; r < end; r += 256u) walk_record(d, a, a.records[r]);
+        uint32_t r = begin + threadIdx.x;
+#pragma unroll
+        for (uint32_t j = 0; j < DOM_KEEP; ++j, r += 256u)
+            if (r < end) walk_ends(d, keep[j]);
+        for (; r < end; r += 256u) walk_record(d, a, a.records[r]);
     }
     __syncthreads();
     for (int i = (int)threadIdx.x; i < T * T * 4; i += 256) {
