@@ -360,31 +360,47 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
     bool use_color = !promote && cc.w > 0.0f;
     F3 accum = splat3(0.0f);
     float weight = 0.0f, h_center = 0.0f, h_cons = 0.0f;
+    // One row of 11 taps at a time: the row's G-buffer words, then the
+    // colours of the taps that pass, are loaded together (one round trip
+    // each) before the taps are summed in the reference's order.
+    constexpr int K = 2 * radius + 1;
     for (int dy = -radius; dy <= radius; ++dy) {
-        int32_t qy = y + dy;
-        if (qy < 0 || qy >= height) continue;
-        for (int dx = -radius; dx <= radius; ++dx) {
-            int32_t qx = x + dx;
-            if (qx < 0 || qx >= width) continue;
-            size_t qpix = (size_t)qy * (uint32_t)width + (uint32_t)qx;
-            F2 d = {i2f(dx), i2f(dy)};
-            float d_par = dot2(d, ts);
-            F2 perp = d - ts * d_par;
+        const int32_t qy = y + dy;
+        const bool row_in = qy >= 0 && qy < height;
+        const size_t row = (size_t)(row_in ? qy : y) * (uint32_t)width;
+        uint64_t qw[K];
+        bool in[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int32_t qx = x + j - radius;
+            in[j] = row_in && qx >= 0 && qx < width;
+            qw[j] = a.center[row + (uint32_t)(in[j] ? qx : x)];
+        }
+        float d_par[K], perp2[K];
+        bool use[K];
+        float4 qc[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const F2 d = {i2f(j - radius), i2f(dy)};
+            d_par[j] = dot2(d, ts);
+            const F2 perp = d - ts * d_par[j];
             // d_perp = sqrt(perp2); correctly rounded sqrt(x) <= 0.5 exactly when
             // x <= 0x3E800001 (0.25 + 2^-25)
-            float perp2 = dot2(perp, perp);
-            uint64_t q_center = a.center[qpix];
-            bool qc_empty = gb_empty(q_center);
-            if (perp2 <= __uint_as_float(0x3E800001u)) {
+            perp2[j] = dot2(perp, perp);
+            const bool qc_empty = gb_empty(qw[j]);
+            if (in[j] && perp2[j] <= __uint_as_float(0x3E800001u)) {
                 if (!qc_empty) h_center = h_center + 1.0f;
-                else if (!gb_empty(a.cons[qpix])) h_cons = h_cons + 1.0f;
+                else if (!gb_empty(a.cons[row + (uint32_t)(x + j - radius)])) h_cons = h_cons + 1.0f;
             }
-            if (qc_empty) continue;
-            const uint32_t qd = gb_depth(q_center);
-            if (qd < q_lo || qd > q_hi) continue;
-            float spatial = fexp(-d_par * d_par * inv_spar2 - perp2 * inv_sperp2);
-            float4 qc = a.shaded[qpix];
-            F3 q_color = {qc.x, qc.y, qc.z};
+            const uint32_t qd = gb_depth(qw[j]);
+            use[j] = in[j] && !qc_empty && qd >= q_lo && qd <= q_hi;
+            qc[j] = a.shaded[row + (uint32_t)(in[j] ? x + j - radius : x)];
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if (!use[j]) continue;
+            float spatial = fexp(-d_par[j] * d_par[j] * inv_spar2 - perp2[j] * inv_sperp2);
+            F3 q_color = {qc[j].x, qc[j].y, qc[j].z};
             float color_w = 1.0f;
             if (use_color) {
                 F3 dc = F3{cc.x, cc.y, cc.z} - q_color;
