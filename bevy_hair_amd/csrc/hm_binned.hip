@@ -10,8 +10,8 @@
 //                    atomic per (workgroup, tile)
 //   k_bin_scan       exclusive scan of the counts; gate words
 //   k_bin_fill       segment ids into per-tile record lists (same histogram,
-//                    one global reservation per (workgroup, tile))
-//   k_fill32         DOM fronts to empty              (only with long segments)
+//                    one global reservation per (workgroup, tile)); then the
+//                    DOM fronts to empty             (only with long segments)
 //   k_dom_long<1>    long segments, global atomic min  (same)
 //   k_dom_tiles<T>   per tile: depth pass (LDS min), front to HBM, opacity
 //                    pass (LDS add), counts to HBM: the whole DOM is written,
@@ -518,6 +518,30 @@ __global__ __launch_bounds__(256) void k_bin_fill(BinArgs a) {
             for_box_tiles(bx[k], a.tiles_x, [&](uint32_t t) { records[atomicAdd(hist + t, 1u)] = id; });
         }
     }
+    // The clear the next kernels need, gated as they are: split camera tiles
+    // take atomic minima into a cleared G-buffer, long DOM segments into
+    // fronts set to the empty depth (no launch of its own when not needed).
+    uint32_t* dst = nullptr;
+    size_t words = 0;
+    uint32_t value = 0u;
+    if (a.fill_clear == 1u && a.meta[BIN_SPLIT] != 0u) {
+        dst = a.gb_slab;
+        words = (size_t)a.gb_words;
+        value = 0xffffffffu;
+    } else if (a.fill_clear == 2u && a.meta[BIN_LONG] != 0u) {
+        dst = a.dom;
+        words = (size_t)a.dom_size * a.dom_size;
+        value = HM_DOM_EMPTY;
+    }
+    if (dst) {
+        const size_t quads = words / 4u;  // both buffers are 16-byte aligned
+        const uint4 v4 = make_uint4(value, value, value, value);
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < quads; i += (size_t)gridDim.x * blockDim.x)
+            reinterpret_cast<uint4*>(dst)[i] = v4;
+        for (size_t i = quads * 4u + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+             i += (size_t)gridDim.x * blockDim.x)
+            dst[i] = value;
+    }
 }
 
 // LDS tile of the map: fronts and 16 counts per texel.
@@ -656,9 +680,8 @@ hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks,
 }
 
 hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks) {
-    const size_t texels = (size_t)a.dom_size * a.dom_size;
-    hipError_t e = launch_fill32(s, a.dom, texels, HM_DOM_EMPTY, long_blocks, a.meta + BIN_LONG);
-    if (e != hipSuccess) return e;
+    // (the fronts were set to empty by k_bin_fill when long segments exist)
+    hipError_t e = hipSuccess;
     hipLaunchKernelGGL(k_dom_long<1>, dim3(long_blocks), dim3(256), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.tile_shift == 4u) {
@@ -869,9 +892,9 @@ __global__ __launch_bounds__(256) void k_cam_long(BinArgs a) {
 
 hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int tile_blocks, int long_blocks) {
     if (a.tile_shift != 4u && a.tile_shift != 5u) return hipErrorInvalidValue;
-    // Split tiles take atomic minima into the G-buffer: clear it first (only then).
-    hipError_t e = launch_fill32(s, a.gb_slab, (size_t)a.gb_words, 0xffffffffu, long_blocks, a.meta + BIN_SPLIT);
-    if (e != hipSuccess) return e;
+    // Split tiles take atomic minima into the G-buffer, which k_bin_fill
+    // has cleared when a split exists.
+    hipError_t e = hipSuccess;
     const uint32_t grid = a.chunk_capacity < (uint32_t)tile_blocks ? a.chunk_capacity : (uint32_t)tile_blocks;
     if (a.tile_shift == 4u) {
         hipLaunchKernelGGL(k_cam_tiles<16>, dim3(grid), dim3(256), 0, s, a);

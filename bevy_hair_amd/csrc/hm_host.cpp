@@ -880,6 +880,7 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     b.chunk_capacity = (uint32_t)std::min<size_t>(chunks, 0xffffffffu);
     b.gb_slab = static_cast<uint32_t*>(ctx->gb_slab);
     b.gb_words = ctx->gb_words;
+    b.fill_clear = pass == HM_PASS_CAMERA ? 1u : 2u;
     b.long_list = ctx->bin.long_list;
     b.long_capacity = (uint32_t)std::min<size_t>(slots * hm::BIN_STRIDE, 0xffffffffu);
     b.tiles_x = tiles_x;

@@ -107,6 +107,7 @@ struct BinArgs {
     uint32_t chunk_capacity;
     uint32_t* gb_slab;              // camera: centre | conservative | beta words, cleared when tiles split
     uint64_t gb_words;
+    uint32_t fill_clear;            // k_bin_fill also clears: 1 the G-buffer if tiles split, 2 the DOM fronts if long
     const hm_strand_ref* reuse_refs;  // DOM: camera work list whose control points cps still holds, or null
     const uint32_t* reuse_indirect;
     uint32_t* long_list;            // segments whose box spans more than box_limit tiles
