@@ -140,7 +140,8 @@ BUF_DOM_LOD, BUF_DOM_REFS, BUF_DOM_INDIRECT = 19, 20, 21
 BUF_CAM_BIN_STATUS, BUF_DOM_BIN_STATUS = 24, 25
 PASS_CAMERA, PASS_DOM = 0, 1
 KERNELS = ["clear", "cam_lod", "cam_scan", "cam_raster", "clear_dom", "dom_lod", "dom_scan",
-           "dom_depth", "dom_opacity", "shade", "filter", "dom_bin", "dom_tiles", "cam_bin", "cam_tiles"]
+           "dom_depth", "dom_opacity", "shade", "filter", "dom_bin", "dom_tiles", "cam_bin", "cam_tiles",
+           "cam_finish", "dom_long_depth", "dom_long_opacity"]
 
 # The exported C ABI: name -> (restype, argtypes). Every symbol include/hair_mi.h declares.
 P = C.c_void_p

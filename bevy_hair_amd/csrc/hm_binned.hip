@@ -679,11 +679,14 @@ hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks,
     return hipGetLastError();
 }
 
-hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks) {
-    // (the fronts were set to empty by k_bin_fill when long segments exist)
-    hipError_t e = hipSuccess;
-    hipLaunchKernelGGL(k_dom_long<1>, dim3(long_blocks), dim3(256), 0, s, a);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+hipError_t launch_dom_long(hipStream_t s, const BinArgs& a, int mode, int long_blocks) {
+    // (before the tiles, k_bin_fill has set the fronts to empty when long segments exist)
+    if (mode == 1) hipLaunchKernelGGL(k_dom_long<1>, dim3(long_blocks), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_dom_long<2>, dim3(long_blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a) {
     if (a.tile_shift == 4u) {
         hipLaunchKernelGGL(k_dom_tiles<16>, dim3(a.tiles), dim3(256), 0, s, a);
     } else if (a.tile_shift == 5u) {
@@ -691,8 +694,6 @@ hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks) {
     } else {
         return hipErrorInvalidValue;
     }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dom_long<2>, dim3(long_blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -890,20 +891,22 @@ __global__ __launch_bounds__(256) void k_cam_long(BinArgs a) {
         walk_cam_record(d, a, a.long_list[i]);
 }
 
-hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int tile_blocks, int long_blocks) {
+hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int tile_blocks) {
     if (a.tile_shift != 4u && a.tile_shift != 5u) return hipErrorInvalidValue;
     // Split tiles take atomic minima into the G-buffer, which k_bin_fill
     // has cleared when a split exists.
-    hipError_t e = hipSuccess;
     const uint32_t grid = a.chunk_capacity < (uint32_t)tile_blocks ? a.chunk_capacity : (uint32_t)tile_blocks;
-    if (a.tile_shift == 4u) {
-        hipLaunchKernelGGL(k_cam_tiles<16>, dim3(grid), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_cam_finish<16>, dim3(a.tiles), dim3(256), 0, s, a);
-    } else {
-        hipLaunchKernelGGL(k_cam_tiles<32>, dim3(grid), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_cam_finish<32>, dim3(a.tiles), dim3(256), 0, s, a);
-    }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (a.tile_shift == 4u) hipLaunchKernelGGL(k_cam_tiles<16>, dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_cam_tiles<32>, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cam_finish(hipStream_t s, const BinArgs& a, int long_blocks) {
+    if (a.tile_shift == 4u) hipLaunchKernelGGL(k_cam_finish<16>, dim3(a.tiles), dim3(256), 0, s, a);
+    else if (a.tile_shift == 5u) hipLaunchKernelGGL(k_cam_finish<32>, dim3(a.tiles), dim3(256), 0, s, a);
+    else return hipErrorInvalidValue;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cam_long, dim3(long_blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }

@@ -943,7 +943,11 @@ static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool
         // One workgroup per chunk for up to 2 records per segment slot.
         const uint64_t usual = (uint64_t)b.tiles + (b.chunk ? (uint64_t)g->capacity * 256u / b.chunk : 0u);
         const int tile_blocks = (int)std::min<uint64_t>(std::max<uint64_t>(usual, 1u), 1u << 20);
-        HM_HIP(ctx, hm::launch_cam_tiles(ctx->stream, b, tile_blocks, ctx->cu_count));
+        HM_HIP(ctx, hm::launch_cam_tiles(ctx->stream, b, tile_blocks));
+    }
+    {
+        Timer t(ctx, HM_K_CAM_FINISH);
+        HM_HIP(ctx, hm::launch_cam_finish(ctx->stream, b, ctx->cu_count));
     }
     const uint32_t* fallback = ctx->bin.meta[HM_PASS_CAMERA] + hm::BIN_FALLBACK;
     if (emit_lists && lists) {
@@ -1006,8 +1010,16 @@ static hm_status dom_impl(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const h
         HM_HIP(ctx, hm::launch_bin(ctx->stream, b, 1, raster_blocks(ctx, g->capacity, 1u), bin_blocks));
     }
     {
+        Timer t(ctx, HM_K_DOM_LONG_DEPTH);
+        HM_HIP(ctx, hm::launch_dom_long(ctx->stream, b, 1, ctx->cu_count));
+    }
+    {
         Timer t(ctx, HM_K_DOM_TILES);
-        HM_HIP(ctx, hm::launch_dom_tiles(ctx->stream, b, ctx->cu_count));
+        HM_HIP(ctx, hm::launch_dom_tiles(ctx->stream, b));
+    }
+    {
+        Timer t(ctx, HM_K_DOM_LONG_OPACITY);
+        HM_HIP(ctx, hm::launch_dom_long(ctx->stream, b, 2, ctx->cu_count));
     }
     if (!b.can_overflow) return HM_OK;
     // Direct kernels, switched on by the device only when the records overflowed.

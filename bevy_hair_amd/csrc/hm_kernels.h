@@ -138,8 +138,10 @@ struct BinArgs {
 };
 
 hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks);
-hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int long_blocks);
-hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int tile_blocks, int long_blocks);
+hipError_t launch_dom_long(hipStream_t s, const BinArgs& a, int mode, int long_blocks);  // mode 1 before, 2 after the tiles
+hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a);
+hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int tile_blocks);
+hipError_t launch_cam_finish(hipStream_t s, const BinArgs& a, int long_blocks);  // split tiles' lists, long segments
 
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes, int blocks);
 // Copies `words` 32-bit words (a graph-safe device copy).

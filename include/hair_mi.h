@@ -212,10 +212,13 @@ typedef enum hm_kernel {
     HM_K_SHADE = 9,
     HM_K_FILTER = 10,
     HM_K_DOM_BIN = 11,          /* binned DOM: control points, tile counts, scan, records */
-    HM_K_DOM_TILES = 12,        /* binned DOM: both passes per tile (+ long segments) */
+    HM_K_DOM_TILES = 12,        /* binned DOM: the tile kernel (both passes per tile) */
     HM_K_CAM_BIN = 13,          /* binned camera pass: control points, tile counts, scan, records */
-    HM_K_CAM_TILES = 14,        /* binned camera pass: per tile G-buffer (+ long segments) */
-    HM_K_COUNT = 15
+    HM_K_CAM_TILES = 14,        /* binned camera pass: the tile kernel (per tile G-buffer) */
+    HM_K_CAM_FINISH = 15,       /* binned camera pass: pixel lists of split tiles, long segments */
+    HM_K_DOM_LONG_DEPTH = 16,   /* binned DOM: long segments, depth pass (before the tiles) */
+    HM_K_DOM_LONG_OPACITY = 17, /* binned DOM: long segments, opacity pass (after the tiles) */
+    HM_K_COUNT = 18
 } hm_kernel;
 
 #define HM_SHADING_PLAIN 0u
