@@ -110,13 +110,20 @@ __device__ __forceinline__ uint32_t camera_box(const View& p, F2 a, F2 b, uint32
     return tx0 | (ty0 << 10) | ((bw - 1u) << 20) | ((bh - 1u) << 26);
 }
 
-// Calls f(tile) for every tile of a box, row by row.
+// Calls f(tile) for every tile of a box, row by row. Most boxes are one
+// tile: that case takes no loop.
 template <class F>
 __device__ __forceinline__ void for_box_tiles(uint32_t box, uint32_t tiles_x, F f) {
     if (box == BIN_NONE) return;
-    const uint32_t bw = ((box >> 20) & 63u) + 1u, bh = ((box >> 26) & 63u) + 1u;
     uint32_t row = ((box >> 10) & 1023u) * tiles_x + (box & 1023u);
+    if ((box >> 20) == 0u) {
+        f(row);
+        return;
+    }
+    const uint32_t bw = ((box >> 20) & 63u) + 1u, bh = ((box >> 26) & 63u) + 1u;
+#pragma nounroll
     for (uint32_t y = 0; y < bh; ++y, row += tiles_x)
+#pragma nounroll
         for (uint32_t x = 0; x < bw; ++x) f(row + x);
 }
 
