@@ -228,14 +228,21 @@ __device__ __forceinline__ void gen_pair(const BinArgs& a, const View& p, uint32
     for (int j = 0; j < S; ++j) {
         const uint32_t idx = (uint32_t)r + 32u * (uint32_t)j;
         pr[j] = project_world(p, cw[j]);
-        if (idx < ncp) {
+        if (idx < ncp && MODE == 1) {
+            // a segment's two end points are 40 contiguous bytes for the DOM tiles
+            float* d = a.dcp + (base + idx) * 5u;
+            d[0] = cw[j].x;
+            d[1] = cw[j].y;
+            d[2] = cw[j].z;
+            d[3] = pr[j].screen.x;
+            d[4] = pr[j].screen.y;
+        }
+        if (idx < ncp && MODE == 0) {
             a.cps[base + idx] = make_float4(cw[j].x, cw[j].y, cw[j].z, pr[j].screen.x);
             a.cpy[base + idx] = pr[j].screen.y;
-            if (MODE == 0) {
-                // raster.wgsl:319-337: world tangents, ao; uv and beta per strand.
-                const F3 wt = dir_world(p, ct[j]);
-                a.cpt[base + idx] = make_float4(wt.x, wt.y, wt.z, ao[j]);
-            }
+            // raster.wgsl:319-337: world tangents, ao; uv and beta per strand.
+            const F3 wt = dir_world(p, ct[j]);
+            a.cpt[base + idx] = make_float4(wt.x, wt.y, wt.z, ao[j]);
         }
     }
     if (MODE == 0 && r == 0 && live) a.sinfo[s] = make_float4(L.uv.x, L.uv.y, L.beta, __uint_as_float(ncp));
@@ -587,7 +594,9 @@ struct RecordEnds {
 };
 
 __device__ __forceinline__ RecordEnds load_record(const BinArgs& a, uint32_t id) {
-    return {a.cps[id], a.cps[id + 1u], a.cpy[id], a.cpy[id + 1u], (id & (BIN_STRIDE - 1u)) == 0u};
+    const float* d = a.dcp + (size_t)id * 5u;
+    return {make_float4(d[0], d[1], d[2], d[3]), make_float4(d[5], d[6], d[7], d[8]), d[4], d[9],
+            (id & (BIN_STRIDE - 1u)) == 0u};
 }
 
 template <class D>

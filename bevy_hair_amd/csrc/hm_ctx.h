@@ -24,6 +24,7 @@ struct hm_groom {
 struct hm_bin_state {
     float4* cps = nullptr;          // [slot * 128 + idx]: world xyz, light-space x
     float* cpy = nullptr;           // light-space (or screen) y
+    float* dcp = nullptr;           // DOM: 5 floats per [slot * 128 + idx]: world xyz, light-space x, y
     float4* cpt = nullptr;          // camera: world tangent xyz, ao
     float4* sinfo = nullptr;        // camera: per strand uv, beta, control points
     uint32_t* seg_box = nullptr;    // per segment id

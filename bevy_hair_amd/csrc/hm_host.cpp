@@ -288,6 +288,7 @@ void free_bin(hm_ctx* c) {
     hm_bin_state& b = c->bin;
     dev_free(b.cps);
     dev_free(b.cpy);
+    dev_free(b.dcp);
     dev_free(b.cpt);
     dev_free(b.sinfo);
     dev_free(b.seg_box);
@@ -317,6 +318,7 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
     if (slots > b.slots) {
         dev_free(b.cps);
         dev_free(b.cpy);
+        dev_free(b.dcp);
         dev_free(b.cpt);
         dev_free(b.sinfo);
         dev_free(b.seg_box);
@@ -325,6 +327,7 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
         size_t segs = slots * hm::BIN_STRIDE;
         if ((e = dev_alloc(&b.cps, segs * sizeof(float4))) != hipSuccess) return e;
         if ((e = dev_alloc(&b.cpy, segs * sizeof(float))) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.dcp, segs * 5 * sizeof(float))) != hipSuccess) return e;
         if ((e = dev_alloc(&b.cpt, segs * sizeof(float4))) != hipSuccess) return e;
         if ((e = dev_alloc(&b.sinfo, slots * sizeof(float4))) != hipSuccess) return e;
         if ((e = dev_alloc(&b.seg_box, segs * 4)) != hipSuccess) return e;
@@ -867,6 +870,7 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     b.capacity = g->capacity;
     b.cps = ctx->bin.cps;
     b.cpy = ctx->bin.cpy;
+    b.dcp = ctx->bin.dcp;
     b.seg_box = ctx->bin.seg_box;
     b.meta = ctx->bin.meta[pass];
     b.records = ctx->bin.records;

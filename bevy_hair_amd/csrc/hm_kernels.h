@@ -94,6 +94,7 @@ struct BinArgs {
     uint32_t capacity;              // refs length
     float4* cps;                    // per control point [slot * BIN_STRIDE + idx]: world xyz, light-space x
     float* cpy;                     // light-space y
+    float* dcp;                     // DOM pass: 5 floats per control point (world xyz, light-space x, y)
     uint32_t* seg_box;              // per segment id: tile box or BIN_NONE
     uint32_t* meta;
     uint32_t* records;              // segment ids grouped by tile
