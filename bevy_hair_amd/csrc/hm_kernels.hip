@@ -360,25 +360,27 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
     bool use_color = !promote && cc.w > 0.0f;
     F3 accum = splat3(0.0f);
     float weight = 0.0f, h_center = 0.0f, h_cons = 0.0f;
-    // One row of 11 taps at a time: the row's G-buffer words, then the
-    // colours of the taps that pass, are loaded together (one round trip
-    // each) before the taps are summed in the reference's order.
+    // One row of 11 taps at a time: the row's G-buffer words and colours
+    // are loaded together (one round trip) before the taps are summed in the
+    // reference's order.
     constexpr int K = 2 * radius + 1;
     for (int dy = -radius; dy <= radius; ++dy) {
         const int32_t qy = y + dy;
         const bool row_in = qy >= 0 && qy < height;
         const size_t row = (size_t)(row_in ? qy : y) * (uint32_t)width;
         uint64_t qw[K];
+        float4 qc[K];
         bool in[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const int32_t qx = x + j - radius;
             in[j] = row_in && qx >= 0 && qx < width;
-            qw[j] = a.center[row + (uint32_t)(in[j] ? qx : x)];
+            const size_t q = row + (uint32_t)(in[j] ? qx : x);
+            qw[j] = a.center[q];
+            qc[j] = a.shaded[q];
         }
         float d_par[K], perp2[K];
         bool use[K];
-        float4 qc[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const F2 d = {i2f(j - radius), i2f(dy)};
@@ -394,7 +396,6 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
             }
             const uint32_t qd = gb_depth(qw[j]);
             use[j] = in[j] && !qc_empty && qd >= q_lo && qd <= q_hi;
-            qc[j] = a.shaded[row + (uint32_t)(in[j] ? x + j - radius : x)];
         }
 #pragma unroll
         for (int j = 0; j < K; ++j) {
