@@ -386,6 +386,23 @@ __device__ __forceinline__ F3 key_multiple_scattering(float theta_h, float depth
     return ms;
 }
 
+// Light i of a light array the kernel only reads, loaded as constant data at
+// a wave-uniform index (scalar loads).
+template <class T>
+__device__ __forceinline__ T uniform_light(const T* lights, uint32_t i) {
+    static_assert(sizeof(T) % 16 == 0, "lights are whole 16-byte vectors");
+    constexpr uint32_t V = sizeof(T) / 16;
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    typedef const __attribute__((address_space(4))) u32x4* const_vec;
+    const const_vec c = (const_vec)lights + (size_t)__builtin_amdgcn_readfirstlane(i) * V;
+    struct Words {
+        u32x4 v[V];
+    } w;
+#pragma unroll
+    for (uint32_t k = 0; k < V; ++k) w.v[k] = c[k];
+    return __builtin_bit_cast(T, w);
+}
+
 // Point light L's contribution (shade.wesl:449-505) with back-lobe terms
 // `lb`; zero where the light's range does not reach the pixel, as the
 // cluster would not list the light.
@@ -420,13 +437,15 @@ __device__ F3 pixel_color(const ShadeArgs& a, const PixelShade& px, const LocalB
     F3 color = splat3(0.0f);
     for (int pass = 0; pass < 2; ++pass) {
         for (uint32_t i = 0; i < a.point_count; ++i) {
-            const hm_point_light& L = a.points[i];
+            // read-only, wave-uniform: through the scalar cache in one load
+            // rather than as dependent vector loads
+            const hm_point_light L = uniform_light(a.points, i);
             if ((pass == 0) != (L.kind == HM_LIGHT_POINT)) continue;
             color = color + point_term(L, px, lb);
         }
     }
     for (uint32_t i = 0; i < a.directional_count; ++i)
-        color = color + directional_term(a.directionals[i], (int)i == key, px, lb, depth, key_ms);
+        color = color + directional_term(uniform_light(a.directionals, i), (int)i == key, px, lb, depth, key_ms);
     return color * a.exposure;
 }
 
