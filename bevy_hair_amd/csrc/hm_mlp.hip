@@ -606,13 +606,14 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
         // after the network without reading them again
         uint64_t word = 0;
         uint32_t beta_word = 0;
+        PixelShade px;  // kept across the network for the light loop
         {
             float f[32];
             for (int k = 0; k < 32; ++k) f[k] = 0.0f;
             if (valid) {
                 word = a.center[pix];
                 beta_word = a.beta[pix];
-                const PixelShade px = pixel_setup(p, word, beta_word, pix % width, pix / width);
+                px = pixel_setup(p, word, beta_word, pix % width, pix / width);
                 depth = mlp_features(p, px, kd, a.dom, f);
                 for (int k = 9; k < MLP_NFEAT; ++k) outside |= f[k] < s_lo[k] || f[k] > s_hi[k];
             }
@@ -675,9 +676,7 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
             }
         }
         F3 c = splat3(0.0f);
-        PixelShade px;
         if (valid) {
-            px = pixel_setup(p, word, beta_word, pix % width, pix / width);
             // the network's dual-scattering terms, or the plain method's
             // (plain_pixel_color; `depth` is its key-light depth)
             const DualScatter ds = (flag == 2u || outside) ? dual_scatter(px.cos_theta_o, px.sigma_a, px.shape)
