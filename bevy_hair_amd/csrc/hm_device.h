@@ -153,6 +153,37 @@ __device__ __forceinline__ void oct_encode(F3 n_in, uint32_t& ox, uint32_t& oy) 
     oy = to_u32(ey * 255.0f + 0.5f);
 }
 
+// oct_encode(safe_normalize(v)) through the hardware reciprocal square root
+// and reciprocal (each within 1 ulp) instead of the IEEE divides and roots.
+// Against the exact chain, the folded coordinates move by < 36 ulps of 1 and
+// the codes' pre-truncation values ex * 255 + 0.5 by < 3.2e-4; when both lie
+// farther than 2^-10 from an integer their truncations are the exact codes,
+// and the function returns true. Otherwise (or for lengths near zero or
+// huge, or components near zero whose sign decides the fold) it returns
+// false and the caller takes the exact path.
+__device__ __forceinline__ bool oct_encode_fast(F3 v, uint32_t& ox, uint32_t& oy) {
+    const float l2 = dot3(v, v);
+    if (!(l2 >= 1e-9f && l2 <= 1e30f)) return false;  // safe_normalize's 1e-10 cut-off, NaN
+    const F3 n = v * __builtin_amdgcn_rsqf(l2);
+    const float ax = fabsf(n.x), ay = fabsf(n.y);
+    if (!(ax >= 0x1p-60f && ay >= 0x1p-60f)) return false;
+    const float r = __builtin_amdgcn_rcpf(ax + ay + fabsf(n.z));
+    float px = n.x * r, py = n.y * r;
+    if (n.z < 0.0f) {
+        const float qx = (1.0f - fabsf(py)) * (px >= 0.0f ? 1.0f : -1.0f);
+        const float qy = (1.0f - fabsf(px)) * (py >= 0.0f ? 1.0f : -1.0f);
+        px = qx;
+        py = qy;
+    }
+    const float dx = clampf(px * 0.5f + 0.5f, 0.0f, 1.0f) * 255.0f + 0.5f;
+    const float dy = clampf(py * 0.5f + 0.5f, 0.0f, 1.0f) * 255.0f + 0.5f;
+    const float fx = dx - floorf(dx), fy = dy - floorf(dy);
+    if (!(fx >= 0x1p-10f && fx <= 1.0f - 0x1p-10f && fy >= 0x1p-10f && fy <= 1.0f - 0x1p-10f)) return false;
+    ox = (uint32_t)dx;
+    oy = (uint32_t)dy;
+    return true;
+}
+
 __device__ __forceinline__ F3 oct_decode(uint32_t tx, uint32_t ty) {
     float ex = rdiv(u2f(tx), 255.0f), ey = rdiv(u2f(ty), 255.0f);
     float fx = ex * 2.0f - 1.0f, fy = ey * 2.0f - 1.0f;

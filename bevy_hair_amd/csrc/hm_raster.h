@@ -280,7 +280,7 @@ struct Drawer {
                 else if (visible)
                     clip_depth = rdiv(cz, cw);
                 if (visible) {
-                    F3 tangent = safe_normalize(mix3(t0, t1, t));
+                    const F3 tangent_raw = mix3(t0, t1, t);
                     uint32_t qu = qu_end, qv = qv_end;
                     if (!uv_fixed) {
                         qu = quant63(mixs(uvw0.x, uvw1.x, t));
@@ -290,7 +290,7 @@ struct Drawer {
                     float ao = mixs(ao0, ao1, t);
                     uint32_t depth = quantize_depth(vd, p.screen.z, p.screen.w);
                     uint32_t ox, oy;
-                    oct_encode(tangent, ox, oy);
+                    if (!oct_encode_fast(tangent_raw, ox, oy)) oct_encode(safe_normalize(tangent_raw), ox, oy);
                     uint32_t ao_q = to_u32(clampf(ao, 0.0f, 1.0f) * 63.0f + 0.5f);
                     if (p.flags[2] == 0u) ao_q = 63u;
                     uint64_t payload = pack_gb(depth, ox, oy, qu, qv, quant63(uvw_z), ao_q);
