@@ -391,10 +391,18 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
     uint32_t* cur = off + tiles + 1u;
     uint32_t* choff = cur + tiles;
     if (tid == 0) split = 0u;
+    // All loads in flight at once: every lane loads (a clamped index, so no
+    // load sits behind a branch and its own wait), then masks.
+    uint32_t ld[PER];
 #pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {  // all loads in flight at once
+    for (uint32_t k = 0; k < PER; ++k) {
         const uint32_t i = tid + 1024u * k;
-        stage[i] = i < tiles ? cnt[i] : 0u;
+        ld[k] = cnt[i < tiles ? i : 0u];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t i = tid + 1024u * k;
+        stage[i] = i < tiles ? ld[k] : 0u;
     }
     __syncthreads();
     const uint32_t b = tid * PER;
