@@ -383,8 +383,12 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
                                                    uint32_t long_capacity, uint32_t list_capacity, uint32_t chunk,
                                                    uint32_t* chunk_tile, uint32_t chunk_capacity) {
     constexpr uint32_t PER = BIN_MAX_TILES / 1024u;
-    __shared__ __align__(16) uint32_t stage[BIN_MAX_TILES];
+    // Tile i sits at word i + i / PER: a thread's run of PER tiles starts
+    // PER + 1 words after its neighbour's, so the run-wise accesses below hit
+    // distinct banks, as do the tile-wise ones.
+    __shared__ uint32_t stage[BIN_MAX_TILES + BIN_MAX_TILES / PER];
     __shared__ uint32_t wave_total[16], wave_chunks[16], split, ok_s;
+    auto pad = [](uint32_t i) { return i + i / PER; };
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t* cnt = meta + BIN_META_WORDS;
     uint32_t* off = meta + BIN_META_WORDS + tiles;
@@ -402,19 +406,13 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
         const uint32_t i = tid + 1024u * k;
-        stage[i] = i < tiles ? ld[k] : 0u;
+        stage[pad(i)] = i < tiles ? ld[k] : 0u;
     }
     __syncthreads();
     const uint32_t b = tid * PER;
     uint32_t v[PER];
 #pragma unroll
-    for (uint32_t q = 0; q < PER / 4u; ++q) {
-        const uint4 w = reinterpret_cast<const uint4*>(stage)[tid * (PER / 4u) + q];
-        v[4 * q] = w.x;
-        v[4 * q + 1] = w.y;
-        v[4 * q + 2] = w.z;
-        v[4 * q + 3] = w.w;
-    }
+    for (uint32_t k = 0; k < PER; ++k) v[k] = stage[pad(b + k)];
     uint32_t sum = 0u, csum = 0u, nch[PER];
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
@@ -446,14 +444,14 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
     bool any_split = false;
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
-        stage[b + k] = r;
+        stage[pad(b + k)] = r;
         r += v[k];
         any_split |= nch[k] > 1u;
     }
     if (any_split) split = 1u;
     __syncthreads();
     for (uint32_t i = tid; i < tiles; i += 1024u) {
-        const uint32_t o = stage[i];
+        const uint32_t o = stage[pad(i)];
         off[i] = o;
         cur[i] = o;
     }
@@ -462,7 +460,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
     // search over them (coalesced chunk_tile stores).
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
-        stage[b + k] = cr;
+        stage[pad(b + k)] = cr;
         cr += nch[k];
     }
     __syncthreads();
@@ -470,13 +468,13 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
                             wave_chunks[5] + wave_chunks[6] + wave_chunks[7] + wave_chunks[8] + wave_chunks[9] +
                             wave_chunks[10] + wave_chunks[11] + wave_chunks[12] + wave_chunks[13] + wave_chunks[14] +
                             wave_chunks[15];
-    for (uint32_t i = tid; i < tiles; i += 1024u) choff[i] = stage[i];
+    for (uint32_t i = tid; i < tiles; i += 1024u) choff[i] = stage[pad(i)];
     const uint32_t nmap = !chunk_tile ? 0u : (chunks < chunk_capacity ? chunks : chunk_capacity);
     for (uint32_t c = tid; c < nmap; c += 1024u) {
         uint32_t lo = 0u, hi = tiles;  // last tile whose first chunk <= c
         while (hi - lo > 1u) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (stage[mid] <= c) lo = mid;
+            if (stage[pad(mid)] <= c) lo = mid;
             else hi = mid;
         }
         chunk_tile[c] = lo;
