@@ -3,12 +3,13 @@
 // pass_mode.x = 1 and then 2 by pass.rs:560-583) are evaluated per light-space
 // tile in LDS, and each tile's finished texels are written to HBM once.
 //
-//   k_fill32         zero the status words and per-tile counts
-//   k_bin_gen        one wave per strand: control points (gen_strand) and
-//                    their light-space projection to HBM, each segment's tile box
+//   k_bin_gen        zero the status words and per-tile counts; two strands
+//                    per wave: control points and their projection to HBM,
+//                    each segment's tile box, long segments to a list
 //   k_bin_count      per-tile counts: LDS histogram per workgroup, one global
 //                    atomic per (workgroup, tile)
-//   k_bin_scan       exclusive scan of the counts; gate words
+//   k_bin_scan       exclusive scan of the counts; gate words; the long-segment
+//                    count to its status word, the append cursor back to zero
 //   k_bin_fill       segment ids into per-tile record lists (same histogram,
 //                    one global reservation per (workgroup, tile)); then the
 //                    DOM fronts to empty             (only with long segments)
@@ -272,7 +273,7 @@ __device__ __forceinline__ void gen_pair(const BinArgs& a, const View& p, uint32
         // Every slot of the strand is written, so the binning passes can walk
         // segment ids without the strand's control-point count.
         if (live) a.seg_box[id] = box;
-        append_long(is_long, id, a.meta + BIN_LONG_COUNT, a.long_list, a.long_capacity);
+        append_long(is_long, id, a.meta + BIN_LONG_APPEND, a.long_list, a.long_capacity);
     }
 #pragma unroll
     for (int j = S; j < (int)(BIN_STRIDE / 32u); ++j)
@@ -282,6 +283,11 @@ __device__ __forceinline__ void gen_pair(const BinArgs& a, const View& p, uint32
 template <int MODE>
 __global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
     const View& p = bin_view(a);
+    // Status words and tile counts to zero for this pass's k_bin_count and
+    // k_bin_scan; the long-segment cursor, the only word written here, is
+    // already zero.
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < BIN_META_WORDS + a.tiles; i += gridDim.x * blockDim.x)
+        if (i != BIN_LONG_APPEND) a.meta[i] = 0u;
     const int lane = (int)(threadIdx.x & 63u), r = lane & 31, gb = lane & 32;
     uint32_t total = a.indirect[0];
     total = total < a.capacity ? total : a.capacity;
@@ -483,7 +489,9 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
         const uint32_t total = r;
         off[tiles] = total;
         choff[tiles] = chunks;
-        const uint32_t longs = meta[BIN_LONG_COUNT];
+        const uint32_t longs = meta[BIN_LONG_APPEND];
+        meta[BIN_LONG_COUNT] = longs;
+        meta[BIN_LONG_APPEND] = 0u;
         const uint32_t ok = (total <= record_capacity && longs <= long_capacity &&
                              meta[BIN_LIST_COUNT] <= list_capacity && chunks <= chunk_capacity)
                                 ? 1u
@@ -686,8 +694,7 @@ __global__ __launch_bounds__(256) void k_dom_long(BinArgs a) {
 
 hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks) {
     if (a.tiles > BIN_MAX_TILES) return hipErrorInvalidValue;
-    hipError_t e = launch_fill32(s, a.meta, BIN_META_WORDS + a.tiles, 0u, 64);
-    if (e != hipSuccess) return e;
+    hipError_t e;
     if (mode == 0) hipLaunchKernelGGL(k_bin_gen<0>, dim3(gen_blocks), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_bin_gen<1>, dim3(gen_blocks), dim3(256), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;

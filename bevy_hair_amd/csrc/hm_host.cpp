@@ -338,6 +338,9 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
         dev_free(b.meta[pass]);
         b.meta_words[pass] = 0;
         if ((e = dev_alloc(&b.meta[pass], meta_words * 4)) != hipSuccess) return e;
+        // k_bin_gen expects the long-segment cursor at zero (k_bin_scan
+        // re-zeroes it after every pass).
+        if ((e = hipMemsetAsync(b.meta[pass], 0, meta_words * 4, c->stream)) != hipSuccess) return e;
         b.meta_words[pass] = meta_words;
     }
     if (records > b.record_capacity) {

@@ -78,6 +78,10 @@ constexpr uint32_t BIN_LONG = 3u;        // binned and long segments exist
 constexpr uint32_t BIN_LONG_COUNT = 4u;
 constexpr uint32_t BIN_LIST_COUNT = 5u;  // (tile, rank) entries the counting workgroups wrote
 constexpr uint32_t BIN_SPLIT = 8u;       // binned and some tile has more than one chunk
+constexpr uint32_t BIN_LONG_APPEND = 9u; // k_bin_gen's long-segment cursor; k_bin_scan moves it to
+                                         // BIN_LONG_COUNT and zeroes it, so it is zero whenever a
+                                         // k_bin_gen starts (k_bin_gen clears every other status
+                                         // word and the tile counts itself)
 constexpr uint32_t BIN_SHADE_SUBS = 16u;                        // camera: shade sub-list counters
 constexpr uint32_t BIN_FILTER_SUBS = BIN_SHADE_SUBS + LIST_SUBS; // camera: filter sub-list counters
 constexpr uint32_t BIN_META_WORDS = BIN_FILTER_SUBS + LIST_SUBS; // then count[tiles], offset[tiles + 1], cursor[tiles],
