@@ -370,6 +370,16 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
     return hipSuccess;
 }
 
+// Binning grid of a pass: workgroups per CU times CUs. The camera pass's
+// tile histogram is small (one LDS word per 32x32 tile), so it takes its own,
+// larger count; the DOM pass's (one word per 16x16 texel tile) limits how many
+// fit on a CU.
+static int bin_grid(const hm_ctx* c, int pass) {
+    const uint32_t cam = c->tuning.cam_bin_blocks_per_cu;
+    const uint32_t per_cu = pass == HM_PASS_CAMERA && cam ? cam : c->tuning.bin_blocks_per_cu;
+    return c->cu_count * (int)per_cu;
+}
+
 int raster_blocks(const hm_ctx* c, uint32_t capacity, uint32_t mode) {
     uint32_t want = (capacity + 3u) / 4u;
     uint32_t bpc = c->tuning.raster_blocks_per_cu[mode < 3u ? mode : 0u];
@@ -854,7 +864,7 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     // the device switches to the direct kernels.
     const size_t worst = std::min<size_t>(slots * 126u * ctx->tuning.bin_box_limit, 0xffffffffu);
     const size_t want_records = ctx->tuning.bin_records ? std::min<size_t>(ctx->tuning.bin_records, worst) : worst;
-    const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
+    const int bin_blocks = bin_grid(ctx, pass);
     // (tile, rank) list entries: at most one per (counting workgroup, tile) and per record.
     const size_t lists = std::min<size_t>(want_records, (size_t)bin_blocks * tiles);
     // Camera tiles split into chunks: at most tiles + records / chunk of them.
@@ -927,7 +937,7 @@ static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool
     }
     hm::BinArgs b;
     HM_HIP(ctx, bin_args(ctx, g, p, HM_PASS_CAMERA, T, tiles_x, tiles_y, b));
-    const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
+    const int bin_blocks = bin_grid(ctx, HM_PASS_CAMERA);
     b.emit_lists = emit_lists ? 1u : 0u;
     b.shade_list = ctx->pix_list;
     b.filter_list = ctx->pix_list2;
@@ -1011,7 +1021,7 @@ static hm_status dom_impl(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const h
         b.reuse_refs = g->refs[HM_PASS_CAMERA];
         b.reuse_indirect = g->indirect[HM_PASS_CAMERA];
     }
-    const int bin_blocks = ctx->cu_count * (int)ctx->tuning.bin_blocks_per_cu;
+    const int bin_blocks = bin_grid(ctx, HM_PASS_DOM);
     {
         Timer t(ctx, HM_K_DOM_BIN);
         HM_HIP(ctx, hm::launch_bin(ctx->stream, b, 1, raster_blocks(ctx, g->capacity, 1u), bin_blocks));
@@ -1289,6 +1299,7 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
         return fail(ctx, HM_ERR_INVALID, "bin_box_limit must be in [1, 64]");
     if (t->bin_blocks_per_cu == 0u || t->bin_blocks_per_cu > 64u)
         return fail(ctx, HM_ERR_INVALID, "bin_blocks_per_cu must be in [1, 64]");
+    if (t->cam_bin_blocks_per_cu > 64u) return fail(ctx, HM_ERR_INVALID, "cam_bin_blocks_per_cu must be in [0, 64]");
 
     ctx->tuning = *t;
     return HM_OK;

@@ -322,7 +322,7 @@ class Renderer:
         return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": list(t.raster_blocks_per_cu),
                 "dom_tile": int(t.dom_tile), "cam_tile": int(t.cam_tile), "bin_box_limit": int(t.bin_box_limit),
                 "bin_records": int(t.bin_records), "bin_blocks_per_cu": int(t.bin_blocks_per_cu),
-                "cam_chunk": int(t.cam_chunk)}
+                "cam_chunk": int(t.cam_chunk), "cam_bin_blocks_per_cu": int(t.cam_bin_blocks_per_cu)}
 
     def get_tuning(self) -> dict:
         t = N.HmTuning()
@@ -330,7 +330,8 @@ class Renderer:
         return self._tuning_dict(t)
 
     def set_tuning(self, raster_flags=None, raster_blocks_per_cu=None, dom_tile=None, bin_box_limit=None,
-                   bin_records=None, bin_blocks_per_cu=None, cam_tile=None, cam_chunk=None):
+                   bin_records=None, bin_blocks_per_cu=None, cam_tile=None, cam_chunk=None,
+                   cam_bin_blocks_per_cu=None):
         """Changes the given launch choices; None keeps the current one."""
         cur = self.get_tuning()
         if raster_flags is not None:
@@ -340,12 +341,13 @@ class Renderer:
             cur["raster_blocks_per_cu"] = [b] * 3 if isinstance(b, int) else list(b)
         for k, v in (("dom_tile", dom_tile), ("cam_tile", cam_tile), ("bin_box_limit", bin_box_limit),
                      ("bin_records", bin_records),
-                     ("bin_blocks_per_cu", bin_blocks_per_cu), ("cam_chunk", cam_chunk)):
+                     ("bin_blocks_per_cu", bin_blocks_per_cu), ("cam_chunk", cam_chunk),
+                     ("cam_bin_blocks_per_cu", cam_bin_blocks_per_cu)):
             if v is not None:
                 cur[k] = int(v)
         t = N.HmTuning((C.c_uint32 * 3)(*cur["raster_flags"]), (C.c_uint32 * 3)(*cur["raster_blocks_per_cu"]),
                        cur["dom_tile"], cur["cam_tile"], cur["bin_box_limit"], cur["bin_records"],
-                       cur["bin_blocks_per_cu"], cur["cam_chunk"])
+                       cur["bin_blocks_per_cu"], cur["cam_chunk"], cur["cam_bin_blocks_per_cu"])
         self._check(N.lib().hm_set_tuning(self.ctx, C.byref(t)), "hm_set_tuning")
 
     def autotune(self, g: GpuGroom, params, opts, reps: int = 3) -> dict:

@@ -373,6 +373,9 @@ typedef struct hm_tuning {
                                       one contiguous run of segments in LDS (1..64) */
     uint32_t cam_chunk;            /* hm_gbuffer: records per tile workgroup; larger tiles are split
                                       over several workgroups (0 = one per tile, else 64..1048576) */
+    uint32_t cam_bin_blocks_per_cu; /* hm_gbuffer's binning grid (0 = bin_blocks_per_cu, else 1..64):
+                                      its per-workgroup tile histogram is small, so it takes more
+                                      workgroups per CU than the DOM pass's */
 } hm_tuning;
 
 hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);

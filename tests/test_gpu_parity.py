@@ -326,7 +326,8 @@ def test_binned_gbuffer_bit_exact(renderer, case, variant):
     g = r.upload(sc.mesh)
     keep = r.get_tuning()
     try:
-        r.set_tuning(cam_tile=tile, bin_box_limit=box, bin_records=records, bin_blocks_per_cu=wg, cam_chunk=chunk)
+        r.set_tuning(cam_tile=tile, bin_box_limit=box, bin_records=records, bin_blocks_per_cu=wg, cam_chunk=chunk,
+                     cam_bin_blocks_per_cu=wg)
         r.lod(g, sc.params[0], N.PASS_CAMERA, w.control_points, w.strand_fraction)
         r.scan(g, sc.params[0], N.PASS_CAMERA)
         want = oracle_gbuffer(case, depth)
