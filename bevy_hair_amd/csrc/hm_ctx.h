@@ -75,7 +75,7 @@ struct hm_ctx {
     hm_directional_light* dirs = nullptr;
     uint32_t dir_count = 0, dir_cap = 0;
     uint64_t* counters = nullptr;
-    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u, 1024u, 4u};
+    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u, 2048u, 4u};
     hm_bin_state bin;
     // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
     uint32_t slots = 0, frame = 0;

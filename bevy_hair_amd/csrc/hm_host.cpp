@@ -1356,12 +1356,14 @@ hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const
         }
     }
     // Camera: direct kernels against binned tiles of either size with 128 to
-    // 1024 records (segments) per tile workgroup, whole hm_gbuffer timed.
+    // 2048 records (segments) per tile workgroup, whole hm_gbuffer timed.
     {
-        const uint32_t variants[9][2] = {{0u, 0u},     {16u, 128u}, {16u, 256u}, {16u, 512u}, {16u, 1024u},
-                                         {32u, 128u}, {32u, 256u}, {32u, 512u}, {32u, 1024u}};
+        constexpr uint32_t kVariants = 11;
+        const uint32_t variants[kVariants][2] = {{0u, 0u},    {16u, 128u},  {16u, 256u}, {16u, 512u},
+                                                 {16u, 1024u}, {16u, 2048u}, {32u, 128u}, {32u, 256u},
+                                                 {32u, 512u},  {32u, 1024u}, {32u, 2048u}};
         float best_cam = 1e30f;
-        for (uint32_t ti = 0; ti < 9; ++ti) {
+        for (uint32_t ti = 0; ti < kVariants; ++ti) {
             ctx->tuning = out;
             ctx->tuning.cam_tile = variants[ti][0];
             if (variants[ti][1]) ctx->tuning.cam_chunk = variants[ti][1];

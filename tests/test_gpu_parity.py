@@ -161,7 +161,7 @@ def test_frame_matches_oracle_and_is_deterministic(renderer):
     assert rel_err(first[4], ref.filtered).max() <= RGB_TOL
 
 
-@pytest.mark.parametrize("cam_tile,chunk", [(16, 1024), (32, 64)])
+@pytest.mark.parametrize("cam_tile,chunk", [(16, 1024), (32, 64), (32, 2048)])
 def test_frame_with_long_camera_segments(renderer, cam_tile, chunk):
     """A box limit of one tile sends most segments down the long path, which
     draws after the tiles have emitted the pixel lists: pixels only long
