@@ -138,6 +138,11 @@ BUF_FILTERED, BUF_FILTERED_DEPTH, BUF_DOM, BUF_SCENE_DEPTH = 5, 6, 7, 8
 BUF_CAMERA_LOD, BUF_CAMERA_REFS, BUF_CAMERA_INDIRECT = 16, 17, 18
 BUF_DOM_LOD, BUF_DOM_REFS, BUF_DOM_INDIRECT = 19, 20, 21
 BUF_CAM_BIN_STATUS, BUF_DOM_BIN_STATUS = 24, 25
+# Status words of a binned pass (hm_kernels.h): 16 header words, then the
+# shade and filter sub-list counters of the camera tiles.
+LIST_SUBS = 64
+BIN_SHADE_SUBS = 16
+BIN_META_WORDS = BIN_SHADE_SUBS + 2 * LIST_SUBS
 PASS_CAMERA, PASS_DOM = 0, 1
 KERNELS = ["clear", "cam_lod", "cam_scan", "cam_raster", "clear_dom", "dom_lod", "dom_scan",
            "dom_depth", "dom_opacity", "shade", "filter", "dom_bin", "dom_tiles", "cam_bin", "cam_tiles",

@@ -35,6 +35,9 @@ struct hm_bin_state {
     // and per-tile count / offset / cursor / chunk offset.
     uint32_t* meta[2] = {nullptr, nullptr};
     size_t meta_words[2] = {0, 0};
+    // A pass's launch sequence failed part way: its status words (the
+    // long-segment cursor among them) are cleared before the pass runs again.
+    bool meta_dirty[2] = {false, false};
     uint32_t* records = nullptr;
     size_t record_capacity = 0;
     uint2* lists = nullptr;         // (tile, rank) entries of the counting workgroups

@@ -309,10 +309,16 @@ void free_bin(hm_ctx* c) {
 hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size_t records, size_t lists,
                       size_t wgs, size_t chunks) {
     hm_bin_state& b = c->bin;
+    hipError_t e;
+    if (b.meta_dirty[pass] && b.meta[pass]) {
+        // (a fill kernel rather than a memset: this may run under graph capture)
+        if ((e = hm::launch_fill32(c->stream, b.meta[pass], b.meta_words[pass], 0u, 1)) != hipSuccess) return e;
+        b.meta_dirty[pass] = false;
+    }
     if (slots <= b.slots && meta_words <= b.meta_words[pass] && records <= b.record_capacity &&
         lists <= b.list_capacity && wgs <= b.wg_capacity && chunks <= b.chunk_capacity)
         return hipSuccess;
-    hipError_t e = hipStreamSynchronize(c->stream);  // earlier frames may still read them
+    e = hipStreamSynchronize(c->stream);  // earlier frames may still read them
     if (e != hipSuccess) return e;
     c->alloc_gen++;
     if (slots > b.slots) {
@@ -340,8 +346,9 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
         if ((e = dev_alloc(&b.meta[pass], meta_words * 4)) != hipSuccess) return e;
         // k_bin_gen expects the long-segment cursor at zero (k_bin_scan
         // re-zeroes it after every pass).
-        if ((e = hipMemsetAsync(b.meta[pass], 0, meta_words * 4, c->stream)) != hipSuccess) return e;
+        if ((e = hm::launch_fill32(c->stream, b.meta[pass], meta_words, 0u, 1)) != hipSuccess) return e;
         b.meta_words[pass] = meta_words;
+        b.meta_dirty[pass] = false;
     }
     if (records > b.record_capacity) {
         dev_free(b.records);
@@ -405,6 +412,15 @@ struct Timer {
         c->ran[base / 2] = 1;
     }
 };
+
+// The direct kernels run for `pass`: its status words say so (records, binned
+// and fallback all zero), so that an earlier binned run's status is never read
+// as this call's. The long-segment cursor and the sub-list counters after it
+// are left alone (the cursor is zero between passes).
+hipError_t mark_direct(hm_ctx* c, int pass) {
+    if (!c->bin.meta[pass]) return hipSuccess;
+    return hm::launch_fill32(c->stream, c->bin.meta[pass], hm::BIN_LONG_APPEND, 0u, 1);
+}
 
 hm_status check_targets(hm_ctx* ctx) {
     if (!ctx) return HM_ERR_INVALID;
@@ -932,6 +948,7 @@ static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool
     const bool binned = T != 0u && tiles_x * tiles_y <= hm::BIN_MAX_TILES &&
                         (uint64_t)g->capacity * hm::BIN_STRIDE <= 0xffffffffull;
     if (!binned) {
+        HM_HIP(ctx, mark_direct(ctx, HM_PASS_CAMERA));
         if ((st = hm_clear(ctx)) != HM_OK) return st;
         return hm_raster(ctx, g, p, HM_PASS_CAMERA);
     }
@@ -953,7 +970,11 @@ static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool
     b.filtered_depth = ctx->filtered_depth;
     {
         Timer t(ctx, HM_K_CAM_BIN);
-        HM_HIP(ctx, hm::launch_bin(ctx->stream, b, 0, raster_blocks(ctx, g->capacity, 0u), bin_blocks));
+        const hipError_t e = hm::launch_bin(ctx->stream, b, 0, raster_blocks(ctx, g->capacity, 0u), bin_blocks);
+        if (e != hipSuccess) {
+            ctx->bin.meta_dirty[HM_PASS_CAMERA] = true;
+            return hip_fail(ctx, e, "launch_bin (camera)");
+        }
     }
     {
         Timer t(ctx, HM_K_CAM_TILES);
@@ -1010,6 +1031,7 @@ static hm_status dom_impl(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const h
     const bool binned = T != 0u && tiles_x * tiles_x <= hm::BIN_MAX_TILES && std::memcmp(&same, dp, sizeof(hm_params)) == 0 &&
                         (uint64_t)g->capacity * hm::BIN_STRIDE <= 0xffffffffull;
     if (!binned) {
+        HM_HIP(ctx, mark_direct(ctx, HM_PASS_DOM));
         if ((st = hm_clear_dom(ctx)) != HM_OK) return st;
         if ((st = hm_raster(ctx, g, dp, HM_PASS_DOM)) != HM_OK) return st;
         return hm_raster(ctx, g, op, HM_PASS_DOM);
@@ -1024,7 +1046,11 @@ static hm_status dom_impl(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const h
     const int bin_blocks = bin_grid(ctx, HM_PASS_DOM);
     {
         Timer t(ctx, HM_K_DOM_BIN);
-        HM_HIP(ctx, hm::launch_bin(ctx->stream, b, 1, raster_blocks(ctx, g->capacity, 1u), bin_blocks));
+        const hipError_t e = hm::launch_bin(ctx->stream, b, 1, raster_blocks(ctx, g->capacity, 1u), bin_blocks);
+        if (e != hipSuccess) {
+            ctx->bin.meta_dirty[HM_PASS_DOM] = true;
+            return hip_fail(ctx, e, "launch_bin (DOM)");
+        }
     }
     {
         Timer t(ctx, HM_K_DOM_LONG_DEPTH);
@@ -1305,14 +1331,13 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
     return HM_OK;
 }
 
-hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm_frame_opts* opts, uint32_t reps,
-                      hm_tuning* best) {
-    if (!ctx || !g || !params || reps == 0) return fail(ctx, HM_ERR_INVALID, "null argument");
+// The search of hm_autotune; `out` starts as the context's tuning and ends as
+// the fastest variant. Leaves ctx->tuning at whatever variant ran last: the
+// caller restores or replaces it.
+static hm_status autotune_search(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm_frame_opts* opts,
+                                 uint32_t reps, hm_tuning& out) {
     hm_status st;
     const uint32_t grids[4] = {4u, 8u, 16u, 32u};
-    hm_tuning keep = ctx->tuning, out = ctx->tuning;
-    uint32_t saved_slots = ctx->slots;
-    ctx->slots = 0;  // no per-kernel events while tuning
     const hm_lod_override* ovr = opts ? &opts->camera_lod : nullptr;
     // Work lists of both passes.
     if ((st = hm_lod(ctx, g, &params[0], HM_PASS_CAMERA, ovr)) != HM_OK) return st;
@@ -1410,11 +1435,22 @@ hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const
         }
     }
 
-    ctx->tuning = out;
-    ctx->slots = saved_slots;
-    (void)keep;
-    if (best) *best = out;
     return HM_OK;
+}
+
+hm_status hm_autotune(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm_frame_opts* opts, uint32_t reps,
+                      hm_tuning* best) {
+    if (!ctx || !g || !params || reps == 0) return fail(ctx, HM_ERR_INVALID, "null argument");
+    const hm_tuning keep = ctx->tuning;
+    const uint32_t saved_slots = ctx->slots;
+    ctx->slots = 0;  // no per-kernel events while tuning
+    hm_tuning out = keep;
+    const hm_status st = autotune_search(ctx, g, params, opts, reps, out);
+    // On any error the context gets back the tuning and timing it had.
+    ctx->slots = saved_slots;
+    ctx->tuning = st == HM_OK ? out : keep;
+    if (st == HM_OK && best) *best = out;
+    return st;
 }
 
 hm_status hm_bench_copy(hm_ctx* ctx, size_t bytes, uint32_t reps, float* gbps) {

@@ -153,7 +153,7 @@ def main(argv=None) -> int:
     ap.add_argument("--workloads", default="c1,c2")
     ap.add_argument("--directions", type=int, default=64)
     ap.add_argument("--samples", type=int, default=4_000_000)
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--epochs", type=int, default=120)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)

@@ -35,6 +35,40 @@ def look_at_world_from_view(eye, focus, up=(0.0, 1.0, 0.0)) -> np.ndarray:
     return m
 
 
+def view_from_world_rigid(world_from_view: np.ndarray) -> np.ndarray:
+    """Inverse of a rigid transform as R^T, -R^T t, with the last row exactly
+    (0, 0, 0, 1) (a general inverse leaves rounding residue there, which the
+    orthographic test of shade.wesl:560 would read as a perspective row)."""
+    r, t = world_from_view[:3, :3], world_from_view[:3, 3]
+    m = np.eye(4)
+    m[:3, :3] = r.T
+    m[:3, 3] = -r.T @ t
+    return m
+
+
+def perspective_rh(fov_y: float, aspect: float, near: float, far: float) -> np.ndarray:
+    """glam Mat4::perspective_rh: forward Z (near -> 0, far -> 1), finite far."""
+    h = 1.0 / np.tan(0.5 * fov_y)
+    r = far / (near - far)
+    m = np.zeros((4, 4))
+    m[0, 0] = h / aspect
+    m[1, 1] = h
+    m[2, 2] = r
+    m[2, 3] = r * near
+    m[3, 2] = -1.0
+    return m
+
+
+def orthographic_rh(left: float, right: float, bottom: float, top: float, near: float, far: float) -> np.ndarray:
+    """glam Mat4::orthographic_rh (z: near -> 0, far -> 1). Bevy's
+    OrthographicProjection passes (far, near) for a reverse-Z depth."""
+    rw, rh, r = 1.0 / (right - left), 1.0 / (top - bottom), 1.0 / (near - far)
+    m = np.eye(4)
+    m[0, 0], m[1, 1], m[2, 2] = 2.0 * rw, 2.0 * rh, r
+    m[0, 3], m[1, 3], m[2, 3] = -(left + right) * rw, -(top + bottom) * rh, r * near
+    return m
+
+
 def perspective_infinite_reverse_rh(fov_y: float, aspect: float, near: float) -> np.ndarray:
     f = 1.0 / np.tan(0.5 * fov_y)
     m = np.zeros((4, 4))
@@ -58,9 +92,17 @@ class View:
     height: int
 
 
+PROJECTIONS = ("perspective", "perspective_forward_z", "orthographic", "orthographic_forward_z")
+
+
 def fit_camera(mesh: groom_mod.HairMesh, width: int, height: int, heading=(150.0, 0.0, 40.0),
-               pitch_deg: float = 15.0, margin: float = 1.1) -> View:
-    """Looks at the groom's bounds centre from a distance that fits its bounding sphere."""
+               pitch_deg: float = 15.0, margin: float = 1.1, projection: str = "perspective") -> View:
+    """Looks at the groom's bounds centre from a distance that fits its bounding sphere.
+
+    projection: "perspective" (Bevy's default: infinite reverse-Z, near 0.1),
+    "perspective_forward_z" (finite forward-Z, near 0.1, far 100),
+    "orthographic" (Bevy's OrthographicProjection: reverse-Z, near 0.1,
+    far 100, the sphere's diameter tall) or "orthographic_forward_z"."""
     lo, hi = mesh.bounds_min.astype(np.float64), mesh.bounds_max.astype(np.float64)
     focus = 0.5 * (lo + hi)
     radius = 0.5 * np.linalg.norm(hi - lo)
@@ -70,8 +112,37 @@ def fit_camera(mesh: groom_mod.HairMesh, width: int, height: int, heading=(150.0
     dist = radius * margin / np.sin(0.5 * FOV_Y)
     eye = focus + h * (dist * np.cos(pitch)) + np.array([0.0, 1.0, 0.0]) * (dist * np.sin(pitch))
     wfv = look_at_world_from_view(eye, focus)
-    proj = perspective_infinite_reverse_rh(FOV_Y, width / height, 0.1)
-    return View(proj @ np.linalg.inv(wfv), wfv, width, height)
+    aspect = width / height
+    if projection == "perspective":
+        proj = perspective_infinite_reverse_rh(FOV_Y, aspect, 0.1)
+        return View(proj @ np.linalg.inv(wfv), wfv, width, height)
+    half = radius * margin
+    if projection == "perspective_forward_z":
+        proj = perspective_rh(FOV_Y, aspect, 0.1, 100.0)
+    elif projection == "orthographic":
+        proj = orthographic_rh(-half * aspect, half * aspect, -half, half, 100.0, 0.1)
+    elif projection == "orthographic_forward_z":
+        proj = orthographic_rh(-half * aspect, half * aspect, -half, half, 0.1, 100.0)
+    else:
+        raise ValueError(f"unknown projection {projection!r}")
+    return View(proj @ view_from_world_rigid(wfv), wfv, width, height)
+
+
+def spot_light(position, target, inner_deg: float, outer_deg: float, color, light_range: float = 3.0) -> dict:
+    """A spot light as Bevy's clustered light record holds it (shade.wesl:312-327):
+    light_custom_data = (dir.x, dir.z, spot scale, spot offset) of the unit
+    direction the light points along, the sign of dir.y in the flags, scale
+    1 / max(cos inner - cos outer, 1e-4) and offset -cos outer * scale."""
+    position = np.asarray(position, dtype=np.float64)
+    d = np.asarray(target, dtype=np.float64) - position
+    d /= np.linalg.norm(d)
+    ci, co = np.cos(np.radians(inner_deg)), np.cos(np.radians(outer_deg))
+    scale = 1.0 / max(ci - co, 1e-4)
+    flags = 2 if d[1] < 0.0 else 0  # HM_LIGHT_FLAG_SPOT_Y_NEGATIVE
+    return {"position_radius": [*position.tolist(), 0.0],
+            "color_inverse_square_range": [*[float(c) for c in color], 1.0 / (light_range * light_range)],
+            "light_custom_data": [float(d[0]), float(d[2]), float(scale), float(-co * scale)],
+            "flags": flags, "kind": 1}
 
 
 @dataclasses.dataclass

@@ -193,9 +193,12 @@ typedef enum hm_buffer {
     HM_BUF_DOM_LOD = 19,
     HM_BUF_DOM_REFS = 20,
     HM_BUF_DOM_INDIRECT = 21,
-    HM_BUF_CAM_BIN_STATUS = 24, /* u32[8] of the last binned hm_gbuffer: records, binned, fallback,
-                                   long segments present, long segment count, -, shade / filter list lengths */
-    HM_BUF_DOM_BIN_STATUS = 25  /* u32[8] of the last binned hm_dom, same layout */
+    HM_BUF_CAM_BIN_STATUS = 24, /* u32[144] of the last hm_gbuffer: [0] records, [1] binned, [2] fallback
+                                   (direct kernels after a record overflow), [3] long segments present,
+                                   [4] long segment count; [1] and [2] both 0 when the direct kernels
+                                   ran by choice; [16..79] / [80..143] the shade / filter sub-list
+                                   lengths of a frame's camera tiles */
+    HM_BUF_DOM_BIN_STATUS = 25  /* u32[144] of the last hm_dom, same header */
 } hm_buffer;
 
 /* Kernel ids for hm_timing_read. */

@@ -33,7 +33,7 @@ def u32s(n):
 
 
 def test_gbuffer_roundtrip_known_word():
-    # gbuffer.rs:241-251: depth 0x123456, oct [3, 250], uvw [1, 17, 63], ao 40.
+    # gbuffer.rs:65-76: depth 0x123456, oct [3, 250], uvw [1, 17, 63], ao 40.
     word = L.or_pack_gb(0x123456, 3, 250, 1, 17, 63, 40)
     assert word == 0x12345603FA051FE8
     out = u32s(7)
@@ -44,7 +44,7 @@ def test_gbuffer_roundtrip_known_word():
 
 
 def test_gbuffer_nearer_is_smaller():
-    # gbuffer.rs:253-269
+    # gbuffer.rs:78-94
     far = L.or_pack_gb(1000, 255, 255, 63, 63, 63, 63)
     near = L.or_pack_gb(999, 0, 0, 0, 0, 0, 0)
     assert near < far < EMPTY64
@@ -112,7 +112,7 @@ def test_control_points_known_answers():
     assert f(0.0) == 9 and f(0.1) == 33 and f(0.262) == 65 and f(1.0) == 65
     for l in np.linspace(0, 1, 401):
         c = f(float(l))
-        # lod.rs:372-386: C - 1 is a power of two, 3 <= C <= 127.
+        # lod.rs:92-106: C - 1 is a power of two, 3 <= C <= 127.
         assert 3 <= c <= 127 and bin(c - 1).count("1") == 1
         # ... and it is the largest 2^k + 1 not above C_raw = max(floor(sqrt(L) * 127), 16).
         c_raw = max(int(np.floor(np.sqrt(np.float32(l)) * np.float32(127.0))), 16)
@@ -137,7 +137,7 @@ def _delta(bundle_id=0):
 
 
 def test_lod_selection_and_strand_count():
-    # lod.rs:356-370. L = diag / R_y * lambda (lambda = 3 in screen_params).
+    # lod.rs:76-90. L = diag / R_y * lambda (lambda = 3 in screen_params).
     rec, _ = _lod_for_span(1080.0, 100)  # L = 3 -> clamped to 1
     assert rec[0] == 100 and rec[1] == 65 and rec[2] == 100
     rec, _ = _lod_for_span(100.0, 100)  # L = 300 / 1080
@@ -166,18 +166,19 @@ def test_scan_order_and_capacity():
     refs, total = oracle.Frame.scan(p, lod, 100)
     assert total == 5
     assert refs.tolist() == [[0, 0, 2, 9], [0, 1, 2, 9], [2, 0, 3, 17], [2, 1, 3, 17], [2, 2, 3, 17]]
-    refs, total = oracle.Frame.scan(p, lod, 3)  # scan.wgsl:293-295 caps at the buffer length
+    refs, total = oracle.Frame.scan(p, lod, 3)  # scan.wgsl:14-16 caps at the buffer length
     assert total == 3 and refs[-1].tolist() == [2, 0, 3, 17]
 
 
 def test_dom_optical_depth_known_answers():
-    # lod.rs:403-414
+    # lod.rs:123-134
     layers = (C.c_uint32 * 16)(100, 40, 7)
     f = L.or_dom_optical_depth
     assert f(layers, 0.0) == 0.0 and f(layers, -1.0) == 0.0
     assert abs(f(layers, 0.5) - 50.0) < 1e-4
     assert abs(f(layers, 1.0) - 100.0) < 1e-4
     assert abs(f(layers, 1.25) - 110.0) < 1e-4
+    assert abs(f(layers, 3.0) - 147.0) < 1e-4
     assert abs(f(layers, 3.0) - 147.0) < 1e-4
 
 
