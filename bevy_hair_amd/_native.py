@@ -205,7 +205,9 @@ ABI = {
     "hm_batch_time": (C.c_int, [P, P, u32, C.POINTER(f32)]),
     "hm_batch_read": (C.c_int, [P, P, u32, P, P]),
     "hm_batch_free": (None, [P, P]),
+    "hm_scan_noalloc": (C.c_int, [P, P, u32, P, u32, P]),
 }
+SCAN_NOALLOC_MAX_BUNDLES = 65536
 
 _lib = None
 

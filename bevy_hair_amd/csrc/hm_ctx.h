@@ -107,6 +107,8 @@ struct hm_lists {
 hm_status hm_fail(hm_ctx* ctx, hm_status st, const std::string& msg);
 hm_status hm_hip_fail(hm_ctx* ctx, hipError_t e, const char* what);
 void hm_mlp_release(hm_ctx* ctx);
+// Looks up the allocation-free scan's scratch address on `device` (current).
+hipError_t hm_noalloc_setup(int device);
 // The shade list for `lists`, compacting it first where needed (enqueued on the stream).
 hm_status hm_shade_list(hm_ctx* ctx, const hm_lists& lists, hm::PixelList* out);
 hm_status hm_shade_mlp_impl(hm_ctx* ctx, const hm_params* p, float exposure, float tolerance, uint32_t* fallback,

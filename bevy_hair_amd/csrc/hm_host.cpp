@@ -468,6 +468,7 @@ hm_status hm_create(int device, hm_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return HM_ERR_NO_DEVICE;
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return HM_ERR_NO_DEVICE;
     if (hipSetDevice(device) != hipSuccess) return HM_ERR_NO_DEVICE;
+    if (hm_noalloc_setup(device) != hipSuccess) return HM_ERR_HIP;
     hm_ctx* c = new hm_ctx();
     c->device = device;
     c->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;

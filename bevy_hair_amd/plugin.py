@@ -207,6 +207,18 @@ class Renderer:
     def scan(self, g: GpuGroom, params, pass_: int):
         self._check(N.lib().hm_scan(self.ctx, g.handle, C.byref(params), pass_), "hm_scan")
 
+    def scan_noalloc(self, lod_ptr: int, bundle_count: int, refs_ptr: int, capacity: int, indirect_ptr: int):
+        """hm_scan over caller-owned device buffers, allocating nothing."""
+        self._check(N.lib().hm_scan_noalloc(self.ctx, lod_ptr, bundle_count, refs_ptr, capacity, indirect_ptr),
+                    "hm_scan_noalloc")
+
+    def device_ptr(self, which: int, groom: GpuGroom | None = None) -> tuple[int, int]:
+        """Device address and size of a library buffer (hm_device_ptr)."""
+        p, n = C.c_void_p(), C.c_size_t()
+        self._check(N.lib().hm_device_ptr(self.ctx, groom.handle if groom else None, which, C.byref(p),
+                                          C.byref(n)), "hm_device_ptr")
+        return int(p.value or 0), int(n.value)
+
     def raster(self, g: GpuGroom, params, pass_: int):
         self._check(N.lib().hm_raster(self.ctx, g.handle, C.byref(params), pass_), "hm_raster")
 

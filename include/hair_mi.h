@@ -401,6 +401,22 @@ hm_status hm_bench_copy(hm_ctx* ctx, size_t bytes, uint32_t reps, float* gbps);
  * atomicMin, 1 u32 atomicMin, 2 u32 atomicAdd; returns atomics per second. */
 hm_status hm_bench_atomics(hm_ctx* ctx, size_t target_bytes, uint32_t kind, uint64_t count, float* per_second);
 
+/* ---- allocation-free scan (no counterpart in the reference) --------------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. hm_scan
+ * (scan.wgsl) over caller-owned device buffers: `lod` holds bundle_count
+ * hm_lod_records (as hm_lod writes them), `refs` capacity hm_strand_refs and
+ * `indirect` 4 words; the results are those hm_scan gives for the same
+ * records, bit for bit. Its scratch (the per-bundle prefix) is a static
+ * device array of HM_SCAN_NOALLOC_MAX_BUNDLES + 1 words (256 KiB) that each
+ * GPU holds until the process ends; more bundles return HM_ERR_CAPACITY.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from any context or stream) race. Allocates nothing; asynchronous on the
+ * context's stream. */
+#define HM_SCAN_NOALLOC_MAX_BUNDLES 65536u
+hm_status hm_scan_noalloc(hm_ctx* ctx, const hm_lod_record* lod, uint32_t bundle_count,
+                          hm_strand_ref* refs, uint32_t capacity, uint32_t* indirect);
+
 /* ---- learned shading (no counterpart in the reference) -------------------
  * A 32 -> 64 -> 64 -> 64 -> 16 MLP (bf16 weights on the matrix cores, f32
  * accumulation) stands in for dual_scatter (shade.wesl:153-190): its outputs
