@@ -52,6 +52,16 @@ def lib() -> C.CDLL:
             "or_dual_scatter": (None, [C.c_double, P, C.c_double, C.c_double, P]),
             "or_hair_bsdf": (None, [P, P, P, P, C.c_double, C.c_double, C.c_double, P]),
             "or_fresnel": (C.c_double, [C.c_double]),
+            "or_scatter_sums": (None, [C.c_double, P, P]),
+            "or_dual_terms": (None, [P, P, C.c_double, C.c_double, P]),
+            "or_bsdf_parts": (None, [P, P, P, P, C.c_double, C.c_double, C.c_double, P]),
+            "or_dom_visibility": (C.c_float, [P, P, P, C.c_float]),
+            "or_spot_mask": (C.c_float, [P, P]),
+            "or_distance_attenuation": (C.c_float, [C.c_float, C.c_float]),
+            "or_reconstruct_world": (None, [P, C.c_int32, C.c_int32, C.c_float, P]),
+            "or_project_world": (None, [P, P, P]),
+            "or_screen_tangent": (None, [P, P, P, P]),
+            "or_pixel_inputs": (None, [P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P]),
             "or_absorption_from_color": (None, [P, C.c_double, P]),
         }
         for name, (res, args) in sig.items():

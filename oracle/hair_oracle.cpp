@@ -699,8 +699,10 @@ template <class T> inline v3<T> lobe_mean(const Lobes<T>& l, v3<T> total, v3<T> 
     return num / vmax(total, splat3(T(1e-5f)));
 }
 
-template <class T> DualScatter<T> dual_scatter(T cos_theta_d, v3<T> sigma_a, const Shape<T>& shape) {
-    // shade.wesl:153-190
+template <class T> struct ScatterSums { Lobes<T> fwd, back; };
+
+// The midpoint-rule azimuthal integrals of dual_scatter (shade.wesl:153-165).
+template <class T> ScatterSums<T> scatter_sums(T cos_theta_d, v3<T> sigma_a) {
     Lobes<T> fwd = {splat3(T(0)), splat3(T(0)), splat3(T(0))};
     Lobes<T> back = fwd;
     const uint32_t steps = 8u;
@@ -714,6 +716,12 @@ template <class T> DualScatter<T> dual_scatter(T cos_theta_d, v3<T> sigma_a, con
     T w = T(kPi) / T((float)steps);
     fwd = {fwd.r * w, fwd.tt * w, fwd.trt * w};
     back = {back.r * w, back.tt * w, back.trt * w};
+    return {fwd, back};
+}
+
+// The dual-scattering terms from the lobe sums (shade.wesl:167-189, Zinke
+// et al. 2008 Eqs. 13-17 as the reference writes them).
+template <class T> DualScatter<T> dual_terms(const Lobes<T>& fwd, const Lobes<T>& back, const Shape<T>& shape) {
     v3<T> a_f = vmin(fwd.r + fwd.tt + fwd.trt, splat3(T(0.95f)));
     v3<T> a_b = back.r + back.tt + back.trt;
     v3<T> beta2 = shape.beta * shape.beta;
@@ -736,9 +744,19 @@ template <class T> DualScatter<T> dual_scatter(T cos_theta_d, v3<T> sigma_a, con
     return {fwd, a_f, alpha_f, beta_f2, a_back, delta_b, sigma_b * sigma_b};
 }
 
+template <class T> DualScatter<T> dual_scatter(T cos_theta_d, v3<T> sigma_a, const Shape<T>& shape) {
+    // shade.wesl:153-190
+    ScatterSums<T> s = scatter_sums(cos_theta_d, sigma_a);
+    return dual_terms(s.fwd, s.back, shape);
+}
+
+// hair_bsdf's terms: single scattering, the local back lobe, the global
+// forward-scattering term (zero unless depth > 1e-3) and exp(-depth).
+template <class T> struct BsdfParts { v3<T> single, back, global; T direct; };
+
 template <class T>
-v3<T> hair_bsdf(v3<T> tangent, v3<T> view, v3<T> light, v3<T> sigma_a, const Shape<T>& shape,
-                const DualScatter<T>& ds, T shadow, T depth) {  // shade.wesl:193-244
+BsdfParts<T> hair_bsdf_parts(v3<T> tangent, v3<T> view, v3<T> light, v3<T> sigma_a, const Shape<T>& shape,
+                             const DualScatter<T>& ds, T depth) {  // shade.wesl:193-242
     v3<T> t = safe_normalize(tangent);
     v3<T> v = safe_normalize(view);
     v3<T> l = safe_normalize(light);
@@ -757,7 +775,7 @@ v3<T> hair_bsdf(v3<T> tangent, v3<T> view, v3<T> light, v3<T> sigma_a, const Sha
     v3<T> single = m.x * n.r + m.y * n.tt + m.z * n.trt;
     v3<T> back = T(2) * ds.a_back * gaussian3(vsqrt(ds.sigma_b2), splat3(theta_h) - ds.delta_b) / T(kPi);
     T direct = std::exp(-depth);
-    v3<T> color = direct * (single + T(0.7f) * back);
+    BsdfParts<T> out = {single, back, splat3(T(0)), direct};
     if (depth > T(1e-3f)) {
         v3<T> sigma_f2 = depth * ds.beta_f2;
         v3<T> transmit = {std::pow(ds.a_f.x, depth), std::pow(ds.a_f.y, depth), std::pow(ds.a_f.z, depth)};
@@ -767,9 +785,17 @@ v3<T> hair_bsdf(v3<T> tangent, v3<T> view, v3<T> light, v3<T> sigma_a, const Sha
                        gaussian3(vsqrt(splat3(beta2.y) + sigma_f2), x - splat3(shape.alpha.y)) * ds.fwd.tt +
                        gaussian3(vsqrt(splat3(beta2.z) + sigma_f2), x - splat3(shape.alpha.z)) * ds.fwd.trt;
         v3<T> back_spread = T(2) * ds.a_back * gaussian3(vsqrt(ds.sigma_b2 + sigma_f2), x - ds.delta_b) / T(kPi);
-        color = color + (T(1) - direct) * transmit * T(0.7f) *
-                            (spread / T(kPi) + T(kPi) * T(0.7f) * back_spread);
+        out.global = (T(1) - direct) * transmit * T(0.7f) * (spread / T(kPi) + T(kPi) * T(0.7f) * back_spread);
     }
+    return out;
+}
+
+template <class T>
+v3<T> hair_bsdf(v3<T> tangent, v3<T> view, v3<T> light, v3<T> sigma_a, const Shape<T>& shape,
+                const DualScatter<T>& ds, T shadow, T depth) {  // shade.wesl:193-244
+    BsdfParts<T> b = hair_bsdf_parts(tangent, view, light, sigma_a, shape, ds, depth);
+    v3<T> color = b.direct * (b.single + T(0.7f) * b.back);
+    if (depth > T(1e-3f)) color = color + b.global;
     return color * shadow;
 }
 
@@ -895,6 +921,28 @@ v3<T> direct_light(const Params<T>& p, const DomView<T>& dv, const Lights& light
     return color;
 }
 
+// The per-pixel inputs shade.wesl:fs derives from a covered pixel's words
+// (shade.wesl:539-571).
+template <class T> struct PixelIn { v3<T> world, tangent, view, albedo; T roughness, beta, view_dist; bool ortho; };
+
+template <class T>
+PixelIn<T> pixel_inputs(const Params<T>& p, uint64_t word, uint32_t beta_word, uint32_t x, uint32_t y) {
+    PixelIn<T> in;
+    Unpacked u = unpack_gb(word);
+    in.view_dist = dequantize_depth(u.depth, p.screen.z, p.screen.w);
+    in.world = reconstruct_world(p, (int32_t)x, (int32_t)y, in.view_dist);
+    in.tangent = oct_decode<T>(u.tx, u.ty);
+    v3<T> uvw = {T((float)u.u) / T(63.0f), T((float)u.v) / T(63.0f), T((float)u.w) / T(63.0f)};
+    in.beta = T((float)(beta_word & 0xffu)) / T(255.0f);
+    in.roughness = fmax_(p.appearance.x * (T(0.65f) + T(0.7f) * uvw.z), T(0.04f));
+    in.albedo = xyz(p.albedo) * (T(0.65f) + T(0.35f) * uvw.z);
+    // Orthographic when the projection's last row is (0, 0, 0, 1).
+    in.ortho = p.clip[3] == T(0) && p.clip[7] == T(0) && p.clip[11] == T(0) && p.clip[15] == T(1);
+    if (in.ortho) in.view = safe_normalize(v3<T>{p.clip[2], p.clip[6], p.clip[10]});
+    else in.view = safe_normalize(xyz(p.camera_pos) - in.world);
+    return in;
+}
+
 template <class T>
 void shade_pixel(const Params<T>& p, const DomView<T>& dv, const Lights& lights, const uint64_t* center,
                  const uint32_t* beta_buf, T exposure, uint32_t x, uint32_t y, float* shaded, float* shaded_depth) {
@@ -907,21 +955,10 @@ void shade_pixel(const Params<T>& p, const DomView<T>& dv, const Lights& lights,
         shaded_depth[pix] = 0.0f;
         return;
     }
-    Unpacked u = unpack_gb(word);
-    T view_dist = dequantize_depth(u.depth, p.screen.z, p.screen.w);
-    v3<T> world = reconstruct_world(p, (int32_t)x, (int32_t)y, view_dist);
-    v3<T> tangent = oct_decode<T>(u.tx, u.ty);
-    v3<T> uvw = {T((float)u.u) / T(63.0f), T((float)u.v) / T(63.0f), T((float)u.w) / T(63.0f)};
-    T beta = T((float)(beta_buf[pix] & 0xffu)) / T(255.0f);
-    T roughness = fmax_(p.appearance.x * (T(0.65f) + T(0.7f) * uvw.z), T(0.04f));
-    T tilt = p.appearance.y;
-    v3<T> albedo = xyz(p.albedo) * (T(0.65f) + T(0.35f) * uvw.z);
-    // Orthographic when the projection's last row is (0, 0, 0, 1).
-    bool ortho = p.clip[3] == T(0) && p.clip[7] == T(0) && p.clip[11] == T(0) && p.clip[15] == T(1);
-    v3<T> view;
-    if (ortho) view = safe_normalize(v3<T>{p.clip[2], p.clip[6], p.clip[10]});
-    else view = safe_normalize(xyz(p.camera_pos) - world);
-    v3<T> color = direct_light(p, dv, lights, world, tangent, view, albedo, roughness, tilt, beta);
+    PixelIn<T> in = pixel_inputs(p, word, beta_buf[pix], x, y);
+    T view_dist = in.view_dist;
+    v3<T> color = direct_light(p, dv, lights, in.world, in.tangent, in.view, in.albedo, in.roughness,
+                               p.appearance.y, in.beta);
     // probe_radiance returns zero without an environment map (shade.wesl:384-394).
     color = color * exposure;
     shaded[pix * 4 + 0] = (float)color.x;
@@ -1292,6 +1329,97 @@ void or_hair_bsdf(const double t[3], const double v[3], const double l[3], const
     out[2] = c.z;
 }
 
+/* scatter_sums (shade.wesl:153-165): out = fwd r, tt, trt then back r, tt, trt, rgb each. */
+void or_scatter_sums(double cos_theta_d, const double sigma_a[3], double out[18]) {
+    ScatterSums<double> s = scatter_sums<double>(cos_theta_d, {sigma_a[0], sigma_a[1], sigma_a[2]});
+    v3<double> parts[6] = {s.fwd.r, s.fwd.tt, s.fwd.trt, s.back.r, s.back.tt, s.back.trt};
+    for (int i = 0; i < 6; ++i) {
+        out[i * 3] = parts[i].x;
+        out[i * 3 + 1] = parts[i].y;
+        out[i * 3 + 2] = parts[i].z;
+    }
+}
+/* dual_terms (shade.wesl:167-189) from given lobe sums fwd / back (r, tt, trt,
+ * rgb each): out = a_f, alpha_f, beta_f2, a_back, delta_b, sigma_b2. */
+void or_dual_terms(const double fwd[9], const double back[9], double roughness, double tilt, double out[18]) {
+    Shape<double> sh = lobe_shape<double>(roughness, tilt);
+    auto lobes = [](const double* a) {
+        return Lobes<double>{{a[0], a[1], a[2]}, {a[3], a[4], a[5]}, {a[6], a[7], a[8]}};
+    };
+    DualScatter<double> d = dual_terms<double>(lobes(fwd), lobes(back), sh);
+    v3<double> parts[6] = {d.a_f, d.alpha_f, d.beta_f2, d.a_back, d.delta_b, d.sigma_b2};
+    for (int i = 0; i < 6; ++i) {
+        out[i * 3] = parts[i].x;
+        out[i * 3 + 1] = parts[i].y;
+        out[i * 3 + 2] = parts[i].z;
+    }
+}
+/* hair_bsdf_parts for one light: out = single, back, global (rgb each), exp(-depth). */
+void or_bsdf_parts(const double t[3], const double v[3], const double l[3], const double albedo[3], double roughness,
+                   double tilt, double depth, double out[10]) {
+    Shape<double> sh = lobe_shape<double>(roughness, tilt);
+    v3<double> tt = {t[0], t[1], t[2]}, vv = {v[0], v[1], v[2]}, ll = {l[0], l[1], l[2]};
+    v3<double> sigma = absorption_from_color<double>({albedo[0], albedo[1], albedo[2]}, roughness);
+    double tv = dot3(tt, vv);
+    double cos_theta_o = std::sqrt(fmax_(1.0 - tv * tv, 1e-6));
+    DualScatter<double> ds = dual_scatter<double>(cos_theta_o, sigma, sh);
+    BsdfParts<double> b = hair_bsdf_parts<double>(tt, vv, ll, sigma, sh, ds, depth);
+    v3<double> parts[3] = {b.single, b.back, b.global};
+    for (int i = 0; i < 3; ++i) {
+        out[i * 3] = parts[i].x;
+        out[i * 3 + 1] = parts[i].y;
+        out[i * 3 + 2] = parts[i].z;
+    }
+    out[9] = b.direct;
+}
+/* dom_visibility (shade.wesl:279-310), float path, for a world point and beta. */
+float or_dom_visibility(const hm_params* prm, const uint32_t* dom, const float world[3], float beta) {
+    Params<float> p = load_params<float>(*prm);
+    DomView<float> dv{p, dom};
+    return dv.visibility({world[0], world[1], world[2]}, beta);
+}
+float or_spot_mask(const hm_point_light* light, const float world[3]) {  // shade.wesl:312-327
+    return spot_mask<float>(*light, {world[0], world[1], world[2]});
+}
+float or_distance_attenuation(float dist2, float inv_range2) { return distance_attenuation<float>(dist2, inv_range2); }
+void or_reconstruct_world(const hm_params* prm, int32_t px, int32_t py, float view_dist, float out[3]) {
+    Params<float> p = load_params<float>(*prm);
+    v3<float> w = reconstruct_world(p, px, py, view_dist);
+    out[0] = w.x;
+    out[1] = w.y;
+    out[2] = w.z;
+}
+/* project_world (common.wgsl:212-219): out = screen x, y, view distance, clip depth, in_front. */
+void or_project_world(const hm_params* prm, const float world[3], float out[5]) {
+    Params<float> p = load_params<float>(*prm);
+    Projected<float> q = project_world(p, v3<float>{world[0], world[1], world[2]});
+    out[0] = q.screen.x;
+    out[1] = q.screen.y;
+    out[2] = q.view_dist;
+    out[3] = q.clip_depth;
+    out[4] = q.in_front ? 1.0f : 0.0f;
+}
+void or_screen_tangent(const hm_params* prm, const float world[3], const float tangent[3], float out[2]) {
+    Params<float> p = load_params<float>(*prm);  // filter.wgsl:9-18
+    v2<float> d = screen_tangent(p, v3<float>{world[0], world[1], world[2]}, v3<float>{tangent[0], tangent[1], tangent[2]});
+    out[0] = d.x;
+    out[1] = d.y;
+}
+/* pixel_inputs: out = world, tangent, view, albedo (3 each), roughness, beta, view distance, ortho. */
+void or_pixel_inputs(const hm_params* prm, uint64_t word, uint32_t beta_word, uint32_t x, uint32_t y, float out[16]) {
+    Params<float> p = load_params<float>(*prm);
+    PixelIn<float> in = pixel_inputs(p, word, beta_word, x, y);
+    v3<float> parts[4] = {in.world, in.tangent, in.view, in.albedo};
+    for (int i = 0; i < 4; ++i) {
+        out[i * 3] = parts[i].x;
+        out[i * 3 + 1] = parts[i].y;
+        out[i * 3 + 2] = parts[i].z;
+    }
+    out[12] = in.roughness;
+    out[13] = in.beta;
+    out[14] = in.view_dist;
+    out[15] = in.ortho ? 1.0f : 0.0f;
+}
 double or_fresnel(double c) { return fresnel<double>(c); }
 
 void or_absorption_from_color(const double color[3], double roughness, double out[3]) {
