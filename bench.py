@@ -37,24 +37,45 @@ SINGLE_KERNELS = ("clear", "cam_raster", "clear_dom", "dom_depth", "dom_opacity"
 
 
 def algorithmic_bytes(kernel: str, w, mesh, counts: dict, hit_pixels: int) -> float:
-    """Algorithmic HBM bytes of one launch (per-unit figures x units)."""
+    """Algorithmic HBM bytes of one launch by the path's own work (per-unit
+    figures x units): clears 20 B per pixel and 68 B per DOM texel; 8 B per
+    u64 and 4 B per u32 atomic, each counted once; 4 B of scene depth per
+    camera write; the groom's compulsory bytes once per pass; shade
+    (8 + 4 + 20 + 272) B per covered pixel; filter 56 B per pixel."""
     P = w.width * w.height
     S = w.dom_size
     B, N = mesh.bundle_count, mesh.strand_count
-    # Each strand reads its ref (16 B), its bundle (16 B), 64 cage positions and
-    # 64 style texels (16 B each); bundles' data is compulsory once per launch.
     groom = B * (16 + 64 * 32 + 1024 * 16) + 16 * N
+    cam_raster = 8.0 * counts["cam"][0] + 12.0 * counts["cam"][1] + 4.0 * counts["cam"][0]
+    dom_atomics = 4.0 * counts["dom1"][2] + 4.0 * counts["dom2"][2]
     if kernel == "clear":
         return 20.0 * P
     if kernel == "clear_dom":
         return 68.0 * S * S
     if kernel == "cam_raster":
-        c = counts["cam"]
-        return 8.0 * c[0] + 12.0 * c[1] + groom
+        return cam_raster + groom
     if kernel == "dom_depth":
         return 4.0 * counts["dom1"][2] + groom
     if kernel == "dom_opacity":
-        return 8.0 * counts["dom2"][2] + groom
+        return 4.0 * counts["dom2"][2]
+    if kernel == "cam_tiles":  # the camera raster's atomics and the G-buffer clear it performs
+        return 20.0 * P + cam_raster
+    if kernel == "dom_tiles":  # both DOM passes' atomics and the DOM clear it performs
+        return 68.0 * S * S + dom_atomics
+    if kernel == "shade":
+        return (8.0 + 4.0 + 20.0 + 272.0) * hit_pixels
+    if kernel == "filter":
+        return 56.0 * P
+    return 0.0
+
+
+def implementation_bytes(kernel: str, w, mesh, counts: dict, hit_pixels: int) -> float:
+    """HBM bytes the binned implementation moves by design (its own control
+    points, tile boxes and records on top of the algorithmic bytes)."""
+    P = w.width * w.height
+    S = w.dom_size
+    B, N = mesh.bundle_count, mesh.strand_count
+    groom = B * (16 + 64 * 32 + 1024 * 16) + 16 * N
     if kernel == "cam_bin":
         # groom in; per control point world position + screen position (20 B)
         # and world tangent + ao (16 B) out; 16 B per strand; tile box out and
@@ -66,18 +87,36 @@ def algorithmic_bytes(kernel: str, w, mesh, counts: dict, hit_pixels: int) -> fl
         # id, both control points (36 B each) and the strand's uv / beta
         return 20.0 * P + 92.0 * counts["cam_records"]
     if kernel == "dom_bin":
-        # groom in, control points out (12 B), tile box out and back (4 + 4 B
+        # groom in, control points out (20 B), tile box out and back (4 + 4 B
         # per segment), one 4 B record per (segment, tile)
-        return groom + 12.0 * counts["dom_cps"] + 8.0 * counts["dom_segments"] + 4.0 * counts["dom_records"]
+        return groom + 20.0 * counts["dom_cps"] + 8.0 * counts["dom_segments"] + 4.0 * counts["dom_records"]
     if kernel == "dom_tiles":
         # the map written once (front + 16 counts per texel); per record its
-        # id and two control points
-        return 68.0 * S * S + 28.0 * counts["dom_records"]
+        # id and its two 20-byte control points
+        return 68.0 * S * S + 44.0 * counts["dom_records"]
     if kernel == "shade":
         return 8.0 * P + 20.0 * P + hit_pixels * (4.0 + 272.0)
-    if kernel == "filter":
-        return 56.0 * P
-    return 0.0
+    return algorithmic_bytes(kernel, w, mesh, counts, hit_pixels)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    """The host cores this process may use (its affinity set), at most 16:
+    the CPU share of one GPU on the measuring machine."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def cpu_baseline(sc, w, threads: int, fraction: int) -> dict:
@@ -119,6 +158,16 @@ def cpu_baseline(sc, w, threads: int, fraction: int) -> dict:
             "sample": f"oracle (C++ restatement, not lavapipe) on every {fraction}th strand of the camera and "
                       f"DOM passes and {rows[1] - rows[0]} of {w.height} rows of shade+filter, scaled x{fraction}",
             "seconds_per_frame": {k: round(v, 3) for k, v in t.items()}}
+
+
+def cpu_baseline_report(sc, w, threads: int, fraction: int) -> dict:
+    """All-core run on 1/fraction of the frame plus a one-thread run on a
+    smaller sample (1/(4 fraction)), both scaled to a full frame."""
+    out = cpu_baseline(sc, w, threads, fraction)
+    one = cpu_baseline(sc, w, 1, 4 * fraction)
+    out["cpu_model"] = cpu_model()
+    out["one_thread"] = {"value": one["value"], "sample": one["sample"], "seconds_per_frame": one["seconds_per_frame"]}
+    return out
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_traffic_pmc.txt")
@@ -340,8 +389,13 @@ def main() -> int:
     # the dominant single kernel (the bin groups time four kernels each)
     dom_k = max((k for k in kt if k in SINGLE_KERNELS), key=kt.get)
     alg = algorithmic_bytes(dom_k, w, sc.mesh, counts, hit)
+    impl = implementation_bytes(dom_k, w, sc.mesh, counts, hit)
     achieved = alg / (kt[dom_k] * 1e-3) / 1e9
     atomics_cam = float(counts["cam"][0] + 2 * counts["cam"][1])
+    atomics_dom = float(counts["dom1"][2] + counts["dom2"][2])
+    per_kernel = {k: {"ms": round(kt[k], 5), "algorithmic_GBs": round(algorithmic_bytes(k, w, sc.mesh, counts, hit)
+                                                                       / (kt[k] * 1e-3) / 1e9, 1)}
+                  for k in ("cam_tiles", "dom_tiles", "shade", "filter") if kt.get(k)}
     result = {
         "metric": METRIC,
         "value": round(fps, 2),
@@ -361,8 +415,9 @@ def main() -> int:
                    "directional_lights": len(sc.dirs), "shading": "plain", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "algorithmic_bytes": alg, "mean_ms": round(kt[dom_k], 5),
+                     "algorithmic_bytes": alg, "implementation_bytes": impl, "mean_ms": round(kt[dom_k], 5),
                      "traffic": measured_traffic(dom_k, tuning, w.name)},
+        "kernel_rooflines": per_kernel,
         "kernels_ms": {k: round(v, 5) for k, v in kt.items()},
         "ms_per_step_with_kernel_events": round(ms_events, 4),
         "tuning": tuning,
@@ -374,7 +429,13 @@ def main() -> int:
             "cam_raster_atomics_per_s": round(atomics_cam / (kt["cam_raster"] * 1e-3), 1)
             if kt["cam_raster"] and not cam_status else None,
             "dom_opacity_atomics_per_s": round(float(counts["dom2"][2]) / (kt["dom_opacity"] * 1e-3), 1)
-            if kt["dom_opacity"] and not bin_status else None},
+            if kt["dom_opacity"] and not bin_status else None,
+            # the binned path's atomics run in LDS: the camera tiles' u64 / u32
+            # minima and the DOM tiles' u32 minima and adds, per second of kernel
+            "cam_tiles_lds_atomics_per_s": round(atomics_cam / (kt["cam_tiles"] * 1e-3), 1)
+            if kt.get("cam_tiles") else None,
+            "dom_tiles_lds_atomics_per_s": round(atomics_dom / (kt["dom_tiles"] * 1e-3), 1)
+            if kt.get("dom_tiles") else None},
     }
     result["mlp"] = mlp_leg
     if not args.no_probes:
@@ -384,9 +445,19 @@ def main() -> int:
         peaks = {"copy_GBs": round(r.bench_copy(1 << 30, 5), 1),
                  "u64_min_per_s": r.bench_atomics(8 * P, 0, 1 << 26),
                  "u32_min_per_s": r.bench_atomics(4 * S * S, 1, 1 << 26),
-                 "u32_add_per_s": r.bench_atomics(64 * S * S, 2, 1 << 26)}
+                 "u32_add_per_s": r.bench_atomics(64 * S * S, 2, 1 << 26),
+                 # LDS atomics over tile-sized arrays: 32x32 u64 words, 16x16x16 u32 counts
+                 "lds_u64_min_per_s": r.bench_lds_atomics(8192, 0, 1 << 30),
+                 "lds_u32_min_per_s": r.bench_lds_atomics(16384, 1, 1 << 30),
+                 "lds_u32_add_per_s": r.bench_lds_atomics(16384, 2, 1 << 30)}
         result["measured_peaks"] = peaks
         rates = result["atomic_rates"]
+        if rates["cam_tiles_lds_atomics_per_s"]:
+            rates["cam_tiles_frac_of_lds_u64_min_peak"] = round(
+                rates["cam_tiles_lds_atomics_per_s"] / peaks["lds_u64_min_per_s"], 4)
+        if rates["dom_tiles_lds_atomics_per_s"]:
+            rates["dom_tiles_frac_of_lds_u32_add_peak"] = round(
+                rates["dom_tiles_lds_atomics_per_s"] / peaks["lds_u32_add_per_s"], 4)
         if rates["cam_raster_atomics_per_s"]:
             rates["cam_raster_frac_of_u64_min_peak"] = round(rates["cam_raster_atomics_per_s"] / peaks["u64_min_per_s"], 4)
         if rates["dom_opacity_atomics_per_s"]:
@@ -396,9 +467,9 @@ def main() -> int:
     result["heavy"] = None if args.no_heavy else heavy_leg_run(r, args)
     r.close()
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         print(f"[bench] cpu baseline on {threads} threads ...", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(sc, w, threads, args.cpu_fraction)
+        result["cpu_baseline"] = cpu_baseline_report(sc, w, threads, args.cpu_fraction)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result))

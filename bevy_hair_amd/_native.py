@@ -188,6 +188,7 @@ ABI = {
     "hm_autotune": (C.c_int, [P, P, C.POINTER(HmParams), C.POINTER(HmFrameOpts), u32, C.POINTER(HmTuning)]),
     "hm_bench_copy": (C.c_int, [P, C.c_size_t, u32, C.POINTER(f32)]),
     "hm_bench_atomics": (C.c_int, [P, C.c_size_t, u32, C.c_uint64, C.POINTER(f32)]),
+    "hm_bench_lds_atomics": (C.c_int, [P, u32, u32, C.c_uint64, C.POINTER(f32)]),
     "hm_mlp_init": (C.c_int, [P, u32]),
     "hm_mlp_set_weights": (C.c_int, [P, P, P]),
     "hm_mlp_get_weights": (C.c_int, [P, P, P]),
@@ -206,6 +207,9 @@ ABI = {
     "hm_batch_read": (C.c_int, [P, P, u32, P, P]),
     "hm_batch_free": (None, [P, P]),
     "hm_scan_noalloc": (C.c_int, [P, P, u32, P, u32, P]),
+    "hm_bind_scene_depth": (C.c_int, [P, P, f32]),
+    "hm_composite": (C.c_int, [P, C.POINTER(HmParams), P, P]),
+    "hm_shadow_factors": (C.c_int, [P, C.POINTER(HmShadowLight), P, u32, P, u32, P]),
 }
 SCAN_NOALLOC_MAX_BUNDLES = 65536
 

@@ -71,7 +71,8 @@ struct hm_ctx {
     uint32_t* pix_list2 = nullptr;  // filter list when the camera tiles write both lists
     size_t list_len = 0;            // entries in pix_list / pix_list2 (capacity)
     uint32_t* pix_count = nullptr;  // sub-list counters: shade, then filter
-    float* scene_depth = nullptr;
+    float* scene_depth = nullptr;          // library-owned copy (hm_set_scene_depth)
+    const float* scene_depth_ext = nullptr; // caller-owned device buffer (hm_bind_scene_depth), used first
     float scene_clear = 0.0f;
     hm_point_light* points = nullptr;
     uint32_t point_count = 0, point_cap = 0;

@@ -635,6 +635,7 @@ hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_
         dev_free(ctx->scene_depth);
         ctx->alloc_gen++;
     }
+    if (width != ctx->width || height != ctx->height) ctx->scene_depth_ext = nullptr;  // bound for the old size
     uint32_t* slab = static_cast<uint32_t*>(ctx->gb_slab);
     ctx->center = reinterpret_cast<uint64_t*>(slab);
     ctx->cons = reinterpret_cast<uint64_t*>(slab + px * 2);
@@ -649,6 +650,7 @@ hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_
 hm_status hm_set_scene_depth(hm_ctx* ctx, const float* host_depth, float clear_value) {
     if (!ctx) return HM_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
+    ctx->scene_depth_ext = nullptr;
     ctx->scene_clear = clear_value;
     if (!host_depth) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -665,6 +667,16 @@ hm_status hm_set_scene_depth(hm_ctx* ctx, const float* host_depth, float clear_v
     }
     HM_HIP(ctx, hipMemcpyAsync(ctx->scene_depth, host_depth, bytes, hipMemcpyHostToDevice, ctx->stream));
     HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HM_OK;
+}
+
+hm_status hm_bind_scene_depth(hm_ctx* ctx, const float* device_depth, float clear_value) {
+    if (!ctx) return HM_ERR_INVALID;
+    if (!device_depth) return hm_set_scene_depth(ctx, nullptr, clear_value);
+    hm_status st = check_targets(ctx);
+    if (st != HM_OK) return st;
+    ctx->scene_depth_ext = device_depth;
+    ctx->scene_clear = clear_value;
     return HM_OK;
 }
 
@@ -840,7 +852,7 @@ static hm_status raster_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int p
     a.cons = ctx->cons;
     a.beta = ctx->beta;
     a.dom = ctx->dom;
-    a.scene_depth = ctx->scene_depth;
+    a.scene_depth = ctx->scene_depth_ext ? ctx->scene_depth_ext : ctx->scene_depth;
     a.scene_clear = ctx->scene_clear;
     a.counters = ctx->counters;
     a.opt = ctx->tuning.raster_flags[p->pass_mode[0] < 3u ? p->pass_mode[0] : 0u];
@@ -930,7 +942,7 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     b.center = ctx->center;
     b.cons = ctx->cons;
     b.beta = ctx->beta;
-    b.scene_depth = ctx->scene_depth;
+    b.scene_depth = ctx->scene_depth_ext ? ctx->scene_depth_ext : ctx->scene_depth;
     b.scene_clear = ctx->scene_clear;
     return hipSuccess;
 }
@@ -1222,7 +1234,10 @@ static bool target_buffer(hm_ctx* ctx, hm_groom* g, hm_buffer which, void** ptr,
         case HM_BUF_FILTERED: *ptr = ctx->filtered; *bytes = px * 16; return ctx->filtered != nullptr;
         case HM_BUF_FILTERED_DEPTH: *ptr = ctx->filtered_depth; *bytes = px * 4; return ctx->filtered_depth != nullptr;
         case HM_BUF_DOM: *ptr = ctx->dom; *bytes = dom; return ctx->dom != nullptr;
-        case HM_BUF_SCENE_DEPTH: *ptr = ctx->scene_depth; *bytes = px * 4; return ctx->scene_depth != nullptr;
+        case HM_BUF_SCENE_DEPTH:
+            *ptr = ctx->scene_depth_ext ? const_cast<float*>(ctx->scene_depth_ext) : ctx->scene_depth;
+            *bytes = px * 4;
+            return *ptr != nullptr;
         case HM_BUF_CAM_BIN_STATUS: *ptr = ctx->bin.meta[0]; *bytes = hm::BIN_META_WORDS * 4; return ctx->bin.meta[0] != nullptr;
         case HM_BUF_DOM_BIN_STATUS: *ptr = ctx->bin.meta[1]; *bytes = hm::BIN_META_WORDS * 4; return ctx->bin.meta[1] != nullptr;
         default: break;
@@ -1496,6 +1511,31 @@ hm_status hm_bench_atomics(hm_ctx* ctx, size_t target_bytes, uint32_t kind, uint
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->frame_ev[0], ctx->frame_ev[1]);
     (void)hipFree(buf);
     if (e != hipSuccess) return hip_fail(ctx, e, "hm_bench_atomics");
+    *per_second = (float)((double)threads * per / (ms * 1e-3));
+    return HM_OK;
+}
+
+hm_status hm_bench_lds_atomics(hm_ctx* ctx, uint32_t lds_bytes, uint32_t kind, uint64_t count, float* per_second) {
+    if (!ctx || !per_second || kind > 2 || count == 0) return HM_ERR_INVALID;
+    const uint32_t word = kind == 0 ? 8u : 4u;
+    if (lds_bytes < 1024u * word || lds_bytes > 65536u) return fail(ctx, HM_ERR_INVALID, "lds_bytes out of range");
+    uint32_t words = 1u;
+    while (words * 2u * word <= lds_bytes) words *= 2u;  // largest power of two that fits
+    (void)hipSetDevice(ctx->device);
+    const uint32_t blocks = (uint32_t)ctx->cu_count * 8u;
+    const uint64_t threads = (uint64_t)blocks * 256u;
+    const uint32_t per = (uint32_t)std::max<uint64_t>(1u, (count + threads - 1) / threads);
+    uint32_t* sink = nullptr;
+    HM_HIP(ctx, hipMalloc(&sink, blocks * 4u));
+    hipError_t e = hm::launch_lds_atomics(ctx->stream, sink, blocks, words, per, kind);  // warm
+    if (e == hipSuccess) e = hipEventRecord(ctx->frame_ev[0], ctx->stream);
+    if (e == hipSuccess) e = hm::launch_lds_atomics(ctx->stream, sink, blocks, words, per, kind);
+    if (e == hipSuccess) e = hipEventRecord(ctx->frame_ev[1], ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->frame_ev[1]);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->frame_ev[0], ctx->frame_ev[1]);
+    (void)hipFree(sink);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hm_bench_lds_atomics");
     *per_second = (float)((double)threads * per / (ms * 1e-3));
     return HM_OK;
 }

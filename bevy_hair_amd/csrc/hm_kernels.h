@@ -152,6 +152,9 @@ hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes, 
 // Copies `words` 32-bit words (a graph-safe device copy).
 hipError_t launch_copy32(hipStream_t s, const void* src, void* dst, size_t words, int max_blocks);
 hipError_t launch_atomics(hipStream_t s, void* buf, size_t words, uint32_t threads, uint32_t per_thread, uint32_t kind);
+// `blocks` workgroups of random LDS atomics over `words` (a power of two) targets each.
+hipError_t launch_lds_atomics(hipStream_t s, uint32_t* sink, uint32_t blocks, uint32_t words, uint32_t per_thread,
+                              uint32_t kind);
 hipError_t launch_fill32(hipStream_t s, uint32_t* dst, size_t words, uint32_t value, int max_blocks,
                          const uint32_t* gate = nullptr);
 hipError_t launch_lod(hipStream_t s, const hm_params& prm, const hm_bundle* bundles, const hm_layer* layers,

@@ -63,6 +63,30 @@ __global__ __launch_bounds__(256) void k_atomics(void* __restrict__ buf, size_t 
     }
 }
 
+// Random-address LDS atomics within one workgroup's array of `words` targets
+// (a power of two): KIND 0 u64 min, 1 u32 min, 2 u32 add. Each lane starts at
+// a hashed target and steps by a fixed odd stride, so the addresses of a
+// wave's lanes stay random (bank conflicts as in the tile kernels) while the
+// address arithmetic costs two VALU operations per atomic.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_lds_atomics(uint32_t* __restrict__ sink, uint32_t words, uint32_t per_thread) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t n32 = KIND == 0 ? 2u * words : words;
+    for (uint32_t i = threadIdx.x; i < n32; i += blockDim.x) lds[i] = KIND == 2 ? 0u : 0xffffffffu;
+    __syncthreads();
+    const uint32_t mask = words - 1u;
+    uint32_t i = pcg(blockIdx.x * blockDim.x + threadIdx.x) & mask;
+    const uint32_t v = pcg(i ^ 0x9e3779b9u);
+    for (uint32_t k = 0; k < per_thread; ++k) {
+        if (KIND == 0) atomicMin(reinterpret_cast<unsigned long long*>(lds) + i, (unsigned long long)v << 8);
+        else if (KIND == 1) atomicMin(lds + i, v);
+        else atomicAdd(lds + i, 1u);
+        i = (i + 613u) & mask;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sink[blockIdx.x] = lds[0] ^ lds[mask];
+}
+
 // ---- lod.wgsl -----------------------------------------------------------------
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -471,6 +495,15 @@ hipError_t launch_copy32(hipStream_t s, const void* src, void* dst, size_t words
 
 hipError_t launch_atomics(hipStream_t s, void* buf, size_t words, uint32_t threads, uint32_t per_thread, uint32_t kind) {
     hipLaunchKernelGGL(k_atomics, dim3((threads + 255) / 256), dim3(256), 0, s, buf, words, per_thread, kind, 12345u);
+    return hipGetLastError();
+}
+
+hipError_t launch_lds_atomics(hipStream_t s, uint32_t* sink, uint32_t blocks, uint32_t words, uint32_t per_thread,
+                              uint32_t kind) {
+    const size_t lds = (size_t)words * (kind == 0 ? 8u : 4u);
+    if (kind == 0) hipLaunchKernelGGL(k_lds_atomics<0>, dim3(blocks), dim3(256), lds, s, sink, words, per_thread);
+    else if (kind == 1) hipLaunchKernelGGL(k_lds_atomics<1>, dim3(blocks), dim3(256), lds, s, sink, words, per_thread);
+    else hipLaunchKernelGGL(k_lds_atomics<2>, dim3(blocks), dim3(256), lds, s, sink, words, per_thread);
     return hipGetLastError();
 }
 

@@ -170,6 +170,20 @@ class Renderer:
             assert d.size == self.width * self.height
             self._check(N.lib().hm_set_scene_depth(self.ctx, N.ptr(d), clear_value), "hm_set_scene_depth")
 
+    def bind_scene_depth(self, device_ptr: int | None, clear_value: float = 0.0):
+        """The scene depth as a caller-owned device buffer (no copy); None unbinds."""
+        self._check(N.lib().hm_bind_scene_depth(self.ctx, device_ptr, clear_value), "hm_bind_scene_depth")
+
+    def composite(self, params, color_ptr: int, depth_ptr: int):
+        """composite.wgsl into device view targets float4[W*H] / float[W*H]."""
+        self._check(N.lib().hm_composite(self.ctx, C.byref(params), color_ptr, depth_ptr), "hm_composite")
+
+    def shadow_factors(self, light: N.HmShadowLight, dom_ptr: int, dom_size: int, points_ptr: int, count: int,
+                       out_ptr: int):
+        """HairShadowMaterial's DOM lookup for `count` float4 world points (device buffers)."""
+        self._check(N.lib().hm_shadow_factors(self.ctx, C.byref(light), dom_ptr, dom_size, points_ptr, count, out_ptr),
+                    "hm_shadow_factors")
+
     def set_lights(self, points: list[dict], directionals: list[dict]):
         pa, np_, da, nd = light_arrays(points, directionals)
         self._check(N.lib().hm_set_lights(self.ctx, pa, np_, da, nd), "hm_set_lights")
@@ -381,6 +395,12 @@ class Renderer:
     def bench_atomics(self, target_bytes: int, kind: int, count: int) -> float:
         v = C.c_float()
         self._check(N.lib().hm_bench_atomics(self.ctx, target_bytes, kind, count, C.byref(v)), "hm_bench_atomics")
+        return float(v.value)
+
+    def bench_lds_atomics(self, lds_bytes: int, kind: int, count: int) -> float:
+        v = C.c_float()
+        self._check(N.lib().hm_bench_lds_atomics(self.ctx, lds_bytes, kind, count, C.byref(v)),
+                    "hm_bench_lds_atomics")
         return float(v.value)
 
     # -- learned shading --

@@ -262,6 +262,16 @@ hm_status hm_targets(hm_ctx* ctx, uint32_t width, uint32_t height, uint32_t dom_
  * no scene geometry: every texel reads `clear_value` (0 for reverse-Z). */
 hm_status hm_set_scene_depth(hm_ctx* ctx, const float* host_depth, float clear_value);
 
+/* Binds a caller-owned device buffer float[W*H] as that scene depth, with no
+ * copy and no synchronization: the view's depth texture as the reference
+ * binds it (pass.rs:365-370), e.g. imported with hipImportExternalMemory, and
+ * the buffer hm_composite writes, so that one groom's composited hair
+ * occludes the next groom's raster. Holds until hm_set_scene_depth, another
+ * binding or a change of the target size; NULL is hm_set_scene_depth(NULL).
+ * The buffer must stay valid while frames that read it run; a batch created
+ * with a binding in force replays that binding. */
+hm_status hm_bind_scene_depth(hm_ctx* ctx, const float* device_depth, float clear_value);
+
 /* The light list shade reads in place of Bevy's view bindings. */
 hm_status hm_set_lights(hm_ctx* ctx, const hm_point_light* points, uint32_t point_count,
                         const hm_directional_light* directionals, uint32_t directional_count);
@@ -334,6 +344,28 @@ hm_status hm_raster_count(hm_ctx* ctx, hm_groom* groom, const hm_params* params,
 hm_status hm_frame(hm_ctx* ctx, hm_groom* groom, const hm_params params[3],
                    const hm_frame_opts* opts);
 
+/* ---- after the frame: composite and shadow receivers ---------------------- */
+
+/* The composite draw of one groom (composite.wgsl:27-46 with the blend and
+ * depth state of pipeline.rs:299-339): the frame's final colour (filtered
+ * when camera->flags[1], else shaded) and depth into the caller's view
+ * targets, target_color float4[W*H] and target_depth float[W*H] (device
+ * pointers). A pixel with alpha >= 0.001 takes the clip depth of its
+ * reconstructed point, clamped to [0, 1]; it passes when >= the stored depth
+ * with reverse Z (camera->pass_mode[2] = 1, pass.rs:916-921), else <=; a
+ * passing pixel writes its depth and blends its colour (rgb src * a + dst *
+ * (1 - a), alpha src + dst * (1 - a)). Asynchronous on the context's stream. */
+hm_status hm_composite(hm_ctx* ctx, const hm_params* camera, float* target_color, float* target_depth);
+
+/* HairShadowMaterial's lookup (hair_shadow.wesl:30-75): the key light's
+ * attenuation by the hair for `count` world points of other surfaces,
+ * factors[i] = visibility at points[i] (float4 per point, w unused). `dom` is
+ * a map of dom_size^2 texels in hm_dom's layout (HM_BUF_DOM), `light` its
+ * ShadowLight (hm_shadow_light_from_params of the DOM params; depth.w = 0
+ * gives 1 everywhere). Device pointers; asynchronous on the context's stream. */
+hm_status hm_shadow_factors(hm_ctx* ctx, const hm_shadow_light* light, const uint32_t* dom, uint32_t dom_size,
+                            const float* points, uint32_t count, float* factors);
+
 /* ---- data access and timing --------------------------------------------- */
 
 /* Copies a buffer to host memory (synchronizes the stream). Groom buffers
@@ -400,6 +432,12 @@ hm_status hm_bench_copy(hm_ctx* ctx, size_t bytes, uint32_t reps, float* gbps);
 /* `count` random-address atomics over a `target_bytes` buffer: kind 0 u64
  * atomicMin, 1 u32 atomicMin, 2 u32 atomicAdd; returns atomics per second. */
 hm_status hm_bench_atomics(hm_ctx* ctx, size_t target_bytes, uint32_t kind, uint64_t count, float* per_second);
+/* About `count` random-address LDS atomics, 256-thread workgroups (8 per CU)
+ * each over its own array of the largest power-of-two word count within
+ * `lds_bytes` (8 KiB to 64 KiB): kind 0 u64 min, 1 u32 min, 2 u32 add;
+ * returns atomics per second over the whole chip (the tile kernels' LDS
+ * atomic peak). */
+hm_status hm_bench_lds_atomics(hm_ctx* ctx, uint32_t lds_bytes, uint32_t kind, uint64_t count, float* per_second);
 
 /* ---- allocation-free scan (no counterpart in the reference) --------------
  * For callers that cannot allocate device memory: code under HIP graph

@@ -62,6 +62,8 @@ def lib() -> C.CDLL:
             "or_project_world": (None, [P, P, P]),
             "or_screen_tangent": (None, [P, P, P, P]),
             "or_pixel_inputs": (None, [P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P]),
+            "or_hair_visibility": (C.c_float, [P, P, C.c_uint32, P]),
+            "or_composite": (None, [P, P, P, P, P]),
             "or_absorption_from_color": (None, [P, C.c_double, P]),
         }
         for name, (res, args) in sig.items():
@@ -135,6 +137,15 @@ class Frame:
         lib().or_shade(_p(params), _p(self.center), _p(self.beta), _p(self.dom), C.cast(points, P), n_points,
                        C.cast(dirs, P), n_dirs, exposure, _p(self.shaded), _p(self.shaded_depth), int(f64), threads,
                        r0, r1)
+
+    def composite(self, params, target_color, target_depth, filtered=None):
+        """composite.wgsl: this frame's final colour (filtered if the camera
+        params' filter flag is set, else shaded) into the view targets."""
+        if filtered is None:
+            filtered = bool(params.flags[1])
+        color = self.filtered if filtered else self.shaded
+        depth = self.filtered_depth if filtered else self.shaded_depth
+        lib().or_composite(_p(params), _p(color), _p(depth), _p(target_color), _p(target_depth))
 
     def filter(self, params, f64=False, threads=1, rows=None):
         r0, r1 = rows if rows else (0, self.height)

@@ -968,6 +968,78 @@ void shade_pixel(const Params<T>& p, const DomView<T>& dv, const Lights& lights,
     shaded_depth[pix] = (float)view_dist;
 }
 
+// ---- hair_shadow.wesl: HairShadowMaterial's receiver lookup -----------------
+
+// hair_shadow.wesl:30-50 with the map size at run time (the shader fixes 512).
+template <class T>
+T receiver_texel_visibility(const hm_shadow_light& L, const uint32_t* dom, uint32_t S, int32_t x, int32_t y, T dist) {
+    if (x < 0 || y < 0 || x >= (int32_t)S || y >= (int32_t)S) return T(1);
+    uint32_t pix = (uint32_t)y * S + (uint32_t)x;
+    uint32_t front_q = dom[pix];
+    if (front_q == HM_DOM_EMPTY) return T(1);
+    T near_ = T(L.depth[0]), far_ = T(L.depth[1]);
+    T z_front = mixs(near_, far_, T((float)front_q) / T(16777214.0f));
+    T thickness = fmax_(far_ - near_, T(1e-3f)) / T((float)HM_DOM_LAYERS);
+    T t = fmax_(dist - z_front, T(0)) / thickness;
+    size_t base = (size_t)S * S + (size_t)pix * HM_DOM_LAYERS;
+    T hits = T(0);
+    for (uint32_t slice = 0; slice < HM_DOM_LAYERS; ++slice)
+        hits = hits + T((float)dom[base + slice]) * clampf(t - T((float)slice), T(0), T(1));
+    return std::exp(-hits * T(L.depth[2]));
+}
+
+template <class T> T hair_visibility(const hm_shadow_light& L, const uint32_t* dom, uint32_t S, v3<T> world) {
+    // hair_shadow.wesl:52-75
+    if (T(L.depth[3]) == T(0)) return T(1);
+    T m[16];
+    for (int i = 0; i < 16; ++i) m[i] = T(L.light_clip_from_world[i]);
+    v4<T> clip = mat_mul(m, v4<T>{world.x, world.y, world.z, T(1)});
+    if (clip.w <= T(0)) return T(1);
+    T ndc_x = clip.x / clip.w, ndc_y = clip.y / clip.w;
+    T ux = ndc_x * T(0.5f) + T(0.5f);
+    T uy = T(1) - (ndc_y * T(0.5f) + T(0.5f));
+    if (ux < T(0) || uy < T(0) || ux > T(1) || uy > T(1)) return T(1);
+    v3<T> eye = {T(L.light_eye[0]), T(L.light_eye[1]), T(L.light_eye[2])};
+    v3<T> fwd = {T(L.light_forward[0]), T(L.light_forward[1]), T(L.light_forward[2])};
+    T dist = dot3(world - eye, fwd);
+    T fx = ux * T((float)S) - T(0.5f);
+    T fy = uy * T((float)S) - T(0.5f);
+    int32_t x0 = to_i32(std::floor(fx)), y0 = to_i32(std::floor(fy));
+    T wx = fx - std::floor(fx), wy = fy - std::floor(fy);
+    T v00 = receiver_texel_visibility(L, dom, S, x0, y0, dist);
+    T v10 = receiver_texel_visibility(L, dom, S, x0 + 1, y0, dist);
+    T v01 = receiver_texel_visibility(L, dom, S, x0, y0 + 1, dist);
+    T v11 = receiver_texel_visibility(L, dom, S, x0 + 1, y0 + 1, dist);
+    return mixs(mixs(v00, v10, wx), mixs(v01, v11, wx), wy);
+}
+
+// ---- composite.wgsl with its pipeline state (pipeline.rs:299-339) -----------
+
+// One fragment: colour with alpha >= 0.001 at the reconstructed point's clip
+// depth (clamped to [0, 1]); depth compare >= with reverse Z (pass.rs:916-921),
+// else <=; depth written; colour alpha-blended (rgb src * a + dst * (1 - a),
+// alpha src + dst * (1 - a)).
+template <class T>
+void composite_pixel(const Params<T>& p, const float* color, const float* depth_buf, float* tc, float* td, int32_t x,
+                     int32_t y) {
+    int32_t width = to_i32(p.screen.x);
+    size_t pix = (size_t)y * (uint32_t)width + (uint32_t)x;
+    const float* c = color + pix * 4;
+    if (c[3] < 0.001f) return;
+    T view_dist = T(depth_buf[pix]);
+    v3<T> world = reconstruct_world(p, x, y, view_dist);
+    v4<T> clip = mat_mul(p.clip, v4<T>{world.x, world.y, world.z, T(1)});
+    T depth = clampf(clip.z / fmax_(clip.w, T(1e-5f)), T(0), T(1));
+    float d = (float)depth;
+    bool pass = p.pass_mode[2] == 1u ? d >= td[pix] : d <= td[pix];
+    if (!pass) return;
+    td[pix] = d;
+    float a = c[3];
+    float* o = tc + pix * 4;
+    for (int k = 0; k < 3; ++k) o[k] = c[k] * a + o[k] * (1.0f - a);
+    o[3] = a + o[3] * (1.0f - a);
+}
+
 // ---- filter.wgsl -----------------------------------------------------------
 
 template <class T> inline v2<T> screen_tangent(const Params<T>& p, v3<T> world, v3<T> tangent) {  // filter.wgsl:9-18
@@ -1419,6 +1491,18 @@ void or_pixel_inputs(const hm_params* prm, uint64_t word, uint32_t beta_word, ui
     out[13] = in.beta;
     out[14] = in.view_dist;
     out[15] = in.ortho ? 1.0f : 0.0f;
+}
+/* hair_visibility (hair_shadow.wesl:52-75) of a world point, map of S x S texels. */
+float or_hair_visibility(const hm_shadow_light* light, const uint32_t* dom, uint32_t S, const float world[3]) {
+    return hair_visibility<float>(*light, dom, S, v3<float>{world[0], world[1], world[2]});
+}
+/* composite.wgsl over the whole screen into target_color (float4) and target_depth. */
+void or_composite(const hm_params* prm, const float* color, const float* depth_buf, float* target_color,
+                  float* target_depth) {
+    Params<float> p = load_params<float>(*prm);
+    int32_t width = (int32_t)prm->screen[0], height = (int32_t)prm->screen[1];
+    for (int32_t y = 0; y < height; ++y)
+        for (int32_t x = 0; x < width; ++x) composite_pixel<float>(p, color, depth_buf, target_color, target_depth, x, y);
 }
 double or_fresnel(double c) { return fresnel<double>(c); }
 

@@ -654,3 +654,104 @@ def test_filter_depth_rejection_promotion_and_closer_rule():
     f2.filter(screen_params(16, 12))
     assert f2.filtered[6 * 16 + 8, 3] == 1.0
     assert f.filtered[6 * 16 + 8, 3] < 1.0
+
+
+# ---- shadow receiver (hair_shadow.wesl:30-75) ---------------------------------------------
+
+
+def receiver_light(has_map=1.0, extinction=0.05):
+    """The light view of dom_params() as a ShadowLight: near 0, far 16."""
+    L = N.HmShadowLight()
+    L.light_clip_from_world[:] = [float(x) for x in np.eye(4).T.reshape(-1)]
+    L.light_eye[:] = [0.0, 0.0, 0.0, 1.0]
+    L.light_forward[:] = [0.0, 0.0, 1.0, 0.0]
+    L.depth[:] = [NEAR, FAR, extinction, has_map]
+    return L
+
+
+def rvis(light, dom, world):
+    return L.or_hair_visibility(C.byref(light), dom.ctypes.data, S_DOM, f3(world))
+
+
+def test_receiver_known_counts_without_slack():
+    """A surface sits behind every strand in front of it: no 0.05-layer slack
+    and no Eq. 8 shift; hits = sum count_k clamp(t - k, 0, 1)."""
+    counts = [3, 5, 8, 2]
+    dom = make_dom({(1, 2): 4.0}, {(1, 2): counts})
+    zf = L.or_dequantize_depth(quantize(4.0), NEAR, FAR)
+    thick = (FAR - NEAR) / 16.0
+    Lr = receiver_light(extinction=0.07)
+    for t, hits in ((0.0, 0.0), (0.25, 0.75), (1.5, 3 + 2.5), (2.5, 3 + 5 + 4), (9.0, 18)):
+        v = rvis(Lr, dom, texel_world(1, 2, zf + t * thick))
+        assert abs(v - math.exp(-0.07 * hits)) < 2e-6, (t, v)
+    # no map yet, empty texel, off the map, behind the light
+    assert rvis(receiver_light(has_map=0.0), dom, texel_world(1, 2, zf + 3 * thick)) == 1.0
+    assert rvis(Lr, dom, texel_world(2, 2, zf + 3 * thick)) == 1.0
+    assert rvis(Lr, dom, (1.2, 0.0, 5.0)) == 1.0
+    q = receiver_light()
+    q.light_clip_from_world[15] = -1.0
+    assert rvis(q, dom, texel_world(1, 2, zf + 3 * thick)) == 1.0
+
+
+def test_receiver_is_the_shade_lookup_without_slack_and_shift():
+    """With beta = 1 and the same extinction the shade's dom_visibility at
+    depth d + 0.05 layer equals the receiver at depth d (to rounding), over a
+    bilinear neighbourhood of four distinct texels."""
+    p = dom_params()
+    fronts = {(1, 1): 2.0, (2, 1): 3.0, (1, 2): 1.5, (2, 2): 4.5}
+    counts = {(1, 1): [5, 5, 5], (2, 1): [1, 2, 3, 4], (1, 2): [9, 0, 9], (2, 2): [2, 2]}
+    dom = make_dom(fronts, counts)
+    Lr = receiver_light(extinction=0.05)
+    thick = (FAR - NEAR) / 16.0
+    for z in (4.8, 6.1, 7.3):
+        for tx, ty in ((0.25, 0.75), (0.6, 0.1)):
+            r = rvis(Lr, dom, texel_world(1, 1, z, tx, ty))
+            s = vis(p, dom, texel_world(1, 1, z + 0.05 * thick, tx, ty))
+            assert abs(r - s) < 5e-6 and r < 1.0
+
+
+# ---- composite (composite.wgsl:27-46, pipeline.rs:299-339) ---------------------------------
+
+
+def composite_case(reverse: bool, colors, view_dists, target_depth, target_color):
+    mesh = synthetic_groom("straight", 64, 1)
+    view = scene.fit_camera(mesh, 8, 4, projection="perspective" if reverse else "perspective_forward_z")
+    cam = plugin.Renderer.make_params(view, plugin.HairGroom(), mesh, 8, [scene.key_light()])[0]
+    color = np.ascontiguousarray(colors, np.float32).reshape(32, 4)
+    depth = np.ascontiguousarray(view_dists, np.float32).reshape(32)
+    tc = np.ascontiguousarray(target_color, np.float32).reshape(32, 4).copy()
+    td = np.ascontiguousarray(target_depth, np.float32).reshape(32).copy()
+    L.or_composite(C.byref(cam), color.ctypes.data, depth.ctypes.data, tc.ctypes.data, td.ctypes.data)
+    return cam, tc, td
+
+
+def test_composite_depth_blend_and_discard():
+    rng = np.random.default_rng(8)
+    colors = rng.uniform(0.0, 1.0, (32, 4)).astype(np.float32)
+    colors[:, 3] = rng.uniform(0.2, 1.0, 32)
+    colors[3, 3] = 0.0009  # below the discard threshold
+    dists = rng.uniform(1.0, 9.0, 32).astype(np.float32)
+    dst = rng.uniform(0.0, 1.0, (32, 4)).astype(np.float32)
+    # reverse Z (Bevy's infinite reverse perspective): clip depth = near / view distance
+    cam, tc, td = composite_case(True, colors, dists, np.full(32, 0.0, np.float32), dst)
+    near = 0.1  # the projection's near plane
+    want_d = near / dists.astype(np.float64)
+    hit = np.arange(32) != 3
+    assert np.allclose(td[hit], want_d[hit], rtol=2e-5)
+    a = colors[:, 3:4].astype(np.float64)
+    want_rgb = colors[:, :3] * a + dst[:, :3] * (1.0 - a)
+    want_a = colors[:, 3] + dst[:, 3] * (1.0 - colors[:, 3])
+    assert np.allclose(tc[hit, :3], want_rgb[hit], rtol=1e-6, atol=1e-7)
+    assert np.allclose(tc[hit, 3], want_a[hit], rtol=1e-6)
+    assert td[3] == 0.0 and (tc[3] == dst[3]).all()  # discarded
+    # a nearer stored depth (larger, reverse Z) keeps the target; equal passes
+    stored = want_d.astype(np.float32).copy()
+    stored[0::2] = np.minimum(1.0, stored[0::2] * 1.01)
+    _, tc2, td2 = composite_case(True, colors, dists, stored, dst)
+    assert (tc2[0::2] == dst[0::2]).all() and (td2[0::2] == stored[0::2]).all()
+    assert not np.allclose(tc2[1::2], dst[1::2])
+    # forward Z compares <=: a nearer (smaller) stored depth keeps the target
+    _, tcf, tdf = composite_case(False, colors, dists, np.full(32, 1.0, np.float32), dst)
+    assert (tdf[hit] < 1.0).all() and (tdf[hit] > 0.0).all()
+    _, tcf2, _ = composite_case(False, colors, dists, np.zeros(32, np.float32), dst)
+    assert (tcf2 == dst).all()
