@@ -210,6 +210,9 @@ ABI = {
     "hm_bind_scene_depth": (C.c_int, [P, P, f32]),
     "hm_composite": (C.c_int, [P, C.POINTER(HmParams), P, P]),
     "hm_shadow_factors": (C.c_int, [P, C.POINTER(HmShadowLight), P, u32, P, u32, P]),
+    "hm_bake": (C.c_int, [P, u32, P, u32, C.POINTER(C.c_void_p)]),
+    "hm_bake_get": (C.c_int, [P, C.POINTER(HmGroomDesc)]),
+    "hm_bake_free": (None, [P]),
 }
 SCAN_NOALLOC_MAX_BUNDLES = 65536
 

@@ -16,7 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhairmi.so")
-SOURCES = ["hm_kernels.hip", "hm_binned.hip", "hm_mlp.hip", "hm_noalloc.hip", "hm_compose.hip", "hm_host.cpp", "hm_mlp_host.cpp",
+SOURCES = ["hm_kernels.hip", "hm_binned.hip", "hm_mlp.hip", "hm_noalloc.hip", "hm_compose.hip", "hm_host.cpp", "hm_bake.cpp", "hm_mlp_host.cpp",
            "hm_batch.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

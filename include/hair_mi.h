@@ -336,6 +336,22 @@ hm_status hm_dom(hm_ctx* ctx, hm_groom* groom, const hm_params* depth_params,
 hm_status hm_raster_count(hm_ctx* ctx, hm_groom* groom, const hm_params* params,
                           int pass, uint64_t counts[4]);
 
+/* ---- offline baker ------------------------------------------------------ */
+
+/* bake_hair_mesh (bake.rs:89-213): explicit strands -> hair-mesh cages with
+ * style volumes and ambient occlusion, the arrays hm_groom_upload takes.
+ * `points` float[point_count][3]; `strands` (first point, point count) pairs
+ * per strand, as HairStrands holds them (hair_file.rs:9-15). Roots are
+ * clustered in a 3D grid (the reference's scalp-mesh clustering is not part
+ * of this entry point). Host computation, no device and no context. */
+typedef struct hm_bake_result hm_bake_result;
+hm_status hm_bake(const float* points, uint32_t point_count, const uint32_t* strands, uint32_t strand_count,
+                  hm_bake_result** out);
+/* The baked mesh as an hm_groom_desc whose pointers stay valid until
+ * hm_bake_free (bundle_count 0 for the empty mesh). */
+hm_status hm_bake_get(const hm_bake_result* result, hm_groom_desc* desc);
+void hm_bake_free(hm_bake_result* result);
+
 /* ---- frame driver ------------------------------------------------------- */
 
 /* One groom of hair_pass (pass.rs:379-619): camera lod/scan, hm_gbuffer,

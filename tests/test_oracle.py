@@ -19,6 +19,7 @@ import pytest
 import oracle
 from bevy_hair_amd import _native as N
 from bevy_hair_amd.groom import synthetic_groom
+from tests.mesh_ref import styled_position
 from tests.oracle_scenes import line_mesh, refs_for, screen_params
 
 L = oracle.lib()
@@ -183,61 +184,6 @@ def test_dom_optical_depth_known_answers():
 
 
 # ---- strand generation vs mesh.rs::styled_position -----------------------------------------
-
-
-def styled_position(mesh, b, uv, w):
-    """float64 restatement of src/mesh.rs:100-175."""
-    bundle = mesh.bundles[b]
-    lay = mesh.layers.astype(np.float64)
-
-    def corners(layer):
-        o = int(bundle[0]) + layer * 4
-        return lay[o:o + 4, :3]
-
-    def bil(q):
-        u, v = uv
-        return q[0] * (1 - u) * (1 - v) + q[1] * u * (1 - v) + q[2] * (1 - u) * v + q[3] * u * v
-
-    def tangent(layer):
-        curr = bil(corners(layer))
-        prev = bil(corners(max(layer - 1, 0)))
-        nxt = bil(corners(min(layer + 1, 15)))
-        if layer == 0:
-            d, s = nxt - curr, np.linalg.norm(nxt - curr)
-        elif layer + 1 >= 16:
-            d, s = curr - prev, np.linalg.norm(curr - prev)
-        else:
-            d, s = nxt - prev, np.linalg.norm(nxt - prev) * 0.5
-        n = np.linalg.norm(d)
-        d = d / n if n > 0 else np.array([0.0, 1.0, 0.0])
-        return d * max(s, 1e-4)
-
-    lf = min(max(w, 0.0), 1.0) * 15.0
-    i0 = min(int(np.floor(lf)), 15)
-    i1 = min(i0 + 1, 15)
-    t = lf - i0
-    p0, p1 = bil(corners(i0)), bil(corners(i1))
-    m0, m1 = tangent(i0), tangent(i1)
-    t2, t3 = t * t, t * t * t
-    base = p0 * (2 * t3 - 3 * t2 + 1) + m0 * (t3 - 2 * t2 + t) + p1 * (-2 * t3 + 3 * t2) + m1 * (t3 - t2)
-    st = mesh.style.astype(np.float64)
-    so = int(bundle[1])
-    u, v, ww = min(max(uv[0], 0), 1) * 7, min(max(uv[1], 0), 1) * 7, min(max(w, 0), 1) * 15
-    u0, v0, w0 = int(np.floor(u)), int(np.floor(v)), int(np.floor(ww))
-    u1, v1, w1 = min(u0 + 1, 7), min(v0 + 1, 7), min(w0 + 1, 15)
-    tu, tv, tw = u - u0, v - v0, ww - w0
-
-    def at(a, b_, c):
-        return st[so + (c * 8 + b_) * 8 + a, :3]
-
-    def lerp(a, b_, t_):
-        return a + (b_ - a) * t_
-
-    c00 = lerp(at(u0, v0, w0), at(u1, v0, w0), tu)
-    c01 = lerp(at(u0, v1, w0), at(u1, v1, w0), tu)
-    c10 = lerp(at(u0, v0, w1), at(u1, v0, w1), tu)
-    c11 = lerp(at(u0, v1, w1), at(u1, v1, w1), tu)
-    return base + lerp(lerp(c00, c01, tv), lerp(c10, c11, tv), tw)
 
 
 def test_eval_cp_matches_mesh_rs_mirror():
