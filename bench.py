@@ -303,6 +303,37 @@ def heavy_leg_run(r, args) -> dict:
     return out
 
 
+def lights_leg_run(r, g, sc, w, args) -> dict:
+    """The same groom and view under 1 key + 1024 short-range point and spot
+    lights, three rect lights and a studio probe: frames/s and shade time
+    with the lights culled per wave (the automatic choice here) and without."""
+    from bevy_hair_amd import scene
+
+    centre = 0.5 * (sc.mesh.bounds_min + sc.mesh.bounds_max).astype(np.float64)
+    radius = 0.5 * float(np.linalg.norm(sc.mesh.bounds_max - sc.mesh.bounds_min))
+    pts = scene.scattered_lights(np.random.default_rng(5), 1024, centre, radius, 0.15, 0.6)
+    keep = r.get_tuning()
+    opts = r.frame_opts(control_points=w.control_points, strand_fraction=w.strand_fraction)
+    out = {"point_and_spot_lights": len(pts), "rect_lights": 3, "probe": "64^2 x 7 specular, 16^2 diffuse"}
+    try:
+        r.set_lights(pts, sc.dirs)
+        r.set_rect_lights(scene.studio_rects(centre, radius))
+        r.set_environment(scene.studio_environment(64, 7, 16, intensity=0.8))
+        steps = max(5, args.steps // 5)
+        for name, cull in (("culled_per_wave", 0), ("every_light_per_pixel", 1)):
+            r.set_tuning(light_cull=cull)
+            for _ in range(2):
+                r.hair_pass(g, sc.params, opts)
+            ms, _, kt = timed_frames(r, g, sc.params, opts, steps)
+            out[name] = {"frames_per_s": round(1000.0 / ms, 2), "shade_ms": round(kt["shade"], 4)}
+    finally:
+        r.set_tuning(**keep)
+        r.set_rect_lights([])
+        r.set_environment(None)
+        r.set_lights(sc.points, sc.dirs)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,6 +349,7 @@ def main() -> int:
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-shading leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view leg")
     ap.add_argument("--no-heavy", action="store_true", help="skip the 2160p heavy-workload leg")
+    ap.add_argument("--no-lights", action="store_true", help="skip the many-lights / probe leg")
     ap.add_argument("--mlp-tolerance", type=float, default=0.0,
                     help="per-tile probe bound of the learned shading (0: trained-range fallback only)")
     args = ap.parse_args()
@@ -462,6 +494,7 @@ def main() -> int:
             rates["cam_raster_frac_of_u64_min_peak"] = round(rates["cam_raster_atomics_per_s"] / peaks["u64_min_per_s"], 4)
         if rates["dom_opacity_atomics_per_s"]:
             rates["dom_opacity_frac_of_u32_add_peak"] = round(rates["dom_opacity_atomics_per_s"] / peaks["u32_add_per_s"], 4)
+    result["many_lights"] = None if args.no_lights else lights_leg_run(r, g, sc, w, args)
     g.free()
     result["batch"] = None if args.no_batch else batch_leg_run(r, args)
     result["heavy"] = None if args.no_heavy else heavy_leg_run(r, args)

@@ -59,6 +59,31 @@ class HmDirectionalLight(C.Structure):
     ]
 
 
+class HmRectLight(C.Structure):
+    _fields_ = [
+        ("position", f32 * 4),
+        ("right", f32 * 4),
+        ("up", f32 * 4),
+        ("color", f32 * 4),
+        ("width", f32),
+        ("height", f32),
+        ("range", f32),
+        ("_pad", f32),
+    ]
+
+
+class HmEnvProbe(C.Structure):
+    _fields_ = [
+        ("specular", C.c_void_p),
+        ("specular_size", u32),
+        ("specular_mips", u32),
+        ("diffuse", C.c_void_p),
+        ("diffuse_size", u32),
+        ("view_rotation", f32 * 4),
+        ("intensity", f32),
+    ]
+
+
 class HmGroomConfig(C.Structure):
     _fields_ = [
         ("world_from_model", f32 * 16),
@@ -112,7 +137,7 @@ class HmBatchView(C.Structure):
 class HmTuning(C.Structure):
     _fields_ = [("raster_flags", u32 * 3), ("raster_blocks_per_cu", u32 * 3), ("dom_tile", u32), ("cam_tile", u32),
                 ("bin_box_limit", u32), ("bin_records", u32), ("bin_blocks_per_cu", u32),
-                ("cam_chunk", u32), ("cam_bin_blocks_per_cu", u32)]
+                ("cam_chunk", u32), ("cam_bin_blocks_per_cu", u32), ("light_cull", u32)]
 
 
 RASTER_PRECHECK, RASTER_XCD = 1, 2
@@ -131,6 +156,8 @@ assert C.sizeof(HmParams) == 432
 assert C.sizeof(HmPointLight) == 64
 assert C.sizeof(HmDirectionalLight) == 48
 assert C.sizeof(HmShadowLight) == 112
+assert C.sizeof(HmRectLight) == 80
+assert C.sizeof(HmEnvProbe) == 48
 
 # hm_buffer
 BUF_CENTER, BUF_CONSERVATIVE, BUF_BETA, BUF_SHADED, BUF_SHADED_DEPTH = 0, 1, 2, 3, 4
@@ -162,6 +189,8 @@ ABI = {
     "hm_targets": (C.c_int, [P, u32, u32, u32]),
     "hm_set_scene_depth": (C.c_int, [P, P, f32]),
     "hm_set_lights": (C.c_int, [P, P, u32, P, u32]),
+    "hm_set_rect_lights": (C.c_int, [P, P, u32]),
+    "hm_set_env_probe": (C.c_int, [P, P]),
     "hm_make_params": (C.c_int, [C.POINTER(HmView), C.POINTER(HmGroomConfig), f32 * 3, f32 * 3, u32,
                                  f32 * 3, u32, u32, u32, C.POINTER(HmParams)]),
     "hm_key_direction": (None, [P, u32, f32 * 3]),

@@ -18,7 +18,7 @@ struct hm_batch {
     hipEvent_t ev[2] = {nullptr, nullptr};
     // the context's buffers the captured launches read: their allocation
     // generation, and the light counts baked into the launches
-    uint64_t alloc_gen = 0;
+    uint64_t alloc_gen = 0, shade_gen = 0;
     uint32_t point_count = 0, dir_count = 0;
 };
 
@@ -117,6 +117,7 @@ hm_status hm_batch_create(hm_ctx* ctx, const hm_batch_view* views, uint32_t coun
         return st;
     }
     b->alloc_gen = ctx->alloc_gen;
+    b->shade_gen = ctx->shade_gen;
     b->point_count = ctx->point_count;
     b->dir_count = ctx->dir_count;
     *out = b;
@@ -129,6 +130,9 @@ hm_status hm_batch_run(hm_ctx* ctx, hm_batch* b, uint32_t reps) {
         return hm_fail(ctx, HM_ERR_NOT_READY, "context buffers were reallocated since the batch was created; create it again");
     if (b->point_count != ctx->point_count || b->dir_count != ctx->dir_count)
         return hm_fail(ctx, HM_ERR_NOT_READY, "the light counts changed since the batch was created; create it again");
+    if (b->shade_gen != ctx->shade_gen)
+        return hm_fail(ctx, HM_ERR_NOT_READY,
+                       "the rect lights or the environment probe changed since the batch was created; create it again");
     for (uint32_t r = 0; r < reps; ++r) BATCH_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
     return HM_OK;
 }

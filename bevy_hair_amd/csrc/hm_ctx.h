@@ -8,6 +8,14 @@
 
 struct hm_mlp_state;  // learned shading (hm_mlp_host.cpp)
 
+// The last per-wave culling choice, for the light list and groom box it
+// was made for.
+struct hm_cull_cache {
+    bool valid = false, cull = false;
+    uint64_t light_gen = 0;
+    float box[6] = {0, 0, 0, 0, 0, 0};
+};
+
 struct hm_groom {
     uint32_t nb = 0, nl = 0, ns = 0, strands = 0, capacity = 0;
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
@@ -78,8 +86,23 @@ struct hm_ctx {
     uint32_t point_count = 0, point_cap = 0;
     hm_directional_light* dirs = nullptr;
     uint32_t dir_count = 0, dir_cap = 0;
+    std::vector<hm_point_light> host_points;  // the point/spot list as last set
+    uint64_t light_gen = 0;                     // bumped by every hm_set_lights
+    hm_cull_cache cull_cache;
+    // World box of the groom being shaded (hm_frame), for the choice to cull
+    // lights per wave; unset outside hm_frame.
+    bool groom_box_set = false;
+    float groom_box[6] = {0, 0, 0, 0, 0, 0};
+    hm_rect_light* rects = nullptr;
+    uint32_t rect_count = 0, rect_cap = 0;
+    float* env_maps = nullptr;  // the probe's specular levels, then its diffuse map (RGBA float)
+    size_t env_bytes = 0;
+    hm::EnvArgs env = {};       // env.on == 0: no environment probe
+    // Bumped when a value the shading launches take by value changes (the
+    // rect light count, the probe's sizes, rotation or intensity).
+    uint64_t shade_gen = 0;
     uint64_t* counters = nullptr;
-    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u, 2048u, 4u};
+    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u, 2048u, 4u, 0u};
     hm_bin_state bin;
     // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
     uint32_t slots = 0, frame = 0;
@@ -110,6 +133,8 @@ hm_status hm_hip_fail(hm_ctx* ctx, hipError_t e, const char* what);
 void hm_mlp_release(hm_ctx* ctx);
 // Looks up the allocation-free scan's scratch address on `device` (current).
 hipError_t hm_noalloc_setup(int device);
+// The light lists and the probe of the context, into shading arguments.
+void hm_shade_lights(hm_ctx* ctx, hm::ShadeArgs& a);
 // The shade list for `lists`, compacting it first where needed (enqueued on the stream).
 hm_status hm_shade_list(hm_ctx* ctx, const hm_lists& lists, hm::PixelList* out);
 hm_status hm_shade_mlp_impl(hm_ctx* ctx, const hm_params* p, float exposure, float tolerance, uint32_t* fallback,

@@ -495,6 +495,8 @@ void hm_destroy(hm_ctx* c) {
     hm_mlp_release(c);
     dev_free(c->points);
     dev_free(c->dirs);
+    dev_free(c->rects);
+    dev_free(c->env_maps);
     dev_free(c->counters);
     for (hipEvent_t e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -703,6 +705,152 @@ hm_status hm_set_lights(hm_ctx* ctx, const hm_point_light* points, uint32_t np, 
     if (nd) HM_HIP(ctx, hipMemcpy(ctx->dirs, dirs, (size_t)nd * sizeof(hm_directional_light), hipMemcpyHostToDevice));
     ctx->point_count = np;
     ctx->dir_count = nd;
+    ctx->host_points.assign(points, points + np);
+    ctx->light_gen++;
+    return HM_OK;
+}
+
+// Whether culling point and spot lights per wave can pay. The culling costs
+// a fixed few hundred instructions per wave and saves the range test of
+// each light that reaches none of the wave's pixels, so it pays when, over
+// the groom's box (grown by 5 %), the lights miss a good share of it: the
+// share of a 3 x 3 x 3 grid of points in the box beyond each light's range,
+// summed over the lights, must reach 16. Without a box (shading called on
+// its own), from 64 lights on. Either choice gives the same colours.
+static bool cull_pays(hm_ctx* ctx) {
+    if (ctx->point_count == 0 || ctx->tuning.light_cull == 1u) return false;
+    if (ctx->tuning.light_cull == 2u) return true;
+    if (!ctx->groom_box_set) return ctx->point_count >= 64u;
+    hm_cull_cache& c = ctx->cull_cache;
+    if (c.valid && c.light_gen == ctx->light_gen && std::memcmp(c.box, ctx->groom_box, sizeof(c.box)) == 0)
+        return c.cull;
+    const float* b = ctx->groom_box;
+    float missed = 0.0f;
+    for (const hm_point_light& L : ctx->host_points) {
+        const float inv = L.color_inverse_square_range[3];
+        if (!(inv > 0.0f)) continue;  // unbounded range
+        int outside = 0;
+        for (int i = 0; i < 27; ++i) {
+            float d2 = 0.0f;
+            for (int k = 0, q = i; k < 3; ++k, q /= 3) {
+                const float x = b[k] + 0.5f * (float)(q % 3) * (b[3 + k] - b[k]) - L.position_radius[k];
+                d2 += x * x;
+            }
+            outside += d2 * inv >= 1.0f;
+        }
+        missed += (float)outside / 27.0f;
+    }
+    c.valid = true;
+    c.light_gen = ctx->light_gen;
+    std::memcpy(c.box, ctx->groom_box, sizeof(c.box));
+    c.cull = missed >= 16.0f;
+    return c.cull;
+}
+
+// The groom's world box for cull_pays while hm_frame shades.
+struct GroomBox {
+    hm_ctx* ctx;
+    GroomBox(hm_ctx* c, const hm_groom* g, const hm_params& cam) : ctx(c) {
+        const float* m = cam.world_from_model;  // column-major
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int corner = 0; corner < 8; ++corner) {
+            const float x = corner & 1 ? g->bmax[0] : g->bmin[0];
+            const float y = corner & 2 ? g->bmax[1] : g->bmin[1];
+            const float z = corner & 4 ? g->bmax[2] : g->bmin[2];
+            for (int k = 0; k < 3; ++k) {
+                const float w = m[k] * x + m[4 + k] * y + m[8 + k] * z + m[12 + k];
+                lo[k] = std::min(lo[k], w);
+                hi[k] = std::max(hi[k], w);
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            const float grow = 0.05f * (hi[k] - lo[k]) + 1e-4f;
+            ctx->groom_box[k] = lo[k] - grow;
+            ctx->groom_box[3 + k] = hi[k] + grow;
+        }
+        ctx->groom_box_set = std::isfinite(lo[0] + lo[1] + lo[2] + hi[0] + hi[1] + hi[2]);
+    }
+    ~GroomBox() { ctx->groom_box_set = false; }
+};
+
+extern "C++" void hm_shade_lights(hm_ctx* ctx, hm::ShadeArgs& a) {
+    a.cull = cull_pays(ctx) ? 1u : 0u;
+    a.points = ctx->points;
+    a.point_count = ctx->point_count;
+    a.directionals = ctx->dirs;
+    a.directional_count = ctx->dir_count;
+    a.rects = ctx->rects;
+    a.rect_count = ctx->rect_count;
+    a.env = ctx->env;
+}
+
+hm_status hm_set_rect_lights(hm_ctx* ctx, const hm_rect_light* rects, uint32_t count) {
+    static_assert(sizeof(hm_rect_light) == 80, "hm_rect_light layout");
+    if (!ctx || (count && !rects)) return fail(ctx, HM_ERR_INVALID, "null rect light array");
+    (void)hipSetDevice(ctx->device);
+    HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (count > ctx->rect_cap) {
+        dev_free(ctx->rects);
+        ctx->rect_cap = 0;
+        ctx->rect_count = 0;
+        ctx->alloc_gen++;
+        HM_HIP(ctx, dev_alloc(&ctx->rects, (size_t)count * sizeof(hm_rect_light)));
+        ctx->rect_cap = count;
+    }
+    if (count) HM_HIP(ctx, hipMemcpy(ctx->rects, rects, (size_t)count * sizeof(hm_rect_light), hipMemcpyHostToDevice));
+    if (count != ctx->rect_count) ctx->shade_gen++;
+    ctx->rect_count = count;
+    return HM_OK;
+}
+
+hm_status hm_set_env_probe(hm_ctx* ctx, const hm_env_probe* probe) {
+    if (!ctx) return HM_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    HM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->shade_gen++;
+    if (!probe) {
+        ctx->env.on = 0u;
+        return HM_OK;
+    }
+    const uint32_t S = probe->specular_size, M = probe->specular_mips, D = probe->diffuse_size;
+    uint32_t levels = 1;
+    while ((S >> levels) != 0u) ++levels;  // floor(log2 S) + 1
+    if (!probe->specular || !probe->diffuse || S == 0u || S > 16384u || D == 0u || D > 16384u || M == 0u ||
+        M > hm::ENV_MAX_MIPS || M > levels)
+        return fail(ctx, HM_ERR_INVALID,
+                    "hm_set_env_probe: cube maps must be non-null, sizes in [1, 16384], 1 <= mips <= "
+                    "min(16, log2(size) + 1)");
+    hm::EnvArgs e = {};
+    size_t texels = 0;
+    for (uint32_t m = 0; m < M; ++m) {
+        e.spec_base[m] = (uint32_t)texels;
+        const size_t sm = S >> m;
+        texels += 6u * sm * sm;
+    }
+    const size_t spec_texels = texels, diff_texels = 6u * (size_t)D * D;
+    if (spec_texels + diff_texels > 0xffffffffull)
+        return fail(ctx, HM_ERR_INVALID, "hm_set_env_probe: cube maps too large");
+    const size_t bytes = (spec_texels + diff_texels) * 16u;
+    if (bytes > ctx->env_bytes) {
+        ctx->env.on = 0u;
+        dev_free(ctx->env_maps);
+        ctx->env_bytes = 0;
+        ctx->alloc_gen++;
+        HM_HIP(ctx, dev_alloc(&ctx->env_maps, bytes));
+        ctx->env_bytes = bytes;
+    }
+    HM_HIP(ctx, hipMemcpy(ctx->env_maps, probe->specular, spec_texels * 16u, hipMemcpyHostToDevice));
+    HM_HIP(ctx, hipMemcpy(ctx->env_maps + spec_texels * 4u, probe->diffuse, diff_texels * 16u,
+                          hipMemcpyHostToDevice));
+    e.specular = reinterpret_cast<const float4*>(ctx->env_maps);
+    e.diffuse = reinterpret_cast<const float4*>(ctx->env_maps + spec_texels * 4u);
+    e.spec_size = S;
+    e.spec_mips = M;
+    e.diff_size = D;
+    for (int k = 0; k < 4; ++k) e.rotation[k] = probe->view_rotation[k];
+    e.intensity = probe->intensity;
+    e.on = 1u;
+    ctx->env = e;
     return HM_OK;
 }
 
@@ -1140,10 +1288,7 @@ static hm_status shade_impl(hm_ctx* ctx, const hm_params* p, float exposure, con
     a.center = ctx->center;
     a.beta = ctx->beta;
     a.dom = ctx->dom;
-    a.points = ctx->points;
-    a.point_count = ctx->point_count;
-    a.directionals = ctx->dirs;
-    a.directional_count = ctx->dir_count;
+    hm_shade_lights(ctx, a);
     a.exposure = exposure;
     a.shaded = ctx->shaded;
     a.shaded_depth = ctx->shaded_depth;
@@ -1211,10 +1356,13 @@ hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm
         const bool same_model = std::memcmp(cam.world_from_model, params[1].world_from_model, 64) == 0;
         if ((st = dom_impl(ctx, g, &params[1], &params[2], lists.from_tiles && same_model)) != HM_OK) return st;
     }
-    if (opts && opts->shading == HM_SHADING_MLP) {
-        if ((st = hm_shade_mlp_impl(ctx, &cam, exposure, opts->mlp_tolerance, nullptr, lists)) != HM_OK) return st;
-    } else if ((st = shade_impl(ctx, &cam, exposure, lists)) != HM_OK) {
-        return st;
+    {
+        GroomBox box(ctx, g, cam);
+        if (opts && opts->shading == HM_SHADING_MLP) {
+            if ((st = hm_shade_mlp_impl(ctx, &cam, exposure, opts->mlp_tolerance, nullptr, lists)) != HM_OK) return st;
+        } else if ((st = shade_impl(ctx, &cam, exposure, lists)) != HM_OK) {
+            return st;
+        }
     }
     if (cam.flags[1]) {
         if ((st = filter_impl(ctx, &cam, lists)) != HM_OK) return st;
@@ -1342,6 +1490,7 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
     if (t->bin_blocks_per_cu == 0u || t->bin_blocks_per_cu > 64u)
         return fail(ctx, HM_ERR_INVALID, "bin_blocks_per_cu must be in [1, 64]");
     if (t->cam_bin_blocks_per_cu > 64u) return fail(ctx, HM_ERR_INVALID, "cam_bin_blocks_per_cu must be in [0, 64]");
+    if (t->light_cull > 2u) return fail(ctx, HM_ERR_INVALID, "light_cull must be 0, 1 or 2");
 
     ctx->tuning = *t;
     return HM_OK;

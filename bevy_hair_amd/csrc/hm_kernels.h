@@ -42,6 +42,19 @@ struct PixelList {
     uint32_t cap;      // entries per sub-list
 };
 
+// The environment probe as the shading kernels read it: RGBA float cube
+// maps in device memory (faces +x, -x, +y, -y, +z, -z; specular levels one
+// after another, level m starting at texel spec_base[m]).
+constexpr uint32_t ENV_MAX_MIPS = 16u;
+struct EnvArgs {
+    const float4* specular;
+    const float4* diffuse;
+    uint32_t spec_size, spec_mips, diff_size, on;
+    float rotation[4];
+    float intensity;
+    uint32_t spec_base[ENV_MAX_MIPS];
+};
+
 struct ShadeArgs {
     hm_params view;
     const uint64_t* center;
@@ -51,10 +64,22 @@ struct ShadeArgs {
     uint32_t point_count;
     const hm_directional_light* directionals;
     uint32_t directional_count;
+    const hm_rect_light* rects;
+    uint32_t rect_count;
     float exposure;
     float4* shaded;
     float* shaded_depth;
+    uint32_t cull;  // cull point and spot lights per wave
+    EnvArgs env;
 };
+
+// Variants of the shading kernels: SHADE_CULL culls point and spot lights
+// per wave; SHADE_EXTRAS evaluates rect lights and the environment probe
+// (compiled out when the view has neither).
+constexpr uint32_t SHADE_CULL = 1u, SHADE_EXTRAS = 2u;
+__host__ __device__ inline uint32_t shade_variant(const ShadeArgs& a) {
+    return (a.cull ? SHADE_CULL : 0u) | ((a.rect_count || a.env.on) ? SHADE_EXTRAS : 0u);
+}
 
 struct FilterArgs {
     hm_params view;

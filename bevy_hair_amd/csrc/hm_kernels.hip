@@ -310,20 +310,40 @@ __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ ce
     }
 }
 
+// F: SHADE_* variant bits (pixel_color).
+template <uint32_t F>
 __global__ __launch_bounds__(256, 3) void k_shade(ShadeArgs a, PixelList L) {
     __shared__ uint32_t pre[LIST_SUBS + 1];
     list_prefix(L, pre);
     const View& p = *reinterpret_cast<const View*>(&a.view);
     const uint32_t width = to_u32(p.screen.x), pixels = width * to_u32(p.screen.y);
     const uint32_t n = pre[LIST_SUBS];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t pix = list_entry(L, pre, i);
-        if (pix >= pixels) continue;
-        uint32_t x = pix % width, y = pix / width;
-        PixelShade px = pixel_setup(p, a.center[pix], a.beta[pix], x, y);
-        F3 color = plain_pixel_color(a, p, px);
-        a.shaded[pix] = make_float4(color.x, color.y, color.z, 1.0f);
-        a.shaded_depth[pix] = px.view_dist;
+    if (!(F & SHADE_CULL)) {
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            const uint32_t pix = list_entry(L, pre, i);
+            if (pix >= pixels) continue;
+            const PixelShade px = pixel_setup(p, a.center[pix], a.beta[pix], pix % width, pix / width);
+            const F3 color = plain_pixel_color<F>(a, p, px, true);
+            a.shaded[pix] = make_float4(color.x, color.y, color.z, 1.0f);
+            a.shaded_depth[pix] = px.view_dist;
+        }
+        return;
+    }
+    // Whole waves step together (the light culling works across the wave):
+    // the grid stride and the block are multiples of 64.
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t first = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); first < n; first += gridDim.x * blockDim.x) {
+        const uint32_t i = first + lane;
+        const uint32_t pix = i < n ? list_entry(L, pre, i) : pixels;
+        const bool valid = pix < pixels;
+        PixelShade px;
+        px.world = splat3(0.0f);
+        if (valid) px = pixel_setup(p, a.center[pix], a.beta[pix], pix % width, pix / width);
+        const F3 color = plain_pixel_color<F>(a, p, px, valid);
+        if (valid) {
+            a.shaded[pix] = make_float4(color.x, color.y, color.z, 1.0f);
+            a.shaded_depth[pix] = px.view_dist;
+        }
     }
 }
 
@@ -566,7 +586,13 @@ hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const
 hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, const PixelList& list, uint32_t pixels, int blocks) {
     if (pixels == 0) return hipSuccess;
     uint32_t want = (pixels + 255) / 256;
-    hipLaunchKernelGGL(k_shade, dim3(want < (uint32_t)blocks ? want : (uint32_t)blocks), dim3(256), 0, s, a, list);
+    const dim3 grid(want < (uint32_t)blocks ? want : (uint32_t)blocks);
+    switch (shade_variant(a)) {
+        case 0u: hipLaunchKernelGGL(k_shade<0u>, grid, dim3(256), 0, s, a, list); break;
+        case SHADE_CULL: hipLaunchKernelGGL(k_shade<SHADE_CULL>, grid, dim3(256), 0, s, a, list); break;
+        case SHADE_EXTRAS: hipLaunchKernelGGL(k_shade<SHADE_EXTRAS>, grid, dim3(256), 0, s, a, list); break;
+        default: hipLaunchKernelGGL(k_shade<SHADE_CULL | SHADE_EXTRAS>, grid, dim3(256), 0, s, a, list); break;
+    }
     return hipGetLastError();
 }
 

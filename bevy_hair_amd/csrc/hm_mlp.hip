@@ -507,9 +507,10 @@ __device__ __forceinline__ F3 lane_read3(F3 v, int lane) {
 
 // pixel_color of one pixel (the same in every lane) with two sets of terms,
 // `net` and `ref`, evaluated by the whole wave: lane i takes the point lights
-// i, i + 64, ... and the directional lights i, i + 64, ...; only the
+// i, i + 64, ..., the directional and rect lights likewise; only the
 // summation order differs from pixel_color. The geometry and single
 // scattering are shared by the two colours.
+template <bool EXTRAS>
 __device__ void color_pair_by_wave(const ShadeArgs& a, const PixelShade& px, const LocalBack& net, F3 net_ms,
                                    const LocalBack& ref, F3 ref_ms, int key, float depth, int lane, F3& got,
                                    F3& want) {
@@ -540,8 +541,18 @@ __device__ void color_pair_by_wave(const ShadeArgs& a, const PixelShade& px, con
             cr = cr + (single + 0.7f * local_back_r(g.theta_h, ref.a_back, ref.delta_b, ref.inv_sb)) * w;
         }
     }
-    got = F3{wave_sum(cn.x), wave_sum(cn.y), wave_sum(cn.z)} * a.exposure;
-    want = F3{wave_sum(cr.x), wave_sum(cr.y), wave_sum(cr.z)} * a.exposure;
+    F3 env = splat3(0.0f);
+    if (EXTRAS) {
+        for (uint32_t i = (uint32_t)lane; i < a.rect_count; i += 64u) {
+            const hm_rect_light& L = a.rects[i];
+            cn = cn + rect_term(L, px, net);
+            cr = cr + rect_term(L, px, ref);
+        }
+        // the probe does not depend on the dual-scattering terms
+        if (a.env.on) env = probe_radiance(a.env, px) * px.ao;
+    }
+    got = (F3{wave_sum(cn.x), wave_sum(cn.y), wave_sum(cn.z)} + env) * a.exposure;
+    want = (F3{wave_sum(cr.x), wave_sum(cr.y), wave_sum(cr.z)} + env) * a.exposure;
 }
 
 // dual_scatter of one pixel (the same in every lane) by the whole wave:
@@ -569,7 +580,9 @@ __device__ DualScatter dual_scatter_by_wave(const PixelShade& px, int lane) {
 // registers) so that its 30 values are not held through it.
 constexpr uint32_t PROBE_STRIDE = 4;
 
+template <uint32_t F>  // SHADE_* variant bits (pixel_color)
 __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs m, PixelList L) {
+    constexpr bool CULL = (F & SHADE_CULL) != 0u;
     __shared__ bf16x8 s_tiles[MLP_TILES * 64];
     __shared__ __align__(16) float s_bias[MLP_NBIAS];
     __shared__ float s_lo[MLP_IN], s_hi[MLP_IN];
@@ -607,6 +620,7 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
         uint64_t word = 0;
         uint32_t beta_word = 0;
         PixelShade px;  // kept across the network for the light loop
+        px.world = splat3(0.0f);
         {
             float f[32];
             for (int k = 0; k < 32; ++k) f[k] = 0.0f;
@@ -669,21 +683,31 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
                 const F3 net_ms = key >= 0 ? key_multiple_scattering(th, pd, pp.shape, dn) : splat3(0.0f);
                 const F3 key_ms = key >= 0 ? key_multiple_scattering(th, pd, pp.shape, ds) : splat3(0.0f);
                 F3 got, ref;
-                color_pair_by_wave(a, pp, local_terms(dn), net_ms, local_terms(ds), key_ms, key, pd, lane, got, ref);
+                color_pair_by_wave<(F & SHADE_EXTRAS) != 0u>(a, pp, local_terms(dn), net_ms, local_terms(ds), key_ms,
+                                                             key, pd, lane, got, ref);
                 const float err = fabsf(got.x - ref.x) + fabsf(got.y - ref.y) + fabsf(got.z - ref.z);
                 const float mag = fabsf(ref.x) + fabsf(ref.y) + fabsf(ref.z);
                 if (err > m.tol * mx(mag, m.floor)) flag = 2u;
             }
         }
         F3 c = splat3(0.0f);
-        if (valid) {
-            // the network's dual-scattering terms, or the plain method's
-            // (plain_pixel_color; `depth` is its key-light depth)
+        if (CULL) {
+            LocalBack lb = {splat3(0.0f), splat3(0.0f), splat3(0.0f)};
+            F3 key_ms = splat3(0.0f);
+            if (valid) {
+                // the network's dual-scattering terms, or the plain method's
+                // (plain_pixel_color; `depth` is its key-light depth)
+                const DualScatter ds = (flag == 2u || outside) ? dual_scatter(px.cos_theta_o, px.sigma_a, px.shape)
+                                                               : network_dual_scatter(o, px.shape);
+                lb = local_terms(ds);
+                if (key >= 0) key_ms = key_multiple_scattering(key_theta_h(px, kd), depth, px.shape, ds);
+            }
+            c = pixel_color<F>(a, px, valid, lb, key, depth, key_ms);  // the whole wave calls
+        } else if (valid) {
             const DualScatter ds = (flag == 2u || outside) ? dual_scatter(px.cos_theta_o, px.sigma_a, px.shape)
                                                            : network_dual_scatter(o, px.shape);
-            const LocalBack lb = local_terms(ds);
             const F3 key_ms = key >= 0 ? key_multiple_scattering(key_theta_h(px, kd), depth, px.shape, ds) : splat3(0.0f);
-            c = pixel_color(a, px, lb, key, depth, key_ms);
+            c = pixel_color<F>(a, px, true, local_terms(ds), key, depth, key_ms);
         }
         if (valid) {
             a.shaded[pix] = make_float4(c.x, c.y, c.z, 1.0f);
@@ -753,7 +777,14 @@ hipError_t launch_shade_mlp(hipStream_t s, const ShadeArgs& a, const MlpShadeArg
     if (m.groups_cap == 0) return hipSuccess;
     const uint32_t want = (m.groups_cap + 3u) / 4u;
     const uint32_t grid = want < (uint32_t)blocks ? want : (uint32_t)blocks;
-    hipLaunchKernelGGL(k_shade_mlp, dim3(grid), dim3(256), 0, s, a, m, L);
+    switch (shade_variant(a)) {
+        case 0u: hipLaunchKernelGGL(k_shade_mlp<0u>, dim3(grid), dim3(256), 0, s, a, m, L); break;
+        case SHADE_CULL: hipLaunchKernelGGL(k_shade_mlp<SHADE_CULL>, dim3(grid), dim3(256), 0, s, a, m, L); break;
+        case SHADE_EXTRAS: hipLaunchKernelGGL(k_shade_mlp<SHADE_EXTRAS>, dim3(grid), dim3(256), 0, s, a, m, L); break;
+        default:
+            hipLaunchKernelGGL(k_shade_mlp<SHADE_CULL | SHADE_EXTRAS>, dim3(grid), dim3(256), 0, s, a, m, L);
+            break;
+    }
     return hipGetLastError();
 }
 

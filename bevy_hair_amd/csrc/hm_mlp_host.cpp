@@ -262,10 +262,7 @@ static void shade_args(hm_ctx* ctx, const hm_params* p, float exposure, hm::Shad
     a.center = ctx->center;
     a.beta = ctx->beta;
     a.dom = ctx->dom;
-    a.points = ctx->points;
-    a.point_count = ctx->point_count;
-    a.directionals = ctx->dirs;
-    a.directional_count = ctx->dir_count;
+    hm_shade_lights(ctx, a);
     a.exposure = exposure;
     a.shaded = ctx->shaded;
     a.shaded_depth = ctx->shaded_depth;

@@ -2,7 +2,9 @@
 //   gaussian3 shade.wesl:51-54, fresnel :56-60, absorption_from_color :70-77,
 //   azimuthal :94-116, lobe_shape :124-128, dual_scatter :153-190,
 //   hair_bsdf :193-244, dom_optical_depth/texel/visibility :246-310,
-//   spot_mask :312-327; Bevy's getDistanceAttenuation restated.
+//   spot_mask :312-327, quat_rotate/cubemap_dir :329-339, probe_radiance
+//   :396-427 (sample_environment :365-382), rect lights :503-518; Bevy's
+//   getRangeFalloff / getDistanceAttenuation restated.
 // Shaded RGB is held to a relative tolerance, so the continuous terms below
 // use the hardware approximations (v_exp_f32, v_log_f32, v_sqrt_f32,
 // v_rcp_f32, v_cos_f32; about 1 ulp) rather than the correctly rounded
@@ -297,10 +299,15 @@ __device__ float dom_visibility(const View& p, const uint32_t* __restrict__ dom,
     return dom_taps_visibility(p, T);
 }
 
-__device__ __forceinline__ float distance_attenuation(float dist2, float inv_range2) {
+// Bevy's range window saturate(1 - (d^2 / r^2)^2)^2: zero from the range on.
+__device__ __forceinline__ float range_falloff(float dist2, float inv_range2) {
     float factor = dist2 * inv_range2;
     float smooth = clampf(1.0f - factor * factor, 0.0f, 1.0f);
-    return smooth * smooth * frcp(mx(dist2, 1e-4f));
+    return smooth * smooth;
+}
+
+__device__ __forceinline__ float distance_attenuation(float dist2, float inv_range2) {
+    return range_falloff(dist2, inv_range2) * frcp(mx(dist2, 1e-4f));
 }
 
 __device__ __forceinline__ float spot_mask(const hm_point_light& L, F3 world) {
@@ -318,7 +325,7 @@ __device__ __forceinline__ float spot_mask(const hm_point_light& L, F3 world) {
 struct PixelShade {
     F3 world, tangent, view, albedo, sigma_a;
     F3 t_n, v_n;  // safe_normalize(tangent), safe_normalize(view)
-    float view_dist, beta, roughness, cos_theta_o;
+    float view_dist, beta, roughness, cos_theta_o, ao;
     Shape shape;
 };
 
@@ -331,6 +338,7 @@ __device__ __forceinline__ PixelShade pixel_setup(const View& p, uint64_t word, 
     s.tangent = oct_decode(gb_tx(word), gb_ty(word));
     float uvw_z = u2f((low >> 6u) & 0x3fu) / 63.0f;
     s.beta = u2f(beta_word & 0xffu) / 255.0f;
+    s.ao = p.flags[2] == 0u ? 1.0f : u2f(low & 0x3fu) / 63.0f;  // shade.wesl:544-547
     s.roughness = mx(p.appearance.x * (0.65f + 0.7f * uvw_z), 0.04f);
     float tilt = p.appearance.y;
     s.albedo = xyz(p.albedo) * (0.65f + 0.35f * uvw_z);
@@ -428,24 +436,203 @@ __device__ __forceinline__ F3 directional_term(const hm_directional_light& L, bo
     return lobe * F3{L.color[0], L.color[1], L.color[2]};
 }
 
-// Colour of one hit pixel (shade.wesl:442-521, 586-588), probe radiance zero
-// (no environment map) and shadow factors 1, from the back-lobe terms `lb`
-// and the key light's multiple scattering `key_ms`. Point lights, then spot
-// lights (the order of Bevy's cluster ranges), then directional lights.
-__device__ F3 pixel_color(const ShadeArgs& a, const PixelShade& px, const LocalBack& lb, int key, float depth,
-                          F3 key_ms) {
-    F3 color = splat3(0.0f);
-    for (int pass = 0; pass < 2; ++pass) {
-        for (uint32_t i = 0; i < a.point_count; ++i) {
-            // read-only, wave-uniform: through the scalar cache in one load
-            // rather than as dependent vector loads
-            const hm_point_light L = uniform_light(a.points, i);
-            if ((pass == 0) != (L.kind == HM_LIGHT_POINT)) continue;
-            color = color + point_term(L, px, lb);
+__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// Rect light L's contribution (shade.wesl:503-518): hair_bsdf with shadow 1
+// and no optical depth, times the rectangle's solid angle and range window.
+__device__ __forceinline__ F3 rect_term(const hm_rect_light& L, const PixelShade& px, const LocalBack& lb) {
+    F3 to_center = F3{L.position[0], L.position[1], L.position[2]} - px.world;
+    float dist2 = mx(dot3(to_center, to_center), 1e-4f);
+    F3 incident = to_center * __builtin_amdgcn_rsqf(dist2);
+    F3 normal = fnormalize(cross3(F3{L.up[0], L.up[1], L.up[2]}, F3{L.right[0], L.right[1], L.right[2]}));
+    float facing = mx(dot3(normal, incident), 0.0f);
+    float falloff = range_falloff(dist2, frcp(mx(L.range * L.range, 1e-4f)));
+    if (facing == 0.0f || falloff == 0.0f) return splat3(0.0f);
+    float solid = L.width * L.height * facing * frcp(dist2);
+    BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, incident);
+    F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
+    return lobe * F3{L.color[0], L.color[1], L.color[2]} * (solid * falloff);
+}
+
+// ---- the environment probe --------------------------------------------------
+
+__device__ __forceinline__ F3 cubemap_dir(const EnvArgs& e, F3 d) {  // quat_rotate, then z negated
+    F3 u = {e.rotation[0], e.rotation[1], e.rotation[2]};
+    F3 t = 2.0f * cross3(u, d);
+    F3 r = d + e.rotation[3] * t + cross3(u, t);
+    return {r.x, r.y, -r.z};
+}
+
+// Vulkan's cube map face selection (ties go to x, then y; a zero or
+// non-finite direction reads the middle of face +x): face and its
+// coordinates s, t in [0, 1].
+struct CubeUV { uint32_t face; float s, t; };
+__device__ __forceinline__ CubeUV cube_face(F3 d) {
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    uint32_t face;
+    float ma, sc, tc;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        face = d.x >= 0.0f ? 0u : 1u;
+        sc = d.x >= 0.0f ? -d.z : d.z;
+        tc = -d.y;
+    } else if (ay >= az) {
+        ma = ay;
+        face = d.y >= 0.0f ? 2u : 3u;
+        sc = d.x;
+        tc = d.y >= 0.0f ? d.z : -d.z;
+    } else {
+        ma = az;
+        face = d.z >= 0.0f ? 4u : 5u;
+        sc = d.z >= 0.0f ? d.x : -d.x;
+        tc = -d.y;
+    }
+    if (!(ma > 0.0f)) return {0u, 0.5f, 0.5f};
+    float r = frcp(ma);
+    return {face, 0.5f * (sc * r + 1.0f), 0.5f * (tc * r + 1.0f)};
+}
+
+// Bilinear inside one face, clamp to edge, RGB of the RGBA texels. The
+// texel coordinate is first held to [-1, size] (which changes no finite
+// result) so that every index stays inside the face.
+__device__ __forceinline__ F3 cube_bilinear(const float4* __restrict__ level, uint32_t size, const CubeUV& c) {
+    const float4* face = level + (size_t)c.face * size * size;
+    const float fs = u2f(size);
+    float u = fminf(fmaxf(c.s * fs - 0.5f, -1.0f), fs);
+    float v = fminf(fmaxf(c.t * fs - 0.5f, -1.0f), fs);
+    float fu = floorf(u), fv = floorf(v);
+    float wx = u - fu, wy = v - fv;
+    int32_t x0 = (int32_t)fu, y0 = (int32_t)fv, hi = (int32_t)size - 1;
+    uint32_t xa = (uint32_t)min(max(x0, 0), hi), xb = (uint32_t)min(max(x0 + 1, 0), hi);
+    uint32_t ya = (uint32_t)min(max(y0, 0), hi), yb = (uint32_t)min(max(y0 + 1, 0), hi);
+    float4 t00 = face[ya * size + xa], t10 = face[ya * size + xb];
+    float4 t01 = face[yb * size + xa], t11 = face[yb * size + xb];
+    F3 top = {mixs(t00.x, t10.x, wx), mixs(t00.y, t10.y, wx), mixs(t00.z, t10.z, wx)};
+    F3 bot = {mixs(t01.x, t11.x, wx), mixs(t01.y, t11.y, wx), mixs(t01.z, t11.z, wx)};
+    return {mixs(top.x, bot.x, wy), mixs(top.y, bot.y, wy), mixs(top.z, bot.z, wy)};
+}
+
+// The specular map at `level` (held to [0, mips - 1]), linear between levels.
+__device__ __forceinline__ F3 cube_sample_spec(const EnvArgs& e, const CubeUV& c, float level) {
+    const float top = u2f(e.spec_mips - 1u);
+    level = fminf(fmaxf(level, 0.0f), top);
+    const float fl = floorf(level);
+    const uint32_t l0 = (uint32_t)fl, l1 = min(l0 + 1u, e.spec_mips - 1u);
+    const F3 a = cube_bilinear(e.specular + e.spec_base[l0], max(e.spec_size >> l0, 1u), c);
+    if (l1 == l0) return a;
+    const F3 b = cube_bilinear(e.specular + e.spec_base[l1], max(e.spec_size >> l1, 1u), c);
+    const float f = level - fl;
+    return {mixs(a.x, b.x, f), mixs(a.y, b.y, f), mixs(a.z, b.z, f)};
+}
+
+// probe_radiance (shade.wesl:396-427) of one hit pixel.
+__device__ F3 probe_radiance(const EnvArgs& e, const PixelShade& px) {
+    const F3 t = px.tangent, v = px.view;
+    const F3 fiber_n = safe_normalize(v - t * dot3(v, t));
+    const float theta_o = fasin(clampf(dot3(t, v), -1.0f, 1.0f));
+    const float theta_i = px.shape.alpha.x - theta_o;  // tilt - theta_o
+    float si, ci;
+    __sincosf(theta_i, &si, &ci);
+    const F3 spec = cube_sample_spec(e, cube_face(cubemap_dir(e, t * si + fiber_n * ci)),
+                                     px.roughness * u2f(e.spec_mips - 1u));
+    const F3 diff = cube_bilinear(e.diffuse, e.diff_size, cube_face(cubemap_dir(e, fiber_n)));
+    const float cos_o = mx(fcos(theta_o), 1e-3f);
+    const float f_r = fresnel(cos_o), f_trt = fresnel(cos_o * 0.5f);
+    const F3 trt = vexp(px.sigma_a * (-3.2f * frcp(cos_o))) * (0.76f * (1.0f - f_trt) * (1.0f - f_trt) * f_trt);
+    // scatter_tint(albedo, 0.5) = (albedo / luminance(albedo))^0.25 (shade.wesl:62-85)
+    const F3 al = px.albedo;
+    const float inv_lum = frcp(mx(al.x * 0.2126f + al.y * 0.7152f + al.z * 0.0722f, 1e-4f));
+    const F3 scatter = al * F3{fpow(al.x * inv_lum, 0.25f), fpow(al.y * inv_lum, 0.25f), fpow(al.z * inv_lum, 0.25f)};
+    return (spec * (splat3(f_r) + trt) + diff * scatter) * e.intensity;
+}
+
+// ---- light culling per wave -------------------------------------------------
+
+// World-space box of the wave's valid pixels (every lane of the wave calls).
+struct WaveBox { F3 lo, hi; };
+__device__ __forceinline__ WaveBox wave_box(F3 w, bool valid) {
+    const float inf = __builtin_huge_valf();
+    F3 lo = valid ? w : splat3(inf), hi = valid ? w : splat3(-inf);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = vmin(lo, F3{__shfl_xor(lo.x, o, 64), __shfl_xor(lo.y, o, 64), __shfl_xor(lo.z, o, 64)});
+        hi = vmax(hi, F3{__shfl_xor(hi.x, o, 64), __shfl_xor(hi.y, o, 64), __shfl_xor(hi.z, o, 64)});
+    }
+    // the same in every lane: held in scalar registers
+    auto u = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+    return {{u(lo.x), u(lo.y), u(lo.z)}, {u(hi.x), u(hi.y), u(hi.z)}};
+}
+
+// Lights base .. base + 63 of the point/spot list (kind point when `points`,
+// else spot) whose range reaches the box, one bit per light (every lane of
+// the wave calls; lane i tests light base + i). A light is dropped only when
+// the box lies beyond its range by a relative margin of 1e-5, far above the
+// rounding of the per-pixel distance: distance_attenuation is then zero at
+// every pixel of the wave, so dropping it changes no colour.
+__device__ __forceinline__ uint64_t reaching_lights(const ShadeArgs& a, const WaveBox& box, uint32_t base,
+                                                    bool points) {
+    const uint32_t j = base + (uint32_t)(threadIdx.x & 63u);
+    bool keep = false;
+    if (j < a.point_count) {
+        const hm_point_light& L = a.points[j];
+        if ((L.kind == HM_LIGHT_POINT) == points) {
+            const F3 c = point_pos(L);
+            const F3 d = vmax(vmax(box.lo - c, c - box.hi), splat3(0.0f));
+            keep = !(dot3(d, d) * L.color_inverse_square_range[3] > 1.00001f);
         }
     }
+    return __ballot(keep);
+}
+
+// Colour of one hit pixel (shade.wesl:442-521, 585-588), Bevy shadow factors
+// 1, from the back-lobe terms `lb` and the key light's multiple scattering
+// `key_ms`. Point lights, then spot lights (the order of Bevy's cluster
+// ranges), then directional, then rect lights; then the probe times the AO.
+// CULL: every lane of the wave calls, `valid` false for lanes without a
+// pixel, and the point and spot lists are culled against the wave's pixels,
+// which is what Bevy's clusters do for a froxel. Otherwise only lanes with
+// a pixel call, and each light's range window is tested per pixel. Rect
+// lights and the probe only in the SHADE_EXTRAS variant.
+template <uint32_t F>
+__device__ F3 pixel_color(const ShadeArgs& a, const PixelShade& px, bool valid, const LocalBack& lb, int key,
+                          float depth, F3 key_ms) {
+    constexpr bool CULL = (F & SHADE_CULL) != 0u;
+    F3 color = splat3(0.0f);
+    if (!CULL) {
+        for (int pass = 0; pass < 2; ++pass) {
+            for (uint32_t i = 0; i < a.point_count; ++i) {
+                // read-only, wave-uniform: through the scalar cache in one
+                // load rather than as dependent vector loads
+                const hm_point_light L = uniform_light(a.points, i);
+                if ((pass == 0) != (L.kind == HM_LIGHT_POINT)) continue;
+                color = color + point_term(L, px, lb);
+            }
+        }
+    } else if (a.point_count) {
+        const WaveBox box = wave_box(px.world, valid);
+        for (int pass = 0; pass < 2; ++pass) {
+            for (uint32_t base = 0; base < a.point_count; base += 64u) {
+                uint64_t reach = reaching_lights(a, box, base, pass == 0);
+                while (reach) {
+                    const uint32_t i = base + (uint32_t)__builtin_ctzll(reach);
+                    reach &= reach - 1u;
+                    // read-only, wave-uniform: through the scalar cache in one
+                    // load rather than as dependent vector loads
+                    const hm_point_light L = uniform_light(a.points, i);
+                    if (valid) color = color + point_term(L, px, lb);
+                }
+            }
+        }
+    }
+    if (!valid) return color;
     for (uint32_t i = 0; i < a.directional_count; ++i)
         color = color + directional_term(uniform_light(a.directionals, i), (int)i == key, px, lb, depth, key_ms);
+    if (F & SHADE_EXTRAS) {
+        for (uint32_t i = 0; i < a.rect_count; ++i) color = color + rect_term(uniform_light(a.rects, i), px, lb);
+        if (a.env.on) color = color + probe_radiance(a.env, px) * px.ao;
+    }
     return color * a.exposure;
 }
 
@@ -456,17 +643,23 @@ __device__ __forceinline__ float key_theta_h(const PixelShade& px, F3 key_dir) {
 
 // The plain method's colour of one hit pixel: dual scattering evaluated
 // (shade.wesl:153-190), the DOM visibility for the first shadow-casting
-// directional light (the key light).
-__device__ F3 plain_pixel_color(const ShadeArgs& a, const View& p, const PixelShade& px) {
-    const DualScatter ds = dual_scatter(px.cos_theta_o, px.sigma_a, px.shape);
+// directional light (the key light). SHADE_CULL: every lane of the wave
+// calls, lanes with `valid` false have no pixel and return zero.
+template <uint32_t F>
+__device__ F3 plain_pixel_color(const ShadeArgs& a, const View& p, const PixelShade& px, bool valid) {
     const int key = key_light(a);
     float depth = 0.0f;
     F3 key_ms = splat3(0.0f);
-    if (key >= 0) {
-        depth = -flog(mx(dom_visibility(p, a.dom, px.world, px.beta), 1e-6f));
-        key_ms = key_multiple_scattering(key_theta_h(px, dir_of(a.directionals[key])), depth, px.shape, ds);
+    LocalBack lb = {splat3(0.0f), splat3(0.0f), splat3(0.0f)};
+    if (valid) {
+        const DualScatter ds = dual_scatter(px.cos_theta_o, px.sigma_a, px.shape);
+        if (key >= 0) {
+            depth = -flog(mx(dom_visibility(p, a.dom, px.world, px.beta), 1e-6f));
+            key_ms = key_multiple_scattering(key_theta_h(px, dir_of(a.directionals[key])), depth, px.shape, ds);
+        }
+        lb = local_terms(ds);
     }
-    return pixel_color(a, px, local_terms(ds), key, depth, key_ms);
+    return pixel_color<F>(a, px, valid, lb, key, depth, key_ms);
 }
 
 }  // namespace hm

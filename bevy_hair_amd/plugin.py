@@ -69,6 +69,34 @@ def directional_light(d: dict) -> N.HmDirectionalLight:
     return L
 
 
+def rect_light(d: dict) -> N.HmRectLight:
+    L = N.HmRectLight()
+    L.position[:] = d["position"]
+    L.right[:] = d["right"]
+    L.up[:] = d["up"]
+    L.color[:] = d["color"]
+    L.width, L.height, L.range = d["width"], d["height"], d["range"]
+    return L
+
+
+def rect_arrays(rects: list[dict]):
+    return (N.HmRectLight * max(1, len(rects)))(*[rect_light(r) for r in rects]), len(rects)
+
+
+class EnvProbeDesc:
+    """hm_env_probe over an EnvironmentMapLight, holding the packed maps."""
+
+    def __init__(self, env):
+        env.validate()
+        self.spec, self.diff = env.packed()
+        d = N.HmEnvProbe()
+        d.specular, d.specular_size, d.specular_mips = N.ptr(self.spec), env.specular[0].shape[1], len(env.specular)
+        d.diffuse, d.diffuse_size = N.ptr(self.diff), env.diffuse.shape[1]
+        d.view_rotation[:] = [float(c) for c in env.rotation]
+        d.intensity = float(env.intensity)
+        self.desc = d
+
+
 def light_arrays(points: list[dict], dirs: list[dict]):
     pa = (N.HmPointLight * max(1, len(points)))(*[point_light(p) for p in points])
     da = (N.HmDirectionalLight * max(1, len(dirs)))(*[directional_light(d) for d in dirs])
@@ -188,6 +216,20 @@ class Renderer:
         pa, np_, da, nd = light_arrays(points, directionals)
         self._check(N.lib().hm_set_lights(self.ctx, pa, np_, da, nd), "hm_set_lights")
         self.directionals = directionals
+
+    def set_rect_lights(self, rects: list[dict]):
+        """The view's rect lights (shade.wesl:503-518); [] removes them."""
+        ra, n = rect_arrays(rects)
+        self._check(N.lib().hm_set_rect_lights(self.ctx, ra, n), "hm_set_rect_lights")
+
+    def set_environment(self, env):
+        """The view's EnvironmentMapLight (probe_radiance x AO is added to
+        every hit pixel); None removes it."""
+        if env is None:
+            self._check(N.lib().hm_set_env_probe(self.ctx, None), "hm_set_env_probe")
+            return
+        d = EnvProbeDesc(env)
+        self._check(N.lib().hm_set_env_probe(self.ctx, C.byref(d.desc)), "hm_set_env_probe")
 
     @staticmethod
     def make_params(view: View, groom: HairGroom, mesh: HairMesh, dom_size: int, directionals: list[dict]):
@@ -353,7 +395,8 @@ class Renderer:
         return {"raster_flags": list(t.raster_flags), "raster_blocks_per_cu": list(t.raster_blocks_per_cu),
                 "dom_tile": int(t.dom_tile), "cam_tile": int(t.cam_tile), "bin_box_limit": int(t.bin_box_limit),
                 "bin_records": int(t.bin_records), "bin_blocks_per_cu": int(t.bin_blocks_per_cu),
-                "cam_chunk": int(t.cam_chunk), "cam_bin_blocks_per_cu": int(t.cam_bin_blocks_per_cu)}
+                "cam_chunk": int(t.cam_chunk), "cam_bin_blocks_per_cu": int(t.cam_bin_blocks_per_cu),
+                "light_cull": int(t.light_cull)}
 
     def get_tuning(self) -> dict:
         t = N.HmTuning()
@@ -362,7 +405,7 @@ class Renderer:
 
     def set_tuning(self, raster_flags=None, raster_blocks_per_cu=None, dom_tile=None, bin_box_limit=None,
                    bin_records=None, bin_blocks_per_cu=None, cam_tile=None, cam_chunk=None,
-                   cam_bin_blocks_per_cu=None):
+                   cam_bin_blocks_per_cu=None, light_cull=None):
         """Changes the given launch choices; None keeps the current one."""
         cur = self.get_tuning()
         if raster_flags is not None:
@@ -373,12 +416,12 @@ class Renderer:
         for k, v in (("dom_tile", dom_tile), ("cam_tile", cam_tile), ("bin_box_limit", bin_box_limit),
                      ("bin_records", bin_records),
                      ("bin_blocks_per_cu", bin_blocks_per_cu), ("cam_chunk", cam_chunk),
-                     ("cam_bin_blocks_per_cu", cam_bin_blocks_per_cu)):
+                     ("cam_bin_blocks_per_cu", cam_bin_blocks_per_cu), ("light_cull", light_cull)):
             if v is not None:
                 cur[k] = int(v)
         t = N.HmTuning((C.c_uint32 * 3)(*cur["raster_flags"]), (C.c_uint32 * 3)(*cur["raster_blocks_per_cu"]),
                        cur["dom_tile"], cur["cam_tile"], cur["bin_box_limit"], cur["bin_records"],
-                       cur["bin_blocks_per_cu"], cur["cam_chunk"], cur["cam_bin_blocks_per_cu"])
+                       cur["bin_blocks_per_cu"], cur["cam_chunk"], cur["cam_bin_blocks_per_cu"], cur["light_cull"])
         self._check(N.lib().hm_set_tuning(self.ctx, C.byref(t)), "hm_set_tuning")
 
     def autotune(self, g: GpuGroom, params, opts, reps: int = 3) -> dict:

@@ -145,6 +145,103 @@ def spot_light(position, target, inner_deg: float, outer_deg: float, color, ligh
             "flags": flags, "kind": 1}
 
 
+def rect_light(position, right, up, width: float, height: float, color, light_range: float = 3.0) -> dict:
+    """A rectangular area light (shade.wesl:503-518): centre, the unit right and
+    up directions of the rectangle (it emits along normalize(up x right)),
+    its size, linear colour and range."""
+    return {"position": [*[float(c) for c in position], 0.0], "right": [*[float(c) for c in right], 0.0],
+            "up": [*[float(c) for c in up], 0.0], "color": [*[float(c) for c in color], 1.0],
+            "width": float(width), "height": float(height), "range": float(light_range)}
+
+
+def rect_facing(position, target, width: float, height: float, color, light_range: float = 3.0) -> dict:
+    """A rect light at `position` whose emitting side faces `target`."""
+    position = np.asarray(position, np.float64)
+    n = np.asarray(target, np.float64) - position
+    n /= np.linalg.norm(n)
+    helper = np.array([0.0, 1.0, 0.0]) if abs(n[1]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    right = np.cross(helper, n)
+    right /= np.linalg.norm(right)
+    up = np.cross(n, right)  # up x right = n
+    return rect_light(position, right, up, width, height, color, light_range)
+
+
+@dataclasses.dataclass
+class EnvironmentMapLight:
+    """The view's environment probe (Bevy's EnvironmentMapLight as shade.wesl
+    reads it): a prefiltered specular cube map with its mip chain, a diffuse
+    (irradiance / pi) cube map, a rotation quaternion (x, y, z, w) and an
+    intensity. Each cube map level is float32 [6, size, size, 4] (RGBA), faces
+    +x, -x, +y, -y, +z, -z, rows from the top of each face."""
+
+    specular: list  # level 0 first; level m has size max(S >> m, 1)
+    diffuse: np.ndarray
+    rotation: tuple = (0.0, 0.0, 0.0, 1.0)
+    intensity: float = 1.0
+
+    def packed(self):
+        spec = np.concatenate([np.ascontiguousarray(l, np.float32).reshape(-1) for l in self.specular])
+        diff = np.ascontiguousarray(self.diffuse, np.float32).reshape(-1)
+        return spec, diff
+
+    def validate(self):
+        S = self.specular[0].shape[1]
+        for m, l in enumerate(self.specular):
+            sm = max(S >> m, 1)
+            if l.shape != (6, sm, sm, 4):
+                raise ValueError(f"specular level {m}: shape {l.shape}, expected {(6, sm, sm, 4)}")
+        D = self.diffuse.shape[1]
+        if self.diffuse.shape != (6, D, D, 4):
+            raise ValueError(f"diffuse shape {self.diffuse.shape}")
+
+
+def cube_texel_dirs(size: int, edge_snap: bool = False) -> np.ndarray:
+    """Unit directions [6, size, size, 3] through the texel centres of a cube
+    map (Vulkan's face orientation). With edge_snap the border texels take
+    the direction of the point on the face edge (and corner) beside them, so a
+    map filled from a continuous function matches across face seams."""
+    c = (np.arange(size) + 0.5) / size
+    if edge_snap and size > 1:
+        c[0], c[-1] = 0.0, 1.0
+    sc = 2.0 * c[None, :] - 1.0 + 0.0 * c[:, None]   # [t, s]
+    tc = 2.0 * c[:, None] - 1.0 + 0.0 * c[None, :]
+    one = np.ones_like(sc)
+    faces = [np.stack([one, -tc, -sc], -1), np.stack([-one, -tc, sc], -1),
+             np.stack([sc, one, tc], -1), np.stack([sc, -one, -tc], -1),
+             np.stack([sc, -tc, one], -1), np.stack([-sc, -tc, -one], -1)]
+    d = np.stack(faces)
+    return d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+def studio_environment(size: int = 64, mips: int = 6, diffuse_size: int = 16, intensity: float = 1.0,
+                       rotation=(0.0, 0.0, 0.0, 1.0)) -> EnvironmentMapLight:
+    """A procedural studio probe: a sky-to-floor gradient with two soft
+    boxes, each level filled from a smooth function of direction whose lobes
+    widen with the level (a stand-in for a GGX prefilter), the diffuse map
+    from a wide version. Seam-continuous (cube_texel_dirs edge_snap)."""
+    def radiance(d, width):
+        sky = np.array([0.55, 0.62, 0.75]) * (0.5 + 0.5 * d[..., 1:2]) + np.array([0.12, 0.1, 0.08]) * (
+            0.5 - 0.5 * d[..., 1:2])
+        out = sky
+        for axis, col in ((np.array([0.6, 0.7, 0.38]), np.array([6.0, 5.6, 5.0])),
+                          (np.array([-0.8, 0.3, -0.52]), np.array([1.5, 2.0, 3.0]))):
+            a = axis / np.linalg.norm(axis)
+            cosang = np.clip(d @ a, -1.0, 1.0)[..., None]
+            out = out + col * np.exp((cosang - 1.0) / (0.02 + width))
+        return out
+
+    levels = []
+    for m in range(mips):
+        sm = max(size >> m, 1)
+        d = cube_texel_dirs(sm, edge_snap=True)
+        rgb = radiance(d, 0.08 * m * m)
+        levels.append(np.concatenate([rgb, np.ones(rgb.shape[:-1] + (1,))], -1).astype(np.float32))
+    d = cube_texel_dirs(diffuse_size, edge_snap=True)
+    rgb = radiance(d, 2.0) * 0.35
+    diffuse = np.concatenate([rgb, np.ones(rgb.shape[:-1] + (1,))], -1).astype(np.float32)
+    return EnvironmentMapLight(levels, diffuse, tuple(rotation), intensity)
+
+
 @dataclasses.dataclass
 class Lights:
     points: np.ndarray  # structured by _native.HmPointLight fields, as a list of dicts
@@ -166,6 +263,38 @@ def point_lights(rng: np.random.Generator, count: int, centre: np.ndarray, box: 
                     "color_inverse_square_range": [*col.tolist(), 1.0 / (light_range * light_range)],
                     "light_custom_data": [0.0, 0.0, 0.0, 0.0], "flags": 0, "kind": 0})
     return out
+
+
+def scattered_lights(rng: np.random.Generator, count: int, centre, radius: float, range_lo: float,
+                     range_hi: float, intensity: float = 0.05) -> list[dict]:
+    """`count` short-range lights, point and spot alternating, uniform in a
+    box 3.2 radii wide around `centre`, ranges U(range_lo, range_hi) x radius:
+    each reaches a small part of a groom of that radius."""
+    centre = np.asarray(centre, np.float64)
+    out = []
+    for i in range(count):
+        pos = centre + rng.uniform(-1.6, 1.6, 3) * radius
+        col = rng.uniform(0.2, 1.0, 3) * intensity
+        rng_ = float(rng.uniform(range_lo, range_hi) * radius)
+        if i % 2:
+            out.append(spot_light(pos, centre + rng.uniform(-0.3, 0.3, 3) * radius, 15.0, 40.0, col, rng_))
+        else:
+            out.append({"position_radius": [*pos.tolist(), 0.0],
+                        "color_inverse_square_range": [*col.tolist(), 1.0 / (rng_ * rng_)],
+                        "light_custom_data": [0.0, 0.0, 0.0, 0.0], "flags": 0, "kind": 0})
+    return out
+
+
+def studio_rects(centre, radius: float) -> list[dict]:
+    """Three rect lights around a groom: a warm key panel, a cool fill and a
+    short-range red kicker below."""
+    centre = np.asarray(centre, np.float64)
+    return [rect_facing(centre + np.array([1.2, 0.8, 0.9]) * radius, centre, 0.3 * radius, 0.2 * radius,
+                        (4.0, 3.5, 3.0), 6.0 * radius),
+            rect_facing(centre + np.array([-1.5, 0.2, 0.6]) * radius, centre, 0.5 * radius, 0.5 * radius,
+                        (0.5, 1.0, 2.0), 6.0 * radius),
+            rect_facing(centre + np.array([0.0, -1.4, -0.9]) * radius, centre, 0.2 * radius, 0.4 * radius,
+                        (2.0, 0.5, 0.5), 1.0 * radius)]
 
 
 @dataclasses.dataclass

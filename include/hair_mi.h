@@ -128,6 +128,33 @@ typedef struct hm_directional_light {
     uint32_t _pad[3];
 } hm_directional_light;
 
+/* A rectangular area light as shade.wesl:503-518 reads Bevy's `rect_lights`
+ * (80 bytes): centre, the rectangle's right and up directions (its normal is
+ * normalize(up x right)), linear colour, width, height and range. */
+typedef struct hm_rect_light {
+    float position[4];
+    float right[4];
+    float up[4];
+    float color[4];
+    float width, height, range, _pad;
+} hm_rect_light;
+
+/* The view's environment light probe (shade.wesl:341-427, Bevy's
+ * light_probes view bindings): RGBA float32 cube maps, faces +x, -x, +y, -y,
+ * +z, -z, each face row-major from its top-left texel (Vulkan's cube map
+ * layout). The specular map holds `specular_mips` levels one after another,
+ * level m with (specular_size >> m)^2 texels per face; it is sampled at level
+ * roughness * (specular_mips - 1) (smallest_specular_mip_level_for_view). */
+typedef struct hm_env_probe {
+    const float* specular;
+    uint32_t specular_size;
+    uint32_t specular_mips;
+    const float* diffuse;
+    uint32_t diffuse_size;
+    float view_rotation[4];     /* quaternion x, y, z, w */
+    float intensity;            /* intensity_for_view */
+} hm_env_probe;
+
 /* HairGroom (lib.rs:32-64) after extract_hair (mod.rs:126-139). */
 typedef struct hm_groom_config {
     float world_from_model[16];
@@ -272,9 +299,19 @@ hm_status hm_set_scene_depth(hm_ctx* ctx, const float* host_depth, float clear_v
  * with a binding in force replays that binding. */
 hm_status hm_bind_scene_depth(hm_ctx* ctx, const float* device_depth, float clear_value);
 
-/* The light list shade reads in place of Bevy's view bindings. */
+/* The light list shade reads in place of Bevy's view bindings. Point and spot
+ * lights are culled per wave of 64 pixels where none of them is in the
+ * light's range, as Bevy's clusters list only the lights that reach a froxel
+ * (the range window is zero beyond the range, so the colours are unchanged). */
 hm_status hm_set_lights(hm_ctx* ctx, const hm_point_light* points, uint32_t point_count,
                         const hm_directional_light* directionals, uint32_t directional_count);
+/* Rectangular area lights (the AREA_LIGHT_LUTS loop, shade.wesl:503-518);
+ * count 0 removes them. Copied to the device. */
+hm_status hm_set_rect_lights(hm_ctx* ctx, const hm_rect_light* rects, uint32_t count);
+/* The environment probe shade adds (probe_radiance * ao, shade.wesl:586);
+ * NULL removes it (the !ENVIRONMENT_MAP shader, which adds nothing). The
+ * cube maps are copied to the device. */
+hm_status hm_set_env_probe(hm_ctx* ctx, const hm_env_probe* probe);
 
 /* ---- host-side parameter construction ----------------------------------- */
 
@@ -427,6 +464,10 @@ typedef struct hm_tuning {
     uint32_t cam_bin_blocks_per_cu; /* hm_gbuffer's binning grid (0 = bin_blocks_per_cu, else 1..64):
                                       its per-workgroup tile histogram is small, so it takes more
                                       workgroups per CU than the DOM pass's */
+    uint32_t light_cull;           /* shading: 0 = cull point and spot lights per wave when some
+                                      light's range may end inside the groom (from 64 lights when
+                                      shading is called on its own), 1 = never, 2 = always; the
+                                      colours are the same either way */
 } hm_tuning;
 
 hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);
@@ -532,9 +573,10 @@ hm_status hm_shade_mlp(hm_ctx* ctx, const hm_params* params, float exposure, flo
  * need), then captures the whole sequence into one HIP graph; hm_batch_run
  * replays it with one launch. The graph holds the addresses of the context's
  * buffers and the light counts: hm_batch_run fails when a buffer was
- * reallocated since (a larger target, more lights, a scene depth...) or the
- * light counts changed - create the batch again. Light values and network
- * weights are read at each replay. The grooms must outlive the batch. Calls
+ * reallocated since (a larger target, more lights, a scene depth...), the
+ * light counts changed or hm_set_env_probe was called - create the batch
+ * again. Light values, cube map texels of an unchanged probe layout and
+ * network weights are read at each replay. The grooms must outlive the batch. Calls
  * that overlap a running batch on the same context race. */
 typedef struct hm_batch hm_batch;
 typedef struct hm_batch_view {

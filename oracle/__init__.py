@@ -45,8 +45,14 @@ def lib() -> C.CDLL:
             "or_lod": (None, [P, P, P, P, P, C.c_int]),
             "or_scan": (C.c_uint32, [P, P, P, C.c_uint32, P]),
             "or_raster": (None, [P, P, P, P, P, C.c_uint32, P, P, P, P, P, C.c_float, P, C.c_int, C.c_int]),
-            "or_shade": (None, [P, P, P, P, P, C.c_uint32, P, C.c_uint32, C.c_float, P, P, C.c_int, C.c_int,
-                                  C.c_uint32, C.c_uint32]),
+            "or_shade": (None, [P, P, P, P, P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, P, C.c_float, P, P,
+                                  C.c_int, C.c_int, C.c_uint32, C.c_uint32]),
+            "or_range_falloff": (C.c_double, [C.c_double, C.c_double]),
+            "or_scatter_tint": (None, [P, C.c_double, P]),
+            "or_cube_face": (None, [P, P]),
+            "or_cube_sample": (None, [P, C.c_uint32, C.c_uint32, P, C.c_double, P]),
+            "or_cubemap_dir": (None, [P, P, P]),
+            "or_probe_radiance": (None, [P, P, P, P, C.c_double, C.c_double, P]),
             "or_filter": (None, [P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_uint32, C.c_uint32]),
             "or_azimuthal": (None, [C.c_double, C.c_double, P, P]),
             "or_dual_scatter": (None, [C.c_double, P, C.c_double, C.c_double, P]),
@@ -132,11 +138,17 @@ class Frame:
                         int(f64), threads)
         return counts
 
+    # Rect lights (hm_rect_light array, count) and the environment probe
+    # (hm_env_probe, its cube maps kept alive by the caller) the shading reads.
+    rects = (None, 0)
+    env = None
+
     def shade(self, params, points, n_points, dirs, n_dirs, exposure=1.0, f64=False, threads=1, rows=None):
         r0, r1 = rows if rows else (0, self.height)
+        rects, n_rects = self.rects
         lib().or_shade(_p(params), _p(self.center), _p(self.beta), _p(self.dom), C.cast(points, P), n_points,
-                       C.cast(dirs, P), n_dirs, exposure, _p(self.shaded), _p(self.shaded_depth), int(f64), threads,
-                       r0, r1)
+                       C.cast(dirs, P), n_dirs, C.cast(rects, P) if rects is not None else None, n_rects,
+                       _p(self.env), exposure, _p(self.shaded), _p(self.shaded_depth), int(f64), threads, r0, r1)
 
     def composite(self, params, target_color, target_depth, filtered=None):
         """composite.wgsl: this frame's final colour (filtered if the camera

@@ -852,11 +852,17 @@ template <class T> struct DomView {
     }
 };
 
-// Bevy's smooth range window (bevy_pbr pbr_lighting.wgsl getDistanceAttenuation).
-template <class T> inline T distance_attenuation(T dist2, T inv_range2) {
+// Bevy's smooth range window (bevy_pbr 0.20.1 pbr_lighting.wgsl
+// getRangeFalloff): saturate(1 - (d^2 / r^2)^2)^2, zero from the range on.
+template <class T> inline T range_falloff(T dist2, T inv_range2) {
     T factor = dist2 * inv_range2;
     T smooth = clampf(T(1) - factor * factor, T(0), T(1));
-    T att = smooth * smooth;
+    return smooth * smooth;
+}
+
+// getDistanceAttenuation: the range window over the squared distance.
+template <class T> inline T distance_attenuation(T dist2, T inv_range2) {
+    T att = range_falloff(dist2, inv_range2);
     return att * T(1) / fmax_(dist2, T(1e-4f));
 }
 
@@ -875,7 +881,132 @@ struct Lights {
     uint32_t np;
     const hm_directional_light* dirs;
     uint32_t nd;
+    const hm_rect_light* rects;
+    uint32_t nr;
+    const hm_env_probe* env;  // null: no environment map
 };
+
+template <class T> inline v3<T> cross3(v3<T> a, v3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// ---- the environment probe (shade.wesl:329-427) ------------------------------
+
+template <class T> inline T luminance(v3<T> c) {  // shade.wesl:62-64
+    return c.x * T(0.2126f) + c.y * T(0.7152f) + c.z * T(0.0722f);
+}
+
+template <class T> inline v3<T> scatter_tint(v3<T> albedo, T shadow) {  // shade.wesl:81-85
+    v3<T> hue = albedo * (T(1) / fmax_(luminance(albedo), T(1e-4f)));
+    T e = T(0.5f) - T(0.5f) * clampf(shadow, T(0), T(1));
+    return {std::pow(hue.x, e), std::pow(hue.y, e), std::pow(hue.z, e)};
+}
+
+template <class T> inline v3<T> quat_rotate(const float q[4], v3<T> d) {  // shade.wesl:329-332
+    v3<T> u = {T(q[0]), T(q[1]), T(q[2])};
+    v3<T> t = T(2) * cross3(u, d);
+    return d + T(q[3]) * t + cross3(u, t);
+}
+
+template <class T> inline v3<T> cubemap_dir(const hm_env_probe& e, v3<T> d) {  // shade.wesl:334-339
+    v3<T> s = quat_rotate(e.view_rotation, d);
+    s.z = -s.z;  // cube maps are left-handed
+    return s;
+}
+
+// The cube face a direction selects and its face coordinates in [0, 1]
+// (Vulkan's cube map face selection table; ties go to x, then y; a zero
+// direction reads the middle of face +x).
+template <class T> struct CubeUV { uint32_t face; T s, t; };
+
+template <class T> CubeUV<T> cube_face(v3<T> d) {
+    T ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    uint32_t face;
+    T ma, sc, tc;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        face = d.x >= T(0) ? 0u : 1u;
+        sc = d.x >= T(0) ? -d.z : d.z;
+        tc = -d.y;
+    } else if (ay >= az) {
+        ma = ay;
+        face = d.y >= T(0) ? 2u : 3u;
+        sc = d.x;
+        tc = d.y >= T(0) ? d.z : -d.z;
+    } else {
+        ma = az;
+        face = d.z >= T(0) ? 4u : 5u;
+        sc = d.z >= T(0) ? d.x : -d.x;
+        tc = -d.y;
+    }
+    if (!(ma > T(0))) return {0u, T(0.5f), T(0.5f)};
+    return {face, T(0.5f) * (sc / ma + T(1)), T(0.5f) * (tc / ma + T(1))};
+}
+
+// Bilinear filtering inside one face with clamp-to-edge (the cube map
+// sampler's linear filter without filtering across faces), RGB of RGBA texels.
+template <class T> v3<T> cube_bilinear(const float* level, uint32_t size, CubeUV<T> c) {
+    const float* face = level + (size_t)c.face * size * size * 4u;
+    T u = c.s * T((float)size) - T(0.5f), v = c.t * T((float)size) - T(0.5f);
+    T fu = std::floor(u), fv = std::floor(v);
+    T wx = u - fu, wy = v - fv;
+    int32_t x0 = (int32_t)fu, y0 = (int32_t)fv;
+    int32_t hi = (int32_t)size - 1;
+    int32_t xs[2] = {std::min(std::max(x0, 0), hi), std::min(std::max(x0 + 1, 0), hi)};
+    int32_t ys[2] = {std::min(std::max(y0, 0), hi), std::min(std::max(y0 + 1, 0), hi)};
+    v3<T> tap[4];
+    for (int k = 0; k < 4; ++k) {
+        const float* q = face + ((size_t)ys[k >> 1] * size + (size_t)xs[k & 1]) * 4u;
+        tap[k] = {T(q[0]), T(q[1]), T(q[2])};
+    }
+    return mix3(mix3(tap[0], tap[1], wx), mix3(tap[2], tap[3], wx), wy);
+}
+
+// textureSampleLevel of a cube map holding `mips` levels one after another
+// (level m: six faces of max(size >> m, 1)^2 texels), linear between levels;
+// the level is clamped to [0, mips - 1].
+template <class T> v3<T> cube_sample(const float* data, uint32_t size, uint32_t mips, v3<T> dir, T level) {
+    CubeUV<T> c = cube_face(dir);
+    level = clampf(level, T(0), T((float)(mips - 1u)));
+    T fl = std::floor(level);
+    uint32_t l0 = (uint32_t)fl, l1 = std::min(l0 + 1u, mips - 1u);
+    T f = level - fl;
+    size_t base = 0;
+    const float* lv[2] = {nullptr, nullptr};
+    uint32_t sz[2] = {0, 0};
+    for (uint32_t m = 0; m <= l1; ++m) {
+        uint32_t sm = std::max(size >> m, 1u);
+        if (m == l0) { lv[0] = data + base; sz[0] = sm; }
+        if (m == l1) { lv[1] = data + base; sz[1] = sm; }
+        base += (size_t)6u * sm * sm * 4u;
+    }
+    v3<T> a = cube_bilinear(lv[0], sz[0], c);
+    if (l1 == l0) return a;
+    return mix3(a, cube_bilinear(lv[1], sz[1], c), f);
+}
+
+// shade.wesl:396-427 (the single-probe sample_environment, shade.wesl:365-382).
+template <class T>
+v3<T> probe_radiance(const hm_env_probe* env, v3<T> tangent, v3<T> fiber_n, v3<T> view, v3<T> albedo, T roughness,
+                     T tilt) {
+    if (!env) return splat3(T(0));
+    T theta_o = std::asin(clampf(dot3(tangent, view), T(-1), T(1)));
+    T theta_i = tilt - theta_o;
+    v3<T> spec_dir = cubemap_dir(*env, tangent * std::sin(theta_i) + fiber_n * std::cos(theta_i));
+    v3<T> diff_dir = cubemap_dir(*env, fiber_n);
+    v3<T> spec = cube_sample(env->specular, env->specular_size, env->specular_mips, spec_dir,
+                             roughness * T((float)(env->specular_mips - 1u)));
+    v3<T> diff = cube_sample(env->diffuse, env->diffuse_size, 1u, diff_dir, T(0));
+    T cos_o = fmax_(std::cos(theta_o), T(1e-3f));
+    T f_r = fresnel(cos_o);
+    T f_trt = fresnel(cos_o * T(0.5f));
+    v3<T> sigma_a = absorption_from_color(albedo, roughness);
+    v3<T> e = sigma_a * (T(-3.2f) / cos_o);
+    v3<T> trt = v3<T>{std::exp(e.x), std::exp(e.y), std::exp(e.z)} *
+                (T(0.76f) * (T(1) - f_trt) * (T(1) - f_trt) * f_trt);
+    v3<T> scatter = albedo * scatter_tint(albedo, T(0.5f));
+    return (spec * (splat3(f_r) + trt) + diff * scatter) * T(env->intensity);
+}
 
 template <class T>
 v3<T> direct_light(const Params<T>& p, const DomView<T>& dv, const Lights& lights, v3<T> world, v3<T> tangent,
@@ -918,12 +1049,26 @@ v3<T> direct_light(const Params<T>& p, const DomView<T>& dv, const Lights& light
         v3<T> lobe = hair_bsdf(tangent, view, incident, sigma_a, shape, ds, T(1), depth);
         color = color + lobe * v3<T>{T(L.color[0]), T(L.color[1]), T(L.color[2])};
     }
+    for (uint32_t i = 0; i < lights.nr; ++i) {  // shade.wesl:503-518
+        const hm_rect_light& L = lights.rects[i];
+        v3<T> to_center = v3<T>{T(L.position[0]), T(L.position[1]), T(L.position[2])} - world;
+        T dist2 = fmax_(dot3(to_center, to_center), T(1e-4f));
+        v3<T> incident = to_center * inverse_sqrt(dist2);
+        v3<T> normal = safe_normalize(cross3(v3<T>{T(L.up[0]), T(L.up[1]), T(L.up[2])},
+                                             v3<T>{T(L.right[0]), T(L.right[1]), T(L.right[2])}));
+        T facing = fmax_(dot3(normal, incident), T(0));
+        T solid = T(L.width) * T(L.height) * facing / dist2;
+        T range2 = fmax_(T(L.range) * T(L.range), T(1e-4f));
+        T falloff = range_falloff(dist2, T(1) / range2);
+        v3<T> lobe = hair_bsdf(tangent, view, incident, sigma_a, shape, ds, T(1), T(0));
+        color = color + lobe * v3<T>{T(L.color[0]), T(L.color[1]), T(L.color[2])} * solid * falloff;
+    }
     return color;
 }
 
 // The per-pixel inputs shade.wesl:fs derives from a covered pixel's words
 // (shade.wesl:539-571).
-template <class T> struct PixelIn { v3<T> world, tangent, view, albedo; T roughness, beta, view_dist; bool ortho; };
+template <class T> struct PixelIn { v3<T> world, tangent, view, albedo; T roughness, beta, view_dist, ao; bool ortho; };
 
 template <class T>
 PixelIn<T> pixel_inputs(const Params<T>& p, uint64_t word, uint32_t beta_word, uint32_t x, uint32_t y) {
@@ -934,6 +1079,7 @@ PixelIn<T> pixel_inputs(const Params<T>& p, uint64_t word, uint32_t beta_word, u
     in.tangent = oct_decode<T>(u.tx, u.ty);
     v3<T> uvw = {T((float)u.u) / T(63.0f), T((float)u.v) / T(63.0f), T((float)u.w) / T(63.0f)};
     in.beta = T((float)(beta_word & 0xffu)) / T(255.0f);
+    in.ao = p.flags[2] == 0u ? T(1) : T((float)u.ao) / T(63.0f);
     in.roughness = fmax_(p.appearance.x * (T(0.65f) + T(0.7f) * uvw.z), T(0.04f));
     in.albedo = xyz(p.albedo) * (T(0.65f) + T(0.35f) * uvw.z);
     // Orthographic when the projection's last row is (0, 0, 0, 1).
@@ -959,7 +1105,13 @@ void shade_pixel(const Params<T>& p, const DomView<T>& dv, const Lights& lights,
     T view_dist = in.view_dist;
     v3<T> color = direct_light(p, dv, lights, in.world, in.tangent, in.view, in.albedo, in.roughness,
                                p.appearance.y, in.beta);
-    // probe_radiance returns zero without an environment map (shade.wesl:384-394).
+    // probe_radiance x ao (shade.wesl:585-586); zero without an environment
+    // map (shade.wesl:384-394).
+    if (lights.env) {
+        v3<T> fiber_n = safe_normalize(in.view - in.tangent * dot3(in.view, in.tangent));
+        color = color + probe_radiance(lights.env, in.tangent, fiber_n, in.view, in.albedo, in.roughness,
+                                       p.appearance.y) * in.ao;
+    }
     color = color * exposure;
     shaded[pix * 4 + 0] = (float)color.x;
     shaded[pix * 4 + 1] = (float)color.y;
@@ -1300,9 +1452,9 @@ void or_raster(const hm_params* prm, const hm_bundle* bundles, const hm_layer* l
 /* shade.wesl:523-593 for every pixel, rows split over `threads`. */
 void or_shade(const hm_params* prm, const uint64_t* center, const uint32_t* beta, const uint32_t* dom,
               const hm_point_light* points, uint32_t np, const hm_directional_light* dirs, uint32_t nd,
-              float exposure, float* shaded, float* shaded_depth, int f64, int threads, uint32_t row_begin,
-              uint32_t row_end) {
-    Lights lights{points, np, dirs, nd};
+              const hm_rect_light* rects, uint32_t nr, const hm_env_probe* env, float exposure, float* shaded,
+              float* shaded_depth, int f64, int threads, uint32_t row_begin, uint32_t row_end) {
+    Lights lights{points, np, dirs, nd, rects, nr, env};
     uint32_t width = (uint32_t)prm->screen[0];
     uint32_t height = std::min(row_end, (uint32_t)prm->screen[1]);
     auto run = [&](uint32_t y0, uint32_t y1) {
@@ -1454,6 +1606,41 @@ float or_spot_mask(const hm_point_light* light, const float world[3]) {  // shad
     return spot_mask<float>(*light, {world[0], world[1], world[2]});
 }
 float or_distance_attenuation(float dist2, float inv_range2) { return distance_attenuation<float>(dist2, inv_range2); }
+double or_range_falloff(double dist2, double inv_range2) { return range_falloff<double>(dist2, inv_range2); }
+
+/* scatter_tint (shade.wesl:81-85). */
+void or_scatter_tint(const double albedo[3], double shadow, double out[3]) {
+    v3<double> r = scatter_tint<double>({albedo[0], albedo[1], albedo[2]}, shadow);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* cube_face: out = face, s, t. */
+void or_cube_face(const double dir[3], double out[3]) {
+    CubeUV<double> c = cube_face<double>({dir[0], dir[1], dir[2]});
+    out[0] = c.face; out[1] = c.s; out[2] = c.t;
+}
+
+/* cube_sample of a cube map with `mips` levels at `level`, no view rotation. */
+void or_cube_sample(const float* data, uint32_t size, uint32_t mips, const double dir[3], double level,
+                    double out[3]) {
+    v3<double> r = cube_sample<double>(data, size, mips, {dir[0], dir[1], dir[2]}, level);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* cubemap_dir (shade.wesl:334-339). */
+void or_cubemap_dir(const hm_env_probe* env, const double dir[3], double out[3]) {
+    v3<double> r = cubemap_dir<double>(*env, {dir[0], dir[1], dir[2]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* probe_radiance (shade.wesl:396-427) with fiber_n formed as shade.wesl:fs does. */
+void or_probe_radiance(const hm_env_probe* env, const double tangent[3], const double view[3],
+                       const double albedo[3], double roughness, double tilt, double out[3]) {
+    v3<double> t = {tangent[0], tangent[1], tangent[2]}, v = {view[0], view[1], view[2]};
+    v3<double> fiber_n = safe_normalize(v - t * dot3(v, t));
+    v3<double> r = probe_radiance<double>(env, t, fiber_n, v, {albedo[0], albedo[1], albedo[2]}, roughness, tilt);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
 void or_reconstruct_world(const hm_params* prm, int32_t px, int32_t py, float view_dist, float out[3]) {
     Params<float> p = load_params<float>(*prm);
     v3<float> w = reconstruct_world(p, px, py, view_dist);

@@ -49,8 +49,24 @@ MUTS = [
  ("every caster gets DOM", "if (dom_left && casts) {", "if (casts) {"),
  ("spot mask on points", "if (pass == 1) c = c * spot_mask<T>(L, world);", "if (pass == 0) c = c * spot_mask<T>(L, world);"),
  ("ortho view sign", "if (in.ortho) in.view = safe_normalize(v3<T>{p.clip[2], p.clip[6], p.clip[10]});", "if (in.ortho) in.view = safe_normalize(v3<T>{p.clip[1], p.clip[5], p.clip[9]});"),
+ ("cube +x sc sign", "sc = d.x >= T(0) ? -d.z : d.z;", "sc = d.x >= T(0) ? d.z : d.z;"),
+ ("cube +y tc sign", "tc = d.y >= T(0) ? d.z : -d.z;", "tc = d.y >= T(0) ? -d.z : -d.z;"),
+ ("cube texel offset", "T u = c.s * T((float)size) - T(0.5f)", "T u = c.s * T((float)size)"),
+ ("cube bilinear weights", "return mix3(mix3(tap[0], tap[1], wx), mix3(tap[2], tap[3], wx), wy);", "return mix3(mix3(tap[0], tap[1], wy), mix3(tap[2], tap[3], wx), wy);"),
+ ("cube level blend", "return mix3(a, cube_bilinear(lv[1], sz[1], c), f);", "return mix3(a, cube_bilinear(lv[1], sz[1], c), T(1) - f);"),
+ ("cubemap no z flip", "    s.z = -s.z;  // cube maps are left-handed\n", ""),
+ ("quat rotate sign", "return d + T(q[3]) * t + cross3(u, t);", "return d - T(q[3]) * t + cross3(u, t);"),
+ ("probe tilt sign", "T theta_i = tilt - theta_o;", "T theta_i = tilt + theta_o;"),
+ ("probe trt half angle", "T f_trt = fresnel(cos_o * T(0.5f));", "T f_trt = fresnel(cos_o);"),
+ ("probe trt absorption", "v3<T> e = sigma_a * (T(-3.2f) / cos_o);", "v3<T> e = sigma_a * (T(-2.0f) / cos_o);"),
+ ("probe tint shadow", "scatter_tint(albedo, T(0.5f))", "scatter_tint(albedo, T(0.0f))"),
+ ("probe level", "roughness * T((float)(env->specular_mips - 1u))", "roughness * T((float)env->specular_mips)"),
+ ("probe no ao", "p.appearance.y) * in.ao;", "p.appearance.y);"),
+ ("rect normal order", "safe_normalize(cross3(v3<T>{T(L.up[0]), T(L.up[1]), T(L.up[2])},\n                                             v3<T>{T(L.right[0]), T(L.right[1]), T(L.right[2])}))", "safe_normalize(cross3(v3<T>{T(L.right[0]), T(L.right[1]), T(L.right[2])},\n                                             v3<T>{T(L.up[0]), T(L.up[1]), T(L.up[2])}))"),
+ ("rect solid angle", "T solid = T(L.width) * T(L.height) * facing / dist2;", "T solid = T(L.width) * T(L.height) * facing;"),
+ ("rect range", "T range2 = fmax_(T(L.range) * T(L.range), T(1e-4f));", "T range2 = fmax_(T(L.range), T(1e-4f));"),
 ]
-TESTS = ["tests/test_oracle.py", "tests/test_oracle_pins.py", "tests/test_branch_cases.py"]
+TESTS = ["tests/test_oracle.py", "tests/test_oracle_pins.py", "tests/test_branch_cases.py", "tests/test_env_pins.py"]
 shutil.copytree(SRC, DST, ignore=shutil.ignore_patterns('.git', '_work', 'gpurun_out', '__pycache__'))
 orig = open(f'{SRC}/oracle/hair_oracle.cpp').read()
 missed = []

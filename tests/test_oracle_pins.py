@@ -553,7 +553,7 @@ def test_direct_light_composition():
     C.memmove(C.byref(cam_dom), C.byref(cam), C.sizeof(N.HmParams))
     cam_dom.dom_info[0] = cam_dom.dom_info[1] = float(S)
     L.or_shade(C.byref(cam_dom), center.ctypes.data, beta.ctypes.data, dom.ctypes.data, C.cast(pa, C.c_void_p), n_p,
-               C.cast(da, C.c_void_p), n_d, 1.5, shaded.ctypes.data, sdepth.ctypes.data, 0, 1, 0, 24)
+               C.cast(da, C.c_void_p), n_d, None, 0, None, 1.5, shaded.ctypes.data, sdepth.ctypes.data, 0, 1, 0, 24)
     vis = L.or_dom_visibility(C.byref(cam_dom), dom.ctypes.data, f3(world), float(inp[13]))
     assert 0.0 < vis < 0.999, "the key light must see some opacity"
     key_depth = -math.log(max(vis, 1e-6))
