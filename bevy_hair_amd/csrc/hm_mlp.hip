@@ -576,8 +576,8 @@ __device__ DualScatter dual_scatter_by_wave(const PixelShade& px, int lane) {
 // method (all 64 lanes on that pixel) and the group is re-shaded by the plain
 // method when the network colour misses it by more than `tol`; which pixels
 // share a group follows the list order, so that re-shading does not. The
-// pixel set-up is redone after the network (from the G-buffer word held in
-// registers) so that its 30 values are not held through it.
+// pixel set-up (PixelShade) is computed once, before the network, and held in
+// registers through it for the light loop.
 constexpr uint32_t PROBE_STRIDE = 4;
 
 template <uint32_t F>  // SHADE_* variant bits (pixel_color)
@@ -615,8 +615,7 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
         const bool valid = pix < pixels;
         float depth = 0.0f;
         bool outside = false;
-        // the pixel's G-buffer word and coverage, kept to redo the set-up
-        // after the network without reading them again
+        // the pixel's G-buffer word and coverage (read once, for the set-up)
         uint64_t word = 0;
         uint32_t beta_word = 0;
         PixelShade px;  // kept across the network for the light loop
