@@ -385,15 +385,17 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
 // `chunk` (at least one chunk per tile, so empty tiles are still written):
 // chunk offsets after the cursors, the tile of every chunk in chunk_tile,
 // and BIN_SPLIT set when some tile has more than one chunk.
+// With `busy` it also lists the tiles that hold records, in tile order, and
+// their number in BIN_BUSY.
 __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tiles, uint32_t record_capacity,
                                                    uint32_t long_capacity, uint32_t list_capacity, uint32_t chunk,
-                                                   uint32_t* chunk_tile, uint32_t chunk_capacity) {
+                                                   uint32_t* chunk_tile, uint32_t chunk_capacity, uint32_t* busy) {
     constexpr uint32_t PER = BIN_MAX_TILES / 1024u;
     // Tile i sits at word i + i / PER: a thread's run of PER tiles starts
     // PER + 1 words after its neighbour's, so the run-wise accesses below hit
     // distinct banks, as do the tile-wise ones.
     __shared__ uint32_t stage[BIN_MAX_TILES + BIN_MAX_TILES / PER];
-    __shared__ uint32_t wave_total[16], wave_chunks[16], split, ok_s;
+    __shared__ uint32_t wave_total[16], wave_chunks[16], wave_busy[16], split, ok_s;
     auto pad = [](uint32_t i) { return i + i / PER; };
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t* cnt = meta + BIN_META_WORDS;
@@ -419,31 +421,41 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
     uint32_t v[PER];
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) v[k] = stage[pad(b + k)];
-    uint32_t sum = 0u, csum = 0u, nch[PER];
+    uint32_t sum = 0u, csum = 0u, bsum = 0u, nch[PER];
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
         sum += v[k];
         nch[k] = b + k < tiles ? (chunk && v[k] > chunk ? (v[k] + chunk - 1u) / chunk : 1u) : 0u;
         csum += nch[k];
+        bsum += (b + k < tiles && v[k] != 0u) ? 1u : 0u;
     }
-    uint32_t inc = sum, cinc = csum;
+    uint32_t inc = sum, cinc = csum, binc = bsum;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(inc, d), cy = __shfl_up(cinc, d);
+        uint32_t y = __shfl_up(inc, d), cy = __shfl_up(cinc, d), by = __shfl_up(binc, d);
         if ((int)lane >= d) {
             inc += y;
             cinc += cy;
+            binc += by;
         }
     }
     if (lane == 63u) {
         wave_total[wave] = inc;
         wave_chunks[wave] = cinc;
+        wave_busy[wave] = binc;
     }
     __syncthreads();
-    uint32_t run = inc - sum, crun = cinc - csum;
+    uint32_t run = inc - sum, crun = cinc - csum, brun = binc - bsum;
     for (uint32_t w = 0; w < wave; ++w) {
         run += wave_total[w];
         crun += wave_chunks[w];
+        brun += wave_busy[w];
+    }
+    if (busy) {
+#pragma unroll
+        for (uint32_t k = 0; k < PER; ++k)
+            if (b + k < tiles && v[k] != 0u) busy[brun++] = b + k;
+        if (tid == 1023u) meta[BIN_BUSY] = brun;
     }
     // Record offsets through LDS, then out (offset and cursor arrays).
     uint32_t r = run, cr = crun;
@@ -590,7 +602,7 @@ struct DomTileSink {
         return out_x | out_y;
     }
     __device__ __forceinline__ uint32_t at(int32_t x, int32_t y) const { return (uint32_t)((y - oy) * T + (x - ox)); }
-    __device__ __forceinline__ void dom_min(size_t, int32_t x, int32_t y, uint32_t q, Counters&) {
+    __device__ __forceinline__ void dom_min(size_t, int32_t x, int32_t y, uint32_t q, float, Counters&) {
         atomicMin(front + at(x, y), q);
     }
     __device__ __forceinline__ uint32_t dom_front(size_t, int32_t x, int32_t y) const { return front[at(x, y)]; }
@@ -680,6 +692,173 @@ __global__ __launch_bounds__(256) void k_dom_tiles(BinArgs a) {
     }
 }
 
+// 16 x 16 DOM tiles, persistent: workgroups of 1024 threads take the tiles
+// that hold records from k_bin_scan's busy list (one counter), then 16-tile
+// runs of the grid whose empty tiles they write (another counter). A busy
+// tile walks its records once, in the depth pass: each sample takes its LDS
+// atomic minimum and is kept in an LDS stash (texel, view distance), from
+// which the opacity pass bins it once every front is final. Only samples the
+// stash cannot hold are walked a second time: a thread's records from the one
+// where its first sample missed the stash on, skipping that record's samples
+// the stash took. Both passes test the same pixels in the same order, so the
+// skipped samples are exactly the stashed ones.
+constexpr uint32_t DOM_STASH = 12288u;  // samples per busy tile (60 KB of LDS)
+constexpr uint32_t DOM_WG = 1024u;
+constexpr uint32_t DOM_NO_FAIL = 0xffffffffu;
+
+struct DomStashSink {
+    static constexpr int T = 16;
+    uint32_t* front;
+    uint32_t* counts;
+    int32_t ox, oy;
+    float* s_dist;
+    uint8_t* s_pix;
+    uint32_t* s_n;
+    uint32_t rec = 0u;        // this thread's ordinal of the record being walked
+    uint32_t in_rec = 0u;     // samples of that record so far
+    uint32_t fail_rec = DOM_NO_FAIL, fail_skip = 0u;
+    uint32_t skip = 0u;       // opacity re-walk: samples still to pass over
+    __device__ __forceinline__ bool accepts(int32_t x, int32_t y) const {
+        return (uint32_t)(x - ox) < (uint32_t)T && (uint32_t)(y - oy) < (uint32_t)T;
+    }
+    __device__ __forceinline__ bool walked_out(int32_t x, int32_t y, int32_t sx, int32_t sy) const {
+        const bool out_x = (sx > 0) ? (x >= ox + T) : (x < ox);
+        const bool out_y = (sy > 0) ? (y >= oy + T) : (y < oy);
+        return out_x | out_y;
+    }
+    __device__ __forceinline__ uint32_t at(int32_t x, int32_t y) const { return (uint32_t)((y - oy) * T + (x - ox)); }
+    __device__ __forceinline__ void dom_min(size_t, int32_t x, int32_t y, uint32_t q, float dist, Counters&) {
+        const uint32_t i = at(x, y);
+        atomicMin(front + i, q);
+        // one stash reservation per wave for the lanes writing now
+        const uint64_t m = __ballot(1);
+        const uint32_t rank = lanes_below(m);
+        uint32_t base = 0u;
+        if (rank == 0u) base = atomicAdd(s_n, (uint32_t)__popcll(m));
+        base = __builtin_amdgcn_readfirstlane(base);
+        const uint32_t slot = base + rank;
+        if (fail_rec == DOM_NO_FAIL) {
+            if (slot < DOM_STASH) {
+                s_dist[slot] = dist;
+                s_pix[slot] = (uint8_t)i;
+            } else {
+                fail_rec = rec;
+                fail_skip = in_rec;
+            }
+        }
+        ++in_rec;
+    }
+    __device__ __forceinline__ uint32_t dom_front(size_t, int32_t x, int32_t y) {
+        if (skip != 0u) {
+            --skip;
+            return HM_DOM_EMPTY;  // binned from the stash already
+        }
+        return front[at(x, y)];
+    }
+    __device__ __forceinline__ void dom_add(size_t, int32_t x, int32_t y, uint32_t slice, Counters&) {
+        atomicAdd(counts + at(x, y) * HM_DOM_LAYERS + slice, 1u);
+    }
+};
+
+__global__ __launch_bounds__(DOM_WG, 8) void k_dom_tiles_busy(BinArgs a) {
+    if (a.meta[BIN_OK] == 0u) return;
+    constexpr int T = 16;
+    __shared__ uint32_t front[T * T];
+    __shared__ __align__(16) uint32_t counts[T * T * HM_DOM_LAYERS];
+    __shared__ float s_dist[DOM_STASH];
+    __shared__ uint8_t s_pix[DOM_STASH];
+    __shared__ uint32_t s_n, s_item, s_fail;
+    const View& p = bin_view(a);
+    const uint32_t tid = threadIdx.x;
+    const int32_t S = (int32_t)a.dom_size;
+    const bool has_long = a.meta[BIN_LONG] != 0u;
+    const uint32_t* cnt = a.meta + BIN_META_WORDS;
+    const uint32_t* off = cnt + a.tiles;
+    const uint32_t* busy = cnt + 4u * a.tiles + 2u;
+    const uint32_t n_busy = a.meta[BIN_BUSY];
+    uint32_t* dom_front = a.dom;
+    uint4* dom_counts = reinterpret_cast<uint4*>(a.dom + (size_t)S * (size_t)S);
+    const DomSlab slab = dom_slab(p.screen.z, p.screen.w);
+    // thread tid: front word tid (tid < 256) and count quad tid of a tile
+    const int32_t fx = (int32_t)(tid % T), fy = (int32_t)((tid / T) % T);
+    const int32_t qx = (int32_t)((tid >> 2) % T), qy = (int32_t)((tid >> 2) / T);
+    for (;;) {
+        if (tid == 0u) s_item = atomicAdd(a.meta + BIN_BUSY_NEXT, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= n_busy) break;
+        const uint32_t tile = busy[item];
+        const int32_t ox = (int32_t)(tile % a.tiles_x) * T, oy = (int32_t)(tile / a.tiles_x) * T;
+        if (tid < (uint32_t)(T * T)) {
+            const int32_t x = ox + fx, y = oy + fy;
+            front[tid] = (has_long && x < S && y < S) ? dom_front[(size_t)y * S + x] : HM_DOM_EMPTY;
+        }
+        reinterpret_cast<uint4*>(counts)[tid] = make_uint4(0u, 0u, 0u, 0u);
+        if (tid == 0u) {
+            s_n = 0u;
+            s_fail = 0u;
+        }
+        __syncthreads();
+        const uint32_t begin = off[tile], end = off[tile + 1u];
+        DomStashSink sink{front, counts, ox, oy, s_dist, s_pix, &s_n};
+        {
+            Drawer<1, DomStashSink> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
+            uint32_t j = 0u;
+            for (uint32_t r = begin + tid; r < end; r += DOM_WG, ++j) {
+                sink.rec = j;
+                sink.in_rec = 0u;
+                walk_record(d, a, a.records[r]);
+            }
+        }
+        if (sink.fail_rec != DOM_NO_FAIL) s_fail = 1u;
+        __syncthreads();
+        if (tid < (uint32_t)(T * T)) {
+            const int32_t x = ox + fx, y = oy + fy;
+            if (x < S && y < S) dom_front[(size_t)y * S + x] = front[tid];
+        }
+        const uint32_t n = s_n < DOM_STASH ? s_n : DOM_STASH;
+        for (uint32_t i = tid; i < n; i += DOM_WG) {
+            const uint32_t px = s_pix[i];
+            const uint32_t fq = front[px];
+            if (fq != HM_DOM_EMPTY) atomicAdd(counts + px * HM_DOM_LAYERS + dom_slice(slab, fq, s_dist[i]), 1u);
+        }
+        if (s_fail != 0u && sink.fail_rec != DOM_NO_FAIL) {
+            Drawer<2, DomStashSink> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
+            uint32_t j = sink.fail_rec;
+            for (uint32_t r = begin + tid + j * DOM_WG; r < end; r += DOM_WG, ++j) {
+                sink.skip = j == sink.fail_rec ? sink.fail_skip : 0u;
+                walk_record(d, a, a.records[r]);
+            }
+        }
+        __syncthreads();
+        {
+            const int32_t x = ox + qx, y = oy + qy;
+            if (x < S && y < S) dom_counts[((size_t)y * S + x) * 4u + (tid & 3u)] = reinterpret_cast<uint4*>(counts)[tid];
+        }
+    }
+    // Tiles without records: empty fronts (unless long segments wrote them)
+    // and zero counts.
+    const uint32_t runs = (a.tiles + 15u) / 16u;
+    for (;;) {
+        __syncthreads();  // s_item's last value has been read
+        if (tid == 0u) s_item = atomicAdd(a.meta + BIN_EMPTY_NEXT, 1u);
+        __syncthreads();
+        const uint32_t run = s_item;
+        if (run >= runs) break;
+        const uint32_t t_end = run * 16u + 16u < a.tiles ? run * 16u + 16u : a.tiles;
+        for (uint32_t t = run * 16u; t < t_end; ++t) {
+            if (cnt[t] != 0u) continue;
+            const int32_t ox = (int32_t)(t % a.tiles_x) * T, oy = (int32_t)(t / a.tiles_x) * T;
+            const int32_t x = ox + qx, y = oy + qy;
+            if (x < S && y < S) dom_counts[((size_t)y * S + x) * 4u + (tid & 3u)] = make_uint4(0u, 0u, 0u, 0u);
+            if (!has_long && tid < (uint32_t)(T * T)) {
+                const int32_t fx2 = ox + fx, fy2 = oy + fy;
+                if (fx2 < S && fy2 < S) dom_front[(size_t)fy2 * S + fx2] = HM_DOM_EMPTY;
+            }
+        }
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_dom_long(BinArgs a) {
     if (a.meta[BIN_LONG] == 0u) return;
@@ -702,7 +881,8 @@ hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks,
     hipLaunchKernelGGL(k_bin_count, dim3(bin_blocks), dim3(256), lds, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, a.meta, a.tiles, a.record_capacity, a.long_capacity,
-                       a.list_capacity, a.chunk, a.chunk_tile, a.chunk_capacity);
+                       a.list_capacity, a.chunk, a.chunk_tile, a.chunk_capacity,
+                       mode == 1 ? a.meta + BIN_META_WORDS + 4u * a.tiles + 2u : nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin_fill, dim3(bin_blocks), dim3(256), lds, s, a);
     return hipGetLastError();
@@ -715,8 +895,10 @@ hipError_t launch_dom_long(hipStream_t s, const BinArgs& a, int mode, int long_b
     return hipGetLastError();
 }
 
-hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a) {
-    if (a.tile_shift == 4u) {
+hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int busy_blocks) {
+    if (a.tile_shift == 4u && busy_blocks > 0) {
+        hipLaunchKernelGGL(k_dom_tiles_busy, dim3(busy_blocks), dim3(DOM_WG), 0, s, a);
+    } else if (a.tile_shift == 4u) {
         hipLaunchKernelGGL(k_dom_tiles<16>, dim3(a.tiles), dim3(256), 0, s, a);
     } else if (a.tile_shift == 5u) {
         hipLaunchKernelGGL(k_dom_tiles<32>, dim3(a.tiles), dim3(256), 0, s, a);

@@ -1047,7 +1047,7 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     // Camera tiles split into chunks: at most tiles + records / chunk of them.
     const uint32_t chunk = pass == HM_PASS_CAMERA ? ctx->tuning.cam_chunk : 0u;
     const size_t chunks = tiles + (chunk ? want_records / chunk + 1u : 0u);
-    hipError_t e = ensure_bin(ctx, pass, slots, hm::BIN_META_WORDS + 4u * (size_t)tiles + 2u, want_records, lists,
+    hipError_t e = ensure_bin(ctx, pass, slots, hm::BIN_META_WORDS + 5u * (size_t)tiles + 2u, want_records, lists,
                               (size_t)bin_blocks, chunks);
     if (e != hipSuccess) return e;
     std::memset(&b, 0, sizeof(b));
@@ -1219,7 +1219,7 @@ static hm_status dom_impl(hm_ctx* ctx, hm_groom* g, const hm_params* dp, const h
     }
     {
         Timer t(ctx, HM_K_DOM_TILES);
-        HM_HIP(ctx, hm::launch_dom_tiles(ctx->stream, b));
+        HM_HIP(ctx, hm::launch_dom_tiles(ctx->stream, b, ctx->cu_count * (int)ctx->tuning.dom_busy_blocks_per_cu));
     }
     {
         Timer t(ctx, HM_K_DOM_LONG_OPACITY);
@@ -1491,6 +1491,7 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
         return fail(ctx, HM_ERR_INVALID, "bin_blocks_per_cu must be in [1, 64]");
     if (t->cam_bin_blocks_per_cu > 64u) return fail(ctx, HM_ERR_INVALID, "cam_bin_blocks_per_cu must be in [0, 64]");
     if (t->light_cull > 2u) return fail(ctx, HM_ERR_INVALID, "light_cull must be 0, 1 or 2");
+    if (t->dom_busy_blocks_per_cu > 2u) return fail(ctx, HM_ERR_INVALID, "dom_busy_blocks_per_cu must be 0, 1 or 2");
 
     ctx->tuning = *t;
     return HM_OK;

@@ -107,10 +107,14 @@ constexpr uint32_t BIN_LONG_APPEND = 9u; // k_bin_gen's long-segment cursor; k_b
                                          // BIN_LONG_COUNT and zeroes it, so it is zero whenever a
                                          // k_bin_gen starts (k_bin_gen clears every other status
                                          // word and the tile counts itself)
+constexpr uint32_t BIN_BUSY = 10u;       // DOM: tiles holding records (k_bin_scan lists them after the
+                                         // chunk offsets)
+constexpr uint32_t BIN_BUSY_NEXT = 11u;  // DOM: next busy-tile list entry a tile workgroup takes
+constexpr uint32_t BIN_EMPTY_NEXT = 12u; // DOM: next 16-tile run whose empty tiles a workgroup writes
 constexpr uint32_t BIN_SHADE_SUBS = 16u;                        // camera: shade sub-list counters
 constexpr uint32_t BIN_FILTER_SUBS = BIN_SHADE_SUBS + LIST_SUBS; // camera: filter sub-list counters
 constexpr uint32_t BIN_META_WORDS = BIN_FILTER_SUBS + LIST_SUBS; // then count[tiles], offset[tiles + 1], cursor[tiles],
-                                         // chunk offset[tiles + 1]
+                                         // chunk offset[tiles + 1], busy tiles[tiles]
 constexpr uint32_t BIN_MAX_TILES = 16384u;  // one LDS word per tile in the binning workgroups
 
 struct BinArgs {
@@ -169,7 +173,9 @@ struct BinArgs {
 
 hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks);
 hipError_t launch_dom_long(hipStream_t s, const BinArgs& a, int mode, int long_blocks);  // mode 1 before, 2 after the tiles
-hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a);
+// 16 x 16 tiles with busy_blocks > 0: the persistent busy-tile kernel on that
+// many workgroups; otherwise one workgroup per tile.
+hipError_t launch_dom_tiles(hipStream_t s, const BinArgs& a, int busy_blocks);
 hipError_t launch_cam_tiles(hipStream_t s, const BinArgs& a, int tile_blocks);
 hipError_t launch_cam_finish(hipStream_t s, const BinArgs& a, int long_blocks);  // split tiles' lists, long segments
 

@@ -171,7 +171,9 @@ struct GlobalSink {
             min32(beta + pix, beta_word);
         }
     }
-    __device__ __forceinline__ void dom_min(size_t pix, int32_t, int32_t, uint32_t q, Counters&) { min32(dom + pix, q); }
+    __device__ __forceinline__ void dom_min(size_t pix, int32_t, int32_t, uint32_t q, float, Counters&) {
+        min32(dom + pix, q);
+    }
     __device__ __forceinline__ uint32_t dom_front(size_t pix, int32_t, int32_t) const { return dom[pix]; }
     __device__ __forceinline__ void dom_add(size_t pix, int32_t, int32_t, uint32_t slice, Counters&) {
         atomicAdd(dom + dom_pixels + pix * HM_DOM_LAYERS + slice, 1u);
@@ -187,10 +189,25 @@ struct CountSink {
         c.cam_writes++;
         if (center_hit) c.cam_centers++;
     }
-    __device__ __forceinline__ void dom_min(size_t, int32_t, int32_t, uint32_t, Counters& c) { c.dom_writes++; }
+    __device__ __forceinline__ void dom_min(size_t, int32_t, int32_t, uint32_t, float, Counters& c) { c.dom_writes++; }
     __device__ __forceinline__ uint32_t dom_front(size_t pix, int32_t, int32_t) const { return dom[pix]; }
     __device__ __forceinline__ void dom_add(size_t, int32_t, int32_t, uint32_t, Counters& c) { c.dom_writes++; }
 };
+
+// The opacity pass's slice of a sample at view distance `dist` behind the
+// texel's front (raster.wgsl:138-146): slab thickness (far - near) / 16 and
+// min(u32(max(dist - z_front, 0) / thickness), 15), exactly.
+struct DomSlab {
+    float near_, far_, thickness, inv_thickness;
+};
+__device__ __forceinline__ DomSlab dom_slab(float near_, float far_) {
+    const float thickness = rdiv(mx(far_ - near_, 1e-3f), 16.0f);
+    return {near_, far_, thickness, rdiv(1.0f, thickness)};
+}
+__device__ __forceinline__ uint32_t dom_slice(const DomSlab& s, uint32_t front_q, float dist) {
+    const float z_front = dequantize_depth(front_q, s.near_, s.far_);
+    return slice_of(mx(dist - z_front, 0.0f), s.thickness, s.inv_thickness);
+}
 
 // ---- segment drawing ------------------------------------------------------------
 
@@ -327,16 +344,13 @@ struct Drawer {
         size_t pix = (size_t)(uint32_t)y * dom_w + (uint32_t)x;
         uint32_t q = quantize_depth(dist, near_, far_);
         if (MODE == 1) {
-            sink.dom_min(pix, x, y, q, c);
+            sink.dom_min(pix, x, y, q, dist, c);
             return;
         }
         uint32_t front_q = sink.dom_front(pix, x, y);  // final: the depth pass has finished
         if (front_q == HM_DOM_EMPTY) return;
-        float z_front = dequantize_depth(front_q, near_, far_);
-        float thickness = rdiv(mx(far_ - near_, 1e-3f), 16.0f);
-        // min(to_u32(max(dist - z_front, 0) / thickness), 15), exactly
-        uint32_t slice = slice_of(mx(dist - z_front, 0.0f), thickness, rdiv(1.0f, thickness));
-        sink.dom_add(pix, x, y, slice, c);
+        const DomSlab slab = dom_slab(near_, far_);
+        sink.dom_add(pix, x, y, dom_slice(slab, front_q, dist), c);
     }
 
     __device__ void dom_segment(F3 p0, F3 p1, bool include_start) {  // raster.wgsl:150-190

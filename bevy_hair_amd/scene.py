@@ -327,6 +327,9 @@ WORKLOADS = {
     # Small parity cases.
     "tiny": Workload("tiny_straight_512x16_128x96", "straight", 512, 7, 128, 96, 128, 17, 0.0, 2),
     "small_lod": Workload("small_wavy_2048_lod_256x160", "wavy", 2048, 11, 256, 160, 256, 0, 0.0, 4),
+    # 20k wavy strands on a 64^2 DOM: tens of thousands of samples per 16 x 16
+    # tile, more than a DOM tile workgroup keeps between its two passes.
+    "dense_dom": Workload("dense_dom_wavy_20000x64_320x180", "wavy", 20_000, 5, 320, 180, 64, 65, 1.0, 1),
 }
 
 

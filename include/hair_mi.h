@@ -468,6 +468,11 @@ typedef struct hm_tuning {
                                       light's range may end inside the groom (from 64 lights when
                                       shading is called on its own), 1 = never, 2 = always; the
                                       colours are the same either way */
+    uint32_t dom_busy_blocks_per_cu; /* hm_dom with 16-texel tiles: 0 = one 256-thread workgroup per
+                                      tile; 1 or 2 = that many persistent 1024-thread workgroups per
+                                      CU, taking the tiles that hold records one at a time (each
+                                      sample walked once, kept in LDS for the opacity pass) and
+                                      then writing the empty tiles */
 } hm_tuning;
 
 hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);

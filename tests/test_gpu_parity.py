@@ -237,24 +237,28 @@ def oracle_dom(name):
     return _DOM_REF[name]
 
 
-# (tile, box limit, record buffer, binning workgroups per CU): binned with few
-# / many long segments, short and long segment runs per workgroup, and a
-# record buffer too small, which must switch to the direct kernels on the device.
-DOM_VARIANTS = [(16, 8, 0, 2), (32, 8, 0, 4), (16, 1, 0, 2), (32, 64, 0, 1), (16, 8, 0, 16), (32, 8, 1, 2)]
+# (tile, box limit, record buffer, binning workgroups per CU, persistent DOM
+# tile workgroups per CU): binned with few / many long segments, short and long
+# segment runs per workgroup, a record buffer too small, which must switch to
+# the direct kernels on the device, and the 16-texel tiles as one workgroup
+# per tile and as the persistent busy-tile kernel.
+DOM_VARIANTS = [(16, 8, 0, 2, 2), (32, 8, 0, 4, 2), (16, 1, 0, 2, 2), (32, 64, 0, 1, 2), (16, 8, 0, 16, 1),
+                (32, 8, 1, 2, 2), (16, 8, 0, 2, 0), (16, 1, 0, 2, 0)]
 
 
-@pytest.mark.parametrize("case", ["tiny", "c1", "small_lod"])
-@pytest.mark.parametrize("variant", DOM_VARIANTS, ids=[f"t{t}-box{b}-rec{n}-wg{w}" for t, b, n, w in DOM_VARIANTS])
+@pytest.mark.parametrize("case", ["tiny", "c1", "small_lod", "dense_dom"])
+@pytest.mark.parametrize("variant", DOM_VARIANTS, ids=[f"t{t}-box{b}-rec{n}-wg{w}-busy{k}" for t, b, n, w, k in DOM_VARIANTS])
 def test_binned_dom_bit_exact(renderer, case, variant):
-    tile, box, records, wg = variant
+    tile, box, records, wg, busy = variant
     sc = make_scene(case)
     w, r = sc.w, renderer
-    print(f"\n[gpu parity] hm_dom case={case} tile={tile} box={box} records={records} wg={wg}", flush=True)
+    print(f"\n[gpu parity] hm_dom case={case} tile={tile} box={box} records={records} wg={wg} busy={busy}", flush=True)
     r.targets(w.width, w.height, w.dom_size)
     g = r.upload(sc.mesh)
     keep = r.get_tuning()
     try:
-        r.set_tuning(dom_tile=tile, bin_box_limit=box, bin_records=records, bin_blocks_per_cu=wg)
+        r.set_tuning(dom_tile=tile, bin_box_limit=box, bin_records=records, bin_blocks_per_cu=wg,
+                     dom_busy_blocks_per_cu=busy)
         r.lod(g, sc.params[1], N.PASS_DOM)
         r.scan(g, sc.params[1], N.PASS_DOM)
         # twice: the second call must not depend on what the first left behind
