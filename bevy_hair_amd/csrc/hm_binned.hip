@@ -264,20 +264,23 @@ __device__ __forceinline__ void gen_pair(const BinArgs& a, const View& p, uint32
                 ef = ef1;
             }
         }
-        const uint32_t id = s * BIN_STRIDE + idx;
+        const uint32_t id = (s << a.seg_shift) + idx;
         bool is_long = false;
         uint32_t box = BIN_NONE;
         if (idx + 1u < ncp && (pr[j].in_front || ef))
             box = MODE == 0 ? camera_box(p, pr[j].screen, e, a.tile_shift, a.box_limit, is_long)
                             : segment_box(p, pr[j].screen, e, a.tile_shift, a.box_limit, is_long);
-        // Every slot of the strand is written, so the binning passes can walk
-        // segment ids without the strand's control-point count.
-        if (live) a.seg_box[id] = box;
+        // Every segment slot of the strand is written, so the binning passes
+        // can walk segment ids without the strand's control-point count (the
+        // host's slot count covers every segment: idx < 2^seg_shift here).
+        if (live && idx < (1u << a.seg_shift)) a.seg_box[id] = box;
         append_long(is_long, id, a.meta + BIN_LONG_APPEND, a.long_list, a.long_capacity);
     }
 #pragma unroll
-    for (int j = S; j < (int)(BIN_STRIDE / 32u); ++j)
-        if (live) a.seg_box[s * BIN_STRIDE + (uint32_t)r + 32u * (uint32_t)j] = BIN_NONE;
+    for (int j = S; j < (int)(BIN_STRIDE / 32u); ++j) {
+        const uint32_t idx = (uint32_t)r + 32u * (uint32_t)j;
+        if (live && idx < (1u << a.seg_shift)) a.seg_box[(s << a.seg_shift) + idx] = BIN_NONE;
+    }
 }
 
 template <int MODE>
@@ -331,7 +334,7 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
     __syncthreads();
     uint32_t total = a.indirect[0];
     total = total < a.capacity ? total : a.capacity;
-    const Run run = workgroup_run(total * BIN_STRIDE);
+    const Run run = workgroup_run(total << a.seg_shift);
     // Thread i takes segment ids first + i + k * blockDim.x: a wave's lanes
     // hold consecutive segments, so the loads are coalesced dwords and, in
     // k_bin_fill, a tile's slots go to consecutive segments in lane order.
@@ -542,7 +545,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(BinArgs a) {
     __syncthreads();
     uint32_t total = a.indirect[0];
     total = total < a.capacity ? total : a.capacity;
-    const Run run = workgroup_run(total * BIN_STRIDE);
+    const Run run = workgroup_run(total << a.seg_shift);
     uint32_t* records = a.records;
     const uint32_t wend = run.end * 4u;
     for (uint32_t r0 = run.first * 4u; r0 < wend; r0 += BIN_SEGS * blockDim.x) {
@@ -620,9 +623,9 @@ struct RecordEnds {
 };
 
 __device__ __forceinline__ RecordEnds load_record(const BinArgs& a, uint32_t id) {
-    const float* d = a.dcp + (size_t)id * 5u;
+    const float* d = a.dcp + (size_t)seg_cp(a, id) * 5u;
     return {make_float4(d[0], d[1], d[2], d[3]), make_float4(d[5], d[6], d[7], d[8]), d[4], d[9],
-            (id & (BIN_STRIDE - 1u)) == 0u};
+            seg_index(a, id) == 0u};
 }
 
 template <class D>
@@ -714,9 +717,11 @@ struct DomStashSink {
     float* s_dist;
     uint8_t* s_pix;
     uint32_t* s_n;
-    uint32_t rec = 0u;        // this thread's ordinal of the record being walked
-    uint32_t in_rec = 0u;     // samples of that record so far
-    uint32_t fail_rec = DOM_NO_FAIL, fail_skip = 0u;
+    // (record ordinal of this thread) << 11 | samples of that record so far
+    // (a walk draws at most 1024), now and at the first sample the stash
+    // could not take
+    uint32_t pos = 0u;
+    uint32_t fail = DOM_NO_FAIL;
     uint32_t skip = 0u;       // opacity re-walk: samples still to pass over
     __device__ __forceinline__ bool accepts(int32_t x, int32_t y) const {
         return (uint32_t)(x - ox) < (uint32_t)T && (uint32_t)(y - oy) < (uint32_t)T;
@@ -737,16 +742,15 @@ struct DomStashSink {
         if (rank == 0u) base = atomicAdd(s_n, (uint32_t)__popcll(m));
         base = __builtin_amdgcn_readfirstlane(base);
         const uint32_t slot = base + rank;
-        if (fail_rec == DOM_NO_FAIL) {
+        if (fail == DOM_NO_FAIL) {
             if (slot < DOM_STASH) {
                 s_dist[slot] = dist;
                 s_pix[slot] = (uint8_t)i;
             } else {
-                fail_rec = rec;
-                fail_skip = in_rec;
+                fail = pos;
             }
         }
-        ++in_rec;
+        ++pos;
     }
     __device__ __forceinline__ uint32_t dom_front(size_t, int32_t x, int32_t y) {
         if (skip != 0u) {
@@ -805,12 +809,11 @@ __global__ __launch_bounds__(DOM_WG, 8) void k_dom_tiles_busy(BinArgs a) {
             Drawer<1, DomStashSink> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
             uint32_t j = 0u;
             for (uint32_t r = begin + tid; r < end; r += DOM_WG, ++j) {
-                sink.rec = j;
-                sink.in_rec = 0u;
+                sink.pos = j << 11;
                 walk_record(d, a, a.records[r]);
             }
         }
-        if (sink.fail_rec != DOM_NO_FAIL) s_fail = 1u;
+        if (sink.fail != DOM_NO_FAIL) s_fail = 1u;
         __syncthreads();
         if (tid < (uint32_t)(T * T)) {
             const int32_t x = ox + fx, y = oy + fy;
@@ -822,11 +825,12 @@ __global__ __launch_bounds__(DOM_WG, 8) void k_dom_tiles_busy(BinArgs a) {
             const uint32_t fq = front[px];
             if (fq != HM_DOM_EMPTY) atomicAdd(counts + px * HM_DOM_LAYERS + dom_slice(slab, fq, s_dist[i]), 1u);
         }
-        if (s_fail != 0u && sink.fail_rec != DOM_NO_FAIL) {
+        if (s_fail != 0u && sink.fail != DOM_NO_FAIL) {
             Drawer<2, DomStashSink> d{p, nullptr, 0.0f, sink, {0, 0, 0, 0}};
-            uint32_t j = sink.fail_rec;
+            const uint32_t j0 = sink.fail >> 11;
+            uint32_t j = j0;
             for (uint32_t r = begin + tid + j * DOM_WG; r < end; r += DOM_WG, ++j) {
-                sink.skip = j == sink.fail_rec ? sink.fail_skip : 0u;
+                sink.skip = j == j0 ? (sink.fail & 2047u) : 0u;
                 walk_record(d, a, a.records[r]);
             }
         }
@@ -936,12 +940,13 @@ struct CamTileSink {
 // the end points' projection, tangents, ao and the strand's uv and beta).
 template <class D>
 __device__ __forceinline__ void walk_cam_record(D& d, const BinArgs& a, uint32_t id) {
-    const float4 c0 = a.cps[id], c1 = a.cps[id + 1u];
-    const float y0 = a.cpy[id], y1 = a.cpy[id + 1u];
-    const float4 t0 = a.cpt[id], t1 = a.cpt[id + 1u];
-    const float4 si = a.sinfo[id / BIN_STRIDE];
+    const uint32_t cp = seg_cp(a, id);
+    const float4 c0 = a.cps[cp], c1 = a.cps[cp + 1u];
+    const float y0 = a.cpy[cp], y1 = a.cpy[cp + 1u];
+    const float4 t0 = a.cpt[cp], t1 = a.cpt[cp + 1u];
+    const float4 si = a.sinfo[seg_strand(a, id)];
     const uint32_t ncp = __float_as_uint(si.w);
-    const uint32_t idx = id & (BIN_STRIDE - 1u);
+    const uint32_t idx = seg_index(a, id);
     const float den = u2f(ncp - 1u > 1u ? ncp - 1u : 1u), inv_den = rdiv(1.0f, den);
     const float w0 = div_by_recip(u2f(idx), den, inv_den), w1 = div_by_recip(u2f(idx + 1u), den, inv_den);
     d.segment_walk(F3{c0.x, c0.y, c0.z}, F3{c1.x, c1.y, c1.z}, F2{c0.w, y0}, F2{c1.w, y1}, F3{t0.x, t0.y, t0.z},
@@ -1029,22 +1034,24 @@ __global__ __launch_bounds__(256) void k_cam_tiles(BinArgs a) {
 }
 
 // Tiles that were split into chunks: their finished words are in the
-// G-buffer; emit the pixel lists.
+// G-buffer; emit the pixel lists (one workgroup per tile; the grid-stride
+// loop only guards a smaller grid).
 template <int T>
 __global__ __launch_bounds__(256) void k_cam_finish(BinArgs a) {
     if (a.meta[BIN_SPLIT] == 0u || !a.emit_lists) return;
-    const uint32_t tile = blockIdx.x;
     const uint32_t* choff = a.meta + BIN_META_WORDS + 3u * a.tiles + 1u;
-    if (choff[tile + 1u] - choff[tile] <= 1u) return;
     __shared__ uint32_t scratch[5];
-    const int32_t ox = (int32_t)(tile % a.tiles_x) * T, oy = (int32_t)(tile / a.tiles_x) * T;
     const int32_t W = (int32_t)a.width, H = (int32_t)a.height;
-    for (int i = (int)threadIdx.x; i < T * T; i += 256) {
-        const int32_t x = ox + i % T, y = oy + i / T;
-        const bool in = x < W && y < H;
-        const size_t g = in ? (size_t)y * (size_t)W + (size_t)x : 0u;
-        const uint64_t c = in ? a.center[g] : ~0ull, k = in ? a.cons[g] : ~0ull;
-        finish_pixel(a, tile, in, g, c, k, 0u, false, scratch);
+    for (uint32_t tile = blockIdx.x; tile < a.tiles; tile += gridDim.x) {
+        if (choff[tile + 1u] - choff[tile] <= 1u) continue;
+        const int32_t ox = (int32_t)(tile % a.tiles_x) * T, oy = (int32_t)(tile / a.tiles_x) * T;
+        for (int i = (int)threadIdx.x; i < T * T; i += 256) {
+            const int32_t x = ox + i % T, y = oy + i / T;
+            const bool in = x < W && y < H;
+            const size_t g = in ? (size_t)y * (size_t)W + (size_t)x : 0u;
+            const uint64_t c = in ? a.center[g] : ~0ull, k = in ? a.cons[g] : ~0ull;
+            finish_pixel(a, tile, in, g, c, k, 0u, false, scratch);
+        }
     }
 }
 

@@ -26,6 +26,9 @@ struct hm_groom {
     hm_strand_ref* refs[2] = {nullptr, nullptr};
     uint32_t* prefix[2] = {nullptr, nullptr};
     uint32_t* indirect[2] = {nullptr, nullptr};
+    // Per pass: the most control points the last hm_lod can have given a
+    // strand (lod.wgsl:38-55 with the override); 127 before any hm_lod.
+    uint32_t ncp_bound[2] = {127u, 127u};
 };
 
 // Work buffers of the binned DOM (sized for the largest groom and map seen).

@@ -962,6 +962,12 @@ hm_status hm_lod(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pass, const h
     uint32_t oc = ovr ? ovr->control_points : 0u;
     float of = ovr ? ovr->strand_fraction : 0.0f;
     HM_HIP(ctx, hm::launch_lod(ctx->stream, *p, g->bundles, g->layers, g->lod[pass], p->pass_mode[1], oc, of));
+    // The control points k_lod can give: 65 for the DOM (control_points_for_lod(1)),
+    // else the override, else 65 with LOD on and 127 with it off.
+    g->ncp_bound[pass] = p->pass_mode[3] == 1u ? 65u
+                         : oc                  ? std::min(std::max(oc, 2u), 127u)
+                         : p->flags[0] != 0u   ? 65u
+                                               : 127u;
     return HM_OK;
 }
 
@@ -1076,7 +1082,10 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     b.gb_words = ctx->gb_words;
     b.fill_clear = pass == HM_PASS_CAMERA ? 1u : 2u;
     b.long_list = ctx->bin.long_list;
-    b.long_capacity = (uint32_t)std::min<size_t>(slots * hm::BIN_STRIDE, 0xffffffffu);
+    // Segment slots per strand: the pass's most segments, rounded up to a power of two (16 .. 128).
+    b.seg_shift = 4u;
+    while ((1u << b.seg_shift) + 1u < g->ncp_bound[pass]) ++b.seg_shift;
+    b.long_capacity = (uint32_t)std::min<size_t>(slots << b.seg_shift, 0xffffffffu);
     b.tiles_x = tiles_x;
     b.tiles = tiles;
     b.tile_shift = T == 16u ? 4u : 5u;

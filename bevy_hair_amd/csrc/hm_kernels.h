@@ -91,8 +91,11 @@ struct FilterArgs {
     float* filtered_depth;
 };
 
-// Binned DOM raster (hm_binned.hip). Segment ids are strand slot * BIN_STRIDE
-// + index of the segment's first control point.
+// Binned raster (hm_binned.hip). Control points of strand slot s sit at
+// s * BIN_STRIDE + index; segment ids are (s << seg_shift) + index of the
+// segment's first control point, with 2^seg_shift (16 .. 128) at least the
+// pass's most segments per strand, so the binning passes walk no more empty
+// segment slots than they must.
 constexpr uint32_t BIN_STRIDE = 128u;
 constexpr uint32_t BIN_NONE = 0xffffffffu;  // no box (behind the view, off the map, or long)
 // Status words at the head of BinArgs::meta.
@@ -147,6 +150,7 @@ struct BinArgs {
     uint32_t* long_list;            // segments whose box spans more than box_limit tiles
     uint32_t long_capacity;
     uint32_t tiles_x, tiles, tile_shift, box_limit;
+    uint32_t seg_shift;             // log2 of the segment slots per strand (4 .. 7)
     uint32_t dom_size;
     uint32_t* dom;                  // fronts S*S, then counts S*S*16
     // camera pass
@@ -170,6 +174,13 @@ struct BinArgs {
     float4* filtered;
     float* filtered_depth;
 };
+
+// Strand slot, index and control point of segment `id`.
+__device__ __forceinline__ uint32_t seg_strand(const BinArgs& a, uint32_t id) { return id >> a.seg_shift; }
+__device__ __forceinline__ uint32_t seg_index(const BinArgs& a, uint32_t id) { return id & ((1u << a.seg_shift) - 1u); }
+__device__ __forceinline__ uint32_t seg_cp(const BinArgs& a, uint32_t id) {
+    return seg_strand(a, id) * BIN_STRIDE + seg_index(a, id);
+}
 
 hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks);
 hipError_t launch_dom_long(hipStream_t s, const BinArgs& a, int mode, int long_blocks);  // mode 1 before, 2 after the tiles
