@@ -247,6 +247,41 @@ def mlp_leg_run(r, g, sc, w, args) -> dict:
             "dtype": "bf16 MFMA, f32 accumulate", "training_samples": int(info.get("samples", 0))}
 
 
+def mlp_c4_leg_run(r, args) -> dict:
+    """The learned-shading configuration's evaluation view: the config-1 groom
+    (4096 straight strands x 16 segments) at 1080p with one shadow-casting
+    directional light. Frames/s and shade time with the plain method and with
+    the learned term, and the learned image's mean relative RGB error."""
+    from bevy_hair_amd import mlp_train
+    from bevy_hair_amd.workload import make_scene
+
+    sc = make_scene("c4")
+    w = sc.w
+    mlp_train.install(r)
+    r.targets(w.width, w.height, w.dom_size)
+    r.set_scene_depth(None, 0.0)
+    r.set_lights(sc.points, sc.dirs)
+    g = r.upload(sc.mesh)
+    out = {"workload": w.name, "lights": len(sc.points) + len(sc.dirs)}
+    try:
+        steps = max(10, args.steps)
+        for name, mlp in (("plain", False), ("mlp", True)):
+            opts = r.frame_opts(control_points=w.control_points, strand_fraction=w.strand_fraction, mlp=mlp,
+                                mlp_tolerance=args.mlp_tolerance)
+            for _ in range(3):
+                r.hair_pass(g, sc.params, opts)
+            ms, _, kt = timed_frames(r, g, sc.params, opts, steps)
+            out[name] = {"frames_per_s": round(1000.0 / ms, 2), "shade_ms": round(kt["shade"], 5)}
+        r.hair_pass(g, sc.params, r.frame_opts(control_points=w.control_points, strand_fraction=w.strand_fraction))
+        r.shade(sc.params[0])
+        plain = r.shaded()[0].copy()
+        out["fallback_tiles"] = r.shade_mlp(sc.params[0], 1.0, args.mlp_tolerance)
+        out["mean_rel_rgb_err"] = round(rel_err(r.shaded()[0], plain), 5)
+    finally:
+        g.free()
+    return out
+
+
 def batch_leg_run(r, args) -> dict:
     """24 views over three grooms (20k x 16, 100k x 64, 150k x 126) at
     1280x720 / 1920x1080 / 2560x1440, learned shading with the trained-range
@@ -496,6 +531,8 @@ def main() -> int:
             rates["dom_opacity_frac_of_u32_add_peak"] = round(rates["dom_opacity_atomics_per_s"] / peaks["u32_add_per_s"], 4)
     result["many_lights"] = None if args.no_lights else lights_leg_run(r, g, sc, w, args)
     g.free()
+    if result["mlp"] is not None:
+        result["mlp"]["c4"] = mlp_c4_leg_run(r, args)
     result["batch"] = None if args.no_batch else batch_leg_run(r, args)
     result["heavy"] = None if args.no_heavy else heavy_leg_run(r, args)
     r.close()

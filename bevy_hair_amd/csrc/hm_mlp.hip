@@ -97,11 +97,33 @@ __device__ __forceinline__ f32x4 mlp_fwd16(const bf16x8* T, const float* bias, b
     return y;
 }
 
-// Forward pass of four 16-pixel column blocks at once, layer by layer: each
-// weight tile is read from `T` once per layer and feeds four independent
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+
+// Two activations rounded to bf16 together (one v_cvt_pk_bf16_f32), then
+// ReLU on their bit patterns: a negative value or -0 has the sign bit set,
+// so it is below zero as a 16-bit integer, and max(x, 0) over packed 16-bit
+// integers gives +0 there and x elsewhere (one v_pk_max_i16). This equals
+// rounding max(x, 0) to bf16 for every non-NaN x, since rounding keeps the sign.
+__device__ __forceinline__ uint32_t relu_pk(float a, float b) {
+    const bf16x2 v = __builtin_convertvector(f32x2{a, b}, bf16x2);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2, v), i16x2{}));
+}
+
+// The next layer's operand from two 16-row output tiles (lane rows 4 g .. 4 g + 3
+// of each): elements 0-3 from `lo`, 4-7 from `hi`.
+__device__ __forceinline__ bf16x8 relu_operand(f32x4 lo, f32x4 hi) {
+    const uint4 w = {relu_pk(lo[0], lo[1]), relu_pk(lo[2], lo[3]), relu_pk(hi[0], hi[1]), relu_pk(hi[2], hi[3])};
+    return __builtin_bit_cast(bf16x8, w);
+}
+
+// Forward pass of NB 16-pixel column blocks at once, layer by layer: each
+// weight tile is read from `T` once per layer and feeds NB independent
 // matrix-core chains; the outputs of block cb are returned in y[cb].
-__device__ __forceinline__ void mlp_fwd64(const bf16x8* T, const float* bias, const bf16x8 b0[4], int lane,
-                                          f32x4 y[4]) {
+template <int NB>
+__device__ __forceinline__ void mlp_fwd_blocks(const bf16x8* T, const float* bias, const bf16x8 b0[NB], int lane,
+                                               f32x4 y[NB]) {
     const int g = lane >> 4;
     // The biases enter as the accumulator input of each layer's first
     // product: lane rows 16 r + 4 g .. + 3 of output tile r.
@@ -113,31 +135,21 @@ __device__ __forceinline__ void mlp_fwd64(const bf16x8* T, const float* bias, co
             bt[r] = f32x4{v.x, v.y, v.z, v.w};
         }
     };
-    f32x4 acc[4][4], bt[4];
+    f32x4 acc[NB][4], bt[4];
     bias_tiles(0, bt);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const bf16x8 w = T[(tile_base(0) + r) * 64 + lane];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w, b0[cb], bt[r]);
+        for (int cb = 0; cb < NB; ++cb) acc[cb][r] = mfma(w, b0[cb], bt[r]);
     }
-    bf16x8 B[4][2];
+    bf16x8 B[NB][2];
 #pragma unroll
     for (int L = 1; L <= 3; ++L) {
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
+        for (int cb = 0; cb < NB; ++cb) {
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                bf16x8 v;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const float lo = acc[cb][2 * c][i];
-                    const float hi = acc[cb][2 * c + 1][i];
-                    v[i] = (__bf16)(lo > 0.0f ? lo : 0.0f);
-                    v[4 + i] = (__bf16)(hi > 0.0f ? hi : 0.0f);
-                }
-                B[cb][c] = v;
-            }
+            for (int c = 0; c < 2; ++c) B[cb][c] = relu_operand(acc[cb][2 * c], acc[cb][2 * c + 1]);
         }
         if (L < 3) {
             bias_tiles(L, bt);
@@ -146,9 +158,9 @@ __device__ __forceinline__ void mlp_fwd64(const bf16x8* T, const float* bias, co
                 const bf16x8 w0 = T[(tile_base(L) + r * 2) * 64 + lane];
                 const bf16x8 w1 = T[(tile_base(L) + r * 2 + 1) * 64 + lane];
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w0, B[cb][0], bt[r]);
+                for (int cb = 0; cb < NB; ++cb) acc[cb][r] = mfma(w0, B[cb][0], bt[r]);
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb) acc[cb][r] = mfma(w1, B[cb][1], acc[cb][r]);
+                for (int cb = 0; cb < NB; ++cb) acc[cb][r] = mfma(w1, B[cb][1], acc[cb][r]);
             }
         }
     }
@@ -157,10 +169,15 @@ __device__ __forceinline__ void mlp_fwd64(const bf16x8* T, const float* bias, co
     const float4 bo4 = reinterpret_cast<const float4*>(bias + 192)[g];
     const f32x4 bo = {bo4.x, bo4.y, bo4.z, bo4.w};
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
+    for (int cb = 0; cb < NB; ++cb) {
         f32x4 v = mfma(w0, B[cb][0], bo);
         y[cb] = mfma(w1, B[cb][1], v);
     }
+}
+
+__device__ __forceinline__ void mlp_fwd64(const bf16x8* T, const float* bias, const bf16x8 b0[4], int lane,
+                                          f32x4 y[4]) {
+    mlp_fwd_blocks<4>(T, bias, b0, lane, y);
 }
 
 __device__ __forceinline__ bf16x8 load_x8(const float* row8) {
@@ -388,30 +405,50 @@ __global__ void k_mlp_mask(float* __restrict__ w, float* __restrict__ b) {
 // The DOM inputs of one pixel from dom_visibility's four taps
 // (shade.wesl:279-310): returns that visibility, bit for bit, and fills
 // `layers` with the transmittance through layers 0..k of the bilinearly
-// weighted layer counts (k = 0..15).
+// weighted layer counts (k = 0..15). The counts stream in four groups of four
+// slices (one 16-byte load per tap and group), each group continuing every
+// tap's optical depth and the weighted layer sums, so that no more than two
+// groups' counts are held at once.
 __device__ float dom_features(const View& p, const uint32_t* __restrict__ dom, F3 world, float beta,
                               float layers[16]) {
     for (int k = 0; k < 16; ++k) layers[k] = 1.0f;
-    DomTaps T;
-    if (!dom_taps(p, dom, world, beta, T)) return 1.0f;
-    const float wts[4] = {(1.0f - T.tx) * (1.0f - T.ty), T.tx * (1.0f - T.ty), (1.0f - T.tx) * T.ty, T.tx * T.ty};
+    DomPos P;
+    if (!dom_pos(p, world, beta, P)) return 1.0f;
+    const uint4* counts = reinterpret_cast<const uint4*>(dom + (size_t)to_u32(p.dom_info.x) * to_u32(p.dom_info.y));
+    bool on[4];
+    float t[4], occ[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const uint32_t f = dom[P.pix[c]];
+        on[c] = P.inside[c] && f != HM_DOM_EMPTY;
+        t[c] = on[c] ? dom_tap_t(p, f, P.dist) : 0.0f;
+        occ[c] = 0.0f;
+    }
+    const float wts[4] = {(1.0f - P.tx) * (1.0f - P.ty), P.tx * (1.0f - P.ty), (1.0f - P.tx) * P.ty, P.tx * P.ty};
     float pre = 0.0f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
+        uint4 v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = counts[(size_t)P.pix[c] * (HM_DOM_LAYERS / 4) + q];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) occ[c] = dom_occ4(occ[c], v[c], t[c], 4 * q);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            float c = 0.0f;
+            float cw = 0.0f;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const uint4 v = T.counts[t][q];
-                const uint32_t cs[4] = {v.x, v.y, v.z, v.w};
-                if (T.front[t] != HM_DOM_EMPTY) c += wts[t] * u2f(cs[k]);
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t cs[4] = {v[c].x, v[c].y, v[c].z, v[c].w};
+                if (on[c]) cw += wts[c] * u2f(cs[k]);
             }
-            pre += c;
-            layers[4 * q + k] = fexp(-pre * p.filter_params.w);
+            pre += cw;
+            layers[4 * q + k] = hexp(-pre * p.filter_params.w);
         }
     }
-    return dom_taps_visibility(p, T);
+    float vis[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) vis[c] = on[c] ? hexp(-occ[c] * p.filter_params.w) : 1.0f;
+    return mixs(mixs(vis[0], vis[1], P.tx), mixs(vis[2], vis[3], P.tx), P.ty);
 }
 
 // Features of one pixel (order and scaling in hm_mlp.h); returns the key
@@ -422,7 +459,7 @@ __device__ float mlp_features(const View& p, const PixelShade& px, F3 key_dir, c
     F3 l = safe_normalize(key_dir);
     f[6] = l.x; f[7] = l.y; f[8] = l.z;
     const float vis = dom_features(p, dom, px.world, px.beta, f + 9);
-    const float depth = -flog(mx(vis, 1e-6f));
+    const float depth = -hlog(mx(vis, 1e-6f));
     f[25] = px.sigma_a.x * MLP_SIGMA_SCALE;
     f[26] = px.sigma_a.y * MLP_SIGMA_SCALE;
     f[27] = px.sigma_a.z * MLP_SIGMA_SCALE;
@@ -520,7 +557,7 @@ __device__ void color_pair_by_wave(const ShadeArgs& a, const PixelShade& px, con
         F3 ltf = point_pos(L) - px.world;
         float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
         if (att == 0.0f) continue;
-        BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, safe_normalize(ltf));
+        BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize(safe_normalize(ltf)));
         F3 single = single_scatter(g, px.sigma_a, px.shape);
         F3 w = point_color(L) * att;
         if (L.kind != HM_LIGHT_POINT) w = w * spot_mask(L, px.world);
@@ -529,11 +566,11 @@ __device__ void color_pair_by_wave(const ShadeArgs& a, const PixelShade& px, con
     }
     for (uint32_t i = (uint32_t)lane; i < a.directional_count; i += 64u) {
         const hm_directional_light& L = a.directionals[i];
-        BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, dir_of(L));
+        BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize(dir_of(L)));
         F3 single = single_scatter(g, px.sigma_a, px.shape);
         F3 w = F3{L.color[0], L.color[1], L.color[2]};
         if ((int)i == key) {
-            single = fexp(-depth) * single;
+            single = hexp(-depth) * single;
             cn = cn + (single + net_ms) * w;
             cr = cr + (single + ref_ms) * w;
         } else {
@@ -567,25 +604,27 @@ __device__ DualScatter dual_scatter_by_wave(const PixelShade& px, int lane) {
 }
 
 // Network shading, one wave per group of 64 consecutive covered pixels, one
-// lane per pixel. Per group: features -> LDS (bf16) -> MFMA forward -> LDS ->
-// dual-scattering terms -> colour (plain_pixel_color with the network's terms
-// in place of dual_scatter's). A pixel with a feature outside the trained
-// range takes the plain method's terms instead (per pixel: its colour does not
-// depend on the order of the covered-pixel list). When `tol` > 0, every
-// PROBE_STRIDE-th range-clean group also shades one probe pixel by the plain
-// method (all 64 lanes on that pixel) and the group is re-shaded by the plain
-// method when the network colour misses it by more than `tol`; which pixels
-// share a group follows the list order, so that re-shading does not. The
-// pixel set-up (PixelShade) is computed once, before the network, and held in
-// registers through it for the light loop.
+// lane per pixel. Per group: the pixel set-up and features -> LDS (bf16) ->
+// MFMA forward (two 16-pixel column blocks at a time, in place) -> LDS ->
+// dual-scattering terms -> the key light's multiple scattering -> colour
+// (pixel_color, the plain method's light loop, with the network's terms in
+// place of dual_scatter's). A pixel with a feature outside the trained range
+// takes the plain method's terms instead, formed by its own lane with
+// dual_scatter (per pixel and by the same code, so its colour does not depend
+// on the order of the covered-pixel list or on its group's other pixels). When
+// `tol` > 0, every PROBE_STRIDE-th range-clean group also shades one probe
+// pixel by the plain method (all 64 lanes on that pixel) and the group is
+// re-shaded by the plain method when the network colour misses it by more
+// than `tol`; which pixels share a group follows the list order, so that
+// re-shading does not.
 constexpr uint32_t PROBE_STRIDE = 4;
 
-template <uint32_t F>  // SHADE_* variant bits (pixel_color)
+// F: SHADE_* variant bits (pixel_color); PROBE: the error-bound probe (tol > 0).
+template <uint32_t F, bool PROBE>
 __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs m, PixelList L) {
     constexpr bool CULL = (F & SHADE_CULL) != 0u;
     __shared__ bf16x8 s_tiles[MLP_TILES * 64];
     __shared__ __align__(16) float s_bias[MLP_NBIAS];
-    __shared__ float s_lo[MLP_IN], s_hi[MLP_IN];
     __shared__ float4 s_io[4][64 * 4];  // per wave: 64 x 32 bf16 features, then 64 x 16 outputs
     __shared__ uint32_t pre[LIST_SUBS + 1];
     list_prefix(L, pre);  // synchronizes the block
@@ -596,10 +635,6 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
         uint4* dst = reinterpret_cast<uint4*>(s_tiles);
         for (int i = threadIdx.x; i < MLP_TILES * 64; i += 256) dst[i] = src[i];
         for (int i = threadIdx.x; i < MLP_NBIAS; i += 256) s_bias[i] = m.bias[i];
-        if (threadIdx.x < MLP_IN) {
-            s_lo[threadIdx.x] = m.lo[threadIdx.x];
-            s_hi[threadIdx.x] = m.hi[threadIdx.x];
-        }
     }
     __syncthreads();
     const View& p = *reinterpret_cast<const View*>(&a.view);
@@ -608,6 +643,11 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
     const int key = key_light(a);
     const F3 kd = key >= 0 ? dir_of(a.directionals[key]) : F3{0.0f, 1.0f, 0.0f};
     float4* io = s_io[wv];
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     for (uint32_t grp = blockIdx.x * 4u + (uint32_t)wv; grp < groups && grp < m.groups_cap;
          grp += gridDim.x * 4u) {
         const uint32_t i = grp * 64u + (uint32_t)lane;
@@ -615,20 +655,15 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
         const bool valid = pix < pixels;
         float depth = 0.0f;
         bool outside = false;
-        // the pixel's G-buffer word and coverage (read once, for the set-up)
-        uint64_t word = 0;
-        uint32_t beta_word = 0;
         PixelShade px;  // kept across the network for the light loop
         px.world = splat3(0.0f);
         {
             float f[32];
             for (int k = 0; k < 32; ++k) f[k] = 0.0f;
             if (valid) {
-                word = a.center[pix];
-                beta_word = a.beta[pix];
-                px = pixel_setup(p, word, beta_word, pix % width, pix / width);
+                px = pixel_setup(p, a.center[pix], a.beta[pix], pix % width, pix / width);
                 depth = mlp_features(p, px, kd, a.dom, f);
-                for (int k = 9; k < MLP_NFEAT; ++k) outside |= f[k] < s_lo[k] || f[k] > s_hi[k];
+                for (int k = 9; k < MLP_NFEAT; ++k) outside |= f[k] < m.lo[k] || f[k] > m.hi[k];
             }
             // row `lane`: 32 bf16 = four 16-byte vectors
 #pragma unroll
@@ -641,30 +676,34 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
         }
         // flag (per group, for the count): 1 when a pixel is outside the
         // trained range; that pixel alone takes the plain method
-        uint32_t flag = __any(outside) ? 1u : 0u;
+        const uint64_t out_lanes = __ballot(outside && valid);
+        uint32_t flag = out_lanes ? 1u : 0u;
         float o[MLP_NOUT] = {};
         if (!__all(outside || !valid)) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            bf16x8 b0[4];
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb) b0[cb] = __builtin_bit_cast(bf16x8, io[(cb * 16 + q) * 4 + g]);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();  // operands read before the outputs overwrite them
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            f32x4 yv[4];
-            mlp_fwd64(s_tiles, s_bias, b0, lane, yv);
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb) io[(cb * 16 + q) * 4 + g] = make_float4(yv[cb][0], yv[cb][1], yv[cb][2], yv[cb][3]);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            wave_sync();
+            // Block cb is pixels 16 cb .. 16 cb + 15: its features are read
+            // from those rows and its outputs written back over them, two
+            // blocks at a time (two blocks' network state in registers).
+#pragma unroll 1
+            for (int cb = 0; cb < 4; cb += 2) {
+                const bf16x8 b0[2] = {__builtin_bit_cast(bf16x8, io[(cb * 16 + q) * 4 + g]),
+                                      __builtin_bit_cast(bf16x8, io[((cb + 1) * 16 + q) * 4 + g])};
+                // An offset the compiler cannot see through: the weight and
+                // bias reads stay inside the loop instead of being hoisted
+                // out of it into registers held across both passes.
+                int off = 0;
+                asm volatile("" : "+v"(off));
+                f32x4 y[2];
+                mlp_fwd_blocks<2>(s_tiles + off, s_bias + off, b0, lane, y);
+                io[(cb * 16 + q) * 4 + g] = make_float4(y[0][0], y[0][1], y[0][2], y[0][3]);
+                io[((cb + 1) * 16 + q) * 4 + g] = make_float4(y[1][0], y[1][1], y[1][2], y[1][3]);
+            }
+            wave_sync();
             const float* y = reinterpret_cast<const float*>(io + lane * 4);
 #pragma unroll
-            for (int k = 0; k < MLP_NOUT; ++k) o[k] = mlp_log_output(k) ? mx(fexp(y[k]) - MLP_LOG_EPS, 0.0f) : y[k];
+            for (int k = 0; k < MLP_NOUT; ++k) o[k] = mlp_log_output(k) ? mx(hexp(y[k]) - MLP_LOG_EPS, 0.0f) : y[k];
         }
-        if (flag == 0u && m.tol > 0.0f && grp % PROBE_STRIDE == 0u) {
+        if (PROBE && flag == 0u && grp % PROBE_STRIDE == 0u) {
             // probe: one pixel of the group, the same in every lane, shaded
             // with the network's terms and with the plain method's
             const uint32_t in_group = n - grp * 64u < 64u ? n - grp * 64u : 64u;
@@ -675,10 +714,9 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
                 const float pd = __shfl(depth, r, 64);
                 float po[MLP_NOUT];
                 for (int k = 0; k < MLP_NOUT; ++k) po[k] = __shfl(o[k], r, 64);
-                const F3 key_dir_p = kd;
                 const DualScatter dn = network_dual_scatter(po, pp.shape);
                 const DualScatter ds = dual_scatter_by_wave(pp, lane);
-                const float th = key >= 0 ? key_theta_h(pp, key_dir_p) : 0.0f;
+                const float th = key >= 0 ? key_theta_h(pp, kd) : 0.0f;
                 const F3 net_ms = key >= 0 ? key_multiple_scattering(th, pd, pp.shape, dn) : splat3(0.0f);
                 const F3 key_ms = key >= 0 ? key_multiple_scattering(th, pd, pp.shape, ds) : splat3(0.0f);
                 F3 got, ref;
@@ -689,33 +727,26 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
                 if (err > m.tol * mx(mag, m.floor)) flag = 2u;
             }
         }
-        F3 c = splat3(0.0f);
-        if (CULL) {
-            LocalBack lb = {splat3(0.0f), splat3(0.0f), splat3(0.0f)};
-            F3 key_ms = splat3(0.0f);
-            if (valid) {
-                // the network's dual-scattering terms, or the plain method's
-                // (plain_pixel_color; `depth` is its key-light depth)
-                const DualScatter ds = (flag == 2u || outside) ? dual_scatter(px.cos_theta_o, px.sigma_a, px.shape)
-                                                               : network_dual_scatter(o, px.shape);
-                lb = local_terms(ds);
-                if (key >= 0) key_ms = key_multiple_scattering(key_theta_h(px, kd), depth, px.shape, ds);
-            }
-            c = pixel_color<F>(a, px, valid, lb, key, depth, key_ms);  // the whole wave calls
-        } else if (valid) {
-            const DualScatter ds = (flag == 2u || outside) ? dual_scatter(px.cos_theta_o, px.sigma_a, px.shape)
-                                                           : network_dual_scatter(o, px.shape);
-            const F3 key_ms = key >= 0 ? key_multiple_scattering(key_theta_h(px, kd), depth, px.shape, ds) : splat3(0.0f);
-            c = pixel_color<F>(a, px, true, local_terms(ds), key, depth, key_ms);
+        // The dual-scattering terms: the network's, or the plain method's for
+        // the group (probe miss) or for the lanes outside the trained range.
+        DualScatter ds;
+        if (valid && (flag == 2u || outside)) ds = dual_scatter(px.cos_theta_o, px.sigma_a, px.shape);
+        else if (valid) ds = network_dual_scatter(o, px.shape);
+        LocalBack lb = {splat3(0.0f), splat3(0.0f), splat3(0.0f)};
+        F3 key_ms = splat3(0.0f);
+        if (valid) {
+            lb = local_terms(ds);
+            if (key >= 0) key_ms = key_multiple_scattering(key_theta_h(px, kd), depth, px.shape, ds);
         }
+        F3 c = splat3(0.0f);
+        if (CULL) c = pixel_color<F>(a, px, valid, lb, key, depth, key_ms);  // the whole wave calls
+        else if (valid) c = pixel_color<F>(a, px, true, lb, key, depth, key_ms);
         if (valid) {
             a.shaded[pix] = make_float4(c.x, c.y, c.z, 1.0f);
             a.shaded_depth[pix] = px.view_dist;
         }
         if (lane == 0) m.group_flags[grp] = flag;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();  // the LDS rows are rewritten by the next group
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();  // the LDS rows are rewritten by the next group
     }
 }
 
@@ -775,13 +806,24 @@ hipError_t launch_shade_mlp(hipStream_t s, const ShadeArgs& a, const MlpShadeArg
                             int blocks) {
     if (m.groups_cap == 0) return hipSuccess;
     const uint32_t want = (m.groups_cap + 3u) / 4u;
-    const uint32_t grid = want < (uint32_t)blocks ? want : (uint32_t)blocks;
+    const dim3 grid(want < (uint32_t)blocks ? want : (uint32_t)blocks);
+    const bool probe = m.tol > 0.0f;
     switch (shade_variant(a)) {
-        case 0u: hipLaunchKernelGGL(k_shade_mlp<0u>, dim3(grid), dim3(256), 0, s, a, m, L); break;
-        case SHADE_CULL: hipLaunchKernelGGL(k_shade_mlp<SHADE_CULL>, dim3(grid), dim3(256), 0, s, a, m, L); break;
-        case SHADE_EXTRAS: hipLaunchKernelGGL(k_shade_mlp<SHADE_EXTRAS>, dim3(grid), dim3(256), 0, s, a, m, L); break;
+        case 0u:
+            if (probe) hipLaunchKernelGGL((k_shade_mlp<0u, true>), grid, dim3(256), 0, s, a, m, L);
+            else hipLaunchKernelGGL((k_shade_mlp<0u, false>), grid, dim3(256), 0, s, a, m, L);
+            break;
+        case SHADE_CULL:
+            if (probe) hipLaunchKernelGGL((k_shade_mlp<SHADE_CULL, true>), grid, dim3(256), 0, s, a, m, L);
+            else hipLaunchKernelGGL((k_shade_mlp<SHADE_CULL, false>), grid, dim3(256), 0, s, a, m, L);
+            break;
+        case SHADE_EXTRAS:
+            if (probe) hipLaunchKernelGGL((k_shade_mlp<SHADE_EXTRAS, true>), grid, dim3(256), 0, s, a, m, L);
+            else hipLaunchKernelGGL((k_shade_mlp<SHADE_EXTRAS, false>), grid, dim3(256), 0, s, a, m, L);
+            break;
         default:
-            hipLaunchKernelGGL(k_shade_mlp<SHADE_CULL | SHADE_EXTRAS>, dim3(grid), dim3(256), 0, s, a, m, L);
+            if (probe) hipLaunchKernelGGL((k_shade_mlp<SHADE_CULL | SHADE_EXTRAS, true>), grid, dim3(256), 0, s, a, m, L);
+            else hipLaunchKernelGGL((k_shade_mlp<SHADE_CULL | SHADE_EXTRAS, false>), grid, dim3(256), 0, s, a, m, L);
             break;
     }
     return hipGetLastError();

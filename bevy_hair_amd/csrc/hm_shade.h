@@ -24,6 +24,13 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float fcos(float x) { return __cosf(x); }
 __device__ __forceinline__ float fpow(float b, float e) { return __expf(e * __logf(b)); }
+// The bare base-2 instructions (v_exp_f32, v_log_f32) for the BSDF and DOM
+// lookup: no range scaling for results below 2^-126, which cannot change a
+// colour held to a relative tolerance (the library forms above add a
+// compare and two selects per call).
+__device__ __forceinline__ float hexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ float hlog(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+__device__ __forceinline__ float hpow(float b, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(b)); }
 // Abramowitz & Stegun 4.4.46 (Hastings): asin x = pi/2 - sqrt(1 - x) P7(x)
 // on [0, 1], |error| <= 2e-8; odd extension for x < 0.
 __device__ __forceinline__ float fasin(float x) {
@@ -52,7 +59,11 @@ struct Lobes { F3 r, tt, trt; };
 struct Shape { F3 alpha, beta, inv_beta; };
 struct DualScatter { Lobes fwd; F3 a_f, beta_f2, a_back, delta_b, sigma_b2; };
 
-__device__ __forceinline__ F3 vexp(F3 a) { return {fexp(a.x), fexp(a.y), fexp(a.z)}; }
+// The BSDF, light terms and dual scattering below are continuous in every
+// input: multiply-adds may fuse there (one rounding instead of two).
+#pragma clang fp contract(fast)
+
+__device__ __forceinline__ F3 vexp(F3 a) { return {hexp(a.x), hexp(a.y), hexp(a.z)}; }
 __device__ __forceinline__ F3 vsqrt(F3 a) { return {fsqrt(a.x), fsqrt(a.y), fsqrt(a.z)}; }
 
 // 1 / max(b, 1e-3) per channel: the reciprocal width of gaussian3.
@@ -85,24 +96,47 @@ __device__ __forceinline__ F3 absorption_from_color(F3 color, float roughness) {
     float d = 5.969f - 0.215f * b + 2.532f * b2 - 10.73f * b3 + 5.574f * b2 * b2 + 0.245f * b3 * b2;
     F3 c = vclamp(color, 1e-4f, 1.0f);
     float rd = frcp(d);
-    F3 l = {flog(c.x) * rd, flog(c.y) * rd, flog(c.z) * rd};
+    F3 l = {hlog(c.x) * rd, hlog(c.y) * rd, hlog(c.z) * rd};
     return l * l;
 }
 
+// The parts of azimuthal that depend on theta_d only (one light or one
+// pixel's quadrature): 1 / cos theta_d, Karis's a = 1 / eta, and the TRT
+// lobe's Fresnel factor (1 - f)^2 f and transmittance exp(-3.2 sigma_a / cos theta_d).
+struct AzimuthalD {
+    float inv_cd, a, trt_f;
+    F3 t_trt;
+};
+__device__ __forceinline__ AzimuthalD azimuthal_d(float cos_theta_d, F3 sigma_a) {
+    AzimuthalD d;
+    d.inv_cd = frcp(cos_theta_d);
+    d.a = frcp(1.19f * d.inv_cd + 0.36f * cos_theta_d);
+    const float f_trt = fresnel(cos_theta_d * 0.5f);
+    d.trt_f = (1.0f - f_trt) * (1.0f - f_trt) * f_trt;
+    d.t_trt = vexp(sigma_a * (-3.2f * d.inv_cd));
+    return d;
+}
+
+// N_R and N_TT of azimuthal (shade.wesl:94-116) at one angle: cos(phi),
+// cos(phi / 2) = sqrt(0.5 + 0.5 cos phi), Karis's offset term 0.6 - 0.8 cos phi
+// and the TT weight exp(-3.65 cos phi - 3.98).
+__device__ __forceinline__ void azimuthal_r_tt(float cos_theta_d, const AzimuthalD& d, F3 sigma_a, float cos_half_phi,
+                                               float h_k, float w_tt, float& n_r, F3& n_tt) {
+    n_r = 0.25f * cos_half_phi * fresnel(cos_theta_d * cos_half_phi);
+    const float h = cos_half_phi * (1.0f + d.a * h_k);
+    const float f_tt = fresnel(cos_theta_d * fsqrt(clampf(1.0f - h * h, 0.0f, 1.0f)));
+    const float s_tt = fsqrt(clampf(1.0f - h * h * d.a * d.a, 0.0f, 1.0f));
+    n_tt = vexp(sigma_a * (-2.0f * s_tt * d.inv_cd)) * (w_tt * (1.0f - f_tt) * (1.0f - f_tt));
+}
+
 __device__ __forceinline__ Lobes azimuthal(float cos_theta_d, float cos_phi, F3 sigma_a) {
-    float cos_half_phi = fsqrt(clampf(0.5f + 0.5f * cos_phi, 0.0f, 1.0f));
-    float n_r = 0.25f * cos_half_phi * fresnel(cos_theta_d * cos_half_phi);
-    float inv_cd = frcp(cos_theta_d);
-    float eta = 1.19f * inv_cd + 0.36f * cos_theta_d;
-    float a = frcp(eta);
-    float h = cos_half_phi * (1.0f + a * (0.6f - 0.8f * cos_phi));
-    float f_tt = fresnel(cos_theta_d * fsqrt(clampf(1.0f - h * h, 0.0f, 1.0f)));
-    float s_tt = fsqrt(clampf(1.0f - h * h * a * a, 0.0f, 1.0f));
-    F3 t_tt = vexp(sigma_a * (-2.0f * s_tt * inv_cd));
-    F3 n_tt = t_tt * (fexp(-3.65f * cos_phi - 3.98f) * (1.0f - f_tt) * (1.0f - f_tt));
-    float f_trt = fresnel(cos_theta_d * 0.5f);
-    F3 t_trt = vexp(sigma_a * (-3.2f * inv_cd));
-    F3 n_trt = t_trt * (fexp(17.0f * cos_phi - 16.78f) * (1.0f - f_trt) * (1.0f - f_trt) * f_trt);
+    const AzimuthalD d = azimuthal_d(cos_theta_d, sigma_a);
+    const float cos_half_phi = fsqrt(clampf(0.5f + 0.5f * cos_phi, 0.0f, 1.0f));
+    float n_r;
+    F3 n_tt;
+    azimuthal_r_tt(cos_theta_d, d, sigma_a, cos_half_phi, 0.6f - 0.8f * cos_phi, hexp(-3.65f * cos_phi - 3.98f), n_r,
+                   n_tt);
+    const F3 n_trt = d.t_trt * (hexp(17.0f * cos_phi - 16.78f) * d.trt_f);
     return {splat3(n_r), n_tt, n_trt};
 }
 
@@ -110,12 +144,43 @@ __device__ __forceinline__ F3 lobe_mean(const Lobes& l, F3 total, F3 q) {
     return fdiv3(l.r * q.x + l.tt * q.y + l.trt * q.z, vmax(total, splat3(1e-5f)));
 }
 
-// Azimuthal sample i of dual_scatter's 16-point quadrature: i < 8 backward
-// (phi = (i + 0.5) pi / 16), i >= 8 forward (phi = pi/2 + (i - 7.5) pi / 16).
+// The 16 azimuthal angles of dual_scatter's quadrature (shade.wesl:158-162):
+// sample i < 8 backward, phi = (i + 0.5) pi / 16; i >= 8 forward,
+// phi = pi / 2 + (i - 7.5) pi / 16. Per sample, rounded to f32 from double
+// precision: cos phi, cos(phi / 2) = sqrt(0.5 + 0.5 cos phi), 0.6 - 0.8 cos phi,
+// exp(-3.65 cos phi - 3.98) (TT) and exp(17 cos phi - 16.78) (TRT).
+struct DualAngle { float cos_phi, cos_half_phi, h_k, w_tt, w_trt; };
+constexpr DualAngle kDualAngles[16] = {
+    {0.9951847195625305f, 0.9987954497337341f, -0.19614776968955994f, 0.0004942721570841968f, 1.1481364965438843f},
+    {0.9569403529167175f, 0.9891765117645264f, -0.1655522882938385f, 0.0005683163763023913f, 0.5992873907089233f},
+    {0.8819212317466736f, 0.9700312614440918f, -0.10553698241710663f, 0.000747323443647474f, 0.16740503907203674f},
+    {0.7730104327201843f, 0.9415440559387207f, -0.01840834692120552f, 0.0011121189454570413f, 0.026283269748091698f},
+    {0.6343932747840881f, 0.903989315032959f, 0.09248538315296173f, 0.0018445213790982962f, 0.002490394050255418f},
+    {0.47139665484428406f, 0.8577286005020142f, 0.22288267314434052f, 0.003343965858221054f, 0.0001559060619911179f},
+    {0.2902846336364746f, 0.803207516670227f, 0.36777228116989136f, 0.006476733833551407f, 7.17318198439898e-06f},
+    {0.0980171337723732f, 0.7409511208534241f, 0.5215862989425659f, 0.013065729290246964f, 2.7302328931000375e-07f},
+    {-0.09801722317934036f, 0.6715589165687561f, 0.6784137487411499f, 0.026722826063632965f, 9.747131990422986e-09f},
+    {-0.2902847230434418f, 0.5956993103027344f, 0.8322277665138245f, 0.05390884354710579f, 3.709921259087423e-10f},
+    {-0.4713968336582184f, 0.5141026973724365f, 0.9771174788475037f, 0.10441295057535172f, 1.706918807686808e-11f},
+    {-0.6343932747840881f, 0.4275550842285156f, 1.1075146198272705f, 0.18929198384284973f, 1.0685851075525887e-12f},
+    {-0.7730104923248291f, 0.3368898332118988f, 1.2184083461761475f, 0.31395310163497925f, 1.0125053735970252e-13f},
+    {-0.8819213509559631f, 0.24298009276390076f, 1.3055371046066284f, 0.4672050476074219f, 1.5896729801110655e-14f},
+    {-0.9569403529167175f, 0.146730437874794f, 1.365552306175232f, 0.614363968372345f, 4.44060410729856e-15f},
+    {-0.9951847195625305f, 0.04906770959496498f, 1.3961477279663086f, 0.7063984870910645f, 2.3178410738991647e-15f},
+};
+// Sums of the TRT weights over the backward and forward samples: N_TRT is
+// that weight times factors that do not depend on phi, so each half of the
+// quadrature sums it in one product.
+constexpr float kDualTrtBack = 1.9437659917028196f, kDualTrtFwd = 1.0136385360493365e-08f;
+
+// Azimuthal sample i of the quadrature (i < 8 backward, i >= 8 forward).
 __device__ __forceinline__ Lobes dual_sample(uint32_t i, float cos_theta_d, F3 sigma_a) {
-    float u = (u2f(i & 7u) + 0.5f) * 0.125f;
-    float phi = i < 8u ? u * 0.5f * kPi : (1.0f + u) * 0.5f * kPi;
-    return azimuthal(cos_theta_d, fcos(phi), sigma_a);
+    const DualAngle s = kDualAngles[i];
+    const AzimuthalD d = azimuthal_d(cos_theta_d, sigma_a);
+    float n_r;
+    F3 n_tt;
+    azimuthal_r_tt(cos_theta_d, d, sigma_a, s.cos_half_phi, s.h_k, s.w_tt, n_r, n_tt);
+    return {splat3(n_r), n_tt, d.t_trt * (s.w_trt * d.trt_f)};
 }
 
 __device__ __forceinline__ Lobes lobes_add(const Lobes& a, const Lobes& b) {
@@ -125,12 +190,28 @@ __device__ __forceinline__ Lobes lobes_add(const Lobes& a, const Lobes& b) {
 // dual_scatter's closing part from the forward and backward quadrature sums.
 __device__ DualScatter dual_finish(Lobes fwd, Lobes back, const Shape& shape);
 
+// dual_scatter (shade.wesl:153-190): the 16 samples unrolled over the angle
+// table; the theta_d parts are formed once and the TRT sums in closed form.
 __device__ DualScatter dual_scatter(float cos_theta_d, F3 sigma_a, const Shape& shape) {
-    Lobes fwd = {splat3(0), splat3(0), splat3(0)}, back = fwd;
-    for (uint32_t i = 0; i < 8u; ++i) {
-        back = lobes_add(back, dual_sample(i, cos_theta_d, sigma_a));
-        fwd = lobes_add(fwd, dual_sample(8u + i, cos_theta_d, sigma_a));
+    const AzimuthalD d = azimuthal_d(cos_theta_d, sigma_a);
+    float r_back = 0.0f, r_fwd = 0.0f;
+    F3 tt_back = splat3(0.0f), tt_fwd = splat3(0.0f);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        float n_r;
+        F3 n_tt;
+        azimuthal_r_tt(cos_theta_d, d, sigma_a, kDualAngles[i].cos_half_phi, kDualAngles[i].h_k, kDualAngles[i].w_tt,
+                       n_r, n_tt);
+        if (i < 8) {
+            r_back += n_r;
+            tt_back = tt_back + n_tt;
+        } else {
+            r_fwd += n_r;
+            tt_fwd = tt_fwd + n_tt;
+        }
     }
+    const Lobes back = {splat3(r_back), tt_back, d.t_trt * (kDualTrtBack * d.trt_f)};
+    const Lobes fwd = {splat3(r_fwd), tt_fwd, d.t_trt * (kDualTrtFwd * d.trt_f)};
     return dual_finish(fwd, back, shape);
 }
 
@@ -158,32 +239,60 @@ __device__ DualScatter dual_finish(Lobes fwd, Lobes back, const Shape& shape) {
     return {fwd, a_f, beta_f2, a_back, delta_b, sigma_b * sigma_b};
 }
 
+// The geometry below follows the oracle's unfused, correctly rounded
+// operations: theta_i = asin(t . l) and the components across the fiber are
+// ill-conditioned where the light or the view runs along the fiber, and an ulp
+// there moves a narrow lobe by far more than the tolerance.
+#pragma clang fp contract(off)
+
 // Angles of one light/view pair around the fiber (shade.wesl:203-215).
 struct BsdfGeom {
     F3 t, l;
     float theta_h, cos_theta_d, cos_phi;
 };
 
-// t and v already normalized (safe_normalize of the tangent and view vectors,
-// once per pixel). cos_phi is formed with the exact operations: near
-// cos_phi = -1 the R lobe takes sqrt(0.5 + 0.5 cos_phi), which turns an ulp
-// of cos_phi into a large relative change.
-__device__ __forceinline__ BsdfGeom bsdf_geom_n(F3 t, F3 v, F3 light) {
-    BsdfGeom g;
-    F3 l = safe_normalize(light);
-    float sin_i = clampf(dot3(t, l), -1.0f, 1.0f);
-    float sin_o = clampf(dot3(t, v), -1.0f, 1.0f);
-    float theta_i = fasin(sin_i), theta_o = fasin(sin_o);
-    g.theta_h = theta_i + theta_o;
-    g.cos_theta_d = mx(fcos(0.5f * fabsf(theta_i - theta_o)), 1e-3f);
-    F3 t_i = l - t * sin_i;
-    F3 t_o = v - t * sin_o;
-    float proj = __builtin_sqrtf(mx(dot3(t_i, t_i) * dot3(t_o, t_o), 1e-8f));
-    g.cos_phi = clampf(rdiv(dot3(t_i, t_o), proj), -1.0f, 1.0f);
+// The view's side of bsdf_geom, once per pixel: t and v normalized
+// (safe_normalize of the tangent and view vectors), sin and asin of theta_o,
+// the view's component across the fiber and its squared length.
+struct ViewGeom {
+    F3 t, v, t_o;
+    float sin_o, theta_o, to2;
+};
+__device__ __forceinline__ ViewGeom view_geom(F3 t, F3 v) {
+    ViewGeom g;
     g.t = t;
+    g.v = v;
+    g.sin_o = clampf(dot3(t, v), -1.0f, 1.0f);
+    g.theta_o = fasin(g.sin_o);
+    g.t_o = v - t * g.sin_o;
+    g.to2 = dot3(g.t_o, g.t_o);
+    return g;
+}
+
+// bsdf_geom for a light direction `l` already normalized. cos_phi is formed
+// with the exact operations: near cos_phi = -1 the R lobe takes
+// sqrt(0.5 + 0.5 cos_phi), which turns an ulp of cos_phi into a large
+// relative change.
+__device__ __forceinline__ BsdfGeom bsdf_geom_u(const ViewGeom& vg, F3 l) {
+    BsdfGeom g;
+    float sin_i = clampf(dot3(vg.t, l), -1.0f, 1.0f);
+    float theta_i = fasin(sin_i);
+    g.theta_h = theta_i + vg.theta_o;
+    g.cos_theta_d = mx(fcos(0.5f * fabsf(theta_i - vg.theta_o)), 1e-3f);
+    F3 t_i = l - vg.t * sin_i;
+    float proj = __builtin_sqrtf(mx(dot3(t_i, t_i) * vg.to2, 1e-8f));
+    g.cos_phi = clampf(rdiv(dot3(t_i, vg.t_o), proj), -1.0f, 1.0f);
+    g.t = vg.t;
     g.l = l;
     return g;
 }
+
+// t and v already normalized; `light` is normalized here (safe_normalize).
+__device__ __forceinline__ BsdfGeom bsdf_geom_n(F3 t, F3 v, F3 light) {
+    return bsdf_geom_u(view_geom(t, v), safe_normalize(light));
+}
+
+#pragma clang fp contract(fast)
 
 // Single scattering M_p * N_p summed over R, TT, TRT (shade.wesl:217-219).
 __device__ __forceinline__ F3 single_scatter(const BsdfGeom& g, F3 sigma_a, const Shape& shape) {
@@ -207,7 +316,7 @@ __device__ __forceinline__ F3 global_spread(float theta_h, float depth, float di
                                             const DualScatter& ds) {
     if (!(depth > 1e-3f)) return splat3(0.0f);
     F3 sigma_f2 = depth * ds.beta_f2;
-    F3 transmit = {fpow(ds.a_f.x, depth), fpow(ds.a_f.y, depth), fpow(ds.a_f.z, depth)};
+    F3 transmit = {hpow(ds.a_f.x, depth), hpow(ds.a_f.y, depth), hpow(ds.a_f.z, depth)};
     F3 beta2 = shape.beta * shape.beta;
     F3 x = splat3(theta_h);
     F3 spread = gaussian3(vsqrt(splat3(beta2.x) + sigma_f2), x - splat3(shape.alpha.x)) * ds.fwd.r +
@@ -218,19 +327,26 @@ __device__ __forceinline__ F3 global_spread(float theta_h, float depth, float di
 }
 
 
-// The four bilinear taps of one DOM lookup (shade.wesl:279-310): every load
-// is issued before any is used, so the lookup costs one memory round trip. A
-// tap outside the map reads texel 0 and is marked empty.
+// The DOM lookup's texel and slice choices keep unfused arithmetic.
+#pragma clang fp contract(off)
+
 struct DomTaps {
     uint32_t front[4];
     uint4 counts[4][4];
     float tx, ty, dist;
 };
 
+// Where one DOM lookup reads: the four taps' texels (a tap outside the map
+// reads texel 0 and is marked outside), the bilinear fractions and the
+// light-space distance shifted by -ln(beta) * dom_scale (shade.wesl:279-299).
 // False when the pixel projects outside the light's clip volume or the DOM
 // is off (visibility 1).
-__device__ __forceinline__ bool dom_taps(const View& p, const uint32_t* __restrict__ dom, F3 world, float beta,
-                                         DomTaps& T) {
+struct DomPos {
+    uint32_t pix[4];
+    bool inside[4];
+    float tx, ty, dist;
+};
+__device__ __forceinline__ bool dom_pos(const View& p, F3 world, float beta, DomPos& P) {
     if (p.flags[3] == 0u) return false;
     float cx = mat_row(p.lclip, 0, world.x, world.y, world.z, 1.0f);
     float cy = mat_row(p.lclip, 1, world.x, world.y, world.z, 1.0f);
@@ -241,48 +357,77 @@ __device__ __forceinline__ bool dom_taps(const View& p, const uint32_t* __restri
     float uy = 1.0f - (ndc_y * 0.5f + 0.5f);
     if (ux < 0.0f || uy < 0.0f || ux > 1.0f || uy > 1.0f) return false;
     uint32_t dom_w = to_u32(p.dom_info.x), dom_h = to_u32(p.dom_info.y);
-    T.dist = dot3(world - xyz(p.light_eye), xyz(p.light_forward)) - (-flog(mx(beta, 1e-4f)) * p.filter_params.z);
+    P.dist = dot3(world - xyz(p.light_eye), xyz(p.light_forward)) - (-hlog(mx(beta, 1e-4f)) * p.filter_params.z);
     float fx = ux * u2f(dom_w) - 0.5f;
     float fy = uy * u2f(dom_h) - 0.5f;
     float flx = floorf(fx), fly = floorf(fy);
     int32_t x0 = to_i32(flx), y0 = to_i32(fly);
-    T.tx = fx - flx;
-    T.ty = fy - fly;
-    const uint4* counts = reinterpret_cast<const uint4*>(dom + (size_t)dom_w * dom_h);
+    P.tx = fx - flx;
+    P.ty = fy - fly;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         int32_t x = x0 + (c & 1), y = y0 + (c >> 1);
-        bool inside = x >= 0 && y >= 0 && x < (int32_t)dom_w && y < (int32_t)dom_h;
-        uint32_t pix = inside ? (uint32_t)y * dom_w + (uint32_t)x : 0u;
-        uint32_t f = dom[pix];
-        T.front[c] = inside ? f : HM_DOM_EMPTY;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) T.counts[c][q] = counts[(size_t)pix * (HM_DOM_LAYERS / 4) + q];
+        P.inside[c] = x >= 0 && y >= 0 && x < (int32_t)dom_w && y < (int32_t)dom_h;
+        P.pix[c] = P.inside[c] ? (uint32_t)y * dom_w + (uint32_t)x : 0u;
     }
     return true;
 }
 
-// Visibility through one tap at light-space distance `dist`.
-__device__ __forceinline__ float dom_tap_visibility(const View& p, uint32_t front_q, const uint4 counts[4], float dist) {
-    if (front_q == HM_DOM_EMPTY) return 1.0f;
+// The four bilinear taps of one DOM lookup: every load is issued before any
+// is used, so the lookup costs one memory round trip.
+__device__ __forceinline__ bool dom_taps(const View& p, const uint32_t* __restrict__ dom, F3 world, float beta,
+                                         DomTaps& T) {
+    DomPos P;
+    if (!dom_pos(p, world, beta, P)) return false;
+    T.tx = P.tx;
+    T.ty = P.ty;
+    T.dist = P.dist;
+    const uint4* counts = reinterpret_cast<const uint4*>(dom + (size_t)to_u32(p.dom_info.x) * to_u32(p.dom_info.y));
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        uint32_t f = dom[P.pix[c]];
+        T.front[c] = P.inside[c] ? f : HM_DOM_EMPTY;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) T.counts[c][q] = counts[(size_t)P.pix[c] * (HM_DOM_LAYERS / 4) + q];
+    }
+    return true;
+}
+
+// The slice position t of light-space distance `dist` behind a tap's front
+// (shade.wesl:265-272): (dist - z_front - 0.05 thickness) / thickness held
+// at >= 0; a NaN t is taken as 0 (it adds no slice).
+__device__ __forceinline__ float dom_tap_t(const View& p, uint32_t front_q, float dist) {
     float near_ = p.dom_info.z, far_ = p.dom_info.w;
     float z_front = dequantize_depth(front_q, near_, far_);
     float thickness = rdiv(mx(far_ - near_, 1e-3f), 16.0f);
     float rel = mx(dist - z_front - thickness * 0.05f, 0.0f);
     float t = rdiv(rel, thickness);
+    return t > 0.0f ? t : 0.0f;
+}
+
+// dom_optical_depth's sum over four slices s0 .. s0 + 3, continued from
+// `occ` in slice order: slice i adds count * min(max(t - i, 0), 1), which is
+// the count itself where i + 1 <= t (t - i >= 1 rounds to >= 1), the count
+// times t - i where i < t < i + 1, and +0 elsewhere: the same additions and
+// roundings as the reference's branches, without them.
+__device__ __forceinline__ float dom_occ4(float occ, uint4 c, float t, int s0) {
+    const uint32_t cs[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float w = __builtin_amdgcn_fmed3f(t - (float)(s0 + k), 0.0f, 1.0f);
+        occ = occ + u2f(cs[k]) * w;
+    }
+    return occ;
+}
+
+// Visibility through one tap at light-space distance `dist`.
+__device__ __forceinline__ float dom_tap_visibility(const View& p, uint32_t front_q, const uint4 counts[4], float dist) {
+    if (front_q == HM_DOM_EMPTY) return 1.0f;
+    const float t = dom_tap_t(p, front_q, dist);
     float occ = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        uint32_t cs[4] = {counts[q].x, counts[q].y, counts[q].z, counts[q].w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float i = (float)(q * 4 + k);
-            float count = u2f(cs[k]);
-            if (i + 1.0f <= t) occ = occ + count;
-            else if (i < t) occ = occ + count * (t - i);
-        }
-    }
-    return fexp(-occ * p.filter_params.w);
+    for (int q = 0; q < 4; ++q) occ = dom_occ4(occ, counts[q], t, 4 * q);
+    return hexp(-occ * p.filter_params.w);
 }
 
 __device__ __forceinline__ float dom_taps_visibility(const View& p, const DomTaps& T) {
@@ -298,6 +443,8 @@ __device__ float dom_visibility(const View& p, const uint32_t* __restrict__ dom,
     if (!dom_taps(p, dom, world, beta, T)) return 1.0f;
     return dom_taps_visibility(p, T);
 }
+
+#pragma clang fp contract(fast)
 
 // Bevy's range window saturate(1 - (d^2 / r^2)^2)^2: zero from the range on.
 __device__ __forceinline__ float range_falloff(float dist2, float inv_range2) {
@@ -321,10 +468,13 @@ __device__ __forceinline__ float spot_mask(const hm_point_light& L, F3 world) {
 }
 
 
+#pragma clang fp contract(off)
+
 // Per-pixel inputs of shade.wesl:fs (shade.wesl:539-571, 442-446).
 struct PixelShade {
     F3 world, tangent, view, albedo, sigma_a;
     F3 t_n, v_n;  // safe_normalize(tangent), safe_normalize(view)
+    ViewGeom vg;  // the view's side of every light's bsdf_geom
     float view_dist, beta, roughness, cos_theta_o, ao;
     Shape shape;
 };
@@ -351,6 +501,7 @@ __device__ __forceinline__ PixelShade pixel_setup(const View& p, uint64_t word, 
     s.shape.inv_beta = gaussian_rcp(s.shape.beta);
     s.t_n = safe_normalize_unit(s.tangent);  // the decoded tangent is normalized
     s.v_n = safe_normalize(s.view);
+    s.vg = view_geom(s.t_n, s.v_n);
     float tv = dot3(s.tangent, s.view);
     s.cos_theta_o = sqrtf(mx(1.0f - tv * tv, 1e-6f));
     s.sigma_a = absorption_from_color(s.albedo, s.roughness);
@@ -374,6 +525,8 @@ __device__ __forceinline__ F3 dir_of(const hm_directional_light& L) {
     return {L.direction_to_light[0], L.direction_to_light[1], L.direction_to_light[2]};
 }
 
+#pragma clang fp contract(fast)
+
 // The dual-scattering terms of the local back lobe that every light uses.
 struct LocalBack {
     F3 a_back, delta_b, inv_sb;  // inv_sb = gaussian_rcp(sqrt(sigma_b^2)), once per pixel
@@ -388,7 +541,7 @@ __device__ __forceinline__ LocalBack local_terms(const DualScatter& ds) {
 // direct * 0.7 * back + [depth > 1e-3] * global spread.
 __device__ __forceinline__ F3 key_multiple_scattering(float theta_h, float depth, const Shape& shape,
                                                       const DualScatter& ds) {
-    const float direct = fexp(-depth);
+    const float direct = hexp(-depth);
     F3 ms = direct * (0.7f * local_back(theta_h, ds.a_back, ds.delta_b, ds.sigma_b2));
     if (depth > 1e-3f) ms = ms + global_spread(theta_h, depth, direct, shape, ds);
     return ms;
@@ -418,7 +571,9 @@ __device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelSha
     F3 ltf = point_pos(L) - px.world;
     float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
     if (att == 0.0f) return splat3(0.0f);
-    BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, safe_normalize(ltf));
+    // safe_normalize twice, as the light loop (shade.wesl:457) and hair_bsdf
+    // (shade.wesl:204) each normalize
+    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize(safe_normalize(ltf)));
     F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
     F3 c = lobe * point_color(L) * att;
     if (L.kind != HM_LIGHT_POINT) c = c * spot_mask(L, px.world);
@@ -429,9 +584,9 @@ __device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelSha
 // `depth`) exp(-depth) * single + key_ms, for the others single + 0.7 * back.
 __device__ __forceinline__ F3 directional_term(const hm_directional_light& L, bool is_key, const PixelShade& px,
                                                const LocalBack& lb, float depth, F3 key_ms) {
-    BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, dir_of(L));
+    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize(dir_of(L)));
     F3 single = single_scatter(g, px.sigma_a, px.shape);
-    F3 lobe = is_key ? fexp(-depth) * single + key_ms
+    F3 lobe = is_key ? hexp(-depth) * single + key_ms
                      : single + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
     return lobe * F3{L.color[0], L.color[1], L.color[2]};
 }
@@ -451,10 +606,12 @@ __device__ __forceinline__ F3 rect_term(const hm_rect_light& L, const PixelShade
     float falloff = range_falloff(dist2, frcp(mx(L.range * L.range, 1e-4f)));
     if (facing == 0.0f || falloff == 0.0f) return splat3(0.0f);
     float solid = L.width * L.height * facing * frcp(dist2);
-    BsdfGeom g = bsdf_geom_n(px.t_n, px.v_n, incident);
+    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize(incident));
     F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
     return lobe * F3{L.color[0], L.color[1], L.color[2]} * (solid * falloff);
 }
+
+#pragma clang fp contract(off)
 
 // ---- the environment probe --------------------------------------------------
 
@@ -544,7 +701,7 @@ __device__ F3 probe_radiance(const EnvArgs& e, const PixelShade& px) {
     // scatter_tint(albedo, 0.5) = (albedo / luminance(albedo))^0.25 (shade.wesl:62-85)
     const F3 al = px.albedo;
     const float inv_lum = frcp(mx(al.x * 0.2126f + al.y * 0.7152f + al.z * 0.0722f, 1e-4f));
-    const F3 scatter = al * F3{fpow(al.x * inv_lum, 0.25f), fpow(al.y * inv_lum, 0.25f), fpow(al.z * inv_lum, 0.25f)};
+    const F3 scatter = al * F3{hpow(al.x * inv_lum, 0.25f), hpow(al.y * inv_lum, 0.25f), hpow(al.z * inv_lum, 0.25f)};
     return (spec * (splat3(f_r) + trt) + diff * scatter) * e.intensity;
 }
 
@@ -585,6 +742,8 @@ __device__ __forceinline__ uint64_t reaching_lights(const ShadeArgs& a, const Wa
     }
     return __ballot(keep);
 }
+
+#pragma clang fp contract(fast)
 
 // Colour of one hit pixel (shade.wesl:442-521, 585-588), Bevy shadow factors
 // 1, from the back-lobe terms `lb` and the key light's multiple scattering
@@ -638,7 +797,7 @@ __device__ F3 pixel_color(const ShadeArgs& a, const PixelShade& px, bool valid, 
 
 // theta_h of the key light at this pixel (bsdf_geom_n's).
 __device__ __forceinline__ float key_theta_h(const PixelShade& px, F3 key_dir) {
-    return bsdf_geom_n(px.t_n, px.v_n, key_dir).theta_h;
+    return bsdf_geom_u(px.vg, safe_normalize(key_dir)).theta_h;
 }
 
 // The plain method's colour of one hit pixel: dual scattering evaluated
@@ -654,12 +813,14 @@ __device__ F3 plain_pixel_color(const ShadeArgs& a, const View& p, const PixelSh
     if (valid) {
         const DualScatter ds = dual_scatter(px.cos_theta_o, px.sigma_a, px.shape);
         if (key >= 0) {
-            depth = -flog(mx(dom_visibility(p, a.dom, px.world, px.beta), 1e-6f));
+            depth = -hlog(mx(dom_visibility(p, a.dom, px.world, px.beta), 1e-6f));
             key_ms = key_multiple_scattering(key_theta_h(px, dir_of(a.directionals[key])), depth, px.shape, ds);
         }
         lb = local_terms(ds);
     }
     return pixel_color<F>(a, px, valid, lb, key, depth, key_ms);
 }
+
+#pragma clang fp contract(off)
 
 }  // namespace hm
