@@ -178,8 +178,10 @@ def measured_traffic(kernel: str, tuning: dict, workload: str):
     summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by
     what the counter reads for a 1 GiB copy kernel); None when the summary
     does not cover this kernel on this workload."""
-    names = {"cam_tiles": f"k_cam_tiles<{tuning.get('cam_tile')}>", "dom_tiles": f"k_dom_tiles<{tuning.get('dom_tile')}>",
-             "shade": "k_shade", "filter": "k_filter"}
+    dom_name = ("k_dom_tiles_busy" if tuning.get("dom_tile") == 16 and tuning.get("dom_busy_blocks_per_cu")
+                else f"k_dom_tiles<{tuning.get('dom_tile')}>")
+    names = {"cam_tiles": f"k_cam_tiles<{tuning.get('cam_tile')}>", "dom_tiles": dom_name,
+             "shade": "k_shade<0u>", "filter": "k_filter"}
     if kernel not in names or not os.path.exists(TRAFFIC_FILE):
         return None
     rows, wl = {}, None
@@ -381,7 +383,9 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the default raster launch choices")
     ap.add_argument("--tuning", default=None,
                     help="launch choices as JSON (hm_tuning fields) instead of autotuning, e.g. for profiling")
-    ap.add_argument("--no-mlp", action="store_true", help="skip the learned-shading leg")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the learned-shading legs")
+    ap.add_argument("--no-mlp-c4", action="store_true",
+                    help="skip the learned-shading leg on the config-1 groom at 1080p")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view leg")
     ap.add_argument("--no-heavy", action="store_true", help="skip the 2160p heavy-workload leg")
     ap.add_argument("--no-lights", action="store_true", help="skip the many-lights / probe leg")
@@ -531,7 +535,7 @@ def main() -> int:
             rates["dom_opacity_frac_of_u32_add_peak"] = round(rates["dom_opacity_atomics_per_s"] / peaks["u32_add_per_s"], 4)
     result["many_lights"] = None if args.no_lights else lights_leg_run(r, g, sc, w, args)
     g.free()
-    if result["mlp"] is not None:
+    if result["mlp"] is not None and not args.no_mlp_c4:
         result["mlp"]["c4"] = mlp_c4_leg_run(r, args)
     result["batch"] = None if args.no_batch else batch_leg_run(r, args)
     result["heavy"] = None if args.no_heavy else heavy_leg_run(r, args)

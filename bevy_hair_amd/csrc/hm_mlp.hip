@@ -424,7 +424,10 @@ __device__ float dom_features(const View& p, const uint32_t* __restrict__ dom, F
         t[c] = on[c] ? dom_tap_t(p, f, P.dist) : 0.0f;
         occ[c] = 0.0f;
     }
-    const float wts[4] = {(1.0f - P.tx) * (1.0f - P.ty), P.tx * (1.0f - P.ty), (1.0f - P.tx) * P.ty, P.tx * P.ty};
+    // bilinear weights, zero for a tap outside the map or with an empty front
+    float wts[4] = {(1.0f - P.tx) * (1.0f - P.ty), P.tx * (1.0f - P.ty), (1.0f - P.tx) * P.ty, P.tx * P.ty};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) wts[c] = on[c] ? wts[c] : 0.0f;
     float pre = 0.0f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -439,7 +442,7 @@ __device__ float dom_features(const View& p, const uint32_t* __restrict__ dom, F
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const uint32_t cs[4] = {v[c].x, v[c].y, v[c].z, v[c].w};
-                if (on[c]) cw += wts[c] * u2f(cs[k]);
+                cw += wts[c] * u2f(cs[k]);
             }
             pre += cw;
             layers[4 * q + k] = hexp(-pre * p.filter_params.w);
@@ -663,7 +666,12 @@ __global__ __launch_bounds__(256, 3) void k_shade_mlp(ShadeArgs a, MlpShadeArgs 
             if (valid) {
                 px = pixel_setup(p, a.center[pix], a.beta[pix], pix % width, pix / width);
                 depth = mlp_features(p, px, kd, a.dom, f);
-                for (int k = 9; k < MLP_NFEAT; ++k) outside |= f[k] < m.lo[k] || f[k] > m.hi[k];
+                // outside the trained box when some feature differs from
+                // min(max(f, lo), hi) (always, for an empty box lo > hi; a NaN
+                // feature differs too)
+                float off_box = 0.0f;
+                for (int k = 9; k < MLP_NFEAT; ++k) off_box += fabsf(f[k] - fminf(fmaxf(f[k], m.lo[k]), m.hi[k]));
+                outside = !(off_box == 0.0f);
             }
             // row `lane`: 32 bf16 = four 16-byte vectors
 #pragma unroll
