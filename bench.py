@@ -211,19 +211,37 @@ def rel_err(got: np.ndarray, ref: np.ndarray, floor: float = 1e-3) -> float:
 
 def timed_frames(r, g, params, opts, steps: int):
     """K frames between one event pair with nothing else on the stream (the
-    throughput), then K more with an event pair around every kernel group
-    (the per-kernel durations, in ms per frame): the per-kernel events
+    throughput, the DOM pass beside the camera pass as the launch choices
+    say), then K more with an event pair around every kernel group and the
+    two passes one after the other (the per-kernel durations, in ms per
+    frame, each kernel with the GPU to itself): the per-kernel events
     themselves cost a few microseconds per frame, so they stay out of the
     first measurement. Returns (ms per frame, ms per frame with the kernel
     events, kernel times)."""
     r.sync()
     ms = r.time_frames(g, params, opts, steps) / steps
-    r.timing(steps)
-    r.sync()
-    ms_events = r.time_frames(g, params, opts, steps) / steps
-    kt = r.kernel_times()
-    r.timing(0)
+    keep = r.get_tuning()["frame_serial"]
+    r.set_tuning(frame_serial=1)
+    try:
+        r.timing(steps)
+        r.sync()
+        ms_events = r.time_frames(g, params, opts, steps) / steps
+        kt = r.kernel_times()
+        r.timing(0)
+    finally:
+        r.set_tuning(frame_serial=keep)
     return ms, ms_events, kt
+
+
+def one_stream_ms(r, g, params, opts, steps: int) -> float:
+    """ms per frame with the DOM pass after the camera pass on one stream."""
+    keep = r.get_tuning()["frame_serial"]
+    r.set_tuning(frame_serial=1)
+    try:
+        r.sync()
+        return r.time_frames(g, params, opts, steps) / steps
+    finally:
+        r.set_tuning(frame_serial=keep)
 
 
 def mlp_leg_run(r, g, sc, w, args) -> dict:
@@ -454,6 +472,7 @@ def main() -> int:
     # Timed region: K frames between two events; then K frames with the
     # per-kernel events that give the kernel breakdown and the roofline.
     ms, ms_events, kt = timed_frames(r, g, sc.params, opts, args.steps)
+    ms_serial = one_stream_ms(r, g, sc.params, opts, args.steps)
     fps = 1000.0 / ms
 
     mlp_leg = None if args.no_mlp else mlp_leg_run(r, g, sc, w, args)
@@ -491,6 +510,7 @@ def main() -> int:
         "kernel_rooflines": per_kernel,
         "kernels_ms": {k: round(v, 5) for k, v in kt.items()},
         "ms_per_step_with_kernel_events": round(ms_events, 4),
+        "ms_per_step_one_stream": round(ms_serial, 4),
         "tuning": tuning,
         "work": {"camera_writes": int(counts["cam"][0]), "camera_center_hits": int(counts["cam"][1]),
                  "camera_atomics": int(atomics_cam), "dom_depth_writes": int(counts["dom1"][2]),

@@ -875,12 +875,13 @@ __global__ __launch_bounds__(256) void k_dom_long(BinArgs a) {
         walk_record(d, a, a.long_list[i]);
 }
 
-hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks) {
+hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks, hipEvent_t after_gen) {
     if (a.tiles > BIN_MAX_TILES) return hipErrorInvalidValue;
     hipError_t e;
     if (mode == 0) hipLaunchKernelGGL(k_bin_gen<0>, dim3(gen_blocks), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_bin_gen<1>, dim3(gen_blocks), dim3(256), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (after_gen && (e = hipEventRecord(after_gen, s)) != hipSuccess) return e;
     const size_t lds = (size_t)a.tiles * sizeof(uint32_t);
     hipLaunchKernelGGL(k_bin_count, dim3(bin_blocks), dim3(256), lds, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -38,31 +38,42 @@ struct hm_bin_state {
     float* dcp = nullptr;           // DOM: 5 floats per [slot * 128 + idx]: world xyz, light-space x, y
     float4* cpt = nullptr;          // camera: world tangent xyz, ao
     float4* sinfo = nullptr;        // camera: per strand uv, beta, control points
-    uint32_t* seg_box = nullptr;    // per segment id
-    uint32_t* long_list = nullptr;
-    size_t slots = 0;               // strand slots cps / seg_box / long_list hold
-    // Per pass (camera, DOM), each grown only by its own pass: status words
-    // (the camera's shade / filter list lengths are read after the DOM pass)
-    // and per-tile count / offset / cursor / chunk offset.
+    size_t slots = 0;               // strand slots cps / cpy / dcp / cpt / sinfo hold
+    // Per pass (camera, DOM), each grown only by its own pass, so that the
+    // two passes of a frame can run at the same time: status words (the
+    // camera's shade / filter list lengths are read after the DOM pass),
+    // per-tile count / offset / cursor / chunk offset, segment boxes,
+    // long-segment lists, tile records and the counting workgroups' lists.
     uint32_t* meta[2] = {nullptr, nullptr};
     size_t meta_words[2] = {0, 0};
     // A pass's launch sequence failed part way: its status words (the
     // long-segment cursor among them) are cleared before the pass runs again.
     bool meta_dirty[2] = {false, false};
-    uint32_t* records = nullptr;
-    size_t record_capacity = 0;
-    uint2* lists = nullptr;         // (tile, rank) entries of the counting workgroups
-    size_t list_capacity = 0;
-    uint2* wg_list = nullptr;       // one per binning workgroup
-    size_t wg_capacity = 0;
-    uint32_t* chunk_tile = nullptr; // tile of each tile-workgroup chunk
+    uint32_t* seg_box[2] = {nullptr, nullptr};    // per segment id
+    uint32_t* long_list[2] = {nullptr, nullptr};
+    size_t seg_slots[2] = {0, 0};                 // strand slots seg_box / long_list hold
+    uint32_t* records[2] = {nullptr, nullptr};
+    size_t record_capacity[2] = {0, 0};
+    uint2* lists[2] = {nullptr, nullptr};         // (tile, rank) entries of the counting workgroups
+    size_t list_capacity[2] = {0, 0};
+    uint2* wg_list[2] = {nullptr, nullptr};       // one per binning workgroup
+    size_t wg_capacity[2] = {0, 0};
+    uint32_t* chunk_tile = nullptr; // camera: tile of each tile-workgroup chunk
     size_t chunk_capacity = 0;
 };
 
 struct hm_ctx {
     int device = 0;
     int cu_count = 256;
+    // The stream every call enqueues on. hm_frame points it at `side` for the
+    // DOM pass, which then runs beside the camera pass and joins `main`
+    // before shading; both streams have the default priority.
     hipStream_t stream = nullptr;
+    hipStream_t main = nullptr, side = nullptr;
+    // hm_frame's fork and join: frame start on main, camera control points
+    // written (the DOM generation reads them), DOM pass done on side.
+    hipEvent_t fork_ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t gen_event = nullptr;  // recorded after the camera generation when set
     uint32_t width = 0, height = 0, dom_size = 0;
     // Target allocations only grow: a smaller size reuses them (the layouts
     // below follow the current size), so views of several sizes can follow
@@ -105,7 +116,7 @@ struct hm_ctx {
     // rect light count, the probe's sizes, rotation or intensity).
     uint64_t shade_gen = 0;
     uint64_t* counters = nullptr;
-    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u, 2048u, 4u, 0u, 2u};
+    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u, 2048u, 4u, 0u, 2u, 0u};
     hm_bin_state bin;
     // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
     uint32_t slots = 0, frame = 0;

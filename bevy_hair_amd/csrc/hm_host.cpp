@@ -291,13 +291,14 @@ void free_bin(hm_ctx* c) {
     dev_free(b.dcp);
     dev_free(b.cpt);
     dev_free(b.sinfo);
-    dev_free(b.seg_box);
-    dev_free(b.long_list);
-    dev_free(b.meta[0]);
-    dev_free(b.meta[1]);
-    dev_free(b.records);
-    dev_free(b.lists);
-    dev_free(b.wg_list);
+    for (int pass = 0; pass < 2; ++pass) {
+        dev_free(b.meta[pass]);
+        dev_free(b.seg_box[pass]);
+        dev_free(b.long_list[pass]);
+        dev_free(b.records[pass]);
+        dev_free(b.lists[pass]);
+        dev_free(b.wg_list[pass]);
+    }
     dev_free(b.chunk_tile);
     b = hm_bin_state();
 }
@@ -305,7 +306,10 @@ void free_bin(hm_ctx* c) {
 // Grows the binning buffers to hold `slots` strand slots, `meta_words`
 // status + per-tile words of pass `pass`, `records` tile records, `lists`
 // (tile, rank) entries, `wgs` binning workgroups and `chunks` tile-workgroup
-// chunks. The other pass's status words are left alone.
+// chunks. Only the shared control-point buffers and pass `pass`'s own
+// buffers are touched; before any of them is replaced, both streams are
+// drained (an earlier frame, or the other pass of this one, may still read
+// them).
 hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size_t records, size_t lists,
                       size_t wgs, size_t chunks) {
     hm_bin_state& b = c->bin;
@@ -315,11 +319,12 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
         if ((e = hm::launch_fill32(c->stream, b.meta[pass], b.meta_words[pass], 0u, 1)) != hipSuccess) return e;
         b.meta_dirty[pass] = false;
     }
-    if (slots <= b.slots && meta_words <= b.meta_words[pass] && records <= b.record_capacity &&
-        lists <= b.list_capacity && wgs <= b.wg_capacity && chunks <= b.chunk_capacity)
+    const bool chunks_ok = pass != HM_PASS_CAMERA || chunks <= b.chunk_capacity;
+    if (slots <= b.slots && slots <= b.seg_slots[pass] && meta_words <= b.meta_words[pass] &&
+        records <= b.record_capacity[pass] && lists <= b.list_capacity[pass] && wgs <= b.wg_capacity[pass] && chunks_ok)
         return hipSuccess;
-    e = hipStreamSynchronize(c->stream);  // earlier frames may still read them
-    if (e != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(c->main)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(c->side)) != hipSuccess) return e;
     c->alloc_gen++;
     if (slots > b.slots) {
         dev_free(b.cps);
@@ -327,8 +332,6 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
         dev_free(b.dcp);
         dev_free(b.cpt);
         dev_free(b.sinfo);
-        dev_free(b.seg_box);
-        dev_free(b.long_list);
         b.slots = 0;
         size_t segs = slots * hm::BIN_STRIDE;
         if ((e = dev_alloc(&b.cps, segs * sizeof(float4))) != hipSuccess) return e;
@@ -336,9 +339,16 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
         if ((e = dev_alloc(&b.dcp, segs * 5 * sizeof(float))) != hipSuccess) return e;
         if ((e = dev_alloc(&b.cpt, segs * sizeof(float4))) != hipSuccess) return e;
         if ((e = dev_alloc(&b.sinfo, slots * sizeof(float4))) != hipSuccess) return e;
-        if ((e = dev_alloc(&b.seg_box, segs * 4)) != hipSuccess) return e;
-        if ((e = dev_alloc(&b.long_list, segs * 4)) != hipSuccess) return e;
         b.slots = slots;
+    }
+    if (slots > b.seg_slots[pass]) {
+        dev_free(b.seg_box[pass]);
+        dev_free(b.long_list[pass]);
+        b.seg_slots[pass] = 0;
+        size_t segs = slots * hm::BIN_STRIDE;
+        if ((e = dev_alloc(&b.seg_box[pass], segs * 4)) != hipSuccess) return e;
+        if ((e = dev_alloc(&b.long_list[pass], segs * 4)) != hipSuccess) return e;
+        b.seg_slots[pass] = slots;
     }
     if (meta_words > b.meta_words[pass]) {
         dev_free(b.meta[pass]);
@@ -350,25 +360,25 @@ hipError_t ensure_bin(hm_ctx* c, int pass, size_t slots, size_t meta_words, size
         b.meta_words[pass] = meta_words;
         b.meta_dirty[pass] = false;
     }
-    if (records > b.record_capacity) {
-        dev_free(b.records);
-        b.record_capacity = 0;
-        if ((e = dev_alloc(&b.records, records * 4)) != hipSuccess) return e;
-        b.record_capacity = records;
+    if (records > b.record_capacity[pass]) {
+        dev_free(b.records[pass]);
+        b.record_capacity[pass] = 0;
+        if ((e = dev_alloc(&b.records[pass], records * 4)) != hipSuccess) return e;
+        b.record_capacity[pass] = records;
     }
-    if (lists > b.list_capacity) {
-        dev_free(b.lists);
-        b.list_capacity = 0;
-        if ((e = dev_alloc(&b.lists, lists * sizeof(uint2))) != hipSuccess) return e;
-        b.list_capacity = lists;
+    if (lists > b.list_capacity[pass]) {
+        dev_free(b.lists[pass]);
+        b.list_capacity[pass] = 0;
+        if ((e = dev_alloc(&b.lists[pass], lists * sizeof(uint2))) != hipSuccess) return e;
+        b.list_capacity[pass] = lists;
     }
-    if (wgs > b.wg_capacity) {
-        dev_free(b.wg_list);
-        b.wg_capacity = 0;
-        if ((e = dev_alloc(&b.wg_list, wgs * sizeof(uint2))) != hipSuccess) return e;
-        b.wg_capacity = wgs;
+    if (wgs > b.wg_capacity[pass]) {
+        dev_free(b.wg_list[pass]);
+        b.wg_capacity[pass] = 0;
+        if ((e = dev_alloc(&b.wg_list[pass], wgs * sizeof(uint2))) != hipSuccess) return e;
+        b.wg_capacity[pass] = wgs;
     }
-    if (chunks > b.chunk_capacity) {
+    if (!chunks_ok) {
         dev_free(b.chunk_tile);
         b.chunk_capacity = 0;
         if ((e = dev_alloc(&b.chunk_tile, chunks * 4)) != hipSuccess) return e;
@@ -472,9 +482,20 @@ hm_status hm_create(int device, hm_ctx** out) {
     hm_ctx* c = new hm_ctx();
     c->device = device;
     c->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->main, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return HM_ERR_HIP;
+    }
+    c->stream = c->main;
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+        hm_destroy(c);
+        return HM_ERR_HIP;
+    }
+    for (int j = 0; j < 3; ++j) {
+        if (hipEventCreateWithFlags(&c->fork_ev[j], hipEventDisableTiming) != hipSuccess) {
+            hm_destroy(c);
+            return HM_ERR_HIP;
+        }
     }
     (void)hipEventCreate(&c->frame_ev[0]);
     (void)hipEventCreate(&c->frame_ev[1]);
@@ -489,7 +510,9 @@ hm_status hm_create(int device, hm_ctx** out) {
 void hm_destroy(hm_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->stream = c->main;
+    if (c->main) (void)hipStreamSynchronize(c->main);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     free_targets(c);
     free_bin(c);
     hm_mlp_release(c);
@@ -502,7 +525,10 @@ void hm_destroy(hm_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     for (int j = 0; j < 2; ++j)
         if (c->frame_ev[j]) (void)hipEventDestroy(c->frame_ev[j]);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    for (hipEvent_t e : c->fork_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->main) (void)hipStreamDestroy(c->main);
     delete c;
 }
 
@@ -1067,21 +1093,21 @@ static hipError_t bin_args(hm_ctx* ctx, hm_groom* g, const hm_params* p, int pas
     b.cps = ctx->bin.cps;
     b.cpy = ctx->bin.cpy;
     b.dcp = ctx->bin.dcp;
-    b.seg_box = ctx->bin.seg_box;
+    b.seg_box = ctx->bin.seg_box[pass];
     b.meta = ctx->bin.meta[pass];
-    b.records = ctx->bin.records;
+    b.records = ctx->bin.records[pass];
     b.record_capacity = (uint32_t)want_records;
     b.can_overflow = want_records < worst ? 1u : 0u;
-    b.lists = ctx->bin.lists;
+    b.lists = ctx->bin.lists[pass];
     b.list_capacity = (uint32_t)std::min<size_t>(lists, 0xffffffffu);
-    b.wg_list = ctx->bin.wg_list;
+    b.wg_list = ctx->bin.wg_list[pass];
     b.chunk = chunk;
     b.chunk_tile = pass == HM_PASS_CAMERA ? ctx->bin.chunk_tile : nullptr;  // DOM tiles are never split
     b.chunk_capacity = (uint32_t)std::min<size_t>(chunks, 0xffffffffu);
     b.gb_slab = static_cast<uint32_t*>(ctx->gb_slab);
     b.gb_words = ctx->gb_words;
     b.fill_clear = pass == HM_PASS_CAMERA ? 1u : 2u;
-    b.long_list = ctx->bin.long_list;
+    b.long_list = ctx->bin.long_list[pass];
     // Segment slots per strand: the pass's most segments, rounded up to a power of two (16 .. 128).
     b.seg_shift = 4u;
     while ((1u << b.seg_shift) + 1u < g->ncp_bound[pass]) ++b.seg_shift;
@@ -1140,7 +1166,8 @@ static hm_status gbuffer_impl(hm_ctx* ctx, hm_groom* g, const hm_params* p, bool
     b.filtered_depth = ctx->filtered_depth;
     {
         Timer t(ctx, HM_K_CAM_BIN);
-        const hipError_t e = hm::launch_bin(ctx->stream, b, 0, raster_blocks(ctx, g->capacity, 0u), bin_blocks);
+        const hipError_t e =
+            hm::launch_bin(ctx->stream, b, 0, raster_blocks(ctx, g->capacity, 0u), bin_blocks, ctx->gen_event);
         if (e != hipSuccess) {
             ctx->bin.meta_dirty[HM_PASS_CAMERA] = true;
             return hip_fail(ctx, e, "launch_bin (camera)");
@@ -1355,15 +1382,62 @@ hm_status hm_frame(hm_ctx* ctx, hm_groom* g, const hm_params params[3], const hm
         }
     } slot{ctx};
     const hm_params& cam = params[0];
+    // The DOM pass runs on the side stream beside the camera pass. It starts
+    // after the work already on main (an earlier frame's shading reads the
+    // map), its generation waits until the camera generation has written the
+    // control points it may reuse, and main waits for all of it before
+    // shading. Every exit, errors included, joins the side stream back.
+    const bool dom_pass = cam.flags[3] != 0u;
+    const bool overlap = dom_pass && ctx->tuning.frame_serial == 0u;
+    struct Side {
+        hm_ctx* c;
+        bool forked = false;
+        hipError_t fork() {
+            hipError_t e = hipEventRecord(c->fork_ev[0], c->main);
+            if (e == hipSuccess) e = hipStreamWaitEvent(c->side, c->fork_ev[0], 0);
+            forked = e == hipSuccess;
+            return e;
+        }
+        hipError_t join() {
+            c->stream = c->main;
+            if (!forked) return hipSuccess;
+            forked = false;
+            hipError_t e = hipEventRecord(c->fork_ev[2], c->side);
+            if (e == hipSuccess) e = hipStreamWaitEvent(c->main, c->fork_ev[2], 0);
+            return e;
+        }
+        ~Side() {
+            (void)join();
+            c->gen_event = nullptr;
+        }
+    } side{ctx};
+    if (overlap) {
+        HM_HIP(ctx, side.fork());
+        ctx->stream = ctx->side;
+        if ((st = hm_lod(ctx, g, &params[1], HM_PASS_DOM, nullptr)) != HM_OK) return st;
+        if ((st = hm_scan(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
+        ctx->stream = ctx->main;
+        ctx->gen_event = ctx->fork_ev[1];
+    }
     if ((st = hm_lod(ctx, g, &cam, HM_PASS_CAMERA, ovr)) != HM_OK) return st;
     if ((st = hm_scan(ctx, g, &cam, HM_PASS_CAMERA)) != HM_OK) return st;
     TileLists lists;
     if ((st = gbuffer_impl(ctx, g, &cam, true, &lists)) != HM_OK) return st;
-    if (cam.flags[3]) {
-        if ((st = hm_lod(ctx, g, &params[1], HM_PASS_DOM, nullptr)) != HM_OK) return st;
-        if ((st = hm_scan(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
+    ctx->gen_event = nullptr;
+    if (dom_pass) {
+        if (overlap) {
+            // (the binned camera pass recorded fork_ev[1] after its generation;
+            // the direct one reuses nothing, and the event follows all of it)
+            if (!lists.from_tiles) HM_HIP(ctx, hipEventRecord(ctx->fork_ev[1], ctx->main));
+            ctx->stream = ctx->side;
+            HM_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev[1], 0));
+        } else {
+            if ((st = hm_lod(ctx, g, &params[1], HM_PASS_DOM, nullptr)) != HM_OK) return st;
+            if ((st = hm_scan(ctx, g, &params[1], HM_PASS_DOM)) != HM_OK) return st;
+        }
         const bool same_model = std::memcmp(cam.world_from_model, params[1].world_from_model, 64) == 0;
         if ((st = dom_impl(ctx, g, &params[1], &params[2], lists.from_tiles && same_model)) != HM_OK) return st;
+        HM_HIP(ctx, side.join());
     }
     {
         GroomBox box(ctx, g, cam);
@@ -1501,6 +1575,7 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
     if (t->cam_bin_blocks_per_cu > 64u) return fail(ctx, HM_ERR_INVALID, "cam_bin_blocks_per_cu must be in [0, 64]");
     if (t->light_cull > 2u) return fail(ctx, HM_ERR_INVALID, "light_cull must be 0, 1 or 2");
     if (t->dom_busy_blocks_per_cu > 2u) return fail(ctx, HM_ERR_INVALID, "dom_busy_blocks_per_cu must be 0, 1 or 2");
+    if (t->frame_serial > 1u) return fail(ctx, HM_ERR_INVALID, "frame_serial must be 0 or 1");
 
     ctx->tuning = *t;
     return HM_OK;

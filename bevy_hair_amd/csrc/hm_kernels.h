@@ -182,7 +182,11 @@ __device__ __forceinline__ uint32_t seg_cp(const BinArgs& a, uint32_t id) {
     return seg_strand(a, id) * BIN_STRIDE + seg_index(a, id);
 }
 
-hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks);
+// Generation, count, scan and fill of one pass; `after_gen`, when given, is
+// recorded once the generation kernel is enqueued (the DOM pass of a frame
+// waits on it before reading the camera's control points).
+hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks,
+                      hipEvent_t after_gen = nullptr);
 hipError_t launch_dom_long(hipStream_t s, const BinArgs& a, int mode, int long_blocks);  // mode 1 before, 2 after the tiles
 // 16 x 16 tiles with busy_blocks > 0: the persistent busy-tile kernel on that
 // many workgroups; otherwise one workgroup per tile.

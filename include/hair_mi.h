@@ -473,6 +473,10 @@ typedef struct hm_tuning {
                                       CU, taking the tiles that hold records one at a time (each
                                       sample walked once, kept in LDS for the opacity pass) and
                                       then writing the empty tiles */
+    uint32_t frame_serial;         /* hm_frame: 0 = the DOM pass on a second stream beside the camera
+                                      pass (its generation after the camera generation, shading after
+                                      both); 1 = the passes one after the other on one stream. The
+                                      results are the same either way */
 } hm_tuning;
 
 hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);
