@@ -1630,12 +1630,12 @@ static hm_status autotune_search(hm_ctx* ctx, hm_groom* g, const hm_params param
             }
         }
     }
-    // Camera: direct kernels against binned tiles of either size with 128 to
+    // Camera: direct kernels against binned tiles of either size with 64 to
     // 2048 records (segments) per tile workgroup, whole hm_gbuffer timed.
     {
-        constexpr uint32_t kVariants = 11;
-        const uint32_t variants[kVariants][2] = {{0u, 0u},    {16u, 128u},  {16u, 256u}, {16u, 512u},
-                                                 {16u, 1024u}, {16u, 2048u}, {32u, 128u}, {32u, 256u},
+        constexpr uint32_t kVariants = 13;
+        const uint32_t variants[kVariants][2] = {{0u, 0u},     {16u, 64u},   {16u, 128u},  {16u, 256u}, {16u, 512u},
+                                                 {16u, 1024u}, {16u, 2048u}, {32u, 64u},   {32u, 128u}, {32u, 256u},
                                                  {32u, 512u},  {32u, 1024u}, {32u, 2048u}};
         float best_cam = 1e30f;
         for (uint32_t ti = 0; ti < kVariants; ++ti) {
@@ -1659,14 +1659,20 @@ static hm_status autotune_search(hm_ctx* ctx, hm_groom* g, const hm_params param
             }
         }
     }
-    // DOM: direct kernels against binned tiles of either size, whole hm_dom timed.
+    // DOM: direct kernels against binned tiles of either size (16-texel tiles
+    // also with the persistent tile workgroups, 1 or 2 per CU, or one
+    // workgroup per tile), whole hm_dom timed.
     if (params[1].flags[3]) {
-        const uint32_t tile_sizes[7][2] = {{0u, 0u}, {16u, 1u}, {16u, 2u}, {16u, 4u}, {32u, 1u}, {32u, 2u}, {32u, 4u}};
+        constexpr uint32_t kVariants = 11;
+        const uint32_t tile_sizes[kVariants][3] = {{0u, 0u, 0u},  {16u, 1u, 2u}, {16u, 2u, 2u}, {16u, 4u, 2u},
+                                                   {16u, 2u, 1u}, {16u, 2u, 0u}, {16u, 1u, 0u}, {16u, 4u, 0u},
+                                                   {32u, 1u, 0u}, {32u, 2u, 0u}, {32u, 4u, 0u}};
         float best_dom = 1e30f;
-        for (uint32_t ti = 0; ti < 7; ++ti) {
+        for (uint32_t ti = 0; ti < kVariants; ++ti) {
             ctx->tuning = out;
             ctx->tuning.dom_tile = tile_sizes[ti][0];
             if (tile_sizes[ti][1]) ctx->tuning.bin_blocks_per_cu = tile_sizes[ti][1];
+            ctx->tuning.dom_busy_blocks_per_cu = tile_sizes[ti][2];
             float total = 0.0f;
             for (uint32_t r = 0; r <= reps; ++r) {
                 HM_HIP(ctx, hipEventRecord(e0, ctx->stream));
@@ -1681,8 +1687,30 @@ static hm_status autotune_search(hm_ctx* ctx, hm_groom* g, const hm_params param
                 best_dom = total;
                 out.dom_tile = ctx->tuning.dom_tile;
                 out.bin_blocks_per_cu = ctx->tuning.bin_blocks_per_cu;
+                out.dom_busy_blocks_per_cu = ctx->tuning.dom_busy_blocks_per_cu;
             }
         }
+    }
+    // Frame: the DOM pass beside the camera pass or after it, whole hm_frame
+    // timed. The two differ by a few percent, so they alternate frame by
+    // frame (a slow drift of the clock counts against both) over more frames.
+    if (params[0].flags[3]) {
+        float total[2] = {0.0f, 0.0f};
+        const uint32_t frames = reps < 4u ? 8u : 2u * reps;
+        for (uint32_t r = 0; r <= frames; ++r) {
+            for (uint32_t serial = 0; serial < 2u; ++serial) {
+                ctx->tuning = out;
+                ctx->tuning.frame_serial = serial;
+                HM_HIP(ctx, hipEventRecord(e0, ctx->stream));
+                if ((st = hm_frame(ctx, g, params, opts)) != HM_OK) return st;
+                HM_HIP(ctx, hipEventRecord(e1, ctx->stream));
+                HM_HIP(ctx, hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) total[serial] += ms;  // the first round warms up
+            }
+        }
+        out.frame_serial = total[1] < total[0] ? 1u : 0u;
     }
 
     return HM_OK;

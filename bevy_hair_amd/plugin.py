@@ -289,13 +289,16 @@ class Renderer:
     def bin_status(self, pass_: int = 0) -> dict:
         """Status words of the last binned hm_gbuffer (pass 0) or hm_dom (pass 1)."""
         s = self.read(N.BUF_CAM_BIN_STATUS if pass_ == 0 else N.BUF_DOM_BIN_STATUS, np.uint32, N.BIN_META_WORDS)
-        # words 16-79 / 80-143: the shade / filter sub-list counters the camera
-        # tiles of a frame fill (zero after a pass that emits no lists)
-        subs = N.BIN_SHADE_SUBS
-        return {"records": int(s[0]), "binned": bool(s[1]), "fallback": bool(s[2]), "long": bool(s[3]),
-                "long_count": int(s[4]),
-                "shade_pixels": int(s[subs:subs + N.LIST_SUBS].astype(np.int64).sum()),
-                "filter_pixels": int(s[subs + N.LIST_SUBS:subs + 2 * N.LIST_SUBS].astype(np.int64).sum())}
+        out = {"records": int(s[0]), "binned": bool(s[1]), "fallback": bool(s[2]), "long": bool(s[3]),
+               "long_count": int(s[4])}
+        if pass_ == 0:
+            # words 16-79 / 80-143: the shade / filter sub-list counters the camera
+            # tiles of a frame fill (zero after a pass that emits no lists); the
+            # DOM pass keeps its tile workgroups' cursors there
+            subs = N.BIN_SHADE_SUBS
+            out["shade_pixels"] = int(s[subs:subs + N.LIST_SUBS].astype(np.int64).sum())
+            out["filter_pixels"] = int(s[subs + N.LIST_SUBS:subs + 2 * N.LIST_SUBS].astype(np.int64).sum())
+        return out
 
     def dom_bin_status(self) -> dict:
         return self.bin_status(1)

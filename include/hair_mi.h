@@ -483,10 +483,12 @@ hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);
 hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* tuning);
 /* Times the raster variants on this groom and view, `reps` launches each
  * from cleared targets, keeps the fastest per pass and returns it: direct
- * kernels (flags x grid size) per pass; camera tiles 16 / 32 x 128 / 256 /
+ * kernels (flags x grid size) per pass; camera tiles 16 / 32 x 64 / 128 / 256 /
  * 512 / 1024 / 2048 records per tile workgroup (cam_chunk); DOM tiles 16 / 32 x
- * binning workgroups per CU 1 / 2 / 4. Leaves the targets dirty: run a frame
- * afterwards. */
+ * binning workgroups per CU 1 / 2 / 4, 16-texel tiles also with persistent tile
+ * workgroups (dom_busy_blocks_per_cu 0 / 1 / 2); then whole frames with the DOM
+ * pass beside or after the camera pass (frame_serial). Leaves the targets
+ * dirty: run a frame afterwards. */
 hm_status hm_autotune(hm_ctx* ctx, hm_groom* groom, const hm_params params[3], const hm_frame_opts* opts,
                       uint32_t reps, hm_tuning* best);
 
