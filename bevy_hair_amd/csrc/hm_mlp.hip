@@ -560,7 +560,7 @@ __device__ void color_pair_by_wave(const ShadeArgs& a, const PixelShade& px, con
         F3 ltf = point_pos(L) - px.world;
         float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
         if (att == 0.0f) continue;
-        BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize(safe_normalize(ltf)));
+        BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(safe_normalize(ltf)));
         F3 single = single_scatter(g, px.sigma_a, px.shape);
         F3 w = point_color(L) * att;
         if (L.kind != HM_LIGHT_POINT) w = w * spot_mask(L, px.world);
@@ -569,7 +569,7 @@ __device__ void color_pair_by_wave(const ShadeArgs& a, const PixelShade& px, con
     }
     for (uint32_t i = (uint32_t)lane; i < a.directional_count; i += 64u) {
         const hm_directional_light& L = a.directionals[i];
-        BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize(dir_of(L)));
+        BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(dir_of(L)));
         F3 single = single_scatter(g, px.sigma_a, px.shape);
         F3 w = F3{L.color[0], L.color[1], L.color[2]};
         if ((int)i == key) {

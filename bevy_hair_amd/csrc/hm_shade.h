@@ -32,7 +32,10 @@ __device__ __forceinline__ float hexp(float x) { return __builtin_amdgcn_exp2f(x
 __device__ __forceinline__ float hlog(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
 __device__ __forceinline__ float hpow(float b, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(b)); }
 // Abramowitz & Stegun 4.4.46 (Hastings): asin x = pi/2 - sqrt(1 - x) P7(x)
-// on [0, 1], |error| <= 2e-8; odd extension for x < 0.
+// on [0, 1], |error| <= 2e-8; odd extension for x < 0. The polynomial's
+// multiply-adds fuse: an error of this size in the result is not amplified
+// (the conditioning near |x| = 1 concerns x itself, which is exact here).
+#pragma clang fp contract(fast)
 __device__ __forceinline__ float fasin(float x) {
     float a = fabsf(x);
     float p = -0.0012624911f;
@@ -46,6 +49,7 @@ __device__ __forceinline__ float fasin(float x) {
     float r = 1.5707963268f - fsqrt(1.0f - a) * p;
     return x < 0.0f ? -r : r;
 }
+#pragma clang fp contract(off)
 __device__ __forceinline__ F3 fnormalize(F3 v) {  // safe_normalize with the hardware reciprocal square root
     float l2 = dot3(v, v);
     if (l2 < 1e-10f) return {0.0f, 1.0f, 0.0f};
@@ -53,6 +57,10 @@ __device__ __forceinline__ F3 fnormalize(F3 v) {  // safe_normalize with the har
     return {v.x * r, v.y * r, v.z * r};
 }
 __device__ __forceinline__ F3 fdiv3(F3 a, F3 b) { return {fdiv(a.x, b.x), fdiv(a.y, b.y), fdiv(a.z, b.z)}; }
+// clamp / max in one instruction each (v_med3_f32, v_max_f32): the same
+// values as clampf / mx for every non-NaN input (a zero keeps its sign).
+__device__ __forceinline__ float sclamp(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }
+__device__ __forceinline__ float smax(float a, float b) { return __builtin_fmaxf(a, b); }
 
 struct Lobes { F3 r, tt, trt; };
 // inv_beta: the reciprocal gaussian3 takes of the widths, formed once per pixel.
@@ -85,7 +93,7 @@ __device__ __forceinline__ float fresnel(float cos_t) {
     const float ior = 1.55f;
     float r0 = (1.0f - ior) / (1.0f + ior);
     float f0 = r0 * r0;
-    float x = 1.0f - clampf(cos_t, 0.0f, 1.0f);
+    float x = 1.0f - sclamp(cos_t, 0.0f, 1.0f);
     float x2 = x * x;
     return f0 + (1.0f - f0) * (x2 * x2 * x);
 }
@@ -124,14 +132,14 @@ __device__ __forceinline__ void azimuthal_r_tt(float cos_theta_d, const Azimutha
                                                float h_k, float w_tt, float& n_r, F3& n_tt) {
     n_r = 0.25f * cos_half_phi * fresnel(cos_theta_d * cos_half_phi);
     const float h = cos_half_phi * (1.0f + d.a * h_k);
-    const float f_tt = fresnel(cos_theta_d * fsqrt(clampf(1.0f - h * h, 0.0f, 1.0f)));
-    const float s_tt = fsqrt(clampf(1.0f - h * h * d.a * d.a, 0.0f, 1.0f));
+    const float f_tt = fresnel(cos_theta_d * fsqrt(sclamp(1.0f - h * h, 0.0f, 1.0f)));
+    const float s_tt = fsqrt(sclamp(1.0f - h * h * d.a * d.a, 0.0f, 1.0f));
     n_tt = vexp(sigma_a * (-2.0f * s_tt * d.inv_cd)) * (w_tt * (1.0f - f_tt) * (1.0f - f_tt));
 }
 
 __device__ __forceinline__ Lobes azimuthal(float cos_theta_d, float cos_phi, F3 sigma_a) {
     const AzimuthalD d = azimuthal_d(cos_theta_d, sigma_a);
-    const float cos_half_phi = fsqrt(clampf(0.5f + 0.5f * cos_phi, 0.0f, 1.0f));
+    const float cos_half_phi = fsqrt(sclamp(0.5f + 0.5f * cos_phi, 0.0f, 1.0f));
     float n_r;
     F3 n_tt;
     azimuthal_r_tt(cos_theta_d, d, sigma_a, cos_half_phi, 0.6f - 0.8f * cos_phi, hexp(-3.65f * cos_phi - 3.98f), n_r,
@@ -275,13 +283,13 @@ __device__ __forceinline__ ViewGeom view_geom(F3 t, F3 v) {
 // relative change.
 __device__ __forceinline__ BsdfGeom bsdf_geom_u(const ViewGeom& vg, F3 l) {
     BsdfGeom g;
-    float sin_i = clampf(dot3(vg.t, l), -1.0f, 1.0f);
+    float sin_i = sclamp(dot3(vg.t, l), -1.0f, 1.0f);
     float theta_i = fasin(sin_i);
     g.theta_h = theta_i + vg.theta_o;
-    g.cos_theta_d = mx(fcos(0.5f * fabsf(theta_i - vg.theta_o)), 1e-3f);
+    g.cos_theta_d = smax(fcos(0.5f * fabsf(theta_i - vg.theta_o)), 1e-3f);
     F3 t_i = l - vg.t * sin_i;
-    float proj = __builtin_sqrtf(mx(dot3(t_i, t_i) * vg.to2, 1e-8f));
-    g.cos_phi = clampf(rdiv(dot3(t_i, vg.t_o), proj), -1.0f, 1.0f);
+    float proj = __builtin_sqrtf(smax(dot3(t_i, t_i) * vg.to2, 1e-8f));
+    g.cos_phi = sclamp(rdiv(dot3(t_i, vg.t_o), proj), -1.0f, 1.0f);
     g.t = vg.t;
     g.l = l;
     return g;
@@ -449,12 +457,12 @@ __device__ float dom_visibility(const View& p, const uint32_t* __restrict__ dom,
 // Bevy's range window saturate(1 - (d^2 / r^2)^2)^2: zero from the range on.
 __device__ __forceinline__ float range_falloff(float dist2, float inv_range2) {
     float factor = dist2 * inv_range2;
-    float smooth = clampf(1.0f - factor * factor, 0.0f, 1.0f);
+    float smooth = sclamp(1.0f - factor * factor, 0.0f, 1.0f);
     return smooth * smooth;
 }
 
 __device__ __forceinline__ float distance_attenuation(float dist2, float inv_range2) {
-    return range_falloff(dist2, inv_range2) * frcp(mx(dist2, 1e-4f));
+    return range_falloff(dist2, inv_range2) * frcp(smax(dist2, 1e-4f));
 }
 
 __device__ __forceinline__ float spot_mask(const hm_point_light& L, F3 world) {
@@ -463,7 +471,7 @@ __device__ __forceinline__ float spot_mask(const hm_point_light& L, F3 world) {
     if (L.flags & HM_LIGHT_FLAG_SPOT_Y_NEGATIVE) sd.y = -sd.y;
     F3 ltf = F3{L.position_radius[0], L.position_radius[1], L.position_radius[2]} - world;
     float cd = dot3(sd * -1.0f, fnormalize(ltf));
-    float att = clampf(cd * L.light_custom_data[2] + L.light_custom_data[3], 0.0f, 1.0f);
+    float att = sclamp(cd * L.light_custom_data[2] + L.light_custom_data[3], 0.0f, 1.0f);
     return att * att;
 }
 
@@ -572,8 +580,9 @@ __device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelSha
     float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
     if (att == 0.0f) return splat3(0.0f);
     // safe_normalize twice, as the light loop (shade.wesl:457) and hair_bsdf
-    // (shade.wesl:204) each normalize
-    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize(safe_normalize(ltf)));
+    // (shade.wesl:204) each normalize; the second one's vector is within a few
+    // ulps of unit length, where safe_normalize_unit gives the same bits cheaper
+    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(safe_normalize(ltf)));
     F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
     F3 c = lobe * point_color(L) * att;
     if (L.kind != HM_LIGHT_POINT) c = c * spot_mask(L, px.world);
@@ -584,7 +593,7 @@ __device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelSha
 // `depth`) exp(-depth) * single + key_ms, for the others single + 0.7 * back.
 __device__ __forceinline__ F3 directional_term(const hm_directional_light& L, bool is_key, const PixelShade& px,
                                                const LocalBack& lb, float depth, F3 key_ms) {
-    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize(dir_of(L)));
+    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(dir_of(L)));
     F3 single = single_scatter(g, px.sigma_a, px.shape);
     F3 lobe = is_key ? hexp(-depth) * single + key_ms
                      : single + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
@@ -606,7 +615,7 @@ __device__ __forceinline__ F3 rect_term(const hm_rect_light& L, const PixelShade
     float falloff = range_falloff(dist2, frcp(mx(L.range * L.range, 1e-4f)));
     if (facing == 0.0f || falloff == 0.0f) return splat3(0.0f);
     float solid = L.width * L.height * facing * frcp(dist2);
-    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize(incident));
+    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(incident));
     F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
     return lobe * F3{L.color[0], L.color[1], L.color[2]} * (solid * falloff);
 }
@@ -797,7 +806,7 @@ __device__ F3 pixel_color(const ShadeArgs& a, const PixelShade& px, bool valid, 
 
 // theta_h of the key light at this pixel (bsdf_geom_n's).
 __device__ __forceinline__ float key_theta_h(const PixelShade& px, F3 key_dir) {
-    return bsdf_geom_u(px.vg, safe_normalize(key_dir)).theta_h;
+    return bsdf_geom_u(px.vg, safe_normalize_unit(key_dir)).theta_h;
 }
 
 // The plain method's colour of one hit pixel: dual scattering evaluated
