@@ -98,6 +98,7 @@ struct hm_ctx {
     float scene_clear = 0.0f;
     hm_point_light* points = nullptr;
     uint32_t point_count = 0, point_cap = 0;
+    uint32_t spot_first = 0;  // the point lights come first in `points`, then the spot lights
     hm_directional_light* dirs = nullptr;
     uint32_t dir_count = 0, dir_cap = 0;
     std::vector<hm_point_light> host_points;  // the point/spot list as last set

@@ -727,11 +727,22 @@ hm_status hm_set_lights(hm_ctx* ctx, const hm_point_light* points, uint32_t np, 
         HM_HIP(ctx, dev_alloc(&ctx->dirs, (size_t)nd * sizeof(hm_directional_light)));
         ctx->dir_cap = nd;
     }
-    if (np) HM_HIP(ctx, hipMemcpy(ctx->points, points, (size_t)np * sizeof(hm_point_light), hipMemcpyHostToDevice));
+    // Point lights first, then spot lights, each kind in the caller's order:
+    // the shading loops walk each kind as one range, in the order of Bevy's
+    // cluster ranges (shade.wesl:452-480).
+    std::vector<hm_point_light> sorted;
+    sorted.reserve(np);
+    for (uint32_t i = 0; i < np; ++i)
+        if (points[i].kind == HM_LIGHT_POINT) sorted.push_back(points[i]);
+    const uint32_t spot_first = (uint32_t)sorted.size();
+    for (uint32_t i = 0; i < np; ++i)
+        if (points[i].kind != HM_LIGHT_POINT) sorted.push_back(points[i]);
+    if (np) HM_HIP(ctx, hipMemcpy(ctx->points, sorted.data(), (size_t)np * sizeof(hm_point_light), hipMemcpyHostToDevice));
     if (nd) HM_HIP(ctx, hipMemcpy(ctx->dirs, dirs, (size_t)nd * sizeof(hm_directional_light), hipMemcpyHostToDevice));
     ctx->point_count = np;
+    ctx->spot_first = spot_first;
     ctx->dir_count = nd;
-    ctx->host_points.assign(points, points + np);
+    ctx->host_points = std::move(sorted);
     ctx->light_gen++;
     return HM_OK;
 }
@@ -803,6 +814,7 @@ extern "C++" void hm_shade_lights(hm_ctx* ctx, hm::ShadeArgs& a) {
     a.cull = cull_pays(ctx) ? 1u : 0u;
     a.points = ctx->points;
     a.point_count = ctx->point_count;
+    a.spot_first = ctx->spot_first;
     a.directionals = ctx->dirs;
     a.directional_count = ctx->dir_count;
     a.rects = ctx->rects;

@@ -60,8 +60,9 @@ struct ShadeArgs {
     const uint64_t* center;
     const uint32_t* beta;
     const uint32_t* dom;
-    const hm_point_light* points;
-    uint32_t point_count;
+    const hm_point_light* points;   // point lights, then spot lights (each in the caller's order)
+    uint32_t point_count;           // both kinds
+    uint32_t spot_first;            // index of the first spot light
     const hm_directional_light* directionals;
     uint32_t directional_count;
     const hm_rect_light* rects;

@@ -575,6 +575,8 @@ __device__ __forceinline__ T uniform_light(const T* lights, uint32_t i) {
 // Point light L's contribution (shade.wesl:449-505) with back-lobe terms
 // `lb`; zero where the light's range does not reach the pixel, as the
 // cluster would not list the light.
+// KIND: 0 point light, 1 spot light, -1 by the light's own kind.
+template <int KIND = -1>
 __device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelShade& px, const LocalBack& lb) {
     F3 ltf = point_pos(L) - px.world;
     float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
@@ -585,7 +587,7 @@ __device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelSha
     BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(safe_normalize(ltf)));
     F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
     F3 c = lobe * point_color(L) * att;
-    if (L.kind != HM_LIGHT_POINT) c = c * spot_mask(L, px.world);
+    if (KIND == 1 || (KIND < 0 && L.kind != HM_LIGHT_POINT)) c = c * spot_mask(L, px.world);
     return c;
 }
 
@@ -731,28 +733,44 @@ __device__ __forceinline__ WaveBox wave_box(F3 w, bool valid) {
     return {{u(lo.x), u(lo.y), u(lo.z)}, {u(hi.x), u(hi.y), u(hi.z)}};
 }
 
-// Lights base .. base + 63 of the point/spot list (kind point when `points`,
-// else spot) whose range reaches the box, one bit per light (every lane of
+// Lights base .. min(base + 63, end - 1) of the point/spot list whose range
+// reaches the box, one bit per light (every lane of
 // the wave calls; lane i tests light base + i). A light is dropped only when
 // the box lies beyond its range by a relative margin of 1e-5, far above the
 // rounding of the per-pixel distance: distance_attenuation is then zero at
 // every pixel of the wave, so dropping it changes no colour.
 __device__ __forceinline__ uint64_t reaching_lights(const ShadeArgs& a, const WaveBox& box, uint32_t base,
-                                                    bool points) {
+                                                    uint32_t end) {
     const uint32_t j = base + (uint32_t)(threadIdx.x & 63u);
     bool keep = false;
-    if (j < a.point_count) {
+    if (j < end) {
         const hm_point_light& L = a.points[j];
-        if ((L.kind == HM_LIGHT_POINT) == points) {
-            const F3 c = point_pos(L);
-            const F3 d = vmax(vmax(box.lo - c, c - box.hi), splat3(0.0f));
-            keep = !(dot3(d, d) * L.color_inverse_square_range[3] > 1.00001f);
-        }
+        const F3 c = point_pos(L);
+        const F3 d = vmax(vmax(box.lo - c, c - box.hi), splat3(0.0f));
+        keep = !(dot3(d, d) * L.color_inverse_square_range[3] > 1.00001f);
     }
     return __ballot(keep);
 }
 
 #pragma clang fp contract(fast)
+
+// Lights begin .. end - 1 of the point/spot list (all point lights, or all
+// spot lights) added to `color` in order. The list is read-only and the index
+// wave-uniform: each light's words come through the scalar cache, and the
+// next light's are requested before this light's terms are formed, so that
+// the load's latency hides behind them.
+template <bool SPOT>
+__device__ __forceinline__ F3 light_range(const ShadeArgs& a, uint32_t begin, uint32_t end, const PixelShade& px,
+                                          const LocalBack& lb, F3 color) {
+    if (begin >= end) return color;
+    hm_point_light L = uniform_light(a.points, begin);
+    for (uint32_t i = begin; i < end; ++i) {
+        const hm_point_light next = uniform_light(a.points, i + 1u < end ? i + 1u : i);
+        color = color + point_term<SPOT ? 1 : 0>(L, px, lb);
+        L = next;
+    }
+    return color;
+}
 
 // Colour of one hit pixel (shade.wesl:442-521, 585-588), Bevy shadow factors
 // 1, from the back-lobe terms `lb` and the key light's multiple scattering
@@ -769,20 +787,14 @@ __device__ F3 pixel_color(const ShadeArgs& a, const PixelShade& px, bool valid, 
     constexpr bool CULL = (F & SHADE_CULL) != 0u;
     F3 color = splat3(0.0f);
     if (!CULL) {
-        for (int pass = 0; pass < 2; ++pass) {
-            for (uint32_t i = 0; i < a.point_count; ++i) {
-                // read-only, wave-uniform: through the scalar cache in one
-                // load rather than as dependent vector loads
-                const hm_point_light L = uniform_light(a.points, i);
-                if ((pass == 0) != (L.kind == HM_LIGHT_POINT)) continue;
-                color = color + point_term(L, px, lb);
-            }
-        }
+        color = light_range<false>(a, 0u, a.spot_first, px, lb, color);
+        color = light_range<true>(a, a.spot_first, a.point_count, px, lb, color);
     } else if (a.point_count) {
         const WaveBox box = wave_box(px.world, valid);
         for (int pass = 0; pass < 2; ++pass) {
-            for (uint32_t base = 0; base < a.point_count; base += 64u) {
-                uint64_t reach = reaching_lights(a, box, base, pass == 0);
+            const uint32_t lo = pass == 0 ? 0u : a.spot_first, hi = pass == 0 ? a.spot_first : a.point_count;
+            for (uint32_t base = lo; base < hi; base += 64u) {
+                uint64_t reach = reaching_lights(a, box, base, hi);
                 while (reach) {
                     const uint32_t i = base + (uint32_t)__builtin_ctzll(reach);
                     reach &= reach - 1u;
