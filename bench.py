@@ -305,7 +305,8 @@ def mlp_c4_leg_run(r, args) -> dict:
 def batch_leg_run(r, args) -> dict:
     """24 views over three grooms (20k x 16, 100k x 64, 150k x 126) at
     1280x720 / 1920x1080 / 2560x1440, learned shading with the trained-range
-    fallback, replayed from one captured HIP graph: views/s."""
+    fallback, replayed from one captured HIP graph: views/s with the views
+    dealt to the tuned number of lanes, and with one lane beside it."""
     from bevy_hair_amd import mlp_train
     from bevy_hair_amd import workload as W
 
@@ -316,16 +317,25 @@ def batch_leg_run(r, args) -> dict:
     views = [(gpu[gi], params, wd, ht, W.BATCH_DOM,
               r.frame_opts(control_points=ncp, strand_fraction=1.0, mlp=True, mlp_tolerance=args.mlp_tolerance))
              for gi, wd, ht, _, params, ncp in bs.views]
-    b = r.batch(views)
+    reps = max(3, min(20, args.steps // 5))
+    lanes = r.get_tuning()["batch_lanes"]
+    per_lanes = {}
     try:
-        b.run(2)
-        reps = max(3, min(20, args.steps // 5))
-        ms = b.time(reps) / reps
+        for n in sorted({lanes, 1}, reverse=True):
+            r.set_tuning(batch_lanes=n)
+            b = r.batch(views)
+            try:
+                b.run(2)
+                per_lanes[n] = b.time(reps) / reps
+            finally:
+                b.free()
     finally:
-        b.free()
+        r.set_tuning(batch_lanes=lanes)
         for gg in gpu:
             gg.free()
+    ms = per_lanes[lanes]
     return {"views": len(views), "views_per_s": round(len(views) * 1000.0 / ms, 1), "ms_per_batch": round(ms, 3),
+            "lanes": lanes, "views_per_s_one_lane": round(len(views) * 1000.0 / per_lanes[1], 1),
             "replays": reps, "grooms": [f"{k} {n}x{ncp - 1}" for k, n, _, ncp in W.BATCH_GROOMS],
             "sizes": [f"{a}x{b_}" for a, b_ in W.BATCH_SIZES], "dom_size": W.BATCH_DOM, "shading": "mlp"}
 

@@ -1,8 +1,15 @@
-// Batched multi-view frames: a fixed list of views rendered back to back on
-// the context's stream and replayed from one captured HIP graph. The views'
-// buffers are grown by one eager pass before the capture, so the captured
-// sequence holds kernel launches only (no allocation, no synchronization, no
-// memset nodes: every clear is a fill kernel).
+// Batched multi-view frames: a fixed list of views dealt in turn to one or
+// more lanes and replayed from one captured HIP graph. A lane is a chain of
+// whole frames on streams and work buffers of its own (the first lane is the
+// context itself, the others are lane contexts that read its lights, probe
+// and network weights); the lanes fork from the context's stream at the start
+// of the graph and join it at the end, so one view's low-occupancy phases
+// (LOD, scans, long-segment and finish launches, the tail of a tile kernel)
+// overlap another view's wide kernels. The views' buffers are grown by one
+// eager pass before the capture, so the captured sequence holds kernel
+// launches only (no allocation, no synchronization, no memset nodes: every
+// clear is a fill kernel).
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -13,9 +20,12 @@
 struct hm_batch {
     std::vector<hm_batch_view> views;
     std::vector<void*> out;  // per view: rgba float4[px], then depth float[px]
+    std::vector<hm_ctx*> lanes;  // lanes[0] is the context; the others are owned here
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr;        // the graph's start, on the context's stream
+    std::vector<hipEvent_t> join;     // per lane after the first: its last copy
     // the context's buffers the captured launches read: their allocation
     // generation, and the light counts baked into the launches
     uint64_t alloc_gen = 0, shade_gen = 0;
@@ -36,27 +46,51 @@ void release(hm_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (hipEvent_t& e : b->ev)
         if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    for (size_t k = 1; k < b->lanes.size(); ++k) hm_destroy(b->lanes[k]);
     for (void* p : b->out)
         if (p) (void)hipFree(p);
     delete b;
 }
 
-// Every view once: targets, frame, copy of the final colour and depth.
-hm_status record(hm_ctx* ctx, hm_batch* b) {
-    for (size_t i = 0; i < b->views.size(); ++i) {
-        const hm_batch_view& v = b->views[i];
-        hm_status st = hm_targets(ctx, v.width, v.height, v.dom_size);
-        if (st != HM_OK) return st;
-        if ((st = hm_frame(ctx, v.groom, v.params, &v.opts)) != HM_OK) return st;
-        const size_t px = (size_t)v.width * v.height;
-        const bool filtered = v.params[0].flags[1] != 0u;
-        const void* rgba = filtered ? (const void*)ctx->filtered : (const void*)ctx->shaded;
-        const void* depth = filtered ? (const void*)ctx->filtered_depth : (const void*)ctx->shaded_depth;
-        char* dst = static_cast<char*>(b->out[i]);
-        BATCH_HIP(ctx, hm::launch_copy32(ctx->stream, rgba, dst, px * 4, ctx->cu_count * 8));
-        BATCH_HIP(ctx, hm::launch_copy32(ctx->stream, depth, dst + px * 16, px, ctx->cu_count * 8));
-    }
+// One view on lane `c`: targets, frame, copy of the final colour and depth.
+hm_status record_view(hm_ctx* c, const hm_batch_view& v, void* out) {
+    hm_status st = hm_targets(c, v.width, v.height, v.dom_size);
+    if (st != HM_OK) return st;
+    if ((st = hm_frame(c, v.groom, v.params, &v.opts)) != HM_OK) return st;
+    const size_t px = (size_t)v.width * v.height;
+    const bool filtered = v.params[0].flags[1] != 0u;
+    const void* rgba = filtered ? (const void*)c->filtered : (const void*)c->shaded;
+    const void* depth = filtered ? (const void*)c->filtered_depth : (const void*)c->shaded_depth;
+    char* dst = static_cast<char*>(out);
+    BATCH_HIP(c, hm::launch_copy32(c->stream, rgba, dst, px * 4, c->cu_count * 8));
+    BATCH_HIP(c, hm::launch_copy32(c->stream, depth, dst + px * 16, px, c->cu_count * 8));
     return HM_OK;
+}
+
+// Every view once, view i on lane i % lanes, between the fork from and the
+// join back into the context's stream (the join is enqueued on every exit,
+// so that a failed capture still ends with every lane joined).
+hm_status record(hm_ctx* ctx, hm_batch* b) {
+    const size_t nl = b->lanes.size();
+    if (nl > 1) {
+        BATCH_HIP(ctx, hipEventRecord(b->fork, ctx->stream));
+        for (size_t k = 1; k < nl; ++k) BATCH_HIP(ctx, hipStreamWaitEvent(b->lanes[k]->stream, b->fork, 0));
+    }
+    hm_status st = HM_OK;
+    for (size_t i = 0; i < b->views.size() && st == HM_OK; ++i) {
+        hm_ctx* c = b->lanes[i % nl];
+        st = record_view(c, b->views[i], b->out[i]);
+        if (st != HM_OK && c != ctx) ctx->err = "batch lane " + std::to_string(i % nl) + ": " + c->err;
+    }
+    for (size_t k = 1; k < nl; ++k) {
+        hipError_t e = hipEventRecord(b->join[k - 1], b->lanes[k]->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, b->join[k - 1], 0);
+        if (e != hipSuccess && st == HM_OK) st = hm_hip_fail(ctx, e, "batch lane join");
+    }
+    return st;
 }
 
 }  // namespace
@@ -78,6 +112,7 @@ hm_status hm_batch_create(hm_ctx* ctx, const hm_batch_view* views, uint32_t coun
     hm_batch* b = new hm_batch();
     b->views.assign(views, views + count);
     b->out.assign(count, nullptr);
+    b->lanes.push_back(ctx);
     hipError_t e = hipSuccess;
     for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
         e = hipMalloc(&b->out[i], (size_t)views[i].width * views[i].height * 20);
@@ -87,13 +122,34 @@ hm_status hm_batch_create(hm_ctx* ctx, const hm_batch_view* views, uint32_t coun
         release(b);
         return hm_hip_fail(ctx, e, "hm_batch_create");
     }
+    // More lanes than views would idle; a scene depth belongs to one target
+    // size in one context and keeps the views in one chain.
+    const bool depth = ctx->scene_depth || ctx->scene_depth_ext;
+    const uint32_t lanes = depth ? 1u : std::min(ctx->tuning.batch_lanes, count);
+    for (uint32_t k = 1; k < lanes; ++k) {
+        hm_ctx* lane = nullptr;
+        hm_status lst = hm_lane_create(ctx, &lane);
+        if (lst != HM_OK) {
+            release(b);
+            return lst;
+        }
+        b->lanes.push_back(lane);
+        hipEvent_t j = nullptr;
+        e = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+        b->join.push_back(j);
+        if (e == hipSuccess && !b->fork) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            release(b);
+            return hm_hip_fail(ctx, e, "hm_batch_create (lane events)");
+        }
+    }
     // Per-kernel timing records events on the stream: keep it out of the graph.
     const uint32_t slots = ctx->slots;
     ctx->slots = 0;
     hm_status st = record(ctx, b);  // eager pass: grows every buffer the views need
-    if (st == HM_OK) {
-        e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) st = hm_hip_fail(ctx, e, "hm_batch_create (eager pass)");
+    for (hm_ctx* c : b->lanes) {
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess && st == HM_OK) st = hm_hip_fail(ctx, e, "hm_batch_create (eager pass)");
     }
     if (st == HM_OK) {
         e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
@@ -160,7 +216,7 @@ hm_status hm_batch_read(hm_ctx* ctx, hm_batch* b, uint32_t view, float* rgba, fl
 
 void hm_batch_free(hm_ctx* ctx, hm_batch* b) {
     if (!b) return;
-    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);  // the lanes join it in every replay
     release(b);
 }
 

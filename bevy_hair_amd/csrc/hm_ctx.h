@@ -117,7 +117,7 @@ struct hm_ctx {
     // rect light count, the probe's sizes, rotation or intensity).
     uint64_t shade_gen = 0;
     uint64_t* counters = nullptr;
-    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u, 2048u, 4u, 0u, 2u, 0u};
+    hm_tuning tuning = {{HM_RASTER_PRECHECK, HM_RASTER_PRECHECK, HM_RASTER_PRECHECK}, {16u, 16u, 16u}, 16u, 16u, 8u, 0u, 2u, 2048u, 4u, 0u, 2u, 0u, 2u};
     hm_bin_state bin;
     // Per-kernel event pairs for `slots` frames: [slot][kernel][begin/end].
     uint32_t slots = 0, frame = 0;
@@ -128,6 +128,9 @@ struct hm_ctx {
     // Bumped whenever a context-owned device buffer is (re)allocated or freed:
     // a captured batch graph holds their addresses.
     uint64_t alloc_gen = 0;
+    // A batch lane (hm_lane_create): the light, rect light and probe buffers
+    // and the network weights are the parent context's, never freed here.
+    bool borrowed = false;
     std::string err;
 };
 
@@ -146,6 +149,14 @@ struct hm_lists {
 hm_status hm_fail(hm_ctx* ctx, hm_status st, const std::string& msg);
 hm_status hm_hip_fail(hm_ctx* ctx, hipError_t e, const char* what);
 void hm_mlp_release(hm_ctx* ctx);
+// A batch lane of `parent`: streams, targets, binning buffers, counters and
+// learned-shading flags of its own; the parent's lights, rect lights, probe
+// and network weights read in place (the parent must outlive the lane and
+// keep those buffers; its allocation generation tells when it did not).
+// No scene depth, no timing slots; freed with hm_destroy.
+hm_status hm_lane_create(hm_ctx* parent, hm_ctx** out);
+// The lane's learned-shading state: the parent's weights, its own flags.
+hm_status hm_mlp_borrow(hm_ctx* lane, const hm_ctx* parent);
 // Looks up the allocation-free scan's scratch address on `device` (current).
 hipError_t hm_noalloc_setup(int device);
 // The light lists and the probe of the context, into shading arguments.

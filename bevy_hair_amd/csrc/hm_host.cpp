@@ -516,6 +516,12 @@ void hm_destroy(hm_ctx* c) {
     free_targets(c);
     free_bin(c);
     hm_mlp_release(c);
+    if (c->borrowed) {  // the parent's buffers
+        c->points = nullptr;
+        c->dirs = nullptr;
+        c->rects = nullptr;
+        c->env_maps = nullptr;
+    }
     dev_free(c->points);
     dev_free(c->dirs);
     dev_free(c->rects);
@@ -531,6 +537,43 @@ void hm_destroy(hm_ctx* c) {
     if (c->main) (void)hipStreamDestroy(c->main);
     delete c;
 }
+
+}  // extern "C"
+
+hm_status hm_lane_create(hm_ctx* parent, hm_ctx** out) {
+    *out = nullptr;
+    hm_ctx* c = nullptr;
+    hm_status st = hm_create(parent->device, &c);
+    if (st != HM_OK) return fail(parent, st, "batch lane: context creation failed");
+    c->borrowed = true;
+    c->tuning = parent->tuning;
+    c->points = parent->points;
+    c->point_count = parent->point_count;
+    c->point_cap = parent->point_cap;
+    c->spot_first = parent->spot_first;
+    c->dirs = parent->dirs;
+    c->dir_count = parent->dir_count;
+    c->dir_cap = parent->dir_cap;
+    c->host_points = parent->host_points;
+    c->light_gen = parent->light_gen;
+    c->rects = parent->rects;
+    c->rect_count = parent->rect_count;
+    c->rect_cap = parent->rect_cap;
+    c->env_maps = parent->env_maps;
+    c->env_bytes = parent->env_bytes;
+    c->env = parent->env;
+    c->shade_gen = parent->shade_gen;
+    c->scene_clear = parent->scene_clear;
+    if ((st = hm_mlp_borrow(c, parent)) != HM_OK) {
+        parent->err = "batch lane: " + c->err;
+        hm_destroy(c);
+        return st;
+    }
+    *out = c;
+    return HM_OK;
+}
+
+extern "C" {
 
 hm_status hm_groom_upload(hm_ctx* ctx, const hm_groom_desc* d, hm_groom** out) {
     if (!ctx || !d || !out) return fail(ctx, HM_ERR_INVALID, "null argument");
@@ -1588,6 +1631,7 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
     if (t->light_cull > 2u) return fail(ctx, HM_ERR_INVALID, "light_cull must be 0, 1 or 2");
     if (t->dom_busy_blocks_per_cu > 2u) return fail(ctx, HM_ERR_INVALID, "dom_busy_blocks_per_cu must be 0, 1 or 2");
     if (t->frame_serial > 1u) return fail(ctx, HM_ERR_INVALID, "frame_serial must be 0 or 1");
+    if (t->batch_lanes < 1u || t->batch_lanes > 4u) return fail(ctx, HM_ERR_INVALID, "batch_lanes must be in [1, 4]");
 
     ctx->tuning = *t;
     return HM_OK;

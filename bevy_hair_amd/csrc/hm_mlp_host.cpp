@@ -37,6 +37,7 @@ struct hm_mlp_state {
     float lo[MLP_IN], hi[MLP_IN];
     bool ready = false;      // weights set
     bool range = false;      // trained feature range known
+    bool borrowed = false;   // a batch lane's: w, b and packed are the parent's
     int step = 0;
 };
 
@@ -137,6 +138,11 @@ hm_status upload_weights(hm_ctx* ctx, hm_mlp_state* s, const float* w, const flo
 void hm_mlp_release(hm_ctx* ctx) {
     if (!ctx || !ctx->mlp) return;
     hm_mlp_state* s = ctx->mlp;
+    if (s->borrowed) {
+        s->w = nullptr;
+        s->b = nullptr;
+        s->packed = nullptr;
+    }
     release(s->w); release(s->b); release(s->packed); release(s->m); release(s->v);
     release(s->gw); release(s->gb); release(s->loss); release(s->act0);
     for (int i = 0; i < 3; ++i) release(s->act[i]);
@@ -144,6 +150,22 @@ void hm_mlp_release(hm_ctx* ctx) {
     release(s->sx); release(s->st); release(s->scount); release(s->flags);
     delete s;
     ctx->mlp = nullptr;
+}
+
+hm_status hm_mlp_borrow(hm_ctx* lane, const hm_ctx* parent) {
+    if (!parent->mlp) return HM_OK;
+    const hm_mlp_state* p = parent->mlp;
+    hm_mlp_state* s = new hm_mlp_state();
+    s->borrowed = true;
+    s->w = p->w;
+    s->b = p->b;
+    s->packed = p->packed;
+    std::memcpy(s->lo, p->lo, sizeof(s->lo));
+    std::memcpy(s->hi, p->hi, sizeof(s->hi));
+    s->ready = p->ready;
+    s->range = p->range;
+    lane->mlp = s;
+    return HM_OK;
 }
 
 extern "C" {

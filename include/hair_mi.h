@@ -477,6 +477,11 @@ typedef struct hm_tuning {
                                       pass (its generation after the camera generation, shading after
                                       both); 1 = the passes one after the other on one stream. The
                                       results are the same either way */
+    uint32_t batch_lanes;          /* hm_batch_create (1..4): views are dealt in turn to this many
+                                      lanes, each a chain on streams and work buffers of its own,
+                                      joined at the end of the captured graph; the first lane is the
+                                      context itself. A batch created while a scene depth is set
+                                      uses one lane. The results are the same either way */
 } hm_tuning;
 
 hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);

@@ -1,5 +1,6 @@
 """The batched multi-view frame (hm_batch_*): a captured HIP graph of several
-views of several grooms at mixed sizes must give, on every replay, exactly the
+views of several grooms at mixed sizes must give, on every replay and for
+every lane count (views dealt to one to three concurrent chains), exactly the
 images the same views give rendered one at a time with hm_frame."""
 from __future__ import annotations
 
@@ -30,9 +31,12 @@ def _views(renderer, meshes, grooms, dirs, mlp):
     return gpu, views
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 3])
 @pytest.mark.parametrize("mlp", [False, True])
-def test_batch_replays_equal_one_view_at_a_time(renderer, mlp):
-    print(f"\n[gpu batch] 6 views, mlp={mlp}", flush=True)
+def test_batch_replays_equal_one_view_at_a_time(renderer, mlp, lanes):
+    print(f"\n[gpu batch] 6 views, mlp={mlp}, lanes={lanes}", flush=True)
+    keep = renderer.get_tuning()["batch_lanes"]
+    renderer.set_tuning(batch_lanes=lanes)
     mlp_train.install(renderer)
     meshes = [synthetic_groom(kind, n, seed) for kind, n, seed, _ in GROOMS]
     grooms = [plugin.HairGroom(filter=True), plugin.HairGroom(filter=False)]
@@ -63,6 +67,60 @@ def test_batch_replays_equal_one_view_at_a_time(renderer, mlp):
         batch.free()
         for g in gpu:
             g.free()
+        renderer.set_tuning(batch_lanes=keep)
+
+
+def test_batch_with_scene_depth(renderer):
+    """With a scene depth set the views stay in one chain (the depth belongs to
+    the context's target): four views of one size in front of a checkerboard
+    depth equal hm_frame with the same depth, on two replays."""
+    print("\n[gpu batch] 4 views over a scene depth, batch_lanes=2", flush=True)
+    keep = renderer.get_tuning()["batch_lanes"]
+    renderer.set_tuning(batch_lanes=2)
+    mesh = synthetic_groom("straight", 4096, 1)
+    groom = plugin.HairGroom(filter=True)
+    rng = np.random.default_rng(5)
+    renderer.set_lights(scene.point_lights(rng, 3, 0.5 * (mesh.bounds_min + mesh.bounds_max)), [scene.key_light()])
+    w, h, dom = 480, 270, 256
+    g = renderer.upload(mesh)
+    views, cams = [], []
+    for v in range(4):
+        cam = scene.fit_camera(mesh, w, h, heading=(120.0 - 50.0 * v, 0.0, 60.0 + 20.0 * v))
+        params = plugin.Renderer.make_params(cam, groom, mesh, dom, [scene.key_light()])
+        views.append((g, params, w, h, dom, renderer.frame_opts(control_points=17, strand_fraction=1.0)))
+        cams.append(cam)
+    # every other 24-pixel square at the clip depth of the groom's centre in the first view
+    c = np.append(0.5 * (mesh.bounds_min + mesh.bounds_max).astype(np.float64), 1.0)
+    clip = cams[0].clip_from_world @ c
+    yy, xx = np.mgrid[0:h, 0:w]
+    depth_map = np.where(((xx // 24) + (yy // 24)) % 2 == 0, np.float32(clip[2] / clip[3]),
+                         np.float32(0.0)).astype(np.float32).ravel()
+    renderer.targets(w, h, dom)
+    renderer.set_scene_depth(depth_map, 0.0)
+    batch = None
+    try:
+        batch = renderer.batch(views)
+        batch.run(1)
+        first = [batch.read(i) for i in range(len(views))]
+        batch.run(1)
+        occluded = 0
+        for i, (gg, params, _, _, _, opts) in enumerate(views):
+            rgba, d = batch.read(i)
+            assert np.array_equal(rgba, first[i][0]) and np.array_equal(d, first[i][1]), f"replay differs, view {i}"
+            renderer.hair_pass(gg, params, opts)
+            img, dd = renderer.filtered()
+            assert np.array_equal(rgba, img) and np.array_equal(d, dd), f"view {i} differs from hm_frame"
+            renderer.set_scene_depth(None, 0.0)
+            renderer.hair_pass(gg, params, opts)
+            occluded += int((renderer.filtered()[0][:, 3] > 0).sum() - (img[:, 3] > 0).sum())
+            renderer.set_scene_depth(depth_map, 0.0)
+        assert occluded > 0, "the scene depth hid no hair pixel"
+    finally:
+        if batch is not None:
+            batch.free()
+        renderer.set_scene_depth(None, 0.0)
+        g.free()
+        renderer.set_tuning(batch_lanes=keep)
 
 
 def test_c5_batch_against_frames_and_oracle(renderer):
