@@ -56,6 +56,38 @@ __device__ __forceinline__ float len2(F2 a) { return __builtin_sqrtf(dot2(a, a))
 __device__ __forceinline__ float len3(F3 a) { return __builtin_sqrtf(dot3(a, a)); }
 __device__ __forceinline__ float rdiv(float a, float b) { return a / b; }
 __device__ __forceinline__ float inv_sqrt(float x) { return 1.0f / __builtin_sqrtf(x); }
+
+// Correctly rounded sqrt and divide on operands where the compiler's IEEE
+// expansions reduce to their core: the same instruction sequences, hence the
+// same bits, without the steps that only act outside these domains.
+// sqrt: for 2^-96 <= x < inf the expansion neither scales x (it does below
+// 2^-96) nor selects x itself (zero, infinity): what is left is the hardware
+// root s and the choice among s and its two neighbours by the signs of the
+// residuals x - s' * s.
+__device__ __forceinline__ float sqrt_rn_mid(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    const float r = rd <= 0.0f ? sd : s;
+    return ru > 0.0f ? su : r;
+}
+// divide: for 2^-60 <= |a|, b <= 2^60 the expansion's v_div_scale leaves both
+// operands as they are, its v_div_fmas is then a plain fma, and v_div_fixup
+// has no zero, infinity or NaN to handle: what is left is the refined
+// reciprocal y of b and two residual corrections of a * y.
+__device__ __forceinline__ float div_rn_mid(float a, float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float y = __builtin_fmaf(__builtin_fmaf(-b, y0, 1.0f), y0, y0);
+    const float q0 = a * y;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
+    return __builtin_fmaf(__builtin_fmaf(-b, q1, a), y, q1);
+}
+__device__ __forceinline__ bool div_mid_ok(float a) {
+    const float m = fabsf(a);
+    return m >= 0x1p-60f && m <= 0x1p60f;
+}
 __device__ __forceinline__ F3 normalize3(F3 v) { return v * inv_sqrt(dot3(v, v)); }
 // normalize3 for a vector expected to be unit length to rounding already
 // (equal to normalize3 for every input; cheaper when the length is near 1).
@@ -200,6 +232,13 @@ __device__ __forceinline__ F3 safe_normalize(F3 v) {
     float l2 = dot3(v, v);
     if (l2 < 1e-10f) return {0.0f, 1.0f, 0.0f};
     return v * inv_sqrt(l2);
+}
+// safe_normalize through the mid-range forms where 1e-10 <= |v|^2 < 2^120
+// (the root then lies in [1e-5, 2^60]); the same results for every v.
+__device__ __forceinline__ F3 safe_normalize_mid(F3 v) {
+    const float l2 = dot3(v, v);
+    if (!(l2 >= 1e-10f && l2 < 0x1p120f)) return safe_normalize(v);
+    return v * div_rn_mid(1.0f, sqrt_rn_mid(l2));
 }
 // safe_normalize for a vector expected to be near unit length (same results).
 __device__ __forceinline__ F3 safe_normalize_unit(F3 v) {

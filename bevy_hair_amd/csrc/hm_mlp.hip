@@ -560,25 +560,25 @@ __device__ void color_pair_by_wave(const ShadeArgs& a, const PixelShade& px, con
         F3 ltf = point_pos(L) - px.world;
         float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
         if (att == 0.0f) continue;
-        BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(safe_normalize(ltf)));
-        F3 single = single_scatter(g, px.sigma_a, px.shape);
+        BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(safe_normalize_mid(ltf)));
+        F3 single = single_lobes(g, px);
         F3 w = point_color(L) * att;
         if (L.kind != HM_LIGHT_POINT) w = w * spot_mask(L, px.world);
-        cn = cn + (single + 0.7f * local_back_r(g.theta_h, net.a_back, net.delta_b, net.inv_sb)) * w;
-        cr = cr + (single + 0.7f * local_back_r(g.theta_h, ref.a_back, ref.delta_b, ref.inv_sb)) * w;
+        cn = cn + (single + back_lobe(g.theta_h, net)) * w;
+        cr = cr + (single + back_lobe(g.theta_h, ref)) * w;
     }
     for (uint32_t i = (uint32_t)lane; i < a.directional_count; i += 64u) {
         const hm_directional_light& L = a.directionals[i];
         BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(dir_of(L)));
-        F3 single = single_scatter(g, px.sigma_a, px.shape);
+        F3 single = single_lobes(g, px);
         F3 w = F3{L.color[0], L.color[1], L.color[2]};
         if ((int)i == key) {
             single = hexp(-depth) * single;
             cn = cn + (single + net_ms) * w;
             cr = cr + (single + ref_ms) * w;
         } else {
-            cn = cn + (single + 0.7f * local_back_r(g.theta_h, net.a_back, net.delta_b, net.inv_sb)) * w;
-            cr = cr + (single + 0.7f * local_back_r(g.theta_h, ref.a_back, ref.delta_b, ref.inv_sb)) * w;
+            cn = cn + (single + back_lobe(g.theta_h, net)) * w;
+            cr = cr + (single + back_lobe(g.theta_h, ref)) * w;
         }
     }
     F3 env = splat3(0.0f);

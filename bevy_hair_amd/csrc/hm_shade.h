@@ -28,7 +28,10 @@ __device__ __forceinline__ float fpow(float b, float e) { return __expf(e * __lo
 // lookup: no range scaling for results below 2^-126, which cannot change a
 // colour held to a relative tolerance (the library forms above add a
 // compare and two selects per call).
-__device__ __forceinline__ float hexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+constexpr float kLog2e = 1.44269504088896341f;
+constexpr float kGaussLog2 = 0.849321800288019f;    // sqrt(log2(e) / 2): exp(-z^2 / 2) = exp2(-(z kGaussLog2)^2)
+constexpr float kInvSqrt2Pi = 0.398942280401432678f;
+__device__ __forceinline__ float hexp(float x) { return __builtin_amdgcn_exp2f(x * kLog2e); }
 __device__ __forceinline__ float hlog(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
 __device__ __forceinline__ float hpow(float b, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(b)); }
 // Abramowitz & Stegun 4.4.46 (Hastings): asin x = pi/2 - sqrt(1 - x) P7(x)
@@ -135,17 +138,6 @@ __device__ __forceinline__ void azimuthal_r_tt(float cos_theta_d, const Azimutha
     const float f_tt = fresnel(cos_theta_d * fsqrt(sclamp(1.0f - h * h, 0.0f, 1.0f)));
     const float s_tt = fsqrt(sclamp(1.0f - h * h * d.a * d.a, 0.0f, 1.0f));
     n_tt = vexp(sigma_a * (-2.0f * s_tt * d.inv_cd)) * (w_tt * (1.0f - f_tt) * (1.0f - f_tt));
-}
-
-__device__ __forceinline__ Lobes azimuthal(float cos_theta_d, float cos_phi, F3 sigma_a) {
-    const AzimuthalD d = azimuthal_d(cos_theta_d, sigma_a);
-    const float cos_half_phi = fsqrt(sclamp(0.5f + 0.5f * cos_phi, 0.0f, 1.0f));
-    float n_r;
-    F3 n_tt;
-    azimuthal_r_tt(cos_theta_d, d, sigma_a, cos_half_phi, 0.6f - 0.8f * cos_phi, hexp(-3.65f * cos_phi - 3.98f), n_r,
-                   n_tt);
-    const F3 n_trt = d.t_trt * (hexp(17.0f * cos_phi - 16.78f) * d.trt_f);
-    return {splat3(n_r), n_tt, n_trt};
 }
 
 __device__ __forceinline__ F3 lobe_mean(const Lobes& l, F3 total, F3 q) {
@@ -288,8 +280,14 @@ __device__ __forceinline__ BsdfGeom bsdf_geom_u(const ViewGeom& vg, F3 l) {
     g.theta_h = theta_i + vg.theta_o;
     g.cos_theta_d = smax(fcos(0.5f * fabsf(theta_i - vg.theta_o)), 1e-3f);
     F3 t_i = l - vg.t * sin_i;
-    float proj = __builtin_sqrtf(smax(dot3(t_i, t_i) * vg.to2, 1e-8f));
-    g.cos_phi = sclamp(rdiv(dot3(t_i, vg.t_o), proj), -1.0f, 1.0f);
+    // proj^2 lies in [1e-8, 16] and proj in [1e-4, 4] (t, v and l are unit
+    // or zero vectors): the mid-range root and, for |dot| >= 2^-60, divide
+    float proj = sqrt_rn_mid(smax(dot3(t_i, t_i) * vg.to2, 1e-8f));
+    const float num = dot3(t_i, vg.t_o);
+    float c = 0.0f;
+    if (div_mid_ok(num)) c = div_rn_mid(num, proj);
+    else c = rdiv(num, proj);
+    g.cos_phi = sclamp(c, -1.0f, 1.0f);
     g.t = vg.t;
     g.l = l;
     return g;
@@ -301,13 +299,6 @@ __device__ __forceinline__ BsdfGeom bsdf_geom_n(F3 t, F3 v, F3 light) {
 }
 
 #pragma clang fp contract(fast)
-
-// Single scattering M_p * N_p summed over R, TT, TRT (shade.wesl:217-219).
-__device__ __forceinline__ F3 single_scatter(const BsdfGeom& g, F3 sigma_a, const Shape& shape) {
-    Lobes n = azimuthal(g.cos_theta_d, g.cos_phi, sigma_a);
-    F3 m = gaussian3_r(shape.inv_beta, splat3(g.theta_h) - shape.alpha);
-    return m.x * n.r + m.y * n.tt + m.z * n.trt;
-}
 
 // Local backscattering lobe (shade.wesl:220); inv_sb = gaussian_rcp(sqrt(sigma_b2)).
 __device__ __forceinline__ F3 local_back_r(float theta_h, F3 a_back, F3 delta_b, F3 inv_sb) {
@@ -485,6 +476,10 @@ struct PixelShade {
     ViewGeom vg;  // the view's side of every light's bsdf_geom
     float view_dist, beta, roughness, cos_theta_o, ao;
     Shape shape;
+    // The light terms' per-pixel factors: -log2(e) sigma_a (absorption as a
+    // base-2 exponent), the Gaussian widths' reciprocals times sqrt(log2(e) / 2)
+    // (M_p = exp2(-((theta_h - alpha_p) gs_p)^2) ms_p) and over sqrt(2 pi).
+    F3 sig2, gs, ms;
 };
 
 __device__ __forceinline__ PixelShade pixel_setup(const View& p, uint64_t word, uint32_t beta_word, uint32_t x,
@@ -513,6 +508,9 @@ __device__ __forceinline__ PixelShade pixel_setup(const View& p, uint64_t word, 
     float tv = dot3(s.tangent, s.view);
     s.cos_theta_o = sqrtf(mx(1.0f - tv * tv, 1e-6f));
     s.sigma_a = absorption_from_color(s.albedo, s.roughness);
+    s.sig2 = s.sigma_a * -kLog2e;
+    s.gs = s.shape.inv_beta * kGaussLog2;
+    s.ms = s.shape.inv_beta * kInvSqrt2Pi;
     return s;
 }
 
@@ -535,13 +533,55 @@ __device__ __forceinline__ F3 dir_of(const hm_directional_light& L) {
 
 #pragma clang fp contract(fast)
 
-// The dual-scattering terms of the local back lobe that every light uses.
+// The dual-scattering terms of the local back lobe that every light uses,
+// once per pixel: 0.7 times the lobe (shade.wesl:220, 232) is
+// kb exp2(-((theta_h - delta_b) gb)^2) with r = gaussian_rcp(sqrt(sigma_b^2)),
+// kb = 0.7 * 2 a_back r / (sqrt(2 pi) pi) and gb = r sqrt(log2(e) / 2).
 struct LocalBack {
-    F3 a_back, delta_b, inv_sb;  // inv_sb = gaussian_rcp(sqrt(sigma_b^2)), once per pixel
+    F3 kb, gb, delta_b;
 };
 
 __device__ __forceinline__ LocalBack local_terms(const DualScatter& ds) {
-    return {ds.a_back, ds.delta_b, gaussian_rcp(vsqrt(ds.sigma_b2))};
+    const F3 r = gaussian_rcp(vsqrt(ds.sigma_b2));
+    return {ds.a_back * r * (1.4f * kInvSqrt2Pi / kPi), r * kGaussLog2, ds.delta_b};
+}
+
+// 0.7 times the local back lobe at theta_h.
+__device__ __forceinline__ F3 back_lobe(float theta_h, const LocalBack& lb) {
+    const F3 z = (splat3(theta_h) - lb.delta_b) * lb.gb;
+    return lb.kb * F3{__builtin_amdgcn_exp2f(-z.x * z.x), __builtin_amdgcn_exp2f(-z.y * z.y),
+                      __builtin_amdgcn_exp2f(-z.z * z.z)};
+}
+
+// Single scattering M_p N_p summed over R, TT and TRT (shade.wesl:217-219,
+// azimuthal :94-116) for one light's angles. The exponentials of each of
+// M_TT N_TT and M_TRT N_TRT are taken as one exp2 per channel: the
+// Gaussian's exponent, the absorption exp(-sigma_a k) and the azimuthal
+// weight exp(c0 cos phi + c1) summed in base 2.
+__device__ __forceinline__ F3 single_lobes(const BsdfGeom& g, const PixelShade& px) {
+    const float ct = g.cos_theta_d, cp = g.cos_phi;
+    const float inv_cd = frcp(ct);
+    const float a = frcp(1.19f * inv_cd + 0.36f * ct);
+    const float f_trt = fresnel(ct * 0.5f);
+    const float chp = fsqrt(sclamp(0.5f + 0.5f * cp, 0.0f, 1.0f));  // cos(phi / 2)
+    const float n_r = 0.25f * chp * fresnel(ct * chp);
+    const float h = chp * (1.0f + a * (0.6f - 0.8f * cp));
+    const float f_tt = fresnel(ct * fsqrt(sclamp(1.0f - h * h, 0.0f, 1.0f)));
+    const float s_tt = fsqrt(sclamp(1.0f - h * h * a * a, 0.0f, 1.0f));
+    const float th = g.theta_h;
+    const float dr = (th - px.shape.alpha.x) * px.gs.x;
+    const float dt = (th - px.shape.alpha.y) * px.gs.y;
+    const float dp = (th - px.shape.alpha.z) * px.gs.z;
+    const float r = __builtin_amdgcn_exp2f(-dr * dr) * px.ms.x * n_r;
+    const float e_tt = cp * (-3.65f * kLog2e) + (-3.98f * kLog2e) - dt * dt;
+    const float e_trt = cp * (17.0f * kLog2e) + (-16.78f * kLog2e) - dp * dp;
+    const float k_tt = 2.0f * s_tt * inv_cd, k_trt = 3.2f * inv_cd;
+    const float w_tt = px.ms.y * (1.0f - f_tt) * (1.0f - f_tt);
+    const float w_trt = px.ms.z * ((1.0f - f_trt) * (1.0f - f_trt) * f_trt);
+    auto ch = [&](float sig2) {
+        return r + w_tt * __builtin_amdgcn_exp2f(sig2 * k_tt + e_tt) + w_trt * __builtin_amdgcn_exp2f(sig2 * k_trt + e_trt);
+    };
+    return {ch(px.sig2.x), ch(px.sig2.y), ch(px.sig2.z)};
 }
 
 // The key light's multiple scattering per unit light colour: hair_bsdf
@@ -584,8 +624,8 @@ __device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelSha
     // safe_normalize twice, as the light loop (shade.wesl:457) and hair_bsdf
     // (shade.wesl:204) each normalize; the second one's vector is within a few
     // ulps of unit length, where safe_normalize_unit gives the same bits cheaper
-    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(safe_normalize(ltf)));
-    F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
+    BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(safe_normalize_mid(ltf)));
+    F3 lobe = single_lobes(g, px) + back_lobe(g.theta_h, lb);
     F3 c = lobe * point_color(L) * att;
     if (KIND == 1 || (KIND < 0 && L.kind != HM_LIGHT_POINT)) c = c * spot_mask(L, px.world);
     return c;
@@ -596,9 +636,8 @@ __device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelSha
 __device__ __forceinline__ F3 directional_term(const hm_directional_light& L, bool is_key, const PixelShade& px,
                                                const LocalBack& lb, float depth, F3 key_ms) {
     BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(dir_of(L)));
-    F3 single = single_scatter(g, px.sigma_a, px.shape);
-    F3 lobe = is_key ? hexp(-depth) * single + key_ms
-                     : single + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
+    F3 single = single_lobes(g, px);
+    F3 lobe = is_key ? hexp(-depth) * single + key_ms : single + back_lobe(g.theta_h, lb);
     return lobe * F3{L.color[0], L.color[1], L.color[2]};
 }
 
@@ -618,7 +657,7 @@ __device__ __forceinline__ F3 rect_term(const hm_rect_light& L, const PixelShade
     if (facing == 0.0f || falloff == 0.0f) return splat3(0.0f);
     float solid = L.width * L.height * facing * frcp(dist2);
     BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(incident));
-    F3 lobe = single_scatter(g, px.sigma_a, px.shape) + 0.7f * local_back_r(g.theta_h, lb.a_back, lb.delta_b, lb.inv_sb);
+    F3 lobe = single_lobes(g, px) + back_lobe(g.theta_h, lb);
     return lobe * F3{L.color[0], L.color[1], L.color[2]} * (solid * falloff);
 }
 
