@@ -278,7 +278,9 @@ __device__ __forceinline__ BsdfGeom bsdf_geom_u(const ViewGeom& vg, F3 l) {
     float sin_i = sclamp(dot3(vg.t, l), -1.0f, 1.0f);
     float theta_i = fasin(sin_i);
     g.theta_h = theta_i + vg.theta_o;
-    g.cos_theta_d = smax(fcos(0.5f * fabsf(theta_i - vg.theta_o)), 1e-3f);
+    // cos(|theta_i - theta_o| / 2): v_cos takes revolutions, and halving the
+    // reciprocal of 2 pi is exact, so this is fcos(0.5 |...|) bit for bit
+    g.cos_theta_d = smax(__builtin_amdgcn_cosf(fabsf(theta_i - vg.theta_o) * (0.5f * 0.15915494309189535f)), 1e-3f);
     F3 t_i = l - vg.t * sin_i;
     // proj^2 lies in [1e-8, 16] and proj in [1e-4, 4] (t, v and l are unit
     // or zero vectors): the mid-range root and, for |dot| >= 2^-60, divide
@@ -584,6 +586,18 @@ __device__ __forceinline__ F3 single_lobes(const BsdfGeom& g, const PixelShade& 
     return {ch(px.sig2.x), ch(px.sig2.y), ch(px.sig2.z)};
 }
 
+// single_lobes + back_lobe, the back lobe's weight taken in a fused multiply-add.
+__device__ __forceinline__ F3 lobes(const BsdfGeom& g, const PixelShade& px, const LocalBack& lb) {
+    const F3 single = single_lobes(g, px);
+    const float th = g.theta_h;
+    auto ch = [&](float sc, float kb, float gb, float db) {
+        const float z = (th - db) * gb;
+        return __builtin_fmaf(kb, __builtin_amdgcn_exp2f(-z * z), sc);
+    };
+    return {ch(single.x, lb.kb.x, lb.gb.x, lb.delta_b.x), ch(single.y, lb.kb.y, lb.gb.y, lb.delta_b.y),
+            ch(single.z, lb.kb.z, lb.gb.z, lb.delta_b.z)};
+}
+
 // The key light's multiple scattering per unit light colour: hair_bsdf
 // (shade.wesl:193-244) minus its direct single scattering,
 // direct * 0.7 * back + [depth > 1e-3] * global spread.
@@ -612,23 +626,24 @@ __device__ __forceinline__ T uniform_light(const T* lights, uint32_t i) {
     return __builtin_bit_cast(T, w);
 }
 
-// Point light L's contribution (shade.wesl:449-505) with back-lobe terms
-// `lb`; zero where the light's range does not reach the pixel, as the
-// cluster would not list the light.
+// Adds point light L's contribution (shade.wesl:449-505) with back-lobe
+// terms `lb` to `color`; nothing where the light's range does not reach the
+// pixel, as the cluster would not list the light.
 // KIND: 0 point light, 1 spot light, -1 by the light's own kind.
 template <int KIND = -1>
-__device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelShade& px, const LocalBack& lb) {
+__device__ __forceinline__ void point_add(const hm_point_light& L, const PixelShade& px, const LocalBack& lb, F3& color) {
     F3 ltf = point_pos(L) - px.world;
     float att = distance_attenuation(dot3(ltf, ltf), L.color_inverse_square_range[3]);
-    if (att == 0.0f) return splat3(0.0f);
+    if (att == 0.0f) return;
     // safe_normalize twice, as the light loop (shade.wesl:457) and hair_bsdf
     // (shade.wesl:204) each normalize; the second one's vector is within a few
     // ulps of unit length, where safe_normalize_unit gives the same bits cheaper
     BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(safe_normalize_mid(ltf)));
-    F3 lobe = single_lobes(g, px) + back_lobe(g.theta_h, lb);
-    F3 c = lobe * point_color(L) * att;
-    if (KIND == 1 || (KIND < 0 && L.kind != HM_LIGHT_POINT)) c = c * spot_mask(L, px.world);
-    return c;
+    const F3 lobe = lobes(g, px, lb);
+    if (KIND == 1 || (KIND < 0 && L.kind != HM_LIGHT_POINT)) att = att * spot_mask(L, px.world);
+    const F3 w = point_color(L) * att;
+    color = {__builtin_fmaf(lobe.x, w.x, color.x), __builtin_fmaf(lobe.y, w.y, color.y),
+             __builtin_fmaf(lobe.z, w.z, color.z)};
 }
 
 // Directional light i's contribution: for the key light (optical depth
@@ -636,8 +651,7 @@ __device__ __forceinline__ F3 point_term(const hm_point_light& L, const PixelSha
 __device__ __forceinline__ F3 directional_term(const hm_directional_light& L, bool is_key, const PixelShade& px,
                                                const LocalBack& lb, float depth, F3 key_ms) {
     BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(dir_of(L)));
-    F3 single = single_lobes(g, px);
-    F3 lobe = is_key ? hexp(-depth) * single + key_ms : single + back_lobe(g.theta_h, lb);
+    F3 lobe = is_key ? hexp(-depth) * single_lobes(g, px) + key_ms : lobes(g, px, lb);
     return lobe * F3{L.color[0], L.color[1], L.color[2]};
 }
 
@@ -657,8 +671,7 @@ __device__ __forceinline__ F3 rect_term(const hm_rect_light& L, const PixelShade
     if (facing == 0.0f || falloff == 0.0f) return splat3(0.0f);
     float solid = L.width * L.height * facing * frcp(dist2);
     BsdfGeom g = bsdf_geom_u(px.vg, safe_normalize_unit(incident));
-    F3 lobe = single_lobes(g, px) + back_lobe(g.theta_h, lb);
-    return lobe * F3{L.color[0], L.color[1], L.color[2]} * (solid * falloff);
+    return lobes(g, px, lb) * F3{L.color[0], L.color[1], L.color[2]} * (solid * falloff);
 }
 
 #pragma clang fp contract(off)
@@ -805,7 +818,7 @@ __device__ __forceinline__ F3 light_range(const ShadeArgs& a, uint32_t begin, ui
     hm_point_light L = uniform_light(a.points, begin);
     for (uint32_t i = begin; i < end; ++i) {
         const hm_point_light next = uniform_light(a.points, i + 1u < end ? i + 1u : i);
-        color = color + point_term<SPOT ? 1 : 0>(L, px, lb);
+        point_add<SPOT ? 1 : 0>(L, px, lb, color);
         L = next;
     }
     return color;
@@ -840,7 +853,7 @@ __device__ F3 pixel_color(const ShadeArgs& a, const PixelShade& px, bool valid, 
                     // read-only, wave-uniform: through the scalar cache in one
                     // load rather than as dependent vector loads
                     const hm_point_light L = uniform_light(a.points, i);
-                    if (valid) color = color + point_term(L, px, lb);
+                    if (valid) point_add(L, px, lb, color);
                 }
             }
         }
