@@ -138,7 +138,8 @@ class HmTuning(C.Structure):
     _fields_ = [("raster_flags", u32 * 3), ("raster_blocks_per_cu", u32 * 3), ("dom_tile", u32), ("cam_tile", u32),
                 ("bin_box_limit", u32), ("bin_records", u32), ("bin_blocks_per_cu", u32),
                 ("cam_chunk", u32), ("cam_bin_blocks_per_cu", u32), ("light_cull", u32),
-                ("dom_busy_blocks_per_cu", u32), ("frame_serial", u32), ("batch_lanes", u32)]
+                ("dom_busy_blocks_per_cu", u32), ("frame_serial", u32), ("batch_lanes", u32),
+                ("filter_tiles", u32)]
 
 
 RASTER_PRECHECK, RASTER_XCD = 1, 2

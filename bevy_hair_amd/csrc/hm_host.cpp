@@ -1419,7 +1419,12 @@ static hm_status filter_impl(hm_ctx* ctx, const hm_params* p, const TileLists& l
         HM_HIP(ctx, hm::launch_compact(ctx->stream, 1, ctx->center, ctx->cons, px, C, ctx->filtered,
                                        ctx->filtered_depth, lists.gate));
     }
-    HM_HIP(ctx, hm::launch_filter(ctx->stream, a, L, px, ctx->cu_count * 8));
+    // With the camera tiles' lists and no overflow gate (the frame's binned
+    // path) the tile kernel filters the same pixels, its taps from LDS.
+    if (lists.from_tiles && !lists.gate && ctx->tuning.filter_tiles)
+        HM_HIP(ctx, hm::launch_filter_tiles(ctx->stream, a, ctx->width, ctx->height));
+    else
+        HM_HIP(ctx, hm::launch_filter(ctx->stream, a, L, px, ctx->cu_count * 8));
     return HM_OK;
 }
 
@@ -1632,6 +1637,7 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* t) {
     if (t->dom_busy_blocks_per_cu > 2u) return fail(ctx, HM_ERR_INVALID, "dom_busy_blocks_per_cu must be 0, 1 or 2");
     if (t->frame_serial > 1u) return fail(ctx, HM_ERR_INVALID, "frame_serial must be 0 or 1");
     if (t->batch_lanes < 1u || t->batch_lanes > 4u) return fail(ctx, HM_ERR_INVALID, "batch_lanes must be in [1, 4]");
+    if (t->filter_tiles > 1u) return fail(ctx, HM_ERR_INVALID, "filter_tiles must be 0 or 1");
 
     ctx->tuning = *t;
     return HM_OK;
@@ -1767,6 +1773,27 @@ static hm_status autotune_search(hm_ctx* ctx, hm_groom* g, const hm_params param
             }
         }
         out.frame_serial = total[1] < total[0] ? 1u : 0u;
+    }
+    // Filter on the binned path: over the camera tiles' pixel lists, or by
+    // screen tiles with the taps in LDS (ahead on small views, where the
+    // per-pixel chain's latency dominates); whole frames, alternating.
+    if (params[0].flags[1] && out.cam_tile != 0u) {
+        float total[2] = {0.0f, 0.0f};
+        const uint32_t frames = reps < 4u ? 8u : 2u * reps;
+        for (uint32_t r = 0; r <= frames; ++r) {
+            for (uint32_t tiles = 0; tiles < 2u; ++tiles) {
+                ctx->tuning = out;
+                ctx->tuning.filter_tiles = tiles;
+                HM_HIP(ctx, hipEventRecord(e0, ctx->stream));
+                if ((st = hm_frame(ctx, g, params, opts)) != HM_OK) return st;
+                HM_HIP(ctx, hipEventRecord(e1, ctx->stream));
+                HM_HIP(ctx, hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) total[tiles] += ms;
+            }
+        }
+        out.filter_tiles = total[1] < total[0] ? 1u : 0u;
     }
 
     return HM_OK;

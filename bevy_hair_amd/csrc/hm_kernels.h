@@ -215,5 +215,9 @@ hipError_t launch_compact(hipStream_t s, int mode, const uint64_t* center, const
                           const PixelList& out, float4* color_out, float* depth_out, const uint32_t* gate = nullptr);
 hipError_t launch_shade(hipStream_t s, const ShadeArgs& a, const PixelList& list, uint32_t pixels, int blocks);
 hipError_t launch_filter(hipStream_t s, const FilterArgs& a, const PixelList& list, uint32_t pixels, int blocks);
+// The filter over every 16 x 16 tile of the screen, for the pixels with a
+// conservative word (the same pixels and results as launch_filter with the
+// camera tiles' lists; it writes nothing for the other pixels).
+hipError_t launch_filter_tiles(hipStream_t s, const FilterArgs& a, uint32_t width, uint32_t height);
 
 }  // namespace hm

@@ -6,7 +6,8 @@
 //   k_scan/k_expand   scan.wgsl:6-24        parallel scan, same StrandRef order
 //   k_raster<MODE>    raster.wgsl:225-337   one wave per strand, lane = segment
 //   k_shade           shade.wesl:523-593    one thread per pixel
-//   k_filter          filter.wgsl:20-135    one thread per pixel
+//   k_filter          filter.wgsl:20-135    one thread per listed pixel
+//   k_filter_tiles    filter.wgsl:20-135    one workgroup per 16 x 16 tile, taps from LDS
 #include "hm_kernels.h"
 
 #include <cstring>
@@ -349,10 +350,25 @@ __global__ __launch_bounds__(256, 3) void k_shade(ShadeArgs a, PixelList L) {
 
 // ---- filter.wgsl ----------------------------------------------------------------
 
-__device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p, int32_t x, int32_t y) {
+// Where filter_pixel takes its taps from: the centre word and colour of the
+// in-screen pixel (qx, qy).
+struct GlobalTaps {
+    const uint64_t* center;
+    const float4* shaded;
+    uint32_t width;
+    __device__ __forceinline__ size_t at(int32_t qx, int32_t qy) const {
+        return (size_t)(uint32_t)qy * width + (uint32_t)qx;
+    }
+    __device__ __forceinline__ uint64_t word(int32_t qx, int32_t qy) const { return center[at(qx, qy)]; }
+    __device__ __forceinline__ float4 color(int32_t qx, int32_t qy) const { return shaded[at(qx, qy)]; }
+};
+
+template <class Taps>
+__device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p, int32_t x, int32_t y,
+                                             const Taps& taps) {
     int32_t width = to_i32(p.screen.x), height = to_i32(p.screen.y);
     size_t pix = (size_t)y * (uint32_t)width + (uint32_t)x;
-    uint64_t center_word = a.center[pix];
+    uint64_t center_word = taps.word(x, y);
     uint64_t cons_word = a.cons[pix];
     bool center_hit = !gb_empty(center_word), cons_hit = !gb_empty(cons_word);
     uint64_t w0 = center_hit ? center_word : cons_word;
@@ -369,14 +385,16 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
         float l = len2(d);
         ts = l < 1e-3f ? F2{1.0f, 0.0f} : F2{rdiv(d.x, l), rdiv(d.y, l)};
     }
-    float4 cc = a.shaded[pix];
+    float4 cc = taps.color(x, y);
     const int radius = 5;
     float s_par = mx(p.filter_params.x, 0.05f), s_perp = mx(p.filter_params.y, 0.02f);
     float sigma_par = 5.0f * s_par, sigma_perp = 5.0f * s_perp;
     float spar2 = sigma_par * sigma_par, sperp2 = sigma_perp * sigma_perp;
-    // Tap weights are continuous: hardware reciprocal and exponential.
-    const float inv_spar2 = frcp(spar2), inv_sperp2 = frcp(sperp2);
-    const float inv_sigma_c2 = 1.0f / (0.9f * 0.9f);
+    // Tap weights are continuous: the spatial and colour Gaussians
+    // (filter.wgsl:99-115) as one exp2 of the summed exponents, the reciprocal
+    // widths with log2(e) folded in.
+    const float k_par = frcp(spar2) * 1.44269504088896341f, k_perp = frcp(sperp2) * 1.44269504088896341f;
+    const float k_col = 1.44269504088896341f / (0.9f * 0.9f);
     float p_norm = depth_code_fraction(depth_q);
     // filter.wgsl's tap depth test |rdiv(q, 16777214) - p_norm| <= 1.45e-3 as
     // an interval of depth codes: the quotient and the difference are
@@ -404,14 +422,13 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
     bool use_color = !promote && cc.w > 0.0f;
     F3 accum = splat3(0.0f);
     float weight = 0.0f, h_center = 0.0f, h_cons = 0.0f;
-    // One row of 11 taps at a time: the row's G-buffer words and colours
-    // are loaded together (one round trip) before the taps are summed in the
-    // reference's order.
+    // One row of 11 taps at a time: the row's centre words and colours are
+    // fetched together before the taps are summed in the reference's order.
     constexpr int K = 2 * radius + 1;
     for (int dy = -radius; dy <= radius; ++dy) {
         const int32_t qy = y + dy;
         const bool row_in = qy >= 0 && qy < height;
-        const size_t row = (size_t)(row_in ? qy : y) * (uint32_t)width;
+        const int32_t ry = row_in ? qy : y;
         uint64_t qw[K];
         float4 qc[K];
         bool in[K];
@@ -419,9 +436,9 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
         for (int j = 0; j < K; ++j) {
             const int32_t qx = x + j - radius;
             in[j] = row_in && qx >= 0 && qx < width;
-            const size_t q = row + (uint32_t)(in[j] ? qx : x);
-            qw[j] = a.center[q];
-            qc[j] = a.shaded[q];
+            const int32_t rx = in[j] ? qx : x;
+            qw[j] = taps.word(rx, ry);
+            qc[j] = taps.color(rx, ry);
         }
         float d_par[K], perp2[K];
         bool use[K];
@@ -436,7 +453,8 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
             const bool qc_empty = gb_empty(qw[j]);
             if (in[j] && perp2[j] <= __uint_as_float(0x3E800001u)) {
                 if (!qc_empty) h_center = h_center + 1.0f;
-                else if (!gb_empty(a.cons[row + (uint32_t)(x + j - radius)])) h_cons = h_cons + 1.0f;
+                else if (!gb_empty(a.cons[(size_t)(uint32_t)ry * (uint32_t)width + (uint32_t)(x + j - radius)]))
+                    h_cons = h_cons + 1.0f;
             }
             const uint32_t qd = gb_depth(qw[j]);
             use[j] = in[j] && !qc_empty && qd >= q_lo && qd <= q_hi;
@@ -444,15 +462,14 @@ __device__ __forceinline__ void filter_pixel(const FilterArgs& a, const View& p,
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             if (!use[j]) continue;
-            float spatial = fexp(-d_par[j] * d_par[j] * inv_spar2 - perp2[j] * inv_sperp2);
-            F3 q_color = {qc[j].x, qc[j].y, qc[j].z};
-            float color_w = 1.0f;
+            float e = __builtin_fmaf(d_par[j] * d_par[j], k_par, perp2[j] * k_perp);
             if (use_color) {
-                F3 dc = F3{cc.x, cc.y, cc.z} - q_color;
-                color_w = fexp(-dot3(dc, dc) * inv_sigma_c2);
+                const float dr = cc.x - qc[j].x, dg = cc.y - qc[j].y, db = cc.z - qc[j].z;
+                e = __builtin_fmaf(__builtin_fmaf(dr, dr, __builtin_fmaf(dg, dg, db * db)), k_col, e);
             }
-            float w = spatial * color_w;
-            accum = accum + q_color * w;
+            const float w = __builtin_amdgcn_exp2f(-e);
+            accum = {__builtin_fmaf(qc[j].x, w, accum.x), __builtin_fmaf(qc[j].y, w, accum.y),
+                     __builtin_fmaf(qc[j].z, w, accum.z)};
             weight = weight + w;
         }
     }
@@ -479,7 +496,68 @@ __global__ __launch_bounds__(256) void k_filter(FilterArgs a, PixelList L) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t pix = list_entry(L, pre, i);
         if (pix >= pixels) continue;
-        filter_pixel(a, p, (int32_t)(pix % width), (int32_t)(pix / width));
+        filter_pixel(a, p, (int32_t)(pix % width), (int32_t)(pix / width), GlobalTaps{a.center, a.shaded, width});
+    }
+}
+
+// The filter by 16 x 16 screen tiles, one workgroup per tile and one thread
+// per pixel: the pixels with a conservative word (the filter list's pixels: a
+// centre hit writes the conservative word too) are gathered, in pixel order,
+// into the first waves, and their taps read from an LDS copy of the tile's
+// 26 x 26 neighbourhood (centre words and colours; empty words off the
+// screen). The same filter_pixel as k_filter: the same results.
+constexpr int FILTER_TILE = 16, FILTER_HALO = FILTER_TILE + 10;
+
+struct LdsTaps {
+    const uint64_t* words;
+    const float4* colors;
+    int32_t bx, by;  // screen position of halo entry 0
+    __device__ __forceinline__ int at(int32_t qx, int32_t qy) const { return (qy - by) * FILTER_HALO + (qx - bx); }
+    __device__ __forceinline__ uint64_t word(int32_t qx, int32_t qy) const { return words[at(qx, qy)]; }
+    __device__ __forceinline__ float4 color(int32_t qx, int32_t qy) const { return colors[at(qx, qy)]; }
+};
+
+__global__ __launch_bounds__(256) void k_filter_tiles(FilterArgs a) {
+    static_assert(FILTER_TILE * FILTER_TILE == 256, "one thread per tile pixel");
+    __shared__ uint64_t s_word[FILTER_HALO * FILTER_HALO];
+    __shared__ float4 s_color[FILTER_HALO * FILTER_HALO];
+    __shared__ uint8_t s_pix[FILTER_TILE * FILTER_TILE];
+    __shared__ uint32_t s_wave[4];
+    const View& p = *reinterpret_cast<const View*>(&a.view);
+    const int32_t W = to_i32(p.screen.x), H = to_i32(p.screen.y);
+    const uint32_t tiles_x = ((uint32_t)W + FILTER_TILE - 1u) / FILTER_TILE;
+    const uint32_t tiles = tiles_x * (((uint32_t)H + FILTER_TILE - 1u) / FILTER_TILE);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int32_t ox = (int32_t)(tile % tiles_x) * FILTER_TILE, oy = (int32_t)(tile / tiles_x) * FILTER_TILE;
+        const int i = (int)threadIdx.x;
+        const int32_t x = ox + (i & (FILTER_TILE - 1)), y = oy + i / FILTER_TILE;
+        const bool cov = x < W && y < H && !gb_empty(a.cons[(size_t)(uint32_t)y * (uint32_t)W + (uint32_t)x]);
+        const uint64_t bal = __ballot(cov);
+        if (lane == 0u) s_wave[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t before = 0u;
+        for (uint32_t w = 0; w < wave; ++w) before += s_wave[w];
+        const uint32_t n = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        if (cov) s_pix[before + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (uint8_t)i;
+        if (n == 0u) {  // the same for every thread
+            __syncthreads();  // s_wave is written again for the next tile
+            continue;
+        }
+        for (int e = i; e < FILTER_HALO * FILTER_HALO; e += 256) {
+            const int32_t qx = ox - 5 + e % FILTER_HALO, qy = oy - 5 + e / FILTER_HALO;
+            const bool in = qx >= 0 && qy >= 0 && qx < W && qy < H;
+            const size_t q = in ? (size_t)(uint32_t)qy * (uint32_t)W + (uint32_t)qx : 0u;
+            s_word[e] = in ? a.center[q] : ~0ull;
+            s_color[e] = in ? a.shaded[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        __syncthreads();
+        if ((uint32_t)i < n) {
+            const uint32_t k = s_pix[i];
+            const LdsTaps taps{s_word, s_color, ox - 5, oy - 5};
+            filter_pixel(a, p, ox + (int32_t)(k & (FILTER_TILE - 1)), oy + (int32_t)(k / FILTER_TILE), taps);
+        }
+        __syncthreads();  // the next tile reuses the LDS copies
     }
 }
 
@@ -600,6 +678,13 @@ hipError_t launch_filter(hipStream_t s, const FilterArgs& a, const PixelList& li
     if (pixels == 0) return hipSuccess;
     uint32_t want = (pixels + 255) / 256;
     hipLaunchKernelGGL(k_filter, dim3(want < (uint32_t)blocks ? want : (uint32_t)blocks), dim3(256), 0, s, a, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_tiles(hipStream_t s, const FilterArgs& a, uint32_t width, uint32_t height) {
+    const uint32_t tiles = ((width + FILTER_TILE - 1u) / FILTER_TILE) * ((height + FILTER_TILE - 1u) / FILTER_TILE);
+    if (tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_filter_tiles, dim3(tiles), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -400,7 +400,8 @@ class Renderer:
                 "bin_records": int(t.bin_records), "bin_blocks_per_cu": int(t.bin_blocks_per_cu),
                 "cam_chunk": int(t.cam_chunk), "cam_bin_blocks_per_cu": int(t.cam_bin_blocks_per_cu),
                 "light_cull": int(t.light_cull), "dom_busy_blocks_per_cu": int(t.dom_busy_blocks_per_cu),
-                "frame_serial": int(t.frame_serial), "batch_lanes": int(t.batch_lanes)}
+                "frame_serial": int(t.frame_serial), "batch_lanes": int(t.batch_lanes),
+                "filter_tiles": int(t.filter_tiles)}
 
     def get_tuning(self) -> dict:
         t = N.HmTuning()
@@ -410,7 +411,7 @@ class Renderer:
     def set_tuning(self, raster_flags=None, raster_blocks_per_cu=None, dom_tile=None, bin_box_limit=None,
                    bin_records=None, bin_blocks_per_cu=None, cam_tile=None, cam_chunk=None,
                    cam_bin_blocks_per_cu=None, light_cull=None, dom_busy_blocks_per_cu=None, frame_serial=None,
-                   batch_lanes=None):
+                   batch_lanes=None, filter_tiles=None):
         """Changes the given launch choices; None keeps the current one."""
         cur = self.get_tuning()
         if raster_flags is not None:
@@ -423,13 +424,14 @@ class Renderer:
                      ("bin_blocks_per_cu", bin_blocks_per_cu), ("cam_chunk", cam_chunk),
                      ("cam_bin_blocks_per_cu", cam_bin_blocks_per_cu), ("light_cull", light_cull),
                      ("dom_busy_blocks_per_cu", dom_busy_blocks_per_cu), ("frame_serial", frame_serial),
-                     ("batch_lanes", batch_lanes)):
+                     ("batch_lanes", batch_lanes), ("filter_tiles", filter_tiles)):
             if v is not None:
                 cur[k] = int(v)
         t = N.HmTuning((C.c_uint32 * 3)(*cur["raster_flags"]), (C.c_uint32 * 3)(*cur["raster_blocks_per_cu"]),
                        cur["dom_tile"], cur["cam_tile"], cur["bin_box_limit"], cur["bin_records"],
                        cur["bin_blocks_per_cu"], cur["cam_chunk"], cur["cam_bin_blocks_per_cu"], cur["light_cull"],
-                       cur["dom_busy_blocks_per_cu"], cur["frame_serial"], cur["batch_lanes"])
+                       cur["dom_busy_blocks_per_cu"], cur["frame_serial"], cur["batch_lanes"],
+                       cur["filter_tiles"])
         self._check(N.lib().hm_set_tuning(self.ctx, C.byref(t)), "hm_set_tuning")
 
     def autotune(self, g: GpuGroom, params, opts, reps: int = 3) -> dict:

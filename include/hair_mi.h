@@ -482,6 +482,10 @@ typedef struct hm_tuning {
                                       joined at the end of the captured graph; the first lane is the
                                       context itself. A batch created while a scene depth is set
                                       uses one lane. The results are the same either way */
+    uint32_t filter_tiles;         /* hm_frame's filter on the binned path: 0 = one thread per pixel of
+                                      the camera tiles' lists, taps from memory; 1 = one workgroup per
+                                      16 x 16 screen tile, taps from an LDS copy of its neighbourhood.
+                                      The results are the same either way */
 } hm_tuning;
 
 hm_status hm_get_tuning(hm_ctx* ctx, hm_tuning* out);
@@ -492,7 +496,8 @@ hm_status hm_set_tuning(hm_ctx* ctx, const hm_tuning* tuning);
  * 512 / 1024 / 2048 records per tile workgroup (cam_chunk); DOM tiles 16 / 32 x
  * binning workgroups per CU 1 / 2 / 4, 16-texel tiles also with persistent tile
  * workgroups (dom_busy_blocks_per_cu 0 / 1 / 2); then whole frames with the DOM
- * pass beside or after the camera pass (frame_serial). Leaves the targets
+ * pass beside or after the camera pass (frame_serial), and with the filter
+ * over the pixel lists or by screen tiles (filter_tiles). Leaves the targets
  * dirty: run a frame afterwards. */
 hm_status hm_autotune(hm_ctx* ctx, hm_groom* groom, const hm_params params[3], const hm_frame_opts* opts,
                       uint32_t reps, hm_tuning* best);

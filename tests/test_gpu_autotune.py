@@ -42,7 +42,7 @@ def test_autotune_choice_and_results(renderer):
         if t["cam_tile"]:
             assert t["cam_chunk"] in (64, 128, 256, 512, 1024, 2048)
         assert t["bin_blocks_per_cu"] in (1, 2, 4)
-        assert t["dom_busy_blocks_per_cu"] in (0, 1, 2) and t["frame_serial"] in (0, 1)
+        assert t["dom_busy_blocks_per_cu"] in (0, 1, 2) and t["frame_serial"] in (0, 1) and t["filter_tiles"] in (0, 1)
         assert all(f in (0, 1, 2, 3) for f in t["raster_flags"])
         assert all(b in (4, 8, 16, 32) for b in t["raster_blocks_per_cu"])
         r.hair_pass(g, sc.params, opts)
