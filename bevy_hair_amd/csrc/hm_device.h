@@ -84,6 +84,19 @@ __device__ __forceinline__ float div_rn_mid(float a, float b) {
     const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
     return __builtin_fmaf(__builtin_fmaf(-b, q1, a), y, q1);
 }
+// div_below_zero / div_above_zero (hm_exact.h, checked there on the host)
+// with the division of their rare branch kept behind it: the compiler turns
+// the branch into a select otherwise, and divides for every call.
+__device__ __forceinline__ bool div_below_zero_rare(float a, float b) {
+    if (fabsf(a) >= 0x1p-60f && b <= 0x1p60f) return a < 0.0f;
+    asm volatile("" : "+v"(a));
+    return a / b < 0.0f;
+}
+__device__ __forceinline__ bool div_above_zero_rare(float a, float b) {
+    if (fabsf(a) >= 0x1p-60f && b <= 0x1p60f) return a > 0.0f;
+    asm volatile("" : "+v"(a));
+    return a / b > 0.0f;
+}
 __device__ __forceinline__ bool div_mid_ok(float a) {
     const float m = fabsf(a);
     return m >= 0x1p-60f && m <= 0x1p60f;

@@ -293,7 +293,7 @@ struct Drawer {
                 float clip_depth = 0.0f;  // cz / cw, formed only when put() compares it per pixel
                 bool visible = vd > p.screen.z;
                 if (visible && sign_only)
-                    visible = !(p.pass_mode[2] == 1u ? div_below_zero(cz, cw) : div_above_zero(cz, cw));
+                    visible = !(p.pass_mode[2] == 1u ? div_below_zero_rare(cz, cw) : div_above_zero_rare(cz, cw));
                 else if (visible)
                     clip_depth = rdiv(cz, cw);
                 if (visible) {
