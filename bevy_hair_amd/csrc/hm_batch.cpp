@@ -21,6 +21,10 @@ struct hm_batch {
     std::vector<hm_batch_view> views;
     std::vector<void*> out;  // per view: rgba float4[px], then depth float[px]
     std::vector<hm_ctx*> lanes;  // lanes[0] is the context; the others are owned here
+    // The groom each view renders: its own on lane 0, on the other lanes a
+    // shadow with per-pass buffers of the lane's own (owned here).
+    std::vector<hm_groom*> grooms;
+    std::vector<hm_groom*> shadows;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -49,17 +53,20 @@ void release(hm_batch* b) {
     if (b->fork) (void)hipEventDestroy(b->fork);
     for (hipEvent_t e : b->join)
         if (e) (void)hipEventDestroy(e);
+    for (size_t k = 1; k < b->lanes.size(); ++k) (void)hipStreamSynchronize(b->lanes[k]->stream);
+    for (hm_groom* g : b->shadows) hm_groom_free(nullptr, g);
     for (size_t k = 1; k < b->lanes.size(); ++k) hm_destroy(b->lanes[k]);
     for (void* p : b->out)
         if (p) (void)hipFree(p);
     delete b;
 }
 
-// One view on lane `c`: targets, frame, copy of the final colour and depth.
-hm_status record_view(hm_ctx* c, const hm_batch_view& v, void* out) {
+// One view on lane `c` with groom `g` (the view's own or the lane's shadow of
+// it): targets, frame, copy of the final colour and depth.
+hm_status record_view(hm_ctx* c, hm_groom* g, const hm_batch_view& v, void* out) {
     hm_status st = hm_targets(c, v.width, v.height, v.dom_size);
     if (st != HM_OK) return st;
-    if ((st = hm_frame(c, v.groom, v.params, &v.opts)) != HM_OK) return st;
+    if ((st = hm_frame(c, g, v.params, &v.opts)) != HM_OK) return st;
     const size_t px = (size_t)v.width * v.height;
     const bool filtered = v.params[0].flags[1] != 0u;
     const void* rgba = filtered ? (const void*)c->filtered : (const void*)c->shaded;
@@ -82,7 +89,7 @@ hm_status record(hm_ctx* ctx, hm_batch* b) {
     hm_status st = HM_OK;
     for (size_t i = 0; i < b->views.size() && st == HM_OK; ++i) {
         hm_ctx* c = b->lanes[i % nl];
-        st = record_view(c, b->views[i], b->out[i]);
+        st = record_view(c, b->grooms[i], b->views[i], b->out[i]);
         if (st != HM_OK && c != ctx) ctx->err = "batch lane " + std::to_string(i % nl) + ": " + c->err;
     }
     for (size_t k = 1; k < nl; ++k) {
@@ -142,6 +149,27 @@ hm_status hm_batch_create(hm_ctx* ctx, const hm_batch_view* views, uint32_t coun
             release(b);
             return hm_hip_fail(ctx, e, "hm_batch_create (lane events)");
         }
+    }
+    // Lanes other than the first render through shadows of the grooms: a
+    // groom's LOD records and strand refs are rewritten by every frame of it,
+    // and two lanes may render the same groom at the same time.
+    b->grooms.resize(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const size_t lane = i % b->lanes.size();
+        b->grooms[i] = views[i].groom;
+        if (lane == 0) continue;
+        hm_groom* shadow = nullptr;
+        for (uint32_t j = 0; j < i && !shadow; ++j)
+            if (j % b->lanes.size() == lane && views[j].groom == views[i].groom) shadow = b->grooms[j];
+        if (!shadow) {
+            hm_status gst = hm_groom_shadow(ctx, views[i].groom, &shadow);
+            if (gst != HM_OK) {
+                release(b);
+                return gst;
+            }
+            b->shadows.push_back(shadow);
+        }
+        b->grooms[i] = shadow;
     }
     // Per-kernel timing records events on the stream: keep it out of the graph.
     const uint32_t slots = ctx->slots;

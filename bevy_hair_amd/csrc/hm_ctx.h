@@ -29,6 +29,9 @@ struct hm_groom {
     // Per pass: the most control points the last hm_lod can have given a
     // strand (lod.wgsl:38-55 with the override); 127 before any hm_lod.
     uint32_t ncp_bound[2] = {127u, 127u};
+    // A batch lane's shadow (hm_groom_shadow): bundles, layers and style are
+    // the groom's own, never freed here.
+    bool borrowed = false;
 };
 
 // Work buffers of the binned DOM (sized for the largest groom and map seen).
@@ -155,6 +158,11 @@ void hm_mlp_release(hm_ctx* ctx);
 // keep those buffers; its allocation generation tells when it did not).
 // No scene depth, no timing slots; freed with hm_destroy.
 hm_status hm_lane_create(hm_ctx* parent, hm_ctx** out);
+// A batch lane's shadow of groom g: g's bundles, layers and style (read
+// only), and per-pass LOD records, strand refs, prefix and indirect words of
+// its own, so that lanes rendering g at the same time do not share them.
+// Freed with hm_groom_free; g must outlive it.
+hm_status hm_groom_shadow(hm_ctx* ctx, const hm_groom* g, hm_groom** out);
 // The lane's learned-shading state: the parent's weights, its own flags.
 hm_status hm_mlp_borrow(hm_ctx* lane, const hm_ctx* parent);
 // Looks up the allocation-free scan's scratch address on `device` (current).

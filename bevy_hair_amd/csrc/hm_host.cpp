@@ -540,6 +540,40 @@ void hm_destroy(hm_ctx* c) {
 
 }  // extern "C"
 
+hm_status hm_groom_shadow(hm_ctx* ctx, const hm_groom* g, hm_groom** out) {
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    hm_groom* s = new hm_groom();
+    s->borrowed = true;
+    s->nb = g->nb;
+    s->nl = g->nl;
+    s->ns = g->ns;
+    s->strands = g->strands;
+    s->capacity = g->capacity;
+    std::memcpy(s->bmin, g->bmin, 12);
+    std::memcpy(s->bmax, g->bmax, 12);
+    s->bundles = g->bundles;
+    s->layers = g->layers;
+    s->style = g->style;
+    hipError_t e = hipSuccess;
+    auto A = [&](auto** p, size_t bytes) {
+        if (e == hipSuccess) e = dev_alloc(p, bytes);
+    };
+    for (int k = 0; k < 2; ++k) {
+        A(&s->lod[k], (size_t)s->nb * sizeof(hm_lod_record));
+        A(&s->refs[k], (size_t)s->capacity * sizeof(hm_strand_ref));
+        A(&s->prefix[k], ((size_t)s->nb + 1) * 4);
+        A(&s->indirect[k], 16);
+    }
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMemset(s->indirect[k], 0, 16);
+    if (e != hipSuccess) {
+        hm_groom_free(nullptr, s);
+        return hip_fail(ctx, e, "batch lane groom");
+    }
+    *out = s;
+    return HM_OK;
+}
+
 hm_status hm_lane_create(hm_ctx* parent, hm_ctx** out) {
     *out = nullptr;
     hm_ctx* c = nullptr;
@@ -640,6 +674,11 @@ void hm_groom_free(hm_ctx* ctx, hm_groom* g) {
     if (ctx) {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
+    }
+    if (g->borrowed) {  // the shadowed groom's buffers
+        g->bundles = nullptr;
+        g->layers = nullptr;
+        g->style = nullptr;
     }
     dev_free(g->bundles);
     dev_free(g->layers);

@@ -70,6 +70,55 @@ def test_batch_replays_equal_one_view_at_a_time(renderer, mlp, lanes):
         renderer.set_tuning(batch_lanes=keep)
 
 
+@pytest.mark.parametrize("lanes", [2, 3])
+def test_batch_lanes_with_per_view_lod(renderer, lanes):
+    """Views of the same groom with the reference's screen-space LOD (no
+    override) at distances and sizes that give each a different LOD record,
+    dealt so that several lanes render one groom at the same time: each lane
+    keeps its own LOD records and strand refs, and every replay equals the
+    views rendered one at a time."""
+    print(f"\n[gpu batch] per-view LOD, lanes={lanes}", flush=True)
+    keep = renderer.get_tuning()["batch_lanes"]
+    renderer.set_tuning(batch_lanes=lanes)
+    meshes = [synthetic_groom("wavy", 2048, 11), synthetic_groom("straight", 3072, 4)]
+    grooms = [plugin.HairGroom(filter=True), plugin.HairGroom(filter=True)]
+    dirs = [scene.key_light()]
+    renderer.set_scene_depth(None, 0.0)
+    renderer.set_lights(scene.point_lights(np.random.default_rng(9), 3,
+                                           0.5 * (meshes[0].bounds_min + meshes[0].bounds_max)), dirs)
+    gpu = [renderer.upload(m) for m in meshes]
+    views = []
+    for v in range(8):
+        gi = (v // 3) % 2  # runs of three views per groom: every lane count puts one groom on several lanes
+        w, h = (640, 360) if v % 2 else (320, 180)
+        cam = scene.fit_camera(meshes[gi], w, h, heading=(120.0 - 35.0 * v, 0.0, 50.0), margin=1.1 + 1.7 * v)
+        params = plugin.Renderer.make_params(cam, grooms[gi], meshes[gi], 256, dirs)
+        views.append((gpu[gi], params, w, h, 256, renderer.frame_opts()))
+    batch = None
+    try:
+        lods = set()
+        want = []
+        for g, params, w, h, dom, opts in views:
+            renderer.targets(w, h, dom)
+            renderer.hair_pass(g, params, opts)
+            want.append([a.copy() for a in renderer.filtered()])
+            lods.add(renderer.lod_records(g, 0).tobytes())
+        assert len(lods) >= 4, "the views should not share their LOD records"
+        batch = renderer.batch(views)
+        for rep in range(2):
+            batch.run(1)
+            for i in range(len(views)):
+                rgba, depth = batch.read(i)
+                assert np.array_equal(rgba, want[i][0]), f"replay {rep}: view {i} colour differs from hm_frame"
+                assert np.array_equal(depth, want[i][1]), f"replay {rep}: view {i} depth differs from hm_frame"
+    finally:
+        if batch is not None:
+            batch.free()
+        for g in gpu:
+            g.free()
+        renderer.set_tuning(batch_lanes=keep)
+
+
 def test_batch_with_scene_depth(renderer):
     """With a scene depth set the views stay in one chain (the depth belongs to
     the context's target): four views of one size in front of a checkerboard
