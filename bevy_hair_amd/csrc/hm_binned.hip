@@ -390,15 +390,22 @@ __global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
 // and BIN_SPLIT set when some tile has more than one chunk.
 // With `busy` it also lists the tiles that hold records, in tile order, and
 // their number in BIN_BUSY.
-__global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tiles, uint32_t record_capacity,
-                                                   uint32_t long_capacity, uint32_t list_capacity, uint32_t chunk,
-                                                   uint32_t* chunk_tile, uint32_t chunk_capacity, uint32_t* busy) {
-    constexpr uint32_t PER = BIN_MAX_TILES / 1024u;
+// NT threads over at most MAXT tiles. A grid of at most 2048 tiles (the
+// camera pass at 1080p) takes the 256-thread, 9 KB variant: a workgroup that
+// small starts as soon as one workgroup of the kernel running beside it on
+// the other stream retires, where a 1024-thread one with 70 KB of LDS waits
+// for a CU to drain (measured: the camera's scan took 61 us instead of 12
+// beside the DOM generation).
+template <uint32_t NT, uint32_t MAXT>
+__global__ __launch_bounds__(NT) void k_bin_scan(uint32_t* meta, uint32_t tiles, uint32_t record_capacity,
+                                                 uint32_t long_capacity, uint32_t list_capacity, uint32_t chunk,
+                                                 uint32_t* chunk_tile, uint32_t chunk_capacity, uint32_t* busy) {
+    constexpr uint32_t PER = MAXT / NT, NW = NT / 64u;
     // Tile i sits at word i + i / PER: a thread's run of PER tiles starts
     // PER + 1 words after its neighbour's, so the run-wise accesses below hit
     // distinct banks, as do the tile-wise ones.
-    __shared__ uint32_t stage[BIN_MAX_TILES + BIN_MAX_TILES / PER];
-    __shared__ uint32_t wave_total[16], wave_chunks[16], wave_busy[16], split, ok_s;
+    __shared__ uint32_t stage[MAXT + MAXT / PER];
+    __shared__ uint32_t wave_total[NW], wave_chunks[NW], wave_busy[NW], split, ok_s;
     auto pad = [](uint32_t i) { return i + i / PER; };
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t* cnt = meta + BIN_META_WORDS;
@@ -411,12 +418,12 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
     uint32_t ld[PER];
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t i = tid + 1024u * k;
+        const uint32_t i = tid + NT * k;
         ld[k] = cnt[i < tiles ? i : 0u];
     }
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t i = tid + 1024u * k;
+        const uint32_t i = tid + NT * k;
         stage[pad(i)] = i < tiles ? ld[k] : 0u;
     }
     __syncthreads();
@@ -458,7 +465,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
 #pragma unroll
         for (uint32_t k = 0; k < PER; ++k)
             if (b + k < tiles && v[k] != 0u) busy[brun++] = b + k;
-        if (tid == 1023u) meta[BIN_BUSY] = brun;
+        if (tid == NT - 1u) meta[BIN_BUSY] = brun;
     }
     // Record offsets through LDS, then out (offset and cursor arrays).
     uint32_t r = run, cr = crun;
@@ -471,7 +478,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
     }
     if (any_split) split = 1u;
     __syncthreads();
-    for (uint32_t i = tid; i < tiles; i += 1024u) {
+    for (uint32_t i = tid; i < tiles; i += NT) {
         const uint32_t o = stage[pad(i)];
         off[i] = o;
         cur[i] = o;
@@ -485,13 +492,12 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
         cr += nch[k];
     }
     __syncthreads();
-    const uint32_t chunks = wave_chunks[0] + wave_chunks[1] + wave_chunks[2] + wave_chunks[3] + wave_chunks[4] +
-                            wave_chunks[5] + wave_chunks[6] + wave_chunks[7] + wave_chunks[8] + wave_chunks[9] +
-                            wave_chunks[10] + wave_chunks[11] + wave_chunks[12] + wave_chunks[13] + wave_chunks[14] +
-                            wave_chunks[15];
-    for (uint32_t i = tid; i < tiles; i += 1024u) choff[i] = stage[pad(i)];
+    uint32_t chunks = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) chunks += wave_chunks[w];
+    for (uint32_t i = tid; i < tiles; i += NT) choff[i] = stage[pad(i)];
     const uint32_t nmap = !chunk_tile ? 0u : (chunks < chunk_capacity ? chunks : chunk_capacity);
-    for (uint32_t c = tid; c < nmap; c += 1024u) {
+    for (uint32_t c = tid; c < nmap; c += NT) {
         uint32_t lo = 0u, hi = tiles;  // last tile whose first chunk <= c
         while (hi - lo > 1u) {
             const uint32_t mid = (lo + hi) >> 1;
@@ -500,7 +506,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* meta, uint32_t tile
         }
         chunk_tile[c] = lo;
     }
-    if (tid == 1023u) {
+    if (tid == NT - 1u) {
         const uint32_t total = r;
         off[tiles] = total;
         choff[tiles] = chunks;
@@ -885,9 +891,14 @@ hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks,
     const size_t lds = (size_t)a.tiles * sizeof(uint32_t);
     hipLaunchKernelGGL(k_bin_count, dim3(bin_blocks), dim3(256), lds, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, a.meta, a.tiles, a.record_capacity, a.long_capacity,
-                       a.list_capacity, a.chunk, a.chunk_tile, a.chunk_capacity,
-                       mode == 1 ? a.meta + BIN_META_WORDS + 4u * a.tiles + 2u : nullptr);
+    uint32_t* busy = mode == 1 ? a.meta + BIN_META_WORDS + 4u * a.tiles + 2u : nullptr;
+    if (a.tiles <= 2048u)
+        hipLaunchKernelGGL((k_bin_scan<256u, 2048u>), dim3(1), dim3(256), 0, s, a.meta, a.tiles, a.record_capacity,
+                           a.long_capacity, a.list_capacity, a.chunk, a.chunk_tile, a.chunk_capacity, busy);
+    else
+        hipLaunchKernelGGL((k_bin_scan<1024u, BIN_MAX_TILES>), dim3(1), dim3(1024), 0, s, a.meta, a.tiles,
+                           a.record_capacity, a.long_capacity, a.list_capacity, a.chunk, a.chunk_tile,
+                           a.chunk_capacity, busy);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin_fill, dim3(bin_blocks), dim3(256), lds, s, a);
     return hipGetLastError();
