@@ -491,8 +491,11 @@ hm_status hm_create(int device, hm_ctx** out) {
         hm_destroy(c);
         return HM_ERR_HIP;
     }
+    // The fork and join events only order the context's two streams on one
+    // device and the host never waits on them, so they leave out the
+    // system-scope fence an event record makes by default.
     for (int j = 0; j < 3; ++j) {
-        if (hipEventCreateWithFlags(&c->fork_ev[j], hipEventDisableTiming) != hipSuccess) {
+        if (hipEventCreateWithFlags(&c->fork_ev[j], hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
             hm_destroy(c);
             return HM_ERR_HIP;
         }
