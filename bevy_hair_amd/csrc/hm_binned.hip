@@ -283,8 +283,14 @@ __device__ __forceinline__ void gen_pair(const BinArgs& a, const View& p, uint32
     }
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
+// MAXS: the most slots a pair of the pass can need, ceil(max ncp / 32), from
+// the host's control-point bound. Passes of at most 96 control points (65 with
+// LOD on) leave out the four-slot instance, whose register arrays set the
+// kernel's register count, and ask for five (camera) or six (DOM) waves per
+// SIMD.
+template <int MODE, int MAXS>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MAXS <= 3 ? (MODE == 1 ? 6 : 5) : 1, 8))) void k_bin_gen(BinArgs a) {
     const View& p = bin_view(a);
     // Status words and tile counts to zero for this pass's k_bin_count and
     // k_bin_scan; the long-segment cursor, the only word written here, is
@@ -313,8 +319,8 @@ __global__ __launch_bounds__(256) void k_bin_gen(BinArgs a) {
         const uint32_t slots = __builtin_amdgcn_readfirstlane((need + 31u) / 32u);
         if (slots <= 1u) gen_pair<MODE, 1>(a, p, s, live, reuse_total, r, gb);
         else if (slots == 2u) gen_pair<MODE, 2>(a, p, s, live, reuse_total, r, gb);
-        else if (slots == 3u) gen_pair<MODE, 3>(a, p, s, live, reuse_total, r, gb);
-        else gen_pair<MODE, 4>(a, p, s, live, reuse_total, r, gb);
+        else if (MAXS <= 3 || slots == 3u) gen_pair<MODE, 3>(a, p, s, live, reuse_total, r, gb);
+        else gen_pair<MODE, MAXS <= 3 ? 3 : 4>(a, p, s, live, reuse_total, r, gb);
     }
 }
 
@@ -884,8 +890,12 @@ __global__ __launch_bounds__(256) void k_dom_long(BinArgs a) {
 hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks, int bin_blocks, hipEvent_t after_gen) {
     if (a.tiles > BIN_MAX_TILES) return hipErrorInvalidValue;
     hipError_t e;
-    if (mode == 0) hipLaunchKernelGGL(k_bin_gen<0>, dim3(gen_blocks), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_bin_gen<1>, dim3(gen_blocks), dim3(256), 0, s, a);
+    // a strand has at most 2^seg_shift + 1 control points
+    const bool four = a.seg_shift > 6u;
+    if (mode == 0 && four) hipLaunchKernelGGL((k_bin_gen<0, 4>), dim3(gen_blocks), dim3(256), 0, s, a);
+    else if (mode == 0) hipLaunchKernelGGL((k_bin_gen<0, 3>), dim3(gen_blocks), dim3(256), 0, s, a);
+    else if (four) hipLaunchKernelGGL((k_bin_gen<1, 4>), dim3(gen_blocks), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_bin_gen<1, 3>), dim3(gen_blocks), dim3(256), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (after_gen && (e = hipEventRecord(after_gen, s)) != hipSuccess) return e;
     const size_t lds = (size_t)a.tiles * sizeof(uint32_t);
