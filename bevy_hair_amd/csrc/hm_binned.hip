@@ -1005,9 +1005,10 @@ __device__ __forceinline__ void finish_pixel(const BinArgs& a, uint32_t tile, bo
 // and takes its segments with LDS atomics. A tile with one chunk is written as
 // it is (this is its clear); the chunks of a larger tile take the minimum into
 // the G-buffer, cleared beforehand, with global atomics, and k_cam_finish
-// then emits that tile's pixel lists.
+// then emits that tile's pixel lists. Six waves per SIMD fit in 79 VGPRs
+// without spills (the compiler's own choice, 87, allowed five); seven spill.
 template <int T>
-__global__ __launch_bounds__(256) void k_cam_tiles(BinArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_cam_tiles(BinArgs a) {
     static_assert((T * T) % 256 == 0, "the write-back loop runs whole waves");
     if (a.meta[BIN_OK] == 0u) return;
     const uint32_t* off = a.meta + BIN_META_WORDS + a.tiles;
