@@ -329,7 +329,8 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 3 ? (MODE == 1 ? 6 : 5) : 1, 8))) voi
 // segment ids, adds each nonzero entry to the global count once, and keeps
 // the count's previous value, its rank within the tile, as a (tile, rank)
 // list for k_bin_fill.
-__global__ __launch_bounds__(256) void k_bin_count(BinArgs a) {
+template <uint32_t NT>
+__global__ __launch_bounds__(NT) void k_bin_count(BinArgs a) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t touched, list_base, list_pos;
     for (uint32_t t = threadIdx.x; t < a.tiles; t += blockDim.x) hist[t] = 0u;
@@ -535,7 +536,8 @@ __global__ __launch_bounds__(NT) void k_bin_scan(uint32_t* meta, uint32_t tiles,
 
 // Same runs as k_bin_count: the workgroup's (tile, rank) list gives the first
 // slot of its block in each touched tile; slots are handed out in LDS.
-__global__ __launch_bounds__(256) void k_bin_fill(BinArgs a) {
+template <uint32_t NT>
+__global__ __launch_bounds__(NT) void k_bin_fill(BinArgs a) {
     if (a.meta[BIN_OK] == 0u) return;
     extern __shared__ uint32_t hist[];
     const uint32_t* off = a.meta + BIN_META_WORDS + a.tiles;
@@ -899,7 +901,13 @@ hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks,
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (after_gen && (e = hipEventRecord(after_gen, s)) != hipSuccess) return e;
     const size_t lds = (size_t)a.tiles * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_bin_count, dim3(bin_blocks), dim3(256), lds, s, a);
+    // A DOM pass's histogram (one word per tile: 64 KB at S = 2048 with 16-texel
+    // tiles) leaves room for two workgroups per CU; with 1024 threads each they
+    // keep eight waves per SIMD busy rather than two. The camera pass keeps
+    // 256 threads (with 1024 the 2160p frame was slower).
+    const bool wide = mode == 1 && a.tiles > 2048u;
+    if (wide) hipLaunchKernelGGL(k_bin_count<1024u>, dim3(bin_blocks), dim3(1024), lds, s, a);
+    else hipLaunchKernelGGL(k_bin_count<256u>, dim3(bin_blocks), dim3(256), lds, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     uint32_t* busy = mode == 1 ? a.meta + BIN_META_WORDS + 4u * a.tiles + 2u : nullptr;
     if (a.tiles <= 2048u)
@@ -910,7 +918,8 @@ hipError_t launch_bin(hipStream_t s, const BinArgs& a, int mode, int gen_blocks,
                            a.record_capacity, a.long_capacity, a.list_capacity, a.chunk, a.chunk_tile,
                            a.chunk_capacity, busy);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_fill, dim3(bin_blocks), dim3(256), lds, s, a);
+    if (wide) hipLaunchKernelGGL(k_bin_fill<1024u>, dim3(bin_blocks), dim3(1024), lds, s, a);
+    else hipLaunchKernelGGL(k_bin_fill<256u>, dim3(bin_blocks), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
