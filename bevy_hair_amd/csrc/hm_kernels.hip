@@ -311,9 +311,11 @@ __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ ce
     }
 }
 
-// F: SHADE_* variant bits (pixel_color).
+// F: SHADE_* variant bits (pixel_color). The plain variant fits four waves
+// per SIMD (128 VGPRs, a few spills outside the light loop); the others spill
+// too much there and keep three.
 template <uint32_t F>
-__global__ __launch_bounds__(256, 3) void k_shade(ShadeArgs a, PixelList L) {
+__global__ __launch_bounds__(256, F == 0u ? 4 : 3) void k_shade(ShadeArgs a, PixelList L) {
     __shared__ uint32_t pre[LIST_SUBS + 1];
     list_prefix(L, pre);
     const View& p = *reinterpret_cast<const View*>(&a.view);
